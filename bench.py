@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Benchmark of the field-evaluation + meshing hot path on one MI355X.
+
+Workload (one "step"): torus R=1.0 r=0.4 with 6-octave ridged gradient noise and a
+Voronoi F1 layer (jitter 0.9) blended by a low-frequency gradient mask, seed 42,
+evaluated in FP32 on a 512^3 grid over [-1.6, 1.6]^3, then marching cubes at iso 0
+(count + emit, vertices deduplicated, with normals).  Inputs are generated on the
+device; nothing crosses PCIe inside the timed region except the 8-byte mesh count that
+marching cubes reads back to size its output.
+
+Prints one JSON line.  `value` = grid samples per second of the whole step (field +
+meshing) in Gsamples/s.  Also reported: the field kernel alone, marching cubes in
+cells/s, the fused forward+adjoint kernel, rooflines, and the CPU oracle timed on a
+bounded slab of the same grid (the reference binary cannot be built: its cage engine
+is absent).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, vendor figure (packed FP32)
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+
+
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(spec, grid, seconds: float):
+    """The CPU oracle (FP32, -O3, OpenMP over z-slabs) on consecutive z-slabs of the
+    same grid until about `seconds` of work; returns Gsamples/s and the sample used."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+
+    from unnatural_planets_amd import spec as S
+
+    o = Oracle()
+    threads = int(o.L.upfo_num_threads())
+    nx, ny, nz = S.grid_dims(grid)
+
+    def run(k0, nslab):
+        sub = S.Grid()
+        for a in range(3):
+            sub.origin[a], sub.spacing[a], sub.dims[a] = grid.origin[a], grid.spacing[a], grid.dims[a]
+        # slab k0..k0+nslab: shift the origin by whole nodes (same node coordinates as
+        # long as origin + k*spacing is recomputed; the slab is a timing sample)
+        sub.origin[2] = np.float32(grid.origin[2]) + np.float32(k0) * np.float32(grid.spacing[2])
+        sub.dims[2] = nslab
+        t = time.perf_counter()
+        o.eval_grid(spec, sub)
+        return time.perf_counter() - t
+
+    t1 = run(nz // 2, 2)
+    per_slab = t1 / 2
+    nslab = int(max(2, min(nz, seconds / max(per_slab, 1e-6))))
+    k0 = max(0, nz // 2 - nslab // 2)
+    t = run(k0, nslab)
+    samples = nx * ny * nslab
+    return {
+        "value": samples / t / 1e9,
+        "unit": "Gsamples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{nslab} z-slabs of the {nx}x{ny}x{nz} grid ({samples} samples, {t:.1f} s) on {threads} "
+                  f"threads of '{cpu_model()}'",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512, help="grid nodes per axis")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the field and "
+              "meshing shard into independent z-slabs; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import unnatural_planets_amd as upf
+    from unnatural_planets_amd import spec as S
+
+    L = upf.lib()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    spec = S.spec_torus_ridged_voronoi(42)
+    grid = S.grid_torus(args.n)
+    nx, ny, nz = S.grid_dims(grid)
+    nodes = nx * ny * nz
+    cells = (nx - 1) * (ny - 1) * (nz - 1)
+    field = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    ws = upf.McWorkspace(grid, dev)
+    spec_p, grid_p = ctypes.byref(spec), ctypes.byref(grid)
+
+    def eval_field():
+        upf.check(L.upf_field_eval_grid(spec_p, grid_p, ctypes.c_void_p(field.data_ptr()), sp), "eval")
+
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+
+    def mc_count():
+        upf.check(L.upf_mc_count(ctypes.c_void_p(field.data_ptr()), grid_p, 0.0, ctypes.c_void_p(ws.buf.data_ptr()),
+                                 ws.bytes, ctypes.byref(nv), ctypes.byref(nt), sp), "count")
+
+    # size the mesh buffers once (outside the timed region)
+    eval_field()
+    mc_count()
+    cap_v, cap_t = int(nv.value * 1.05) + 1024, int(nt.value * 1.05) + 1024
+    pos = torch.empty((cap_v, 3), dtype=torch.float32, device=dev)
+    nrm = torch.empty((cap_v, 3), dtype=torch.float32, device=dev)
+    idx = torch.empty((cap_t, 3), dtype=torch.int32, device=dev)
+    mb = S.MeshBuffers()
+    mb.positions, mb.normals, mb.indices = pos.data_ptr(), nrm.data_ptr(), idx.data_ptr()
+    mb.cap_vertices, mb.cap_triangles = cap_v, cap_t
+
+    def mc_emit():
+        mb.n_vertices, mb.n_triangles = nv.value, nt.value
+        upf.check(L.upf_mc_emit(ctypes.c_void_p(field.data_ptr()), grid_p, 0.0, ctypes.byref(mb),
+                                ctypes.c_void_p(ws.buf.data_ptr()), ws.bytes, sp), "emit")
+
+    def step():
+        eval_field()
+        mc_count()
+        mc_emit()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    V, T = int(nv.value), int(nt.value)
+
+    # per-phase timing with HIP events on the stream the library launches on
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    reps = max(3, args.steps)
+    t_field = timed(eval_field, reps)
+    t_mc = timed(lambda: (mc_count(), mc_emit()), reps)
+
+    flops = float(L.upf_flops_per_sample(spec_p))
+    field_tflops = flops * nodes / t_field / 1e12
+    mc_bytes = 4 * nodes + 24 * V + 12 * T
+    mc_gbs = mc_bytes / t_mc / 1e9
+
+    out = {
+        "metric": "SDF+noise Gsamples/s & MC cells/s at 512^3; fwd+adjoint vs plain, % FP32 peak",
+        "value": round(nodes / t_step / 1e9, 4),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (procedural field on a uniform grid, seed 42)",
+        "config": {"workload": f"torus R=1 r=0.4 + 6-oct ridged gradient + Voronoi F1 (jitter 0.9), "
+                               f"mask-blended, seed 42, {nx}^3 FP32 grid over [-1.6,1.6]^3, marching cubes iso 0",
+                   "grid": [nx, ny, nz], "vertices": V, "triangles": T, "parallelism": "single GPU"},
+        "field_gsamples_per_s": round(nodes / t_field / 1e9, 4),
+        "field_ms": round(t_field * 1e3, 4),
+        "flops_per_sample": flops,
+        "mc_cells_per_s": round(cells / t_mc, 1),
+        "mc_ms": round(t_mc * 1e3, 4),
+        "roofline": {"bound": "valu", "kernel": "k_eval_grid", "achieved": round(field_tflops, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None},
+        "roofline_mc": {"bound": "hbm", "kernel": "marching cubes (classify+scan+emit)", "achieved": round(mc_gbs, 1),
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(mc_gbs / PEAK_HBM_GBS, 4),
+                        "algorithmic_bytes": mc_bytes},
+    }
+
+    if not args.no_adjoint:
+        dLdf = torch.ones((nz, ny, nx), dtype=torch.float32, device=dev)
+        dfdx = torch.empty((nz, ny, nx, 3), dtype=torch.float32, device=dev)
+        dth = torch.zeros((S.THETA_COUNT,), dtype=torch.float32, device=dev)
+
+        def adj():
+            upf.check(L.upf_field_adjoint_grid(spec_p, grid_p, ctypes.c_void_p(dLdf.data_ptr()),
+                                               ctypes.c_void_p(field.data_ptr()), ctypes.c_void_p(dfdx.data_ptr()),
+                                               ctypes.c_void_p(dth.data_ptr()), sp), "adjoint")
+        adj()
+        t_adj = timed(adj, max(2, args.steps // 2))
+        aflops = float(L.upf_adjoint_flops_per_sample(spec_p))
+        out["adjoint"] = {
+            "gsamples_per_s": round(nodes / t_adj / 1e9, 4),
+            "ms": round(t_adj * 1e3, 4),
+            "flops_per_sample": aflops,
+            "tflops": round(aflops * nodes / t_adj / 1e12, 3),
+            "frac_fp32_peak": round(aflops * nodes / t_adj / 1e12 / PEAK_FP32_TFLOPS, 4),
+            "vs_plain_time_ratio": round(t_adj / t_field, 3),
+        }
+        del dLdf, dfdx
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(spec, grid, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

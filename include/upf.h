@@ -1,0 +1,266 @@
+/*
+ * upf.h — C-ABI of libupf, the MI355X (gfx950) field-evaluation and meshing path
+ * of unnatural-planets.
+ *
+ * The reference hands its engine per-point callbacks; this library takes whole
+ * grids, point sets, meshes and atlases instead:
+ *
+ *   reference boundary                                         replaced by
+ *   ---------------------------------------------------------  -----------------------------
+ *   MarchingCubes::updateByPosition(Delegate<Real(const Vec3&)>)   upf_field_eval_grid
+ *     bound to terrainSdfLand/Water/Navigation                      (compose = LAND/WATER/NAV)
+ *     (sources/meshGeneration.cpp:38, modesConfiguration.cpp:248-279)
+ *   MarchingCubes::makeMesh()  (sources/meshGeneration.cpp:39)    upf_mc_extract
+ *   terrainSdf*(pos) called point-wise (tileProperties.cpp:258,
+ *     coloringModes.cpp:51-57, meshGeneration.cpp:68)              upf_field_eval_points
+ *   meshGenerateTexture(mesh, {w, h, generator = pixel})
+ *     (sources/textures.cpp:65-71, pixel at :24-43)                upf_bake
+ *   (no counterpart: gradient of the field w.r.t. position and     upf_field_adjoint,
+ *    spec parameters)                                              upf_field_adjoint_grid
+ *   (no counterpart: many chunks in one launch)                    upf_batch_*
+ *
+ * Conventions
+ *  - Every device pointer is caller-allocated HIP device memory; the library never
+ *    allocates or frees caller memory.  Scratch comes in as an explicit workspace
+ *    (upf_*_workspace_bytes tells how much).
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).  Calls are
+ *    stream-ordered.  Calls that must return counts (upf_mc_extract, upf_bake_atlas)
+ *    synchronise the stream before they return.
+ *  - Errors are int status codes; nothing throws across the ABI.
+ *  - No global mutable state: seeds live in the spec (the reference draws them from a
+ *    process-global RNG, sources/math.cpp:73-77).
+ *
+ * Field definition (all FP32):
+ *   elev(p) = layer stack (see upf_layer)
+ *   LAND        f = shape(p) - elevation_scale * elev(p)             (modesConfiguration.cpp:248-258)
+ *   WATER       f = shape(p)                                         (:260-267)
+ *   NAVIGATION  f = shape(p) - elevation_scale * max(elev(p), 0)     (:269-279)
+ *   ELEVATION   f = shape(p) / elevation_scale                       (:230-237, ratio 10)
+ *   ELEVATION_RAW f = elev(p)                                        (:239-246)
+ * Negative is inside.  The reference uses elevation_scale = 1/10 (meshElevationRatio,
+ * modesConfiguration.cpp:227).
+ *
+ * Grid: node (i,j,k) sits at origin + (i,j,k)*spacing, each coordinate computed as
+ * origin[a] + (float)i * spacing[a] with two IEEE roundings (no fused multiply-add);
+ * storage is float32 row-major [k][j][i] (x fastest).
+ */
+#ifndef UPF_H_INCLUDED
+#define UPF_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UPF_VERSION 1
+
+#define UPF_MAX_LAYERS 4
+#define UPF_MAX_OCTAVES 8
+#define UPF_MAX_SHAPE_PARAMS 8
+
+/* ---- status codes ------------------------------------------------------------ */
+enum upf_status {
+  UPF_OK = 0,
+  UPF_ERR_INVALID_VALUE = 1, /* NaN/Inf in a spec parameter or in the produced field
+                                (modesConfiguration.cpp:234-277 throws on !valid) */
+  UPF_ERR_BAD_SPEC = 2,      /* unknown shape/noise/fractal/blend/compose id, bad counts */
+  UPF_ERR_CAPACITY = 3,      /* output buffers too small; required counts written back */
+  UPF_ERR_WORKSPACE = 4,     /* workspace missing or too small */
+  UPF_ERR_BAD_GRID = 5,      /* dims < 2 or too large, spacing not finite */
+  UPF_ERR_HIP = 6,           /* a HIP runtime call failed */
+  UPF_ERR_NULL = 7,          /* a required pointer is NULL */
+  UPF_ERR_EMPTY_MESH = 8     /* used by host mirrors: meshGenerateBase* on an empty mesh
+                                (meshGeneration.cpp:250-251, 303-304) */
+};
+
+/* ---- spec enums ---------------------------------------------------------------- */
+enum upf_shape_id {
+  UPF_SHAPE_NONE = 0,    /* shape(p) = 0                                              */
+  UPF_SHAPE_SPHERE = 1,  /* params: r                    |p| - r   (sdf.cpp:20-23)        */
+  UPF_SHAPE_BOX = 2,     /* params: hx, hy, hz, rounding  box - rounding (sdf.cpp:40-48)  */
+  UPF_SHAPE_TORUS = 3,   /* params: R, r   ring in the xz-plane, axis y (sdf.cpp:81-84)   */
+  UPF_SHAPE_CAPSULE = 4, /* params: h, r   segment z in [-h/2, h/2] (sdf.cpp:25-28 swaps
+                                            y and z before cage's y-axis capsule)          */
+  UPF_SHAPE_CYLINDER = 5,/* params: h, r   axis z, |z| <= h/2   (sdf.cpp:30-38 swizzled)  */
+  UPF_SHAPE_PLANE = 6,   /* params: nx, ny, nz (unit), d     dot(p,n) - d  (sdf.cpp:10-18)*/
+  UPF_SHAPE_OCTAHEDRON = 7, /* params: s, rounding     (sdf.cpp:55-58)                    */
+  UPF_SHAPE_DOUBLE_TORUS = 8,  /* params: R, r, k   smoothMin of two tori (sdf.cpp:292-299)  */
+  UPF_SHAPE_TORUS_CROSS = 9,   /* params: R, r, k   (sdf.cpp:301-308)                      */
+  UPF_SHAPE_H2O = 10,          /* no params, metre scale  (sdf.cpp:141-149)                */
+  UPF_SHAPE_BOWL = 11,         /* no params, metre scale  (sdf.cpp:310-313)                */
+  UPF_SHAPE_INSIDE_CUBE = 12,  /* no params, metre scale  (sdf.cpp:315-318)                */
+  UPF_SHAPE_WORMHOLE = 13,     /* no params, metre scale  (sdf.cpp:399-404)                */
+  UPF_SHAPE_PIPE = 14,         /* no params, metre scale  (sdf.cpp:355-366)                */
+  UPF_SHAPE_HEMISPHERES = 15,  /* no params, metre scale  (sdf.cpp:383-397)                */
+  UPF_SHAPE_COUNT = 16
+};
+
+enum upf_noise_id {
+  UPF_NOISE_GRADIENT = 1, /* 3D lattice gradient noise, quintic fade, 12-edge gradients   */
+  UPF_NOISE_VALUE = 2,    /* 3D lattice value noise, quintic fade                         */
+  UPF_NOISE_CELLULAR = 3, /* F1 Euclidean distance to jittered feature points, 27 cells   */
+  UPF_NOISE_COUNT = 4
+};
+
+enum upf_fractal_id {
+  UPF_FRACTAL_NONE = 0,   /* one octave, amp[0] = 1                                       */
+  UPF_FRACTAL_FBM = 1,    /* sum_k amp[k] * n_k                                            */
+  UPF_FRACTAL_RIDGED = 2, /* sum_k amp[k] * (1 - 2|n_k|)                                  */
+  UPF_FRACTAL_COUNT = 3
+};
+
+enum upf_blend_id {
+  UPF_BLEND_ADD = 0,  /* elev += weight * v                                               */
+  UPF_BLEND_MASK = 1, /* m = saturate(weight * v + 0.5)   (sets the blend mask, elev kept) */
+  UPF_BLEND_LERP = 2, /* elev += m * (weight * v - elev)  (lerp towards this layer by m)  */
+  UPF_BLEND_COUNT = 3
+};
+
+enum upf_compose_id {
+  UPF_COMPOSE_LAND = 0,
+  UPF_COMPOSE_WATER = 1,
+  UPF_COMPOSE_NAVIGATION = 2,
+  UPF_COMPOSE_ELEVATION = 3,
+  UPF_COMPOSE_ELEVATION_RAW = 4,
+  UPF_COMPOSE_COUNT = 5
+};
+
+/* One noise layer.  Octave k samples noise with seed (seed + k) at p * freq[k] and is
+ * weighted by amp[k]; the per-octave arrays are part of the interface so that the
+ * adjoint can report d/d freq[k] and d/d amp[k] and so that a caller can set them
+ * directly.  upf_layer_init() fills them from (frequency, lacunarity, gain) the way
+ * the reference's NoiseFunctionCreateConfig does (sources/elevationModes.cpp:83-87):
+ *   freq[0] = frequency, freq[k] = freq[k-1] * lacunarity        (float, in order)
+ *   amp[0]  = 1 / sum_{j<octaves} |gain|^j,  amp[k] = amp[k-1] * gain
+ * (gain may be negative, sources/elevationModes.cpp:85). */
+typedef struct upf_layer {
+  int32_t noise;    /* upf_noise_id   */
+  int32_t fractal;  /* upf_fractal_id */
+  int32_t octaves;  /* 1..UPF_MAX_OCTAVES */
+  int32_t blend;    /* upf_blend_id   */
+  uint32_t seed;
+  float weight;     /* layer amplitude (the "Voronoi weight" of the adjoint) */
+  float jitter;     /* cellular only: feature-point jitter in [0, 1]          */
+  float reserved;
+  float freq[UPF_MAX_OCTAVES];
+  float amp[UPF_MAX_OCTAVES];
+} upf_layer;
+
+typedef struct upf_field_spec {
+  int32_t shape;    /* upf_shape_id   */
+  int32_t compose;  /* upf_compose_id */
+  int32_t n_layers; /* 0..UPF_MAX_LAYERS */
+  int32_t flags;    /* reserved, 0 */
+  float shape_params[UPF_MAX_SHAPE_PARAMS];
+  float elevation_scale;
+  float reserved[3];
+  upf_layer layers[UPF_MAX_LAYERS];
+} upf_field_spec;
+
+typedef struct upf_grid {
+  float origin[3];
+  float spacing[3];
+  int32_t dims[3]; /* nodes per axis, >= 2 */
+  int32_t reserved;
+} upf_grid;
+
+/* Gradient of a scalar loss w.r.t. the spec's continuous parameters (the θ of the
+ * adjoint).  Same indexing as upf_field_spec. */
+typedef struct upf_theta_grad {
+  float shape_params[UPF_MAX_SHAPE_PARAMS];
+  float weight[UPF_MAX_LAYERS];
+  float freq[UPF_MAX_LAYERS][UPF_MAX_OCTAVES];
+  float amp[UPF_MAX_LAYERS][UPF_MAX_OCTAVES];
+} upf_theta_grad; /* 76 floats */
+#define UPF_THETA_COUNT 76
+
+/* Marching-cubes output.  positions/normals: float[cap_vertices][3]; indices:
+ * uint32[cap_triangles][3].  normals may be NULL.  n_vertices / n_triangles are
+ * written back (also when UPF_ERR_CAPACITY is returned, then as the required size). */
+typedef struct upf_mesh_buffers {
+  float* positions;
+  float* normals;
+  uint32_t* indices;
+  int64_t cap_vertices;
+  int64_t cap_triangles;
+  int64_t n_vertices;
+  int64_t n_triangles;
+} upf_mesh_buffers;
+
+/* ---- library ------------------------------------------------------------------- */
+int upf_version(void);
+const char* upf_status_string(int status);
+
+/* Fill layer->freq/amp from (frequency, lacunarity, gain) as described at upf_layer.
+ * Replaces newNoiseFunction(NoiseFunctionCreateConfig) (sources/elevationModes.cpp:16-27). */
+int upf_layer_init(upf_layer* layer, int32_t noise, int32_t fractal, int32_t octaves,
+                   float frequency, float lacunarity, float gain, float weight,
+                   uint32_t seed, float jitter, int32_t blend);
+
+/* Checks ids, counts and that every parameter is finite.  Replaces the choose*Function
+ * validation (sources/modesConfiguration.cpp:131-135, 190-194) and the NaN throws. */
+int upf_spec_validate(const upf_field_spec* spec);
+
+/* FP32 flops per sample of the forward field, by the counting rule FMA = 2,
+ * add/sub/mul/div/sqrt = 1, compare/select/floor/convert/integer = 0. */
+double upf_flops_per_sample(const upf_field_spec* spec);
+/* Same for the fused forward + adjoint (value, d/dx, d/dθ accumulation). */
+double upf_adjoint_flops_per_sample(const upf_field_spec* spec);
+
+/* f at every grid node -> d_field[dims[2]][dims[1]][dims[0]].
+ * Replaces cubes->updateByPosition(Delegate(terrainSdf*)) (sources/meshGeneration.cpp:38). */
+int upf_field_eval_grid(const upf_field_spec* spec, const upf_grid* grid, float* d_field,
+                        void* stream);
+
+/* f at n points d_xyz[n][3] -> d_f[n]; if d_dfdx != NULL also the analytic gradient
+ * d_dfdx[n][3] (exact normals).  Replaces point-wise terrainSdf* calls
+ * (sources/coloringModes.cpp:51-57,122-151; sources/tileProperties.cpp:258-273). */
+int upf_field_eval_points(const upf_field_spec* spec, const float* d_xyz, int64_t n,
+                          float* d_f, float* d_dfdx, void* stream);
+
+/* Fused forward + adjoint at n points.  d_f (nullable) gets f, d_dfdx (nullable) gets
+ * df/dx, and d_dLdtheta (a device upf_theta_grad, nullable) gets sum_i dLdf[i] * df_i/dθ
+ * ADDED to its current contents (the caller zeroes it).  d_dLdf may be NULL only when
+ * d_dLdtheta is NULL.  No counterpart in the reference. */
+int upf_field_adjoint(const upf_field_spec* spec, const float* d_xyz, int64_t n,
+                      const float* d_dLdf, float* d_f, float* d_dfdx, float* d_dLdtheta,
+                      void* stream);
+
+/* The same over grid nodes (positions generated, not read). */
+int upf_field_adjoint_grid(const upf_field_spec* spec, const upf_grid* grid,
+                           const float* d_dLdf, float* d_f, float* d_dfdx,
+                           float* d_dLdtheta, void* stream);
+
+/* Marching cubes over a field produced by upf_field_eval_grid (or any FP32 grid in the
+ * same layout).  Case code bit c set <=> f(corner c) < iso.  Vertex = linear zero
+ * crossing on the cell edge that owns it; vertices are numbered by (owning node linear
+ * index, axis x<y<z), triangles by (cell linear index, table order).  Normals are the
+ * normalised grid gradient (central differences, one-sided at the border) interpolated
+ * along the edge; they point towards increasing f (outward).  Triangles wind
+ * counter-clockwise seen from outside.  Synchronises `stream`.
+ * Replaces cubes->makeMesh() (sources/meshGeneration.cpp:39). */
+size_t upf_mc_workspace_bytes(const upf_grid* grid);
+int upf_mc_extract(const float* d_field, const upf_grid* grid, float iso,
+                   upf_mesh_buffers* out, void* d_workspace, size_t workspace_bytes,
+                   void* stream);
+
+/* The two phases of upf_mc_extract, for callers that size outputs exactly:
+ * upf_mc_count classifies and scans (leaving its state in the workspace), synchronises
+ * the stream and returns the vertex/triangle counts; upf_mc_emit then writes the mesh
+ * from that state (same field, grid, iso and workspace; asynchronous). */
+int upf_mc_count(const float* d_field, const upf_grid* grid, float iso, void* d_workspace,
+                 size_t workspace_bytes, int64_t* n_vertices, int64_t* n_triangles, void* stream);
+int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out,
+                void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Case code of every cell -> d_codes[dims[2]-1][dims[1]-1][dims[0]-1] (uint8). */
+int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,
+                      uint8_t* d_codes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UPF_H_INCLUDED */

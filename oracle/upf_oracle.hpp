@@ -1,0 +1,352 @@
+// upf_oracle.hpp -- CPU oracle for the field-evaluation and meshing path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke check
+// and the CPU-baseline leg of bench.py may load it.  The product path (libupf) never
+// includes, links or calls it.
+//
+// It restates, in plain scalar C++ templated on the number type T, what the reference
+// computes per grid node and per mesh cell:
+//   T = float                  the FP32 parity target (built with -ffp-contract=off)
+//   T = double                 the FP64 error reference
+//   T = Dual<K>                forward-mode derivatives: df/dx and df/dθ, an
+//                              independent check of the hand-written adjoint
+//   T = CountedFloat           tallies FP32 flops (FMA = 2, add/sub/mul/div/sqrt = 1,
+//                              compare/select/floor/convert/integer = 0)
+//
+// Reference anchors (paths relative to the reference checkout):
+//   field composition         sources/modesConfiguration.cpp:227-279
+//   base shapes               sources/sdf.cpp:20-48,81-84 (wrapping cage primitives)
+//   math helpers              sources/math.cpp:8-38
+//   Voronoi (craters)         sources/voronoi.cpp:10-80
+//   grid sweep / MC driver    sources/meshGeneration.cpp:19-45
+// The noise, the SDF primitives, the hash and marching cubes live in the cage engine,
+// which is not present in the reference checkout; for those the oracle states the
+// definitions this project chose (named published methods, see each function), and
+// parity with cage itself is unpinned.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../include/upf.h"
+
+namespace upfo {
+
+// ============================================================================ numbers
+inline float val(float x) { return x; }
+inline double val(double x) { return x; }
+
+template <int K>
+struct Dual {
+  double v = 0;
+  double d[K] = {};
+  Dual() = default;
+  Dual(double x) : v(x) {}  // NOLINT: implicit constant promotion
+};
+template <int K> double val(const Dual<K>& x) { return x.v; }
+template <int K> Dual<K> operator+(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r(a.v + b.v); for (int i = 0; i < K; i++) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int K> Dual<K> operator-(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r(a.v - b.v); for (int i = 0; i < K; i++) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int K> Dual<K> operator-(const Dual<K>& a) {
+  Dual<K> r(-a.v); for (int i = 0; i < K; i++) r.d[i] = -a.d[i]; return r; }
+template <int K> Dual<K> operator*(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r(a.v * b.v); for (int i = 0; i < K; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int K> Dual<K> operator/(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r(a.v / b.v); for (int i = 0; i < K; i++) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v); return r; }
+template <int K> Dual<K> sqrt_(const Dual<K>& a) {
+  Dual<K> r(std::sqrt(a.v));
+  const double s = a.v > 0 ? 0.5 / r.v : 0.0;
+  for (int i = 0; i < K; i++) r.d[i] = a.d[i] * s;
+  return r;
+}
+template <int K> Dual<K> abs_(const Dual<K>& a) { return a.v < 0 ? -a : a; }
+template <int K> Dual<K> floor_(const Dual<K>& a) { return Dual<K>(std::floor(a.v)); }
+template <int K> Dual<K> rint_(const Dual<K>& a) { return Dual<K>(std::nearbyint(a.v)); }
+
+// Counting float: every arithmetic op bumps a per-thread counter.
+struct CountedFloat {
+  float v = 0;
+  static long long& count() { static thread_local long long n = 0; return n; }
+  CountedFloat() = default;
+  CountedFloat(float x) : v(x) {}  // NOLINT
+};
+inline float val(const CountedFloat& x) { return x.v; }
+inline CountedFloat operator+(CountedFloat a, CountedFloat b) { CountedFloat::count()++; return a.v + b.v; }
+inline CountedFloat operator-(CountedFloat a, CountedFloat b) { CountedFloat::count()++; return a.v - b.v; }
+inline CountedFloat operator-(CountedFloat a) { return -a.v; }  // sign modifier: free
+inline CountedFloat operator*(CountedFloat a, CountedFloat b) { CountedFloat::count()++; return a.v * b.v; }
+inline CountedFloat operator/(CountedFloat a, CountedFloat b) { CountedFloat::count()++; return a.v / b.v; }
+inline CountedFloat sqrt_(CountedFloat a) { CountedFloat::count()++; return std::sqrt(a.v); }
+inline CountedFloat abs_(CountedFloat a) { return std::fabs(a.v); }
+inline CountedFloat floor_(CountedFloat a) { return std::floor(a.v); }
+inline CountedFloat rint_(CountedFloat a) { return std::nearbyint(a.v); }
+
+inline float sqrt_(float a) { return std::sqrt(a); }
+inline double sqrt_(double a) { return std::sqrt(a); }
+inline float abs_(float a) { return std::fabs(a); }
+inline double abs_(double a) { return std::fabs(a); }
+inline float floor_(float a) { return std::floor(a); }
+inline double floor_(double a) { return std::floor(a); }
+inline float rint_(float a) { return std::nearbyint(a); }    // round half to even
+inline double rint_(double a) { return std::nearbyint(a); }
+
+template <class T> T max_(const T& a, const T& b) { return val(a) < val(b) ? b : a; }
+template <class T> T min_(const T& a, const T& b) { return val(b) < val(a) ? b : a; }
+template <class T> T clamp_(const T& x, const T& lo, const T& hi) { return min_(max_(x, lo), hi); }
+template <class T> T saturate_(const T& x) { return clamp_(x, T(0.0f), T(1.0f)); }
+
+// ============================================================================ hash
+// Lattice hash: the prime-multiply-xor hash of FastNoise Lite (MIT; the enum set
+// the reference configures, sources/elevationModes.cpp:16-27, is FastNoise Lite's):
+//   h = (seed ^ x*501125321 ^ y*1136930381 ^ z*1720413743) * 0x27d4eb2d   (mod 2^32)
+constexpr uint32_t kPrimeX = 501125321u;
+constexpr uint32_t kPrimeY = 1136930381u;
+constexpr uint32_t kPrimeZ = 1720413743u;
+constexpr uint32_t kHashMul = 0x27d4eb2du;
+
+inline uint32_t hash3(uint32_t seed, int32_t x, int32_t y, int32_t z) {
+  uint32_t h = seed ^ ((uint32_t)x * kPrimeX) ^ ((uint32_t)y * kPrimeY) ^ ((uint32_t)z * kPrimeZ);
+  return h * kHashMul;
+}
+
+// cage's integer hash (used by sources/voronoi.cpp:17,22) is not in the checkout;
+// this project uses the murmur3 32-bit finaliser (Appleby, public domain).
+inline uint32_t hash1(uint32_t k) {
+  k ^= k >> 16; k *= 0x85ebca6bu; k ^= k >> 13; k *= 0xc2b2ae35u; k ^= k >> 16; return k;
+}
+
+// ============================================================================ noise
+// Quintic fade 6t^5 - 15t^4 + 10t^3 (Perlin 2002), evaluated as t*t*t*(t*(t*6-15)+10).
+template <class T> T fade(const T& t) {
+  return t * t * t * (t * (t * T(6.0f) - T(15.0f)) + T(10.0f));
+}
+template <class T> T lerp_(const T& a, const T& b, const T& t) { return a + t * (b - a); }
+
+// Gradient from the top 4 hash bits, Perlin's improved-noise set: the 12 cube-edge
+// directions with (1,1,0),(-1,1,0),(0,-1,1),(0,-1,-1) repeated (Perlin 2002).
+template <class T> T grad_dot(uint32_t h, const T& x, const T& y, const T& z) {
+  const uint32_t g = h >> 28;
+  const T u = g < 8 ? x : y;
+  const T v = g < 4 ? y : ((g == 12 || g == 14) ? x : z);
+  return ((g & 1) ? -u : u) + ((g & 2) ? -v : v);
+}
+
+// 3D gradient noise at lattice coordinates q (already scaled by the frequency).
+template <class T> T noise_gradient(uint32_t seed, const T q[3]) {
+  const T f0 = floor_(q[0]), f1 = floor_(q[1]), f2 = floor_(q[2]);
+  const int32_t ix = (int32_t)val(f0), iy = (int32_t)val(f1), iz = (int32_t)val(f2);
+  const T tx = q[0] - f0, ty = q[1] - f1, tz = q[2] - f2;
+  const T sx = tx - T(1.0f), sy = ty - T(1.0f), sz = tz - T(1.0f);
+  const T ux = fade(tx), uy = fade(ty), uz = fade(tz);
+  const T n000 = grad_dot(hash3(seed, ix, iy, iz), tx, ty, tz);
+  const T n100 = grad_dot(hash3(seed, ix + 1, iy, iz), sx, ty, tz);
+  const T n010 = grad_dot(hash3(seed, ix, iy + 1, iz), tx, sy, tz);
+  const T n110 = grad_dot(hash3(seed, ix + 1, iy + 1, iz), sx, sy, tz);
+  const T n001 = grad_dot(hash3(seed, ix, iy, iz + 1), tx, ty, sz);
+  const T n101 = grad_dot(hash3(seed, ix + 1, iy, iz + 1), sx, ty, sz);
+  const T n011 = grad_dot(hash3(seed, ix, iy + 1, iz + 1), tx, sy, sz);
+  const T n111 = grad_dot(hash3(seed, ix + 1, iy + 1, iz + 1), sx, sy, sz);
+  const T x00 = lerp_(n000, n100, ux), x10 = lerp_(n010, n110, ux);
+  const T x01 = lerp_(n001, n101, ux), x11 = lerp_(n011, n111, ux);
+  const T y0 = lerp_(x00, x10, uy), y1 = lerp_(x01, x11, uy);
+  return lerp_(y0, y1, uz);
+}
+
+// Lattice value in [-1, 1): the hash read as a signed 32-bit integer times 2^-31.
+inline float lattice_value(uint32_t h) { return (float)(int32_t)h * 4.656612873077393e-10f; }
+
+template <class T> T noise_value(uint32_t seed, const T q[3]) {
+  const T f0 = floor_(q[0]), f1 = floor_(q[1]), f2 = floor_(q[2]);
+  const int32_t ix = (int32_t)val(f0), iy = (int32_t)val(f1), iz = (int32_t)val(f2);
+  const T tx = q[0] - f0, ty = q[1] - f1, tz = q[2] - f2;
+  const T ux = fade(tx), uy = fade(ty), uz = fade(tz);
+  const T v000 = T(lattice_value(hash3(seed, ix, iy, iz)));
+  const T v100 = T(lattice_value(hash3(seed, ix + 1, iy, iz)));
+  const T v010 = T(lattice_value(hash3(seed, ix, iy + 1, iz)));
+  const T v110 = T(lattice_value(hash3(seed, ix + 1, iy + 1, iz)));
+  const T v001 = T(lattice_value(hash3(seed, ix, iy, iz + 1)));
+  const T v101 = T(lattice_value(hash3(seed, ix + 1, iy, iz + 1)));
+  const T v011 = T(lattice_value(hash3(seed, ix, iy + 1, iz + 1)));
+  const T v111 = T(lattice_value(hash3(seed, ix + 1, iy + 1, iz + 1)));
+  const T x00 = lerp_(v000, v100, ux), x10 = lerp_(v010, v110, ux);
+  const T x01 = lerp_(v001, v101, ux), x11 = lerp_(v011, v111, ux);
+  const T y0 = lerp_(x00, x10, uy), y1 = lerp_(x01, x11, uy);
+  return lerp_(y0, y1, uz);
+}
+
+// Cellular F1 (Worley 1996): Euclidean distance to the nearest feature point among the
+// 27 lattice cells around rint(q).  Cell k holds one point at k + jitter*(r - 0.5),
+// r in [0,1]^3 from three 10-bit fields of m = h ^ (h >> 15).
+template <class T> T noise_cellular(uint32_t seed, const T q[3], const T& jitter) {
+  const T c0 = rint_(q[0]), c1 = rint_(q[1]), c2 = rint_(q[2]);
+  const int32_t cx = (int32_t)val(c0), cy = (int32_t)val(c1), cz = (int32_t)val(c2);
+  const T js = jitter / T(1023.0f);
+  const T j0 = jitter * T(0.5f);
+  // lattice coordinate of each neighbour column minus j0, per axis
+  T kf[3][3];
+  for (int d = 0; d < 3; d++) {
+    kf[0][d] = T((float)(cx + d - 1)) - j0;
+    kf[1][d] = T((float)(cy + d - 1)) - j0;
+    kf[2][d] = T((float)(cz + d - 1)) - j0;
+  }
+  T best = T(0.0f);
+  float bestv = INFINITY;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int32_t kx = cx + dx, ky = cy + dy, kz = cz + dz;
+        const uint32_t h = hash3(seed, kx, ky, kz);
+        const uint32_t m = h ^ (h >> 15);
+        const T px = T((float)(m & 1023u)) * js + kf[0][dx + 1];
+        const T py = T((float)((m >> 10) & 1023u)) * js + kf[1][dy + 1];
+        const T pz = T((float)((m >> 20) & 1023u)) * js + kf[2][dz + 1];
+        const T ex = px - q[0], ey = py - q[1], ez = pz - q[2];
+        const T d2 = ex * ex + ey * ey + ez * ez;
+        if ((float)val(d2) < bestv) { bestv = (float)val(d2); best = d2; }
+      }
+  return sqrt_(best);
+}
+
+// ============================================================================ spec in T
+template <class T>
+struct LayerT {
+  int noise = 0, fractal = 0, octaves = 0, blend = 0;
+  uint32_t seed = 0;
+  T weight, jitter;
+  T freq[UPF_MAX_OCTAVES], amp[UPF_MAX_OCTAVES];
+};
+
+template <class T>
+struct SpecT {
+  int shape = 0, compose = 0, n_layers = 0;
+  T shape_params[UPF_MAX_SHAPE_PARAMS];
+  T elevation_scale;
+  LayerT<T> layers[UPF_MAX_LAYERS];
+};
+
+template <class T>
+SpecT<T> spec_to(const upf_field_spec& s) {
+  SpecT<T> r;
+  r.shape = s.shape; r.compose = s.compose; r.n_layers = s.n_layers;
+  for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++) r.shape_params[i] = T(s.shape_params[i]);
+  r.elevation_scale = T(s.elevation_scale);
+  for (int l = 0; l < UPF_MAX_LAYERS; l++) {
+    const upf_layer& a = s.layers[l];
+    LayerT<T>& b = r.layers[l];
+    b.noise = a.noise; b.fractal = a.fractal; b.octaves = a.octaves; b.blend = a.blend;
+    b.seed = a.seed; b.weight = T(a.weight); b.jitter = T(a.jitter);
+    for (int k = 0; k < UPF_MAX_OCTAVES; k++) { b.freq[k] = T(a.freq[k]); b.amp[k] = T(a.amp[k]); }
+  }
+  return r;
+}
+
+// ============================================================================ shapes
+// Closed forms (Quilez's distance-function catalogue) for the cage primitives the
+// reference wraps in sources/sdf.cpp.  Parameters as documented in upf.h.
+template <class T> T shape_eval(const SpecT<T>& s, const T p[3]) {
+  const T* a = s.shape_params;
+  switch (s.shape) {
+    case UPF_SHAPE_NONE: return T(0.0f);
+    case UPF_SHAPE_SPHERE:  // sdf.cpp:20-23
+      return sqrt_(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - a[0];
+    case UPF_SHAPE_BOX: {   // sdf.cpp:40-48: sdfBox(pos, half) - rounding
+      const T qx = abs_(p[0]) - a[0], qy = abs_(p[1]) - a[1], qz = abs_(p[2]) - a[2];
+      const T mx = max_(qx, T(0.0f)), my = max_(qy, T(0.0f)), mz = max_(qz, T(0.0f));
+      const T outside = sqrt_(mx * mx + my * my + mz * mz);
+      const T inside = min_(max_(qx, max_(qy, qz)), T(0.0f));
+      return outside + inside - a[3];
+    }
+    case UPF_SHAPE_TORUS: { // sdf.cpp:81-84
+      const T ring = sqrt_(p[0] * p[0] + p[2] * p[2]) - a[0];
+      return sqrt_(ring * ring + p[1] * p[1]) - a[1];
+    }
+    case UPF_SHAPE_CAPSULE: {  // sdf.cpp:25-28
+      const T h2 = a[0] * T(0.5f);
+      const T zc = p[2] - clamp_(p[2], -h2, h2);
+      return sqrt_(p[0] * p[0] + p[1] * p[1] + zc * zc) - a[1];
+    }
+    default: return T(NAN);
+  }
+}
+
+// ============================================================================ layers
+template <class T> T octave_noise(const LayerT<T>& L, int k, const T p[3]) {
+  const T q[3] = {p[0] * L.freq[k], p[1] * L.freq[k], p[2] * L.freq[k]};
+  const uint32_t seed = L.seed + (uint32_t)k;
+  switch (L.noise) {
+    case UPF_NOISE_GRADIENT: return noise_gradient(seed, q);
+    case UPF_NOISE_VALUE: return noise_value(seed, q);
+    case UPF_NOISE_CELLULAR: return noise_cellular(seed, q, L.jitter);
+    default: return T(NAN);
+  }
+}
+
+template <class T> T layer_value(const LayerT<T>& L, const T p[3]) {
+  const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
+  T acc = T(0.0f);
+  for (int k = 0; k < oct; k++) {
+    const T n = octave_noise(L, k, p);
+    if (L.fractal == UPF_FRACTAL_RIDGED)
+      acc = acc + L.amp[k] * (T(1.0f) - T(2.0f) * abs_(n));
+    else if (L.fractal == UPF_FRACTAL_NONE)
+      acc = n;
+    else
+      acc = acc + L.amp[k] * n;
+  }
+  return acc;
+}
+
+template <class T> T elevation_eval(const SpecT<T>& s, const T p[3]) {
+  T acc = T(0.0f), m = T(0.0f);
+  for (int l = 0; l < s.n_layers; l++) {
+    const LayerT<T>& L = s.layers[l];
+    const T v = L.weight * layer_value(L, p);
+    if (L.blend == UPF_BLEND_ADD) acc = acc + v;
+    else if (L.blend == UPF_BLEND_MASK) m = saturate_(v + T(0.5f));
+    else acc = acc + m * (v - acc);
+  }
+  return acc;
+}
+
+// modesConfiguration.cpp:230-279
+template <class T> T field_eval(const SpecT<T>& s, const T p[3]) {
+  switch (s.compose) {
+    case UPF_COMPOSE_LAND: return shape_eval(s, p) - s.elevation_scale * elevation_eval(s, p);
+    case UPF_COMPOSE_WATER: return shape_eval(s, p);
+    case UPF_COMPOSE_NAVIGATION:
+      return shape_eval(s, p) - s.elevation_scale * max_(elevation_eval(s, p), T(0.0f));
+    case UPF_COMPOSE_ELEVATION: return shape_eval(s, p) / s.elevation_scale;
+    case UPF_COMPOSE_ELEVATION_RAW: return elevation_eval(s, p);
+    default: return T(NAN);
+  }
+}
+
+// Node position: origin + (float)i * spacing, two roundings (no FMA).
+inline float node_coord(float origin, float spacing, int i) {
+  const float m = (float)i * spacing;
+  return origin + m;
+}
+
+// ============================================================================ math.cpp
+// sources/math.cpp:8-38, restated.
+template <class T> T rescale(T v, T ia, T ib, T oa, T ob) { return (v - ia) / (ib - ia) * (ob - oa) + oa; }
+template <class T> T sharp_edge(T v, T p) {
+  return rescale(clamp_(v, T(0.5f) - p, T(0.5f) + p), T(0.5f) - p, T(0.5f) + p, T(0.0f), T(1.0f));
+}
+template <class T> T smootherstep_(T x) { return x * x * x * (x * (x * T(6.0f) - T(15.0f)) + T(10.0f)); }
+template <class T> T terrace(T x, T steepness) {
+  const T f = floor_(x);
+  const T t = smootherstep_(saturate_((x - f) * max_(steepness, T(1.0f)))) + f;
+  return lerp_(x, t, saturate_(steepness));
+}
+template <class T> T smooth_min(T a, T b, T k) {
+  const T h = saturate_((b - a) / k * T(0.5f) + T(0.5f));
+  return lerp_(b, a, h) - k * h * (T(1.0f) - h);
+}
+template <class T> T smooth_max(T a, T b, T k) { return -smooth_min(-a, -b, k); }
+
+}  // namespace upfo

@@ -1,0 +1,194 @@
+// upf_oracle_capi.cpp -- C entry points of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+// Loaded by tests/ through ctypes and by bench.py's CPU-baseline leg; never by libupf.
+#include <cstdlib>
+#include <memory>
+
+#include "upf_oracle.hpp"
+#include "upf_oracle_mc.hpp"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using namespace upfo;
+
+namespace {
+template <class T>
+void eval_grid(const upf_field_spec* spec, const upf_grid* g, T* out) {
+  const SpecT<T> s = spec_to<T>(*spec);
+  const int nx = g->dims[0], ny = g->dims[1], nz = g->dims[2];
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const T p[3] = {T(node_coord(g->origin[0], g->spacing[0], i)),
+                        T(node_coord(g->origin[1], g->spacing[1], j)),
+                        T(node_coord(g->origin[2], g->spacing[2], k))};
+        out[((int64_t)k * ny + j) * nx + i] = field_eval(s, p);
+      }
+}
+
+template <int K>
+void seed_theta(SpecT<Dual<K>>& s) {
+  int c = 0;
+  for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++) s.shape_params[i].d[c++] = 1;
+  for (int l = 0; l < UPF_MAX_LAYERS; l++) s.layers[l].weight.d[c++] = 1;
+  for (int l = 0; l < UPF_MAX_LAYERS; l++)
+    for (int k = 0; k < UPF_MAX_OCTAVES; k++) s.layers[l].freq[k].d[c++] = 1;
+  for (int l = 0; l < UPF_MAX_LAYERS; l++)
+    for (int k = 0; k < UPF_MAX_OCTAVES; k++) s.layers[l].amp[k].d[c++] = 1;
+}
+}  // namespace
+
+extern "C" {
+
+int upfo_version(void) { return UPF_VERSION; }
+
+uint32_t upfo_hash3(uint32_t seed, int32_t x, int32_t y, int32_t z) { return hash3(seed, x, y, z); }
+uint32_t upfo_hash1(uint32_t k) { return hash1(k); }
+
+// One octave of raw noise at lattice coordinates q (no frequency scaling).
+double upfo_noise_f64(int noise, uint32_t seed, const double* q, double jitter) {
+  switch (noise) {
+    case UPF_NOISE_GRADIENT: return noise_gradient<double>(seed, q);
+    case UPF_NOISE_VALUE: return noise_value<double>(seed, q);
+    case UPF_NOISE_CELLULAR: return noise_cellular<double>(seed, q, jitter);
+    default: return NAN;
+  }
+}
+float upfo_noise_f32(int noise, uint32_t seed, const float* q, float jitter) {
+  switch (noise) {
+    case UPF_NOISE_GRADIENT: return noise_gradient<float>(seed, q);
+    case UPF_NOISE_VALUE: return noise_value<float>(seed, q);
+    case UPF_NOISE_CELLULAR: return noise_cellular<float>(seed, q, jitter);
+    default: return NAN;
+  }
+}
+
+void upfo_eval_points_f32(const upf_field_spec* spec, const float* xyz, int64_t n, float* out) {
+  const SpecT<float> s = spec_to<float>(*spec);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) out[i] = field_eval(s, xyz + 3 * i);
+}
+
+void upfo_eval_points_f64(const upf_field_spec* spec, const float* xyz, int64_t n, double* out) {
+  const SpecT<double> s = spec_to<double>(*spec);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    const double p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    out[i] = field_eval(s, p);
+  }
+}
+
+void upfo_eval_grid_f32(const upf_field_spec* spec, const upf_grid* g, float* out) { eval_grid<float>(spec, g, out); }
+void upfo_eval_grid_f64(const upf_field_spec* spec, const upf_grid* g, double* out) { eval_grid<double>(spec, g, out); }
+
+int upfo_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void upfo_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+// Forward-mode df/dx (FP64 duals) at n points: f -> out_f[n], grad -> out_g[n][3].
+void upfo_grad_points(const upf_field_spec* spec, const float* xyz, int64_t n, double* out_f, double* out_g) {
+  const SpecT<Dual<3>> s = spec_to<Dual<3>>(*spec);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    Dual<3> p[3] = {Dual<3>(xyz[3 * i]), Dual<3>(xyz[3 * i + 1]), Dual<3>(xyz[3 * i + 2])};
+    for (int a = 0; a < 3; a++) p[a].d[a] = 1;
+    const Dual<3> f = field_eval(s, p);
+    out_f[i] = f.v;
+    for (int a = 0; a < 3; a++) out_g[3 * i + a] = f.d[a];
+  }
+}
+
+// sum_i dLdf[i] * df_i/dθ (FP64 duals) -> out[UPF_THETA_COUNT] in upf_theta_grad order.
+void upfo_theta_grad(const upf_field_spec* spec, const float* xyz, int64_t n, const float* dLdf, double* out) {
+  SpecT<Dual<UPF_THETA_COUNT>> s = spec_to<Dual<UPF_THETA_COUNT>>(*spec);
+  seed_theta(s);
+  for (int j = 0; j < UPF_THETA_COUNT; j++) out[j] = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const Dual<UPF_THETA_COUNT> p[3] = {Dual<UPF_THETA_COUNT>(xyz[3 * i]), Dual<UPF_THETA_COUNT>(xyz[3 * i + 1]),
+                                        Dual<UPF_THETA_COUNT>(xyz[3 * i + 2])};
+    const Dual<UPF_THETA_COUNT> f = field_eval(s, p);
+    for (int j = 0; j < UPF_THETA_COUNT; j++) out[j] += (double)dLdf[i] * f.d[j];
+  }
+}
+
+// FP32 flops of one field evaluation at point xyz (counting rule in upf_oracle.hpp).
+long long upfo_flops_at(const upf_field_spec* spec, const float* xyz) {
+  const SpecT<CountedFloat> s = spec_to<CountedFloat>(*spec);
+  const CountedFloat p[3] = {xyz[0], xyz[1], xyz[2]};
+  CountedFloat::count() = 0;
+  (void)field_eval(s, p);
+  return CountedFloat::count();
+}
+
+// Oracle restatement of the per-octave expansion documented at upf_layer.
+void upfo_layer_expand(int fractal, int octaves, float frequency, float lacunarity, float gain,
+                       float* freq, float* amp) {
+  const int oct = fractal == UPF_FRACTAL_NONE ? 1 : octaves;
+  const float g = std::fabs(gain);
+  float sum = 1.0f, a = g;
+  for (int i = 1; i < oct; i++) { sum += a; a *= g; }
+  float f = frequency, w = 1.0f / sum;
+  if (fractal == UPF_FRACTAL_NONE) w = 1.0f;
+  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+    if (k < oct) { freq[k] = f; amp[k] = w; f *= lacunarity; w *= gain; }
+    else { freq[k] = 0.0f; amp[k] = 0.0f; }
+  }
+}
+
+// ---- marching cubes
+void upfo_mc_table(int* ntris, int* tris) {
+  const McTable& t = table();
+  for (int c = 0; c < 256; c++) {
+    ntris[c] = t.ntris[c];
+    for (int i = 0; i < 15; i++) tris[c * 15 + i] = t.tris[c][i];
+  }
+}
+
+void upfo_case_codes(const float* field, const upf_grid* g, float iso, uint8_t* out) {
+  GridView v{field, g->dims[0], g->dims[1], g->dims[2],
+             {g->origin[0], g->origin[1], g->origin[2]}, {g->spacing[0], g->spacing[1], g->spacing[2]}};
+  const int cx = v.nx - 1, cy = v.ny - 1, cz = v.nz - 1;
+  for (int k = 0; k < cz; k++)
+    for (int j = 0; j < cy; j++)
+      for (int i = 0; i < cx; i++) out[((int64_t)k * cy + j) * cx + i] = (uint8_t)case_code(v, i, j, k, iso);
+}
+
+void* upfo_mc_run(const float* field, const upf_grid* g, float iso, int with_normals) {
+  GridView v{field, g->dims[0], g->dims[1], g->dims[2],
+             {g->origin[0], g->origin[1], g->origin[2]}, {g->spacing[0], g->spacing[1], g->spacing[2]}};
+  return new McMesh(mc_extract(v, iso, with_normals != 0));
+}
+void upfo_mc_counts(void* h, int64_t* nv, int64_t* nt) {
+  const McMesh* m = static_cast<McMesh*>(h);
+  *nv = (int64_t)m->positions.size() / 3;
+  *nt = (int64_t)m->indices.size() / 3;
+}
+void upfo_mc_copy(void* h, float* pos, float* nrm, uint32_t* idx) {
+  const McMesh* m = static_cast<McMesh*>(h);
+  if (pos) std::memcpy(pos, m->positions.data(), m->positions.size() * sizeof(float));
+  if (nrm && !m->normals.empty()) std::memcpy(nrm, m->normals.data(), m->normals.size() * sizeof(float));
+  if (idx) std::memcpy(idx, m->indices.data(), m->indices.size() * sizeof(uint32_t));
+}
+void upfo_mc_free(void* h) { delete static_cast<McMesh*>(h); }
+
+// ---- sources/math.cpp helpers (FP64), for closed-form checks
+double upfo_rescale(double v, double ia, double ib, double oa, double ob) { return rescale<double>(v, ia, ib, oa, ob); }
+double upfo_sharp_edge(double v, double p) { return sharp_edge<double>(v, p); }
+double upfo_terrace(double x, double s) { return terrace<double>(x, s); }
+double upfo_smooth_min(double a, double b, double k) { return smooth_min<double>(a, b, k); }
+double upfo_smooth_max(double a, double b, double k) { return smooth_max<double>(a, b, k); }
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+"""ctypes wrapper of oracle/liboracle.so, the CPU restatement used as the checker."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from unnatural_planets_amd.spec import THETA_COUNT, FieldSpec, Grid, grid_dims
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_P = ctypes.c_void_p
+_SP = ctypes.POINTER(FieldSpec)
+_GP = ctypes.POINTER(Grid)
+
+
+def _f32p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    def __init__(self, path: str = PATH):
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        sig = {
+            "upfo_hash3": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+            "upfo_hash1": (ctypes.c_uint32, [ctypes.c_uint32]),
+            "upfo_noise_f64": (ctypes.c_double, [ctypes.c_int, ctypes.c_uint32, _P, ctypes.c_double]),
+            "upfo_noise_f32": (ctypes.c_float, [ctypes.c_int, ctypes.c_uint32, _P, ctypes.c_float]),
+            "upfo_eval_points_f32": (None, [_SP, _P, ctypes.c_int64, _P]),
+            "upfo_eval_points_f64": (None, [_SP, _P, ctypes.c_int64, _P]),
+            "upfo_eval_grid_f32": (None, [_SP, _GP, _P]),
+            "upfo_eval_grid_f64": (None, [_SP, _GP, _P]),
+            "upfo_grad_points": (None, [_SP, _P, ctypes.c_int64, _P, _P]),
+            "upfo_theta_grad": (None, [_SP, _P, ctypes.c_int64, _P, _P]),
+            "upfo_flops_at": (ctypes.c_longlong, [_SP, _P]),
+            "upfo_layer_expand": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, _P, _P]),
+            "upfo_mc_table": (None, [_P, _P]),
+            "upfo_case_codes": (None, [_P, _GP, ctypes.c_float, _P]),
+            "upfo_mc_run": (ctypes.c_void_p, [_P, _GP, ctypes.c_float, ctypes.c_int]),
+            "upfo_mc_counts": (None, [_P, _P, _P]),
+            "upfo_mc_copy": (None, [_P, _P, _P, _P]),
+            "upfo_mc_free": (None, [_P]),
+            "upfo_num_threads": (ctypes.c_int, []),
+            "upfo_set_num_threads": (None, [ctypes.c_int]),
+            "upfo_rescale": (ctypes.c_double, [ctypes.c_double] * 5),
+            "upfo_sharp_edge": (ctypes.c_double, [ctypes.c_double] * 2),
+            "upfo_terrace": (ctypes.c_double, [ctypes.c_double] * 2),
+            "upfo_smooth_min": (ctypes.c_double, [ctypes.c_double] * 3),
+            "upfo_smooth_max": (ctypes.c_double, [ctypes.c_double] * 3),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        self.L = L
+
+    # ---- noise / hash
+    def hash3(self, seed, x, y, z):
+        return self.L.upfo_hash3(seed, x, y, z)
+
+    def noise(self, noise: int, seed: int, q, jitter: float = 1.0, f32: bool = False):
+        if f32:
+            a = np.ascontiguousarray(q, dtype=np.float32)
+            return self.L.upfo_noise_f32(noise, seed, _f32p(a), jitter)
+        a = np.ascontiguousarray(q, dtype=np.float64)
+        return self.L.upfo_noise_f64(noise, seed, _f32p(a), jitter)
+
+    # ---- field
+    def eval_points(self, s: FieldSpec, xyz, f64: bool = False):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        n = xyz.shape[0]
+        out = np.empty(n, dtype=np.float64 if f64 else np.float32)
+        fn = self.L.upfo_eval_points_f64 if f64 else self.L.upfo_eval_points_f32
+        fn(ctypes.byref(s), _f32p(xyz), n, _f32p(out))
+        return out
+
+    def eval_grid(self, s: FieldSpec, g: Grid, f64: bool = False):
+        nx, ny, nz = grid_dims(g)
+        out = np.empty((nz, ny, nx), dtype=np.float64 if f64 else np.float32)
+        fn = self.L.upfo_eval_grid_f64 if f64 else self.L.upfo_eval_grid_f32
+        fn(ctypes.byref(s), ctypes.byref(g), _f32p(out))
+        return out
+
+    def grad_points(self, s: FieldSpec, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        n = xyz.shape[0]
+        f = np.empty(n, dtype=np.float64)
+        g = np.empty((n, 3), dtype=np.float64)
+        self.L.upfo_grad_points(ctypes.byref(s), _f32p(xyz), n, _f32p(f), _f32p(g))
+        return f, g
+
+    def theta_grad(self, s: FieldSpec, xyz, dLdf):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+        dLdf = np.ascontiguousarray(dLdf, dtype=np.float32)
+        out = np.empty(THETA_COUNT, dtype=np.float64)
+        self.L.upfo_theta_grad(ctypes.byref(s), _f32p(xyz), xyz.shape[0], _f32p(dLdf), _f32p(out))
+        return out
+
+    def flops_at(self, s: FieldSpec, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        return int(self.L.upfo_flops_at(ctypes.byref(s), _f32p(a)))
+
+    def layer_expand(self, fractal, octaves, frequency, lacunarity, gain):
+        fr = np.zeros(8, dtype=np.float32)
+        am = np.zeros(8, dtype=np.float32)
+        self.L.upfo_layer_expand(fractal, octaves, frequency, lacunarity, gain, _f32p(fr), _f32p(am))
+        return fr, am
+
+    # ---- marching cubes
+    def mc_table(self):
+        nt = np.zeros(256, dtype=np.int32)
+        tr = np.zeros((256, 15), dtype=np.int32)
+        self.L.upfo_mc_table(_f32p(nt), _f32p(tr))
+        return nt, tr
+
+    def case_codes(self, field, g: Grid, iso=0.0):
+        nx, ny, nz = grid_dims(g)
+        field = np.ascontiguousarray(field, dtype=np.float32)
+        out = np.empty((nz - 1, ny - 1, nx - 1), dtype=np.uint8)
+        self.L.upfo_case_codes(_f32p(field), ctypes.byref(g), iso, _f32p(out))
+        return out
+
+    def mc(self, field, g: Grid, iso=0.0, normals=True):
+        field = np.ascontiguousarray(field, dtype=np.float32)
+        h = self.L.upfo_mc_run(_f32p(field), ctypes.byref(g), iso, 1 if normals else 0)
+        nv, nt = ctypes.c_int64(), ctypes.c_int64()
+        self.L.upfo_mc_counts(h, ctypes.byref(nv), ctypes.byref(nt))
+        pos = np.empty((nv.value, 3), dtype=np.float32)
+        nrm = np.empty((nv.value, 3), dtype=np.float32)
+        idx = np.empty((nt.value, 3), dtype=np.uint32)
+        self.L.upfo_mc_copy(h, _f32p(pos), _f32p(nrm) if normals else None, _f32p(idx))
+        self.L.upfo_mc_free(h)
+        return pos, (nrm if normals else None), idx
+
+
+def mesh_edge_stats(idx: np.ndarray):
+    """(n_unique_edges, n_boundary_edges, n_bad_edges, oriented_ok) for a triangle list.
+    An interior edge must appear exactly twice, once in each direction."""
+    idx = np.asarray(idx, dtype=np.int64)
+    e = np.concatenate([idx[:, [0, 1]], idx[:, [1, 2]], idx[:, [2, 0]]], axis=0)
+    key = np.minimum(e[:, 0], e[:, 1]) * (1 << 32) + np.maximum(e[:, 0], e[:, 1])
+    direction = (e[:, 0] < e[:, 1]).astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    key, direction = key[order], direction[order]
+    uniq, start, counts = np.unique(key, return_index=True, return_counts=True)
+    dsum = np.add.reduceat(direction, start)
+    boundary = int(np.sum(counts == 1))
+    bad = int(np.sum(counts > 2))
+    oriented = bool(np.all(dsum[counts == 2] == 1))
+    return len(uniq), boundary, bad, oriented

@@ -1,0 +1,257 @@
+"""unnatural_planets_amd -- MI355X (gfx950) field evaluation and meshing for unnatural-planets.
+
+Host-side mirror of the reference's interface for this path, over the C-ABI of
+libupf.so.  torch supplies device memory and streams only; every computation runs in
+the library's HIP kernels, and every call raises when libupf.so is missing.
+
+Reference functions mirrored (paths relative to the reference checkout):
+  terrain_sdf_land / water / navigation / elevation / elevation_raw
+      sources/modesConfiguration.cpp:230-279 (point-wise field)
+  mesh_generate_generic          sources/meshGeneration.cpp:32-45
+  mesh_generate_base_land        sources/meshGeneration.cpp:47-55
+  mesh_generate_base_water       sources/meshGeneration.cpp:57-98 (without the vertex mask)
+  mesh_generate_base_navigation  sources/meshGeneration.cpp:100-107
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+from . import spec as spec_mod
+from ._lib import UpfError, check, lib, status_string  # noqa: F401
+from .spec import (FieldSpec, Grid, Layer, MeshBuffers, copy_spec, field_spec, grid_dims,  # noqa: F401
+                   layer, uniform_grid)
+
+__all__ = [
+    "FieldSpec", "Grid", "Layer", "Mesh", "UpfError", "field_spec", "layer", "uniform_grid",
+    "field_eval_grid", "field_eval_points", "field_adjoint", "field_adjoint_grid", "mc_extract",
+    "mc_case_codes", "terrain_sdf_land", "terrain_sdf_water", "terrain_sdf_navigation",
+    "terrain_sdf_elevation", "terrain_sdf_elevation_raw", "mesh_generate_generic",
+    "mesh_generate_base_land", "mesh_generate_base_water", "mesh_generate_base_navigation",
+    "reference_grid", "flops_per_sample", "adjoint_flops_per_sample",
+]
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream(stream=None):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev(device):
+    torch = _torch()
+    return torch.device(device if device is not None else "cuda")
+
+
+def _check_f32(t, name, shape=None):
+    torch = _torch()
+    if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous float32 CUDA tensor")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
+def flops_per_sample(s: FieldSpec) -> float:
+    return float(lib().upf_flops_per_sample(ctypes.byref(s)))
+
+
+def adjoint_flops_per_sample(s: FieldSpec) -> float:
+    return float(lib().upf_adjoint_flops_per_sample(ctypes.byref(s)))
+
+
+def reference_grid(debug: bool = False) -> Grid:
+    """The reference's sweep box [-2000, 2000]^3 at 500^3 nodes (110^3 in debug builds),
+    sources/meshGeneration.cpp:19-28,35-36."""
+    return uniform_grid(-2000.0, 2000.0, 110 if debug else 500)
+
+
+# ---------------------------------------------------------------------------- field
+def field_eval_grid(s: FieldSpec, g: Grid, out=None, stream=None):
+    """f at every node of g -> float32 tensor [nz, ny, nx] (upf_field_eval_grid)."""
+    torch = _torch()
+    nx, ny, nz = grid_dims(g)
+    if out is None:
+        out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+    _check_f32(out, "out", (nz, ny, nx))
+    check(lib().upf_field_eval_grid(ctypes.byref(s), ctypes.byref(g), _ptr(out), _stream(stream)),
+          "upf_field_eval_grid")
+    return out
+
+
+def field_eval_points(s: FieldSpec, xyz, with_grad: bool = False, stream=None):
+    """f (and df/dx when with_grad) at points xyz [n, 3] (upf_field_eval_points)."""
+    torch = _torch()
+    _check_f32(xyz, "xyz")
+    n = xyz.shape[0]
+    f = torch.empty((n,), dtype=torch.float32, device=xyz.device)
+    g = torch.empty((n, 3), dtype=torch.float32, device=xyz.device) if with_grad else None
+    check(lib().upf_field_eval_points(ctypes.byref(s), _ptr(xyz), n, _ptr(f), _ptr(g), _stream(stream)),
+          "upf_field_eval_points")
+    return (f, g) if with_grad else f
+
+
+def field_adjoint(s: FieldSpec, xyz, dLdf=None, want_f=True, want_dfdx=True, dtheta=None, stream=None):
+    """Fused forward + adjoint at points (upf_field_adjoint).  Returns (f, dfdx, dtheta);
+    dtheta is a float32 [76] tensor in upf_theta_grad order (sum over points of
+    dLdf * df/dθ), None when dLdf is None."""
+    torch = _torch()
+    _check_f32(xyz, "xyz")
+    n = xyz.shape[0]
+    f = torch.empty((n,), dtype=torch.float32, device=xyz.device) if want_f else None
+    g = torch.empty((n, 3), dtype=torch.float32, device=xyz.device) if want_dfdx else None
+    if dLdf is not None:
+        _check_f32(dLdf, "dLdf", (n,))
+        if dtheta is None:
+            dtheta = torch.zeros((spec_mod.THETA_COUNT,), dtype=torch.float32, device=xyz.device)
+    else:
+        dtheta = None
+    check(lib().upf_field_adjoint(ctypes.byref(s), _ptr(xyz), n, _ptr(dLdf), _ptr(f), _ptr(g), _ptr(dtheta),
+                                  _stream(stream)), "upf_field_adjoint")
+    return f, g, dtheta
+
+
+def field_adjoint_grid(s: FieldSpec, gr: Grid, dLdf=None, want_f=True, want_dfdx=True, stream=None):
+    """Fused forward + adjoint over grid nodes (upf_field_adjoint_grid)."""
+    torch = _torch()
+    nx, ny, nz = grid_dims(gr)
+    f = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda") if want_f else None
+    g = torch.empty((nz, ny, nx, 3), dtype=torch.float32, device="cuda") if want_dfdx else None
+    dtheta = None
+    if dLdf is not None:
+        _check_f32(dLdf, "dLdf", (nz, ny, nx))
+        dtheta = torch.zeros((spec_mod.THETA_COUNT,), dtype=torch.float32, device="cuda")
+    check(lib().upf_field_adjoint_grid(ctypes.byref(s), ctypes.byref(gr), _ptr(dLdf), _ptr(f), _ptr(g),
+                                       _ptr(dtheta), _stream(stream)), "upf_field_adjoint_grid")
+    return f, g, dtheta
+
+
+def _with_compose(s: FieldSpec, compose: str) -> FieldSpec:
+    r = copy_spec(s)
+    r.compose = spec_mod.COMPOSE[compose]
+    return r
+
+
+def terrain_sdf_land(s, xyz):
+    """terrainSdfLand at points (sources/modesConfiguration.cpp:248-258)."""
+    return field_eval_points(_with_compose(s, "land"), xyz)
+
+
+def terrain_sdf_water(s, xyz):
+    """terrainSdfWater (sources/modesConfiguration.cpp:260-267)."""
+    return field_eval_points(_with_compose(s, "water"), xyz)
+
+
+def terrain_sdf_navigation(s, xyz):
+    """terrainSdfNavigation (sources/modesConfiguration.cpp:269-279)."""
+    return field_eval_points(_with_compose(s, "navigation"), xyz)
+
+
+def terrain_sdf_elevation(s, xyz):
+    """terrainSdfElevation = shape / elevation_scale (sources/modesConfiguration.cpp:230-237)."""
+    return field_eval_points(_with_compose(s, "elevation"), xyz)
+
+
+def terrain_sdf_elevation_raw(s, xyz):
+    """terrainSdfElevationRaw (sources/modesConfiguration.cpp:239-246)."""
+    return field_eval_points(_with_compose(s, "elevation_raw"), xyz)
+
+
+# ---------------------------------------------------------------------------- meshing
+@dataclass
+class Mesh:
+    positions: object  # float32 [V, 3]
+    normals: object    # float32 [V, 3] or None
+    indices: object    # int32 [T, 3] (uint32 values)
+
+    @property
+    def n_vertices(self) -> int:
+        return int(self.positions.shape[0])
+
+    @property
+    def n_triangles(self) -> int:
+        return int(self.indices.shape[0])
+
+
+class McWorkspace:
+    """Reusable marching-cubes workspace for one grid shape."""
+
+    def __init__(self, g: Grid, device=None):
+        torch = _torch()
+        self.bytes = int(lib().upf_mc_workspace_bytes(ctypes.byref(g)))
+        if self.bytes == 0:
+            raise UpfError(spec_mod.ERR_BAD_GRID, "upf_mc_workspace_bytes")
+        self.buf = torch.empty((self.bytes,), dtype=torch.uint8, device=_dev(device))
+
+
+def mc_extract(field, g: Grid, iso: float = 0.0, normals: bool = True, workspace: McWorkspace | None = None,
+               stream=None) -> Mesh:
+    """Marching cubes over field [nz, ny, nx] (upf_mc_count + upf_mc_emit)."""
+    torch = _torch()
+    nx, ny, nz = grid_dims(g)
+    _check_f32(field, "field", (nz, ny, nx))
+    ws = workspace or McWorkspace(g, field.device)
+    nv, nt = ctypes.c_int64(0), ctypes.c_int64(0)
+    st = _stream(stream)
+    check(lib().upf_mc_count(_ptr(field), ctypes.byref(g), iso, _ptr(ws.buf), ws.bytes, ctypes.byref(nv),
+                             ctypes.byref(nt), st), "upf_mc_count")
+    pos = torch.empty((nv.value, 3), dtype=torch.float32, device=field.device)
+    nrm = torch.empty((nv.value, 3), dtype=torch.float32, device=field.device) if normals else None
+    idx = torch.empty((nt.value, 3), dtype=torch.int32, device=field.device)
+    mb = MeshBuffers()
+    mb.positions = pos.data_ptr() if nv.value else None
+    mb.normals = nrm.data_ptr() if (normals and nv.value) else None
+    mb.indices = idx.data_ptr() if nt.value else None
+    mb.cap_vertices, mb.cap_triangles = nv.value, nt.value
+    mb.n_vertices, mb.n_triangles = nv.value, nt.value
+    check(lib().upf_mc_emit(_ptr(field), ctypes.byref(g), iso, ctypes.byref(mb), _ptr(ws.buf), ws.bytes, st),
+          "upf_mc_emit")
+    return Mesh(pos, nrm, idx)
+
+
+def mc_case_codes(field, g: Grid, iso: float = 0.0, stream=None):
+    torch = _torch()
+    nx, ny, nz = grid_dims(g)
+    _check_f32(field, "field", (nz, ny, nx))
+    out = torch.empty((nz - 1, ny - 1, nx - 1), dtype=torch.uint8, device=field.device)
+    check(lib().upf_mc_case_codes(_ptr(field), ctypes.byref(g), iso, _ptr(out), _stream(stream)),
+          "upf_mc_case_codes")
+    return out
+
+
+def mesh_generate_generic(s: FieldSpec, g: Grid, iso: float = 0.0, normals: bool = True) -> Mesh:
+    """Grid sweep + marching cubes (sources/meshGeneration.cpp:32-45).  The reference then
+    runs meshRemoveDisconnected and meshFlipNormals; here normals already point
+    outward (towards increasing f), so no flip is needed."""
+    f = field_eval_grid(s, g)
+    return mc_extract(f, g, iso, normals)
+
+
+def mesh_generate_base_land(s: FieldSpec, g: Grid | None = None) -> Mesh:
+    """sources/meshGeneration.cpp:47-55: land mesh, raising on an empty result."""
+    m = mesh_generate_generic(_with_compose(s, "land"), g or reference_grid())
+    if m.n_triangles == 0:
+        raise UpfError(spec_mod.ERR_EMPTY_MESH, "generated empty base land mesh")
+    return m
+
+
+def mesh_generate_base_water(s: FieldSpec, g: Grid | None = None) -> Mesh:
+    """sources/meshGeneration.cpp:57-60: water mesh (the vertex mask of :62-95 is not applied)."""
+    return mesh_generate_generic(_with_compose(s, "water"), g or reference_grid())
+
+
+def mesh_generate_base_navigation(s: FieldSpec, g: Grid | None = None) -> Mesh:
+    """sources/meshGeneration.cpp:100-107: navigation mesh, raising on an empty result."""
+    m = mesh_generate_generic(_with_compose(s, "navigation"), g or reference_grid())
+    if m.n_triangles == 0:
+        raise UpfError(spec_mod.ERR_EMPTY_MESH, "generated empty base navigation mesh")
+    return m

@@ -1,0 +1,81 @@
+"""ctypes binding of libupf.so (the C-ABI declared in include/upf.h).
+
+The library is loaded from this package directory; there is no fallback: when it is
+missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .spec import FieldSpec, Grid, Layer, MeshBuffers
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libupf.so")
+
+_lib = None
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "upf_version": (ctypes.c_int, []),
+    "upf_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "upf_layer_init": (ctypes.c_int, [ctypes.POINTER(Layer), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_uint32, ctypes.c_float, ctypes.c_int32]),
+    "upf_spec_validate": (ctypes.c_int, [ctypes.POINTER(FieldSpec)]),
+    "upf_flops_per_sample": (ctypes.c_double, [ctypes.POINTER(FieldSpec)]),
+    "upf_adjoint_flops_per_sample": (ctypes.c_double, [ctypes.POINTER(FieldSpec)]),
+    "upf_field_eval_grid": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_void_p,
+                                           ctypes.c_void_p]),
+    "upf_field_eval_points": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.c_void_p, ctypes.c_int64,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "upf_field_adjoint": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
+    "upf_field_adjoint_grid": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p]),
+    "upf_mc_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Grid)]),
+    "upf_mc_extract": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float,
+                                      ctypes.POINTER(MeshBuffers), ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.c_void_p]),
+    "upf_mc_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_void_p,
+                                    ctypes.c_size_t, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.c_void_p]),
+    "upf_mc_emit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float,
+                                   ctypes.POINTER(MeshBuffers), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "upf_mc_case_codes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
+}
+
+
+class UpfError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where}: {status_string(status)} (status {status})")
+
+
+def lib():
+    """The loaded libupf.so.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libupf.so is not built ({LIB_PATH}); run unnatural_planets_amd.build.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_string(status: int) -> str:
+    return lib().upf_status_string(status).decode()
+
+
+def check(status: int, where: str) -> None:
+    if status != 0:
+        raise UpfError(status, where)

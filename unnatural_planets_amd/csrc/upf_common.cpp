@@ -1,0 +1,262 @@
+// upf_common.cpp -- spec validation and conversion, layer expansion, the flop table
+// and status strings of libupf (host code).
+#include <cmath>
+#include <cstring>
+
+#include "upf_internal.hpp"
+
+namespace upf {
+
+namespace {
+bool finite(float v) { return std::isfinite(v); }
+
+int shape_params_used(int shape) {
+  switch (shape) {
+    case UPF_SHAPE_NONE: return 0;
+    case UPF_SHAPE_SPHERE: return 1;
+    case UPF_SHAPE_BOX: return 4;
+    case UPF_SHAPE_TORUS: return 2;
+    case UPF_SHAPE_CAPSULE: return 2;
+    default: return -1;
+  }
+}
+}  // namespace
+
+int to_device(const upf_field_spec* in, DSpec* out) {
+  if (!in || !out) return UPF_ERR_NULL;
+  if (shape_params_used(in->shape) < 0) return UPF_ERR_BAD_SPEC;
+  if (in->compose < 0 || in->compose >= UPF_COMPOSE_COUNT) return UPF_ERR_BAD_SPEC;
+  if (in->n_layers < 0 || in->n_layers > UPF_MAX_LAYERS) return UPF_ERR_BAD_SPEC;
+  for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++)
+    if (!finite(in->shape_params[i])) return UPF_ERR_INVALID_VALUE;
+  if (!finite(in->elevation_scale)) return UPF_ERR_INVALID_VALUE;
+  if (in->compose == UPF_COMPOSE_ELEVATION && in->elevation_scale == 0.0f) return UPF_ERR_INVALID_VALUE;
+  std::memset(out, 0, sizeof(*out));
+  out->shape = in->shape;
+  out->compose = in->compose;
+  out->n_layers = in->n_layers;
+  for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++) out->sp[i] = in->shape_params[i];
+  out->escale = in->elevation_scale;
+  out->all_add = 1;
+  for (int l = 0; l < in->n_layers; l++) {
+    const upf_layer& a = in->layers[l];
+    DLayer& b = out->L[l];
+    if (a.noise < 1 || a.noise >= UPF_NOISE_COUNT) return UPF_ERR_BAD_SPEC;
+    if (a.fractal < 0 || a.fractal >= UPF_FRACTAL_COUNT) return UPF_ERR_BAD_SPEC;
+    if (a.blend < 0 || a.blend >= UPF_BLEND_COUNT) return UPF_ERR_BAD_SPEC;
+    const int oct = a.fractal == UPF_FRACTAL_NONE ? 1 : a.octaves;
+    if (oct < 1 || oct > UPF_MAX_OCTAVES) return UPF_ERR_BAD_SPEC;
+    if (!finite(a.weight) || !finite(a.jitter)) return UPF_ERR_INVALID_VALUE;
+    for (int k = 0; k < oct; k++)
+      if (!finite(a.freq[k]) || !finite(a.amp[k])) return UPF_ERR_INVALID_VALUE;
+    b.noise = a.noise;
+    b.fractal = a.fractal;
+    b.octaves = oct;
+    b.blend = a.blend;
+    b.seed = a.seed;
+    b.weight = a.weight;
+    b.jitter = a.jitter;
+    b.js = a.jitter / 1023.0f;
+    b.j0 = a.jitter * 0.5f;
+    for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+      b.freq[k] = k < oct ? a.freq[k] : 0.0f;
+      b.amp[k] = k < oct ? a.amp[k] : 0.0f;
+    }
+    if (a.blend != UPF_BLEND_ADD) out->all_add = 0;
+  }
+  return UPF_OK;
+}
+
+int to_device(const upf_grid* in, DGrid* out) {
+  if (!in || !out) return UPF_ERR_NULL;
+  int64_t total = 1;
+  for (int a = 0; a < 3; a++) {
+    if (in->dims[a] < 2 || in->dims[a] > 65535) return UPF_ERR_BAD_GRID;
+    if (!finite(in->origin[a]) || !finite(in->spacing[a]) || !(in->spacing[a] > 0.0f)) return UPF_ERR_BAD_GRID;
+    total *= in->dims[a];
+    out->o[a] = in->origin[a];
+    out->s[a] = in->spacing[a];
+    out->n[a] = in->dims[a];
+  }
+  if (total > (int64_t)1 << 34) return UPF_ERR_BAD_GRID;
+  out->pad = 0;
+  return UPF_OK;
+}
+
+}  // namespace upf
+
+// ---------------------------------------------------------------- flop table
+// FP32 flops per sample by the counting rule (FMA = 2, add/sub/mul/div/sqrt = 1,
+// compare/select/floor/convert/integer = 0), following the forward formulas in
+// upf_device.hpp term by term.
+namespace {
+double noise_flops(int noise) {
+  switch (noise) {
+    // q = p*f (3) + t (3) + t-1 (3) + 3 fades (3*7) + 8 corner dots (8) + 7 lerps (7*3)
+    case UPF_NOISE_GRADIENT: return 3 + 3 + 3 + 21 + 8 + 21;
+    // q (3) + t (3) + fades (21) + lerps (21)
+    case UPF_NOISE_VALUE: return 3 + 3 + 21 + 21;
+    // q (3) + js, j0 (2) + k - j0 per axis value (9) + 27 cells x (3 fma + 3 sub + 5 d2)
+    // + sqrt (1)
+    case UPF_NOISE_CELLULAR: return 3 + 2 + 9 + 27 * (6 + 3 + 5) + 1;
+    default: return 0;
+  }
+}
+
+double shape_flops(int shape) {
+  switch (shape) {
+    case UPF_SHAPE_SPHERE: return 7;
+    case UPF_SHAPE_BOX: return 11;
+    case UPF_SHAPE_TORUS: return 10;
+    case UPF_SHAPE_CAPSULE: return 9;
+    default: return 0;
+  }
+}
+
+double elevation_flops(const upf_field_spec* s) {
+  double f = 0;
+  for (int l = 0; l < s->n_layers; l++) {
+    const upf_layer& L = s->layers[l];
+    const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
+    const double per = noise_flops(L.noise) +
+                       (L.fractal == UPF_FRACTAL_RIDGED ? 4 : (L.fractal == UPF_FRACTAL_FBM ? 2 : 0));
+    f += oct * per + 1;  // weight
+    f += L.blend == UPF_BLEND_ADD ? 1 : (L.blend == UPF_BLEND_MASK ? 1 : 3);
+  }
+  return f;
+}
+// Fused forward + adjoint (k_adjoint with the θ accumulation on), tallied from the
+// formulas of upf_device_grad.hpp by the same rule.  Each θ component costs its own
+// products plus one add for the sum over samples (the cross-lane reduction is a
+// per-wave tree, counted once per sample).
+double noise_d_flops(int noise) {
+  switch (noise) {
+    // q 3, t 3, t-1 3, fades 21, corner gradients 16, corner dots 40, value lerps 18,
+    // gradient blend 63, dn/du 19, fade' 12, chain 6, final lerp 3
+    case UPF_NOISE_GRADIENT: return 3 + 3 + 3 + 21 + 16 + 40 + 18 + 63 + 19 + 12 + 6 + 3;
+    // q 3, t 3, fades 21, lerps 18, dx 18, dy 10, dz 6, final lerp 3
+    case UPF_NOISE_VALUE: return 3 + 3 + 21 + 18 + 18 + 10 + 6 + 3;
+    // forward cellular 393, 1/r, 3 products
+    case UPF_NOISE_CELLULAR: return noise_flops(UPF_NOISE_CELLULAR) + 1 + 3;
+    default: return 0;
+  }
+}
+
+double shape_d_flops(int shape) {
+  switch (shape) {
+    case UPF_SHAPE_SPHERE: return 11;
+    case UPF_SHAPE_BOX: return 18;
+    case UPF_SHAPE_TORUS: return 18;
+    case UPF_SHAPE_CAPSULE: return 15;
+    default: return 0;
+  }
+}
+
+int shape_nparams(int shape) {
+  switch (shape) {
+    case UPF_SHAPE_SPHERE: return 1;
+    case UPF_SHAPE_BOX: return 4;
+    case UPF_SHAPE_TORUS: return 2;
+    case UPF_SHAPE_CAPSULE: return 2;
+    default: return 0;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int upf_version(void) { return UPF_VERSION; }
+
+double upf_adjoint_flops_per_sample(const upf_field_spec* s) {
+  if (!s) return 0;
+  const bool has_shape = s->compose != UPF_COMPOSE_ELEVATION_RAW;
+  const bool has_elev = s->compose == UPF_COMPOSE_LAND || s->compose == UPF_COMPOSE_NAVIGATION ||
+                        s->compose == UPF_COMPOSE_ELEVATION_RAW;
+  double f = 0;
+  if (has_shape) f += shape_d_flops(s->shape) + 3 + 3 * shape_nparams(s->shape);
+  if (!has_elev || s->n_layers == 0) return f + 1;
+  bool all_add = true;
+  for (int l = 0; l < s->n_layers; l++) all_add &= s->layers[l].blend == UPF_BLEND_ADD;
+  if (!all_add || s->compose == UPF_COMPOSE_NAVIGATION) f += elevation_flops(s) + 4 * s->n_layers;
+  for (int l = 0; l < s->n_layers; l++) {
+    const upf_layer& L = s->layers[l];
+    const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
+    // per octave: value accumulate 2, position-gradient scale 2 + 3 fma, q.grad 5,
+    // d/dfreq 2 + 1, d/damp 1 + 1, ridged term and slope 3
+    double per = noise_d_flops(L.noise) + 2 + 2 + 6 + 5 + 3;
+    if (L.fractal != UPF_FRACTAL_NONE) per += 2;
+    if (L.fractal == UPF_FRACTAL_RIDGED) per += 3;
+    f += oct * per;
+    f += 2 + 2 + 4;  // coefficients, elevation accumulate, d/dweight
+  }
+  return f + 2;
+}
+
+const char* upf_status_string(int status) {
+  switch (status) {
+    case UPF_OK: return "ok";
+    case UPF_ERR_INVALID_VALUE: return "invalid value (NaN or Inf)";
+    case UPF_ERR_BAD_SPEC: return "bad spec (unknown id or count out of range)";
+    case UPF_ERR_CAPACITY: return "output capacity too small";
+    case UPF_ERR_WORKSPACE: return "workspace missing or too small";
+    case UPF_ERR_BAD_GRID: return "bad grid";
+    case UPF_ERR_HIP: return "HIP runtime error";
+    case UPF_ERR_NULL: return "null pointer";
+    case UPF_ERR_EMPTY_MESH: return "empty mesh";
+    default: return "unknown status";
+  }
+}
+
+int upf_layer_init(upf_layer* layer, int32_t noise, int32_t fractal, int32_t octaves, float frequency,
+                   float lacunarity, float gain, float weight, uint32_t seed, float jitter, int32_t blend) {
+  if (!layer) return UPF_ERR_NULL;
+  if (noise < 1 || noise >= UPF_NOISE_COUNT || fractal < 0 || fractal >= UPF_FRACTAL_COUNT || blend < 0 ||
+      blend >= UPF_BLEND_COUNT)
+    return UPF_ERR_BAD_SPEC;
+  const int oct = fractal == UPF_FRACTAL_NONE ? 1 : octaves;
+  if (oct < 1 || oct > UPF_MAX_OCTAVES) return UPF_ERR_BAD_SPEC;
+  if (!std::isfinite(frequency) || !std::isfinite(lacunarity) || !std::isfinite(gain) || !std::isfinite(weight) ||
+      !std::isfinite(jitter))
+    return UPF_ERR_INVALID_VALUE;
+  std::memset(layer, 0, sizeof(*layer));
+  layer->noise = noise;
+  layer->fractal = fractal;
+  layer->octaves = oct;
+  layer->blend = blend;
+  layer->seed = seed;
+  layer->weight = weight;
+  layer->jitter = jitter;
+  // fractal bounding 1 / sum |gain|^k, accumulated in FP32 in octave order
+  const float g = std::fabs(gain);
+  float sum = 1.0f, a = g;
+  for (int i = 1; i < oct; i++) { sum += a; a *= g; }
+  float w = fractal == UPF_FRACTAL_NONE ? 1.0f : 1.0f / sum;
+  float f = frequency;
+  for (int k = 0; k < oct; k++) {
+    layer->freq[k] = f;
+    layer->amp[k] = w;
+    f *= lacunarity;
+    w *= gain;
+  }
+  return UPF_OK;
+}
+
+int upf_spec_validate(const upf_field_spec* spec) {
+  upf::DSpec d;
+  return upf::to_device(spec, &d);
+}
+
+double upf_flops_per_sample(const upf_field_spec* s) {
+  if (!s) return 0;
+  const double sh = shape_flops(s->shape);
+  switch (s->compose) {
+    case UPF_COMPOSE_LAND: return sh + elevation_flops(s) + 2;
+    case UPF_COMPOSE_WATER: return sh;
+    case UPF_COMPOSE_NAVIGATION: return sh + elevation_flops(s) + 2;
+    case UPF_COMPOSE_ELEVATION: return sh + 1;
+    case UPF_COMPOSE_ELEVATION_RAW: return elevation_flops(s);
+    default: return 0;
+  }
+}
+
+}  // extern "C"

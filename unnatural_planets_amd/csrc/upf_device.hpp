@@ -1,0 +1,249 @@
+// upf_device.hpp -- gfx950 device functions of the field path: lattice hash, noise,
+// base shapes, layer stack and field composition (forward values).
+//
+// Everything is FP32 VALU work; the integer lattice (floor, hash, gradient selection)
+// is bit-exact with the CPU restatement, and the floating-point parts agree to a few
+// ulps.  This file is compiled with -ffp-contract=off: every fused multiply-add is
+// written as fmaf() on purpose, everything else rounds per operation.
+//
+// Reference anchors: field composition sources/modesConfiguration.cpp:230-279, base
+// shapes sources/sdf.cpp:20-48,81-84, layer configs sources/elevationModes.cpp:16-27.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/upf.h"
+
+namespace upf {
+
+// Kernel-side copy of a layer: the spec's layer plus the cellular constants
+// js = jitter/1023 and j0 = jitter*0.5, precomputed on the host in FP32.
+struct DLayer {
+  int32_t noise, fractal, octaves, blend;
+  uint32_t seed;
+  float weight, jitter, js, j0, pad;
+  float freq[UPF_MAX_OCTAVES];
+  float amp[UPF_MAX_OCTAVES];
+};
+
+struct DSpec {
+  int32_t shape, compose, n_layers, all_add;
+  float sp[UPF_MAX_SHAPE_PARAMS];
+  float escale;
+  float pad[3];
+  DLayer L[UPF_MAX_LAYERS];
+};
+
+struct DGrid {
+  float o[3], s[3];
+  int32_t n[3];
+  int32_t pad;
+};
+
+// ------------------------------------------------------------------ hash and lattice
+constexpr uint32_t kPX = 501125321u;
+constexpr uint32_t kPY = 1136930381u;
+constexpr uint32_t kPZ = 1720413743u;
+constexpr uint32_t kHM = 0x27d4eb2du;
+
+__device__ __forceinline__ float node_coord(float o, float s, int i) {
+  const float m = (float)i * s;
+  return o + m;
+}
+
+// Quintic fade t^3 (t (6t - 15) + 10).
+__device__ __forceinline__ float fade(float t) {
+  const float a = fmaf(t, 6.0f, -15.0f);
+  const float b = fmaf(t, a, 10.0f);
+  return (t * t) * t * b;
+}
+
+// d fade / dt = 30 t^2 (t - 1)^2
+__device__ __forceinline__ float dfade(float t) {
+  const float a = t * (t - 1.0f);
+  return 30.0f * a * a;
+}
+
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+// Dot product with gradient g = h >> 28 of Perlin's 16-entry improved-noise set:
+// u = g<8 ? x : y;  v = g<4 ? y : (g==12 || g==14 ? x : z);  ±u ± v by bits 0 and 1.
+__device__ __forceinline__ float gdot(uint32_t h, float x, float y, float z) {
+  const uint32_t g = h >> 28;
+  const float u = g < 8u ? x : y;
+  const float v = g < 4u ? y : ((g == 12u || g == 14u) ? x : z);
+  const float su = __uint_as_float(__float_as_uint(u) ^ ((g & 1u) << 31));
+  const float sv = __uint_as_float(__float_as_uint(v) ^ ((g & 2u) << 30));
+  return su + sv;
+}
+
+// Gradient vector of hash h, components in {-1, 0, 1}.
+__device__ __forceinline__ void gvec(uint32_t h, float& gx, float& gy, float& gz) {
+  const uint32_t g = h >> 28;
+  const float su = (g & 1u) ? -1.0f : 1.0f;
+  const float sv = (g & 2u) ? -1.0f : 1.0f;
+  const bool ux = g < 8u;                       // u axis: x (else y)
+  const bool vy = g < 4u;                       // v axis: y
+  const bool vx = !vy && (g == 12u || g == 14u);  // v axis: x
+  const bool vz = !vy && !vx;                   // v axis: z
+  gx = (ux ? su : 0.0f) + (vx ? sv : 0.0f);
+  gy = (ux ? 0.0f : su) + (vy ? sv : 0.0f);
+  gz = vz ? sv : 0.0f;
+}
+
+// Corner hashes of the lattice cell (ix, iy, iz): h[c] for c = x + 2y + 4z.
+__device__ __forceinline__ void cell_hashes(uint32_t seed, int ix, int iy, int iz, uint32_t h[8]) {
+  const uint32_t x0 = (uint32_t)ix * kPX, x1 = x0 + kPX;
+  const uint32_t y0 = (uint32_t)iy * kPY, y1 = y0 + kPY;
+  const uint32_t z0 = (uint32_t)iz * kPZ, z1 = z0 + kPZ;
+  const uint32_t s00 = seed ^ y0 ^ z0, s10 = seed ^ y1 ^ z0;
+  const uint32_t s01 = seed ^ y0 ^ z1, s11 = seed ^ y1 ^ z1;
+  h[0] = (s00 ^ x0) * kHM; h[1] = (s00 ^ x1) * kHM;
+  h[2] = (s10 ^ x0) * kHM; h[3] = (s10 ^ x1) * kHM;
+  h[4] = (s01 ^ x0) * kHM; h[5] = (s01 ^ x1) * kHM;
+  h[6] = (s11 ^ x0) * kHM; h[7] = (s11 ^ x1) * kHM;
+}
+
+// ------------------------------------------------------------------ noise (values)
+__device__ __forceinline__ float noise_gradient(uint32_t seed, float qx, float qy, float qz) {
+  const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+  uint32_t h[8];
+  cell_hashes(seed, (int)fx, (int)fy, (int)fz, h);
+  const float tx = qx - fx, ty = qy - fy, tz = qz - fz;
+  const float sx = tx - 1.0f, sy = ty - 1.0f, sz = tz - 1.0f;
+  const float ux = fade(tx), uy = fade(ty), uz = fade(tz);
+  const float x00 = lerpf(gdot(h[0], tx, ty, tz), gdot(h[1], sx, ty, tz), ux);
+  const float x10 = lerpf(gdot(h[2], tx, sy, tz), gdot(h[3], sx, sy, tz), ux);
+  const float x01 = lerpf(gdot(h[4], tx, ty, sz), gdot(h[5], sx, ty, sz), ux);
+  const float x11 = lerpf(gdot(h[6], tx, sy, sz), gdot(h[7], sx, sy, sz), ux);
+  return lerpf(lerpf(x00, x10, uy), lerpf(x01, x11, uy), uz);
+}
+
+__device__ __forceinline__ float lattice_value(uint32_t h) { return (float)(int32_t)h * 4.656612873077393e-10f; }
+
+__device__ __forceinline__ float noise_value(uint32_t seed, float qx, float qy, float qz) {
+  const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+  uint32_t h[8];
+  cell_hashes(seed, (int)fx, (int)fy, (int)fz, h);
+  const float ux = fade(qx - fx), uy = fade(qy - fy), uz = fade(qz - fz);
+  const float x00 = lerpf(lattice_value(h[0]), lattice_value(h[1]), ux);
+  const float x10 = lerpf(lattice_value(h[2]), lattice_value(h[3]), ux);
+  const float x01 = lerpf(lattice_value(h[4]), lattice_value(h[5]), ux);
+  const float x11 = lerpf(lattice_value(h[6]), lattice_value(h[7]), ux);
+  return lerpf(lerpf(x00, x10, uy), lerpf(x01, x11, uy), uz);
+}
+
+// Cellular F1 over the 27 cells around rint(q); returns the squared distance and the
+// offset (q - nearest point) for the gradient.
+__device__ __forceinline__ float cellular_d2(uint32_t seed, float qx, float qy, float qz, float js,
+                                             float j0, float& ex, float& ey, float& ez) {
+  const float cxf = rintf(qx), cyf = rintf(qy), czf = rintf(qz);
+  const int cx = (int)cxf, cy = (int)cyf, cz = (int)czf;
+  float best = __builtin_inff();
+  ex = ey = ez = 0.0f;
+  const uint32_t xb = (uint32_t)(cx - 1) * kPX, yb = (uint32_t)(cy - 1) * kPY, zb = (uint32_t)(cz - 1) * kPZ;
+#pragma unroll
+  for (int dz = 0; dz < 3; dz++) {
+    const uint32_t zp = zb + (uint32_t)dz * kPZ;
+    const float kz = (float)(cz - 1 + dz) - j0;
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) {
+      const uint32_t yzs = seed ^ (yb + (uint32_t)dy * kPY) ^ zp;
+      const float ky = (float)(cy - 1 + dy) - j0;
+#pragma unroll
+      for (int dx = 0; dx < 3; dx++) {
+        const float kx = (float)(cx - 1 + dx) - j0;
+        const uint32_t h = (yzs ^ (xb + (uint32_t)dx * kPX)) * kHM;
+        const uint32_t m = h ^ (h >> 15);
+        const float px = fmaf((float)(m & 1023u), js, kx);
+        const float py = fmaf((float)((m >> 10) & 1023u), js, ky);
+        const float pz = fmaf((float)((m >> 20) & 1023u), js, kz);
+        const float dxv = px - qx, dyv = py - qy, dzv = pz - qz;
+        const float d2 = fmaf(dzv, dzv, fmaf(dyv, dyv, dxv * dxv));
+        if (d2 < best) { best = d2; ex = -dxv; ey = -dyv; ez = -dzv; }
+      }
+    }
+  }
+  return best;
+}
+
+__device__ __forceinline__ float noise_cellular(uint32_t seed, float qx, float qy, float qz, float js, float j0) {
+  float ex, ey, ez;
+  return __builtin_sqrtf(cellular_d2(seed, qx, qy, qz, js, j0, ex, ey, ez));
+}
+
+// ------------------------------------------------------------------ shapes
+__device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, float z) {
+  switch (s.shape) {
+    case UPF_SHAPE_SPHERE:
+      return __builtin_sqrtf(fmaf(z, z, fmaf(y, y, x * x))) - s.sp[0];
+    case UPF_SHAPE_BOX: {
+      const float qx = fabsf(x) - s.sp[0], qy = fabsf(y) - s.sp[1], qz = fabsf(z) - s.sp[2];
+      const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
+      const float outside = __builtin_sqrtf(fmaf(mz, mz, fmaf(my, my, mx * mx)));
+      const float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+      return outside + inside - s.sp[3];
+    }
+    case UPF_SHAPE_TORUS: {
+      const float ring = __builtin_sqrtf(fmaf(z, z, x * x)) - s.sp[0];
+      return __builtin_sqrtf(fmaf(y, y, ring * ring)) - s.sp[1];
+    }
+    case UPF_SHAPE_CAPSULE: {
+      const float h2 = s.sp[0] * 0.5f;
+      const float zc = z - fminf(fmaxf(z, -h2), h2);
+      return __builtin_sqrtf(fmaf(zc, zc, fmaf(y, y, x * x))) - s.sp[1];
+    }
+    default:
+      return 0.0f;
+  }
+}
+
+// ------------------------------------------------------------------ layers
+__device__ __forceinline__ float octave_noise(const DLayer& L, int k, float x, float y, float z) {
+  const float f = L.freq[k];
+  const float qx = x * f, qy = y * f, qz = z * f;
+  const uint32_t seed = L.seed + (uint32_t)k;
+  if (L.noise == UPF_NOISE_GRADIENT) return noise_gradient(seed, qx, qy, qz);
+  if (L.noise == UPF_NOISE_VALUE) return noise_value(seed, qx, qy, qz);
+  return noise_cellular(seed, qx, qy, qz, L.js, L.j0);
+}
+
+__device__ __forceinline__ float layer_value(const DLayer& L, float x, float y, float z) {
+  if (L.fractal == UPF_FRACTAL_NONE) return octave_noise(L, 0, x, y, z);
+  float acc = 0.0f;
+  if (L.fractal == UPF_FRACTAL_RIDGED) {
+    for (int k = 0; k < L.octaves; k++) {
+      const float n = octave_noise(L, k, x, y, z);
+      acc = fmaf(L.amp[k], fmaf(-2.0f, fabsf(n), 1.0f), acc);
+    }
+  } else {
+    for (int k = 0; k < L.octaves; k++) acc = fmaf(L.amp[k], octave_noise(L, k, x, y, z), acc);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float elevation_eval(const DSpec& s, float x, float y, float z) {
+  float acc = 0.0f, m = 0.0f;
+  for (int l = 0; l < s.n_layers; l++) {
+    const DLayer& L = s.L[l];
+    const float v = L.weight * layer_value(L, x, y, z);
+    if (L.blend == UPF_BLEND_ADD) acc = acc + v;
+    else if (L.blend == UPF_BLEND_MASK) m = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
+    else acc = fmaf(m, v - acc, acc);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float field_eval(const DSpec& s, float x, float y, float z) {
+  switch (s.compose) {
+    case UPF_COMPOSE_LAND: return shape_eval(s, x, y, z) - s.escale * elevation_eval(s, x, y, z);
+    case UPF_COMPOSE_WATER: return shape_eval(s, x, y, z);
+    case UPF_COMPOSE_NAVIGATION:
+      return shape_eval(s, x, y, z) - s.escale * fmaxf(elevation_eval(s, x, y, z), 0.0f);
+    case UPF_COMPOSE_ELEVATION: return shape_eval(s, x, y, z) / s.escale;
+    default: return elevation_eval(s, x, y, z);
+  }
+}
+
+}  // namespace upf

@@ -1,0 +1,170 @@
+// upf_field.hip -- field evaluation kernels (forward and fused forward+adjoint) and
+// their C-ABI entry points.
+//
+// Replaces the per-node callback the reference hands to cage's marching cubes,
+// cubes->updateByPosition(Delegate<Real(const Vec3&)>(terrainSdfLand))
+// (sources/meshGeneration.cpp:38), and the point-wise terrainSdf* calls
+// (sources/coloringModes.cpp:51-57,122-151).
+#include <hip/hip_runtime.h>
+
+#include "upf_device_grad.hpp"
+#include "upf_internal.hpp"
+
+namespace upf {
+
+// One thread per node.  A workgroup covers 64 consecutive x nodes of 4 rows, so each
+// wave writes 256 contiguous bytes.
+constexpr int kGridBX = 64;
+constexpr int kGridBY = 4;
+
+__global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g, float* __restrict__ out) {
+  const int i = blockIdx.x * kGridBX + threadIdx.x;
+  const int j = blockIdx.y * kGridBY + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= g.n[0] || j >= g.n[1]) return;
+  const float x = node_coord(g.o[0], g.s[0], i);
+  const float y = node_coord(g.o[1], g.s[1], j);
+  const float z = node_coord(g.o[2], g.s[2], k);
+  out[((size_t)k * g.n[1] + j) * g.n[0] + i] = field_eval(s, x, y, z);
+}
+
+__global__ __launch_bounds__(256) void k_eval_points(const DSpec s, const float* __restrict__ xyz, int64_t n,
+                                                     float* __restrict__ f) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  f[i] = field_eval(s, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+}
+
+// Fused forward + adjoint over a grid-stride loop; every lane of a block runs the same
+// number of iterations so the wave reductions inside ThetaAcc see all 64 lanes.
+template <bool THETA, bool GRID>
+__global__ __launch_bounds__(256) void k_adjoint(const DSpec s, const DGrid g, const float* __restrict__ xyz,
+                                                 int64_t n, const float* __restrict__ dLdf, float* __restrict__ fout,
+                                                 float* __restrict__ dfdx, float* __restrict__ dLdtheta) {
+  __shared__ float th[4][UPF_THETA_COUNT];
+  const int wave = threadIdx.x >> 6;
+  if (THETA) {
+    for (int t = threadIdx.x; t < 4 * UPF_THETA_COUNT; t += 256) (&th[0][0])[t] = 0.0f;
+    __syncthreads();
+  }
+  ThetaAcc acc{th[wave]};
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t base = (int64_t)blockIdx.x * 256; base < n; base += stride) {
+    const int64_t idx = base + threadIdx.x;
+    const bool valid = idx < n;
+    const int64_t i = valid ? idx : n - 1;
+    float x, y, z;
+    if (GRID) {
+      const int64_t nxy = (int64_t)g.n[0] * g.n[1];
+      const int kk = (int)(i / nxy);
+      const int64_t r = i - (int64_t)kk * nxy;
+      const int jj = (int)(r / g.n[0]);
+      const int ii = (int)(r - (int64_t)jj * g.n[0]);
+      x = node_coord(g.o[0], g.s[0], ii);
+      y = node_coord(g.o[1], g.s[1], jj);
+      z = node_coord(g.o[2], g.s[2], kk);
+    } else {
+      x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
+    }
+    const float w = (THETA && valid) ? dLdf[i] : 0.0f;
+    float gr[3];
+    const float f = field_grad<THETA>(s, x, y, z, w, gr, acc);
+    if (valid) {
+      if (fout) fout[i] = f;
+      if (dfdx) { dfdx[3 * i] = gr[0]; dfdx[3 * i + 1] = gr[1]; dfdx[3 * i + 2] = gr[2]; }
+    }
+  }
+  if (THETA) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < UPF_THETA_COUNT; t += 256) {
+      const float v = th[0][t] + th[1][t] + th[2][t] + th[3][t];
+      if (v != 0.0f) atomicAdd(dLdtheta + t, v);
+    }
+  }
+}
+
+namespace {
+int adjoint_blocks(int64_t n) {
+  // enough workgroups to fill 256 CUs several times; capped so the per-block θ
+  // flush stays a few hundred thousand atomics at most
+  const int64_t want = (n + 255) / 256;
+  return (int)(want < 4096 ? (want > 0 ? want : 1) : 4096);
+}
+}  // namespace
+
+}  // namespace upf
+
+using namespace upf;
+
+extern "C" {
+
+int upf_field_eval_grid(const upf_field_spec* spec, const upf_grid* grid, float* d_field, void* stream) {
+  if (!d_field) return UPF_ERR_NULL;
+  DSpec s;
+  DGrid g;
+  int st = to_device(spec, &s);
+  if (st) return st;
+  if ((st = to_device(grid, &g))) return st;
+  const dim3 blocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + kGridBY - 1) / kGridBY, g.n[2]);
+  hipLaunchKernelGGL(k_eval_grid, blocks, dim3(kGridBX, kGridBY, 1), 0, as_stream(stream), s, g, d_field);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_field_eval_points(const upf_field_spec* spec, const float* d_xyz, int64_t n, float* d_f, float* d_dfdx,
+                          void* stream) {
+  if (n < 0) return UPF_ERR_BAD_GRID;
+  if (n == 0) return UPF_OK;
+  if (!d_xyz || !d_f) return UPF_ERR_NULL;
+  DSpec s;
+  const int st = to_device(spec, &s);
+  if (st) return st;
+  if (d_dfdx) return upf_field_adjoint(spec, d_xyz, n, nullptr, d_f, d_dfdx, nullptr, stream);
+  hipLaunchKernelGGL(k_eval_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), s, d_xyz,
+                     n, d_f);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_field_adjoint(const upf_field_spec* spec, const float* d_xyz, int64_t n, const float* d_dLdf, float* d_f,
+                      float* d_dfdx, float* d_dLdtheta, void* stream) {
+  if (n < 0) return UPF_ERR_BAD_GRID;
+  if (n == 0) return UPF_OK;
+  if (!d_xyz) return UPF_ERR_NULL;
+  if (d_dLdtheta && !d_dLdf) return UPF_ERR_NULL;
+  DSpec s;
+  const int st = to_device(spec, &s);
+  if (st) return st;
+  DGrid g{};
+  const int nb = adjoint_blocks(n);
+  if (d_dLdtheta)
+    hipLaunchKernelGGL((k_adjoint<true, false>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, d_xyz, n, d_dLdf,
+                       d_f, d_dfdx, d_dLdtheta);
+  else
+    hipLaunchKernelGGL((k_adjoint<false, false>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, d_xyz, n,
+                       d_dLdf, d_f, d_dfdx, d_dLdtheta);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_field_adjoint_grid(const upf_field_spec* spec, const upf_grid* grid, const float* d_dLdf, float* d_f,
+                           float* d_dfdx, float* d_dLdtheta, void* stream) {
+  if (d_dLdtheta && !d_dLdf) return UPF_ERR_NULL;
+  DSpec s;
+  DGrid g;
+  int st = to_device(spec, &s);
+  if (st) return st;
+  if ((st = to_device(grid, &g))) return st;
+  const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  const int nb = adjoint_blocks(n);
+  if (d_dLdtheta)
+    hipLaunchKernelGGL((k_adjoint<true, true>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, nullptr, n, d_dLdf,
+                       d_f, d_dfdx, d_dLdtheta);
+  else
+    hipLaunchKernelGGL((k_adjoint<false, true>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, nullptr, n,
+                       d_dLdf, d_f, d_dfdx, d_dLdtheta);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+"""ctypes mirror of the POD structs and enums in include/upf.h, plus spec builders.
+
+The builders mirror how the reference configures its fields:
+  * `layer(...)` ~ NoiseFunctionCreateConfig + newNoiseFunction
+    (reference sources/elevationModes.cpp:16-27, 83-87);
+  * `FieldSpec.compose` ~ terrainSdfLand/Water/Navigation/Elevation/ElevationRaw
+    (reference sources/modesConfiguration.cpp:230-279);
+  * `shape=` ~ the base-shape functions of sources/sdf.cpp.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+MAX_LAYERS = 4
+MAX_OCTAVES = 8
+MAX_SHAPE_PARAMS = 8
+THETA_COUNT = 76
+
+# status
+OK, ERR_INVALID_VALUE, ERR_BAD_SPEC, ERR_CAPACITY, ERR_WORKSPACE, ERR_BAD_GRID, ERR_HIP, \
+    ERR_NULL, ERR_EMPTY_MESH = range(9)
+
+SHAPE = {"none": 0, "sphere": 1, "box": 2, "torus": 3, "capsule": 4}
+NOISE = {"gradient": 1, "value": 2, "cellular": 3}
+FRACTAL = {"none": 0, "fbm": 1, "ridged": 2}
+BLEND = {"add": 0, "mask": 1, "lerp": 2}
+COMPOSE = {"land": 0, "water": 1, "navigation": 2, "elevation": 3, "elevation_raw": 4}
+
+
+class Layer(ctypes.Structure):
+    _fields_ = [
+        ("noise", ctypes.c_int32), ("fractal", ctypes.c_int32), ("octaves", ctypes.c_int32),
+        ("blend", ctypes.c_int32), ("seed", ctypes.c_uint32), ("weight", ctypes.c_float),
+        ("jitter", ctypes.c_float), ("reserved", ctypes.c_float),
+        ("freq", ctypes.c_float * MAX_OCTAVES), ("amp", ctypes.c_float * MAX_OCTAVES),
+    ]
+
+
+class FieldSpec(ctypes.Structure):
+    _fields_ = [
+        ("shape", ctypes.c_int32), ("compose", ctypes.c_int32), ("n_layers", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("shape_params", ctypes.c_float * MAX_SHAPE_PARAMS),
+        ("elevation_scale", ctypes.c_float), ("reserved", ctypes.c_float * 3),
+        ("layers", Layer * MAX_LAYERS),
+    ]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("spacing", ctypes.c_float * 3),
+                ("dims", ctypes.c_int32 * 3), ("reserved", ctypes.c_int32)]
+
+
+class MeshBuffers(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+                ("indices", ctypes.c_void_p), ("cap_vertices", ctypes.c_int64),
+                ("cap_triangles", ctypes.c_int64), ("n_vertices", ctypes.c_int64),
+                ("n_triangles", ctypes.c_int64)]
+
+
+assert ctypes.sizeof(Layer) == 96
+assert ctypes.sizeof(FieldSpec) == 16 + 32 + 16 + 4 * 96
+assert ctypes.sizeof(Grid) == 40
+
+
+def f32(x: float) -> float:
+    """Round a Python float to the nearest float32."""
+    return ctypes.c_float(x).value
+
+
+def expand_octaves(fractal: int, octaves: int, frequency: float, lacunarity: float, gain: float):
+    """Per-octave (freq, amp) in float32, in the order documented at upf_layer."""
+    oct_ = 1 if fractal == FRACTAL["none"] else octaves
+    g = f32(abs(f32(gain)))
+    s, a = f32(1.0), g
+    for _ in range(1, oct_):
+        s = f32(s + a)
+        a = f32(a * g)
+    w = f32(1.0) if fractal == FRACTAL["none"] else f32(f32(1.0) / s)
+    f = f32(frequency)
+    freq, amp = [0.0] * MAX_OCTAVES, [0.0] * MAX_OCTAVES
+    for k in range(oct_):
+        freq[k], amp[k] = f, w
+        f = f32(f * f32(lacunarity))
+        w = f32(w * f32(gain))
+    return freq, amp
+
+
+def layer(noise="gradient", fractal="fbm", octaves=4, frequency=1.0, lacunarity=2.0, gain=0.5,
+          weight=1.0, seed=0, jitter=1.0, blend="add") -> Layer:
+    L = Layer()
+    L.noise = NOISE[noise] if isinstance(noise, str) else int(noise)
+    L.fractal = FRACTAL[fractal] if isinstance(fractal, str) else int(fractal)
+    L.octaves = 1 if L.fractal == 0 else int(octaves)
+    L.blend = BLEND[blend] if isinstance(blend, str) else int(blend)
+    L.seed = int(seed) & 0xFFFFFFFF
+    L.weight = weight
+    L.jitter = jitter
+    freq, amp = expand_octaves(L.fractal, L.octaves, frequency, lacunarity, gain)
+    for k in range(MAX_OCTAVES):
+        L.freq[k], L.amp[k] = freq[k], amp[k]
+    return L
+
+
+def field_spec(shape="sphere", shape_params=(1.0,), layers=(), compose="land",
+               elevation_scale=1.0) -> FieldSpec:
+    s = FieldSpec()
+    s.shape = SHAPE[shape] if isinstance(shape, str) else int(shape)
+    s.compose = COMPOSE[compose] if isinstance(compose, str) else int(compose)
+    for i, v in enumerate(shape_params):
+        s.shape_params[i] = v
+    s.elevation_scale = elevation_scale
+    s.n_layers = len(layers)
+    for i, L in enumerate(layers):
+        s.layers[i] = L
+    return s
+
+
+def uniform_grid(lo: float, hi: float, n: int) -> Grid:
+    """n^3 nodes over [lo, hi]^3: origin lo, spacing (hi - lo)/(n - 1) in float32."""
+    g = Grid()
+    sp = f32(f32(f32(hi) - f32(lo)) / f32(n - 1))
+    for a in range(3):
+        g.origin[a] = lo
+        g.spacing[a] = sp
+        g.dims[a] = n
+    return g
+
+
+def grid_dims(g: Grid):
+    return int(g.dims[0]), int(g.dims[1]), int(g.dims[2])
+
+
+def copy_spec(s: FieldSpec) -> FieldSpec:
+    r = FieldSpec()
+    ctypes.memmove(ctypes.byref(r), ctypes.byref(s), ctypes.sizeof(FieldSpec))
+    return r
+
+
+def theta_index(kind: str, layer_idx: int = 0, octave: int = 0) -> int:
+    """Index into the 76-float upf_theta_grad vector."""
+    if kind == "shape":
+        return layer_idx  # shape parameter index
+    if kind == "weight":
+        return MAX_SHAPE_PARAMS + layer_idx
+    if kind == "freq":
+        return MAX_SHAPE_PARAMS + MAX_LAYERS + layer_idx * MAX_OCTAVES + octave
+    if kind == "amp":
+        return MAX_SHAPE_PARAMS + MAX_LAYERS + MAX_LAYERS * MAX_OCTAVES + layer_idx * MAX_OCTAVES + octave
+    raise KeyError(kind)
+
+
+# ---------------------------------------------------------------------------- workloads
+def spec_sphere_fbm(seed: int = 1234) -> FieldSpec:
+    """Sphere r=1 plus 0.05 x 4-octave gradient fBm (f0 2.0, lacunarity 2.0, gain 0.5)."""
+    return field_spec("sphere", (1.0,), [layer("gradient", "fbm", 4, 2.0, 2.0, 0.5, 0.05, seed)])
+
+
+def grid_sphere_fbm(n: int = 128) -> Grid:
+    return uniform_grid(-1.3, 1.3, n)
+
+
+def spec_torus_ridged_voronoi(seed: int = 42) -> FieldSpec:
+    """Torus R=1.0 r=0.4; 6-octave ridged gradient noise and a Voronoi F1 layer (jitter 0.9)
+    blended by a 1-octave low-frequency gradient mask."""
+    return field_spec("torus", (1.0, 0.4), [
+        layer("gradient", "ridged", 6, 2.0, 2.0, 0.5, 0.06, seed),
+        layer("gradient", "none", 1, 0.75, 2.0, 0.5, 0.5, seed + 101, blend="mask"),
+        layer("cellular", "none", 1, 6.0, 2.0, 0.5, 0.08, seed + 202, jitter=0.9, blend="lerp"),
+    ])
+
+
+def grid_torus(n: int = 512) -> Grid:
+    return uniform_grid(-1.6, 1.6, n)
+
+
+def spec_torus_fbm_voronoi(seed: int = 7) -> FieldSpec:
+    """Torus R=1, r=0.4 + 4-octave gradient fBm + cellular F1 layer (all blended by add):
+    the 11-parameter θ = {4 amp, 4 freq, Voronoi weight, R, r} of the adjoint workload."""
+    return field_spec("torus", (1.0, 0.4), [
+        layer("gradient", "fbm", 4, 2.0, 2.0, 0.5, 0.05, seed),
+        layer("cellular", "none", 1, 4.0, 2.0, 0.5, 0.03, seed + 101, jitter=0.9),
+    ])
