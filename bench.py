@@ -95,7 +95,7 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the field and "
-              "meshing shard into independent z-slabs; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "meshing shard into independent z-slabs). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import torch
@@ -118,8 +118,11 @@ def main() -> int:
     ws = upf.McWorkspace(grid, dev)
     spec_p, grid_p = ctypes.byref(spec), ctypes.byref(grid)
 
-    def eval_field():
-        upf.check(L.upf_field_eval_grid(spec_p, grid_p, ctypes.c_void_p(field.data_ptr()), sp), "eval")
+    def eval_field(variant=upf.EVAL_AUTO):
+        opts = upf.EvalOpts()
+        opts.variant = variant
+        upf.check(L.upf_field_eval_grid_ex(spec_p, grid_p, ctypes.c_void_p(field.data_ptr()), ctypes.byref(opts), sp),
+                  "eval")
 
     nv, nt = ctypes.c_int64(), ctypes.c_int64()
 
@@ -171,6 +174,7 @@ def main() -> int:
 
     reps = max(3, args.steps)
     t_field = timed(eval_field, reps)
+    t_field_lane = timed(lambda: eval_field(upf.EVAL_PER_LANE), reps)
     t_mc = timed(lambda: (mc_count(), mc_emit()), reps)
 
     flops = float(L.upf_flops_per_sample(spec_p))
@@ -196,10 +200,11 @@ def main() -> int:
                    "grid": [nx, ny, nz], "vertices": V, "triangles": T, "parallelism": "single GPU"},
         "field_gsamples_per_s": round(nodes / t_field / 1e9, 4),
         "field_ms": round(t_field * 1e3, 4),
+        "field_per_lane_ms": round(t_field_lane * 1e3, 4),
         "flops_per_sample": flops,
         "mc_cells_per_s": round(cells / t_mc, 1),
         "mc_ms": round(t_mc * 1e3, 4),
-        "roofline": {"bound": "valu", "kernel": "k_eval_grid", "achieved": round(field_tflops, 3),
+        "roofline": {"bound": "valu", "kernel": "k_eval_grid_row", "achieved": round(field_tflops, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4),
                      "traffic": None},
         "roofline_mc": {"bound": "hbm", "kernel": "marching cubes (classify+scan+emit)", "achieved": round(mc_gbs, 1),

@@ -214,6 +214,22 @@ double upf_adjoint_flops_per_sample(const upf_field_spec* spec);
 int upf_field_eval_grid(const upf_field_spec* spec, const upf_grid* grid, float* d_field,
                         void* stream);
 
+/* Launch options of the grid evaluation (the knobs the batch autotuner searches). */
+enum upf_eval_variant {
+  UPF_EVAL_AUTO = 0,      /* wave-cooperative rows: lattice points hashed once per wave */
+  UPF_EVAL_ROWS = 1,      /* same as AUTO                                               */
+  UPF_EVAL_PER_LANE = 2   /* every lane hashes its own corners / cells                 */
+};
+typedef struct upf_eval_opts {
+  int32_t variant; /* upf_eval_variant */
+  int32_t reserved[3];
+} upf_eval_opts;
+
+/* upf_field_eval_grid with explicit launch options (NULL = defaults).  No counterpart
+ * in the reference; same results for every variant, bit for bit. */
+int upf_field_eval_grid_ex(const upf_field_spec* spec, const upf_grid* grid, float* d_field,
+                           const upf_eval_opts* opts, void* stream);
+
 /* f at n points d_xyz[n][3] -> d_f[n]; if d_dfdx != NULL also the analytic gradient
  * d_dfdx[n][3] (exact normals).  Replaces point-wise terrainSdf* calls
  * (sources/coloringModes.cpp:51-57,122-151; sources/tileProperties.cpp:258-273). */

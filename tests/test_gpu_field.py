@@ -138,3 +138,37 @@ def test_adjoint_theta_matches_fp64_duals(upf, oracle):
         err = np.abs(th - ref) / scale
         print(f"theta max err {err.max():.3e} (scale {scale:.3g})")
         assert err.max() <= 1e-3
+
+
+def _grid(o, sp, dims):
+    g = S.Grid()
+    for a in range(3):
+        g.origin[a], g.spacing[a], g.dims[a] = o[a], sp[a], dims[a]
+    return g
+
+
+@pytest.mark.parametrize("case", ["sphere_fbm", "torus_ridged_voronoi", "value_ridged", "cell_highfreq",
+                                  "grad_highfreq", "ragged"])
+def test_row_kernel_bitwise_equals_per_lane(upf, case):
+    """The wave-cooperative row kernel and the per-lane kernel agree bit for bit, on the
+    LDS path, on the per-lane fallback (lattice span too wide for the wave's slice) and
+    on rows that end mid-wave."""
+    torch = _torch()
+    g = S.uniform_grid(-1.6, 1.6, 70)
+    if case == "sphere_fbm":
+        s = S.spec_sphere_fbm(1234)
+    elif case == "torus_ridged_voronoi":
+        s = S.spec_torus_ridged_voronoi(42)
+    elif case == "value_ridged":
+        s = S.field_spec("box", (0.6, 0.7, 0.8, 0.1), [S.layer("value", "ridged", 7, 1.5, 2.2, 0.6, 0.1, 9)])
+    elif case == "cell_highfreq":  # > 7 cells per row: per-lane fallback for the cellular layer
+        s = S.field_spec("sphere", (1.0,), [S.layer("cellular", "fbm", 3, 40.0, 2.0, 0.5, 0.05, 3, jitter=0.7)])
+    elif case == "grad_highfreq":  # > 62 lattice x per row at the top octaves
+        s = S.field_spec("sphere", (1.0,), [S.layer("gradient", "fbm", 8, 9.0, 2.0, 0.5, 0.05, 4)])
+    else:
+        s = S.spec_torus_ridged_voronoi(7)
+        g = _grid((-1.5, -1.2, -0.7), (0.021, 0.033, 0.05), (131, 45, 29))
+    a = upf.field_eval_grid(s, g, variant=upf.EVAL_ROWS)
+    b = upf.field_eval_grid(s, g, variant=upf.EVAL_PER_LANE)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32))

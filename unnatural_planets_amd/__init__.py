@@ -76,15 +76,24 @@ def reference_grid(debug: bool = False) -> Grid:
 
 
 # ---------------------------------------------------------------------------- field
-def field_eval_grid(s: FieldSpec, g: Grid, out=None, stream=None):
-    """f at every node of g -> float32 tensor [nz, ny, nx] (upf_field_eval_grid)."""
+EVAL_AUTO, EVAL_ROWS, EVAL_PER_LANE = 0, 1, 2
+
+
+class EvalOpts(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+
+def field_eval_grid(s: FieldSpec, g: Grid, out=None, stream=None, variant: int = EVAL_AUTO):
+    """f at every node of g -> float32 tensor [nz, ny, nx] (upf_field_eval_grid_ex)."""
     torch = _torch()
     nx, ny, nz = grid_dims(g)
     if out is None:
         out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
     _check_f32(out, "out", (nz, ny, nx))
-    check(lib().upf_field_eval_grid(ctypes.byref(s), ctypes.byref(g), _ptr(out), _stream(stream)),
-          "upf_field_eval_grid")
+    opts = EvalOpts()
+    opts.variant = variant
+    check(lib().upf_field_eval_grid_ex(ctypes.byref(s), ctypes.byref(g), _ptr(out), ctypes.byref(opts),
+                                       _stream(stream)), "upf_field_eval_grid_ex")
     return out
 
 

@@ -27,6 +27,8 @@ SIGNATURES = {
     "upf_adjoint_flops_per_sample": (ctypes.c_double, [ctypes.POINTER(FieldSpec)]),
     "upf_field_eval_grid": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_void_p,
                                            ctypes.c_void_p]),
+    "upf_field_eval_grid_ex": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p]),
     "upf_field_eval_points": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.c_void_p, ctypes.c_int64,
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "upf_field_adjoint": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.c_void_p, ctypes.c_int64,

@@ -16,12 +16,12 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libupf.so")
 
 SOURCES = ["upf_common.cpp", "upf_field.hip", "upf_mc.hip"]
-HEADERS = ["upf_device.hpp", "upf_device_grad.hpp", "upf_internal.hpp", "mc_table.inc",
+HEADERS = ["upf_device.hpp", "upf_device_grad.hpp", "upf_device_row.hpp", "upf_internal.hpp", "upf_mc.hpp", "mc_table.inc",
            os.path.join("..", "..", "include", "upf.h")]
 
 # -ffp-contract=off: every fused multiply-add in the kernels is an explicit fmaf(), so the
 # lattice coordinates round exactly as in the CPU restatement.
-# -packed-fp32-ops off: see DESIGN.md (packed FP32 forms are not used on this target).
+# -packed-fp32-ops off: the compiler must not form packed FP32 (v_pk_*_f32) instructions.
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-munsafe-fp-atomics",

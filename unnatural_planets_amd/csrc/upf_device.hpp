@@ -236,13 +236,16 @@ __device__ __forceinline__ float elevation_eval(const DSpec& s, float x, float y
 }
 
 __device__ __forceinline__ float field_eval(const DSpec& s, float x, float y, float z) {
+  const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
+                         s.compose == UPF_COMPOSE_ELEVATION_RAW;
+  const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval(s, x, y, z);
+  const float elev = need_elev ? elevation_eval(s, x, y, z) : 0.0f;
   switch (s.compose) {
-    case UPF_COMPOSE_LAND: return shape_eval(s, x, y, z) - s.escale * elevation_eval(s, x, y, z);
-    case UPF_COMPOSE_WATER: return shape_eval(s, x, y, z);
-    case UPF_COMPOSE_NAVIGATION:
-      return shape_eval(s, x, y, z) - s.escale * fmaxf(elevation_eval(s, x, y, z), 0.0f);
-    case UPF_COMPOSE_ELEVATION: return shape_eval(s, x, y, z) / s.escale;
-    default: return elevation_eval(s, x, y, z);
+    case UPF_COMPOSE_LAND: return shape - s.escale * elev;
+    case UPF_COMPOSE_WATER: return shape;
+    case UPF_COMPOSE_NAVIGATION: return shape - s.escale * fmaxf(elev, 0.0f);
+    case UPF_COMPOSE_ELEVATION: return shape / s.escale;
+    default: return elev;
   }
 }
 

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "upf_device_grad.hpp"
+#include "upf_device_row.hpp"
 #include "upf_internal.hpp"
 
 namespace upf {
@@ -26,6 +27,28 @@ __global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g,
   const float y = node_coord(g.o[1], g.s[1], j);
   const float z = node_coord(g.o[2], g.s[2], k);
   out[((size_t)k * g.n[1] + j) * g.n[0] + i] = field_eval(s, x, y, z);
+}
+
+// Wave-cooperative rows (upf_device_row.hpp): each wave evaluates 64 x nodes of one
+// row; lanes past the end of the row stand in for the last node so that all 64 lanes
+// take part in the shared lattice work, and only real nodes are stored.
+__global__ __launch_bounds__(256) void k_eval_grid_row(const DSpec s, const DGrid g, float* __restrict__ out) {
+  __shared__ float4 s_slice[kGridBY][kRowSlice];
+  __shared__ float4 s_gtab[16];
+  if (threadIdx.y == 0 && threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * kGridBX + lane;
+  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * kGridBY + threadIdx.y);
+  const int k = blockIdx.z;
+  if (j >= g.n[1]) return;
+  const int ic = min(i, g.n[0] - 1);
+  const float x = node_coord(g.o[0], g.s[0], ic);
+  const float y = node_coord(g.o[1], g.s[1], j);
+  const float z = node_coord(g.o[2], g.s[2], k);
+  const RowCtx c{s_slice[threadIdx.y], s_gtab, lane};
+  const float v = row_field(c, s, x, y, z);
+  if (i < g.n[0]) out[((size_t)k * g.n[1] + j) * g.n[0] + i] = v;
 }
 
 __global__ __launch_bounds__(256) void k_eval_points(const DSpec s, const float* __restrict__ xyz, int64_t n,
@@ -98,17 +121,27 @@ using namespace upf;
 
 extern "C" {
 
-int upf_field_eval_grid(const upf_field_spec* spec, const upf_grid* grid, float* d_field, void* stream) {
+int upf_field_eval_grid_ex(const upf_field_spec* spec, const upf_grid* grid, float* d_field,
+                           const upf_eval_opts* opts, void* stream) {
   if (!d_field) return UPF_ERR_NULL;
   DSpec s;
   DGrid g;
   int st = to_device(spec, &s);
   if (st) return st;
   if ((st = to_device(grid, &g))) return st;
+  const int variant = opts ? opts->variant : UPF_EVAL_AUTO;
+  if (variant < 0 || variant > UPF_EVAL_PER_LANE) return UPF_ERR_BAD_SPEC;
   const dim3 blocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + kGridBY - 1) / kGridBY, g.n[2]);
-  hipLaunchKernelGGL(k_eval_grid, blocks, dim3(kGridBX, kGridBY, 1), 0, as_stream(stream), s, g, d_field);
+  if (variant == UPF_EVAL_PER_LANE)
+    hipLaunchKernelGGL(k_eval_grid, blocks, dim3(kGridBX, kGridBY, 1), 0, as_stream(stream), s, g, d_field);
+  else
+    hipLaunchKernelGGL(k_eval_grid_row, blocks, dim3(kGridBX, kGridBY, 1), 0, as_stream(stream), s, g, d_field);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
+}
+
+int upf_field_eval_grid(const upf_field_spec* spec, const upf_grid* grid, float* d_field, void* stream) {
+  return upf_field_eval_grid_ex(spec, grid, d_field, nullptr, stream);
 }
 
 int upf_field_eval_points(const upf_field_spec* spec, const float* d_xyz, int64_t n, float* d_f, float* d_dfdx,

@@ -3,185 +3,202 @@
 // Replaces cage's MarchingCubes::makeMesh() (sources/meshGeneration.cpp:39).
 //
 // Layout.  Nodes are cut into "groups" of 64 consecutive x nodes of one row (a row of
-// nx nodes has gpr = ceil(nx/64) groups), one wavefront per group, lane = x offset.
-//  classify  each lane reads its node and the +x/+y/+z neighbours (and the other
-//            cell corners), the wave ballots the inside bits into a 64-bit sign mask
-//            S[g] and counts the crossing edges the group's nodes own (+x, +y, +z) and
-//            the triangles of the cells whose min corner it holds;
-//  scan      exclusive prefix over the per-group (vertex, triangle) counts;
-//  emit      groups with work recompute their crossings, write each owned vertex at
-//            voff[g] + rank (rank = popcounts of the lower lanes' crossing masks, then
-//            the axis order x<y<z), and write each triangle's three vertex ids.  The
-//            id of an edge owned by another group is rebuilt from that group's sign
-//            mask and its +x/+y/+z neighbour masks, so one pass deduplicates vertices
-//            without a hash table or a dense edge-id array.
-// HBM traffic: the field is read once by classify (neighbour reads hit L2) plus the
-// crossing cells' corners again in emit; scratch is 16 B per group (S, counts,
-// offsets), about 0.25 B per node.
+// nx nodes has gpr = ceil(nx/64) groups); a wavefront handles a group, lane = x offset.
+//  classify  a workgroup owns one 64-wide column of 4 rows and sweeps a z range,
+//            keeping the current z plane in registers (each step loads only the next
+//            plane: rows j and j+1 at x and x+1).  Per group it ballots the inside bits
+//            into a 64-bit sign mask S[g] and packs the counts of the crossing edges
+//            the group's nodes own (+x, +y, +z) and of the triangles of the cells whose
+//            min corner it holds into one word;
+//  scan      exclusive prefix of (vertices, triangles, active groups); writes the dense
+//            per-group vertex offset and the compacted list of active groups;
+//  emit      one wave per active group writes each owned vertex at voff[g] + rank
+//            (popcounts of the lower lanes' crossing masks, then axis order x<y<z) and
+//            each triangle's three vertex ids.  An edge owned by a neighbouring group is
+//            resolved from that group's crossing masks, rebuilt from the sign masks, so
+//            one pass deduplicates vertices with neither a hash table nor a dense
+//            per-edge id array.
+// Scratch is 24 B per group (S, packed counts, voff, active entry), ~0.4 B per node.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 
 #include "upf_internal.hpp"
+#include "upf_mc.hpp"
 
 namespace upf {
 
 #define UPF_MC_QUAL __constant__
 #include "mc_table.inc"
 #undef UPF_MC_QUAL
-#define c_ntris upf_mc_ntris
-#define c_tris upf_mc_tris
 
-// edge -> (owner corner offset x,y,z, axis)
+// edge -> (owner corner offset x, y, z, axis)
 __constant__ int8_t c_edge_owner[12][4] = {
     {0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
     {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
 
-struct McDims {
-  int nx, ny, nz, gpr;
-  int64_t nxy;
-  int64_t groups;
-};
-
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__device__ __forceinline__ int case_code(const float* __restrict__ f, int64_t n, int nx, int64_t nxy, float iso) {
-  const float c0 = f[n], c1 = f[n + 1], c3 = f[n + nx], c2 = f[n + nx + 1];
-  const float c4 = f[n + nxy], c5 = f[n + nxy + 1], c7 = f[n + nxy + nx], c6 = f[n + nxy + nx + 1];
-  return (c0 < iso) | ((c1 < iso) << 1) | ((c2 < iso) << 2) | ((c3 < iso) << 3) | ((c4 < iso) << 4) |
-         ((c5 < iso) << 5) | ((c6 < iso) << 6) | ((c7 < iso) << 7);
+// packed per-group counts: vertices (bits 0-7, <= 192), triangles (8-16, <= 320), active (17)
+__device__ __forceinline__ uint32_t pack_counts(int nv, int nt) {
+  return (uint32_t)nv | ((uint32_t)nt << 8) | ((nv | nt) ? (1u << 17) : 0u);
 }
 
 // ------------------------------------------------------------------ classify
 __global__ __launch_bounds__(256) void k_mc_classify(const float* __restrict__ f, const McDims d, float iso,
-                                                     uint64_t* __restrict__ S, uint2* __restrict__ counts) {
-  const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (g >= d.groups) return;
+                                                     uint64_t* __restrict__ S, uint32_t* __restrict__ counts) {
+  __shared__ uint8_t s_ntris[256];
+  s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
+  __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int xb = (int)(g % d.gpr);
-  const int64_t row = g / d.gpr;
-  const int j = (int)(row % d.ny), k = (int)(row / d.ny);
-  const int i = xb * 64 + lane;
-  const bool valid = i < d.nx;
-  const int64_t n = row * d.nx + i;
-  const float f0 = valid ? f[n] : 0.0f;
-  const bool s0 = valid && f0 < iso;
-  const bool hx = valid && i + 1 < d.nx, hy = valid && j + 1 < d.ny, hz = valid && k + 1 < d.nz;
-  const bool cx = hx && ((f[n + 1] < iso) != s0);
-  const bool cy = hy && ((f[n + d.nx] < iso) != s0);
-  const bool cz = hz && ((f[n + d.nxy] < iso) != s0);
-  int nt = 0;
-  if (hx && hy && hz) nt = c_ntris[case_code(f, n, d.nx, d.nxy, iso)];
-  const uint64_t sm = ballot64(s0);
-  const int nv = __popcll(ballot64(cx)) + __popcll(ballot64(cy)) + __popcll(ballot64(cz));
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) nt += __shfl_xor(nt, off, 64);
-  if (lane == 0) {
-    S[g] = sm;
-    counts[g] = make_uint2((unsigned)nv, (unsigned)nt);
+  const int xb = blockIdx.x;
+  const int j = uniform(blockIdx.y * 4 + (threadIdx.x >> 6));
+  if (j >= d.ny) return;
+  const int k0 = blockIdx.z * kMcClassZ;
+  const int k1 = min(k0 + kMcClassZ, d.nz);
+  const int x = xb * 64 + lane;
+  const bool valid = x < d.nx;
+  const int xc = min(x, d.nx - 1), x1 = min(x + 1, d.nx - 1);
+  const int j1 = min(j + 1, d.ny - 1);
+  const bool hx = valid && x + 1 < d.nx, hy = valid && j + 1 < d.ny;
+  const int64_t r0 = (int64_t)j * d.nx, r1 = (int64_t)j1 * d.nx;
+  // current plane: a0 = (x, j), a1 = (x+1, j), a2 = (x+1, j+1), a3 = (x, j+1)
+  const float* p = f + (int64_t)k0 * d.nxy;
+  float a0 = p[r0 + xc], a1 = p[r0 + x1], a2 = p[r1 + x1], a3 = p[r1 + xc];
+  for (int k = k0; k < k1; k++) {
+    const bool hz = valid && k + 1 < d.nz;
+    const float* q = f + (int64_t)min(k + 1, d.nz - 1) * d.nxy;
+    const float b0 = q[r0 + xc], b1 = q[r0 + x1], b2 = q[r1 + x1], b3 = q[r1 + xc];
+    const bool s0 = valid && a0 < iso;
+    const bool cx = hx && ((a1 < iso) != s0);
+    const bool cy = hy && ((a3 < iso) != s0);
+    const bool cz = hz && ((b0 < iso) != s0);
+    int nt = 0;
+    if (hx && hy && hz) {
+      const int code = (a0 < iso) | ((a1 < iso) << 1) | ((a2 < iso) << 2) | ((a3 < iso) << 3) | ((b0 < iso) << 4) |
+                       ((b1 < iso) << 5) | ((b2 < iso) << 6) | ((b3 < iso) << 7);
+      nt = s_ntris[code];
+    }
+    const uint64_t sm = ballot64(s0);
+    const int nv = __popcll(ballot64(cx)) + __popcll(ballot64(cy)) + __popcll(ballot64(cz));
+    const int ntot = __popcll(ballot64(nt & 1)) + 2 * __popcll(ballot64(nt & 2)) + 4 * __popcll(ballot64(nt & 4));
+    if (lane == 0) {
+      const int64_t g = ((int64_t)k * d.ny + j) * d.gpr + xb;
+      S[g] = sm;
+      counts[g] = pack_counts(nv, ntot);
+    }
+    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
   }
 }
 
 // ------------------------------------------------------------------ scan
+// Three running sums at once: x = vertices, y = triangles, z = active groups.
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-__device__ __forceinline__ uint2 add2(uint2 a, uint2 b) { return make_uint2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, 0); }
+__device__ __forceinline__ uint4 unpack_counts(uint32_t c) {
+  return make_uint4(c & 0xFFu, (c >> 8) & 0x1FFu, (c >> 17) & 1u, 0);
+}
 
-// Block-wide exclusive scan of one uint2 per thread; returns the block total in *total.
-__device__ __forceinline__ uint2 block_exclusive_scan(uint2 v, uint2* total) {
-  __shared__ uint2 wsum[kScanThreads / 64];
+// Block-wide exclusive scan of one uint4 per thread; the block total goes to *total.
+__device__ __forceinline__ uint4 block_exclusive_scan(uint4 v, uint4* total) {
+  __shared__ uint4 wsum[kScanThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint2 inc = v;
+  uint4 inc = v;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
-    const unsigned ax = __shfl_up(inc.x, off, 64), ay = __shfl_up(inc.y, off, 64);
-    if (lane >= off) { inc.x += ax; inc.y += ay; }
+    const unsigned ax = __shfl_up(inc.x, off, 64), ay = __shfl_up(inc.y, off, 64), az = __shfl_up(inc.z, off, 64);
+    if (lane >= off) { inc.x += ax; inc.y += ay; inc.z += az; }
   }
   if (lane == 63) wsum[wave] = inc;
   __syncthreads();
-  uint2 before = make_uint2(0, 0), all = make_uint2(0, 0);
+  uint4 before = make_uint4(0, 0, 0, 0), all = make_uint4(0, 0, 0, 0);
   for (int w = 0; w < kScanThreads / 64; w++) {
-    if (w < wave) before = add2(before, wsum[w]);
-    all = add2(all, wsum[w]);
+    if (w < wave) before = add4(before, wsum[w]);
+    all = add4(all, wsum[w]);
   }
   __syncthreads();
   *total = all;
-  return make_uint2(before.x + inc.x - v.x, before.y + inc.y - v.y);
+  return make_uint4(before.x + inc.x - v.x, before.y + inc.y - v.y, before.z + inc.z - v.z, 0);
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint2* __restrict__ in, int64_t n,
-                                                             uint2* __restrict__ tile_sums) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint32_t* __restrict__ counts, int64_t n,
+                                                             uint4* __restrict__ tile_sums) {
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  uint2 s = make_uint2(0, 0);
+  uint4 s = make_uint4(0, 0, 0, 0);
 #pragma unroll
   for (int t = 0; t < kScanItems; t++)
-    if (base + t < n) s = add2(s, in[base + t]);
-  uint2 total;
+    if (base + t < n) s = add4(s, unpack_counts(counts[base + t]));
+  uint4 total;
   (void)block_exclusive_scan(s, &total);
   if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
-// Single workgroup: exclusive scan of the tile sums in place, grand total to *total.
-__global__ __launch_bounds__(kScanThreads) void k_scan_top(uint2* __restrict__ sums, int64_t n,
-                                                           uint2* __restrict__ total) {
-  uint2 carry = make_uint2(0, 0);
+// One workgroup: exclusive scan of the tile sums in place, grand total to *total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(uint4* __restrict__ sums, int64_t n,
+                                                           uint4* __restrict__ total) {
+  uint4 carry = make_uint4(0, 0, 0, 0);
   for (int64_t base = 0; base < n; base += kScanTile) {
     const int64_t b = base + (int64_t)threadIdx.x * kScanItems;
-    uint2 v[kScanItems];
-    uint2 s = make_uint2(0, 0);
+    uint4 v[kScanItems];
+    uint4 s = make_uint4(0, 0, 0, 0);
 #pragma unroll
     for (int t = 0; t < kScanItems; t++) {
-      v[t] = (b + t < n) ? sums[b + t] : make_uint2(0, 0);
-      s = add2(s, v[t]);
+      v[t] = (b + t < n) ? sums[b + t] : make_uint4(0, 0, 0, 0);
+      s = add4(s, v[t]);
     }
-    uint2 chunk;
-    uint2 ex = add2(block_exclusive_scan(s, &chunk), carry);
+    uint4 chunk;
+    uint4 ex = add4(block_exclusive_scan(s, &chunk), carry);
 #pragma unroll
     for (int t = 0; t < kScanItems; t++) {
       if (b + t < n) sums[b + t] = ex;
-      ex = add2(ex, v[t]);
+      ex = add4(ex, v[t]);
     }
-    carry = add2(carry, chunk);
+    carry = add4(carry, chunk);
   }
   if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint2* __restrict__ in, int64_t n,
-                                                             const uint2* __restrict__ tile_off,
-                                                             uint2* __restrict__ out) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ counts, int64_t n,
+                                                             const uint4* __restrict__ tile_off,
+                                                             uint32_t* __restrict__ voff, uint2* __restrict__ act) {
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  uint2 v[kScanItems];
-  uint2 s = make_uint2(0, 0);
+  uint4 v[kScanItems];
+  uint4 s = make_uint4(0, 0, 0, 0);
 #pragma unroll
   for (int t = 0; t < kScanItems; t++) {
-    v[t] = (base + t < n) ? in[base + t] : make_uint2(0, 0);
-    s = add2(s, v[t]);
+    v[t] = (base + t < n) ? unpack_counts(counts[base + t]) : make_uint4(0, 0, 0, 0);
+    s = add4(s, v[t]);
   }
-  uint2 total;
-  uint2 ex = add2(block_exclusive_scan(s, &total), tile_off[blockIdx.x]);
+  uint4 total;
+  uint4 ex = add4(block_exclusive_scan(s, &total), tile_off[blockIdx.x]);
 #pragma unroll
   for (int t = 0; t < kScanItems; t++) {
-    if (base + t < n) out[base + t] = ex;
-    ex = add2(ex, v[t]);
+    if (base + t < n) {
+      voff[base + t] = ex.x;
+      if (v[t].z) act[ex.z] = make_uint2((uint32_t)(base + t), ex.y);
+    }
+    ex = add4(ex, v[t]);
   }
 }
 
 // ------------------------------------------------------------------ emit
-// Crossing masks of group gg (row-aligned, so +y and +z neighbour groups share its
-// lane alignment).  Lanes past nx hold 0 sign bits and are masked.
 struct Cross {
   uint64_t bx, by, bz;
+  uint32_t voff, pad;
 };
 
-__device__ __forceinline__ Cross group_crossings(const uint64_t* __restrict__ S, const McDims& d, int64_t gg) {
-  const int xb = (int)(gg % d.gpr);
-  const int64_t row = gg / d.gpr;
-  const int j = (int)(row % d.ny), k = (int)(row / d.ny);
+// Crossing masks of group (xb, j, k) (row-aligned, so its +y and +z neighbour groups
+// share its lane alignment).  Lanes past nx hold 0 sign bits and are masked.
+__device__ __forceinline__ Cross group_crossings(const uint64_t* __restrict__ S, const uint32_t* __restrict__ voff,
+                                                 const McDims& d, int xb, int j, int k) {
+  const int64_t gg = ((int64_t)k * d.ny + j) * d.gpr + xb;
   const int first = xb * 64;
-  const int nval = min(64, d.nx - first);                 // lanes inside the row
-  const int nvx = min(64, d.nx - 1 - first);              // lanes with an x neighbour
+  const int nval = min(64, d.nx - first);     // lanes inside the row
+  const int nvx = min(64, d.nx - 1 - first);  // lanes with an x neighbour
   const uint64_t vmask = nval >= 64 ? ~0ull : ((1ull << nval) - 1);
   const uint64_t xmask = nvx >= 64 ? ~0ull : (nvx > 0 ? ((1ull << nvx) - 1) : 0ull);
   const uint64_t s = S[gg];
@@ -190,6 +207,8 @@ __device__ __forceinline__ Cross group_crossings(const uint64_t* __restrict__ S,
   c.bx = (s ^ ((s >> 1) | (snext << 63))) & xmask;
   c.by = (j + 1 < d.ny) ? ((s ^ S[gg + d.gpr]) & vmask) : 0ull;
   c.bz = (k + 1 < d.nz) ? ((s ^ S[gg + (int64_t)d.gpr * d.ny]) & vmask) : 0ull;
+  c.voff = voff[gg];
+  c.pad = 0;
   return c;
 }
 
@@ -210,88 +229,114 @@ __device__ __forceinline__ void node_gradient(const float* __restrict__ f, const
 
 __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, const McDims d, const DGrid g,
                                                  float iso, const uint64_t* __restrict__ S,
-                                                 const uint2* __restrict__ counts, const uint2* __restrict__ offs,
-                                                 float* __restrict__ pos, float* __restrict__ nrm,
-                                                 uint32_t* __restrict__ idx) {
-  const int64_t gid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (gid >= d.groups) return;
-  const uint2 cnt = counts[gid];
-  if (cnt.x == 0 && cnt.y == 0) return;
-  const uint2 off = offs[gid];
+                                                 const uint32_t* __restrict__ counts,
+                                                 const uint32_t* __restrict__ voff, const uint2* __restrict__ act,
+                                                 const uint4* __restrict__ total, float* __restrict__ pos,
+                                                 float* __restrict__ nrm, uint32_t* __restrict__ idx) {
+  __shared__ Cross s_nb[4][8];
+  const int wave = uniform(threadIdx.x >> 6);
+  const int64_t a = (int64_t)blockIdx.x * 4 + wave;
+  if (a >= (int64_t)total->z) return;
   const int lane = threadIdx.x & 63;
-  const int xb = (int)(gid % d.gpr);
-  const int64_t row = gid / d.gpr;
-  const int j = (int)(row % d.ny), k = (int)(row / d.ny);
+  const uint2 entry = act[a];
+  const uint32_t gid = (uint32_t)uniform((int)entry.x);
+  const uint32_t toff = (uint32_t)uniform((int)entry.y);
+  const uint32_t row = gid / (uint32_t)d.gpr;
+  const int xb = (int)(gid - row * (uint32_t)d.gpr);
+  const int k = (int)(row / (uint32_t)d.ny);
+  const int j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
+  const uint32_t c = counts[gid];
+  const int cnt_v = c & 0xFF, cnt_t = (c >> 8) & 0x1FF;
+
+  // crossing masks of the (up to) 8 groups a cell of this group takes edges from:
+  // slot = gx | dy << 1 | dz << 2, gx = 1 for the next group along x
+  if (lane < 8) {
+    const int gx = lane & 1, dy = (lane >> 1) & 1, dz = lane >> 2;
+    if (xb + gx < d.gpr && j + dy < d.ny && k + dz < d.nz)
+      s_nb[wave][lane] = group_crossings(S, voff, d, xb + gx, j + dy, k + dz);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
   const int i = xb * 64 + lane;
   const bool valid = i < d.nx;
-  const int64_t n = row * d.nx + i;
+  const int64_t n = (int64_t)k * d.nxy + (int64_t)j * d.nx + i;
+  const uint64_t lt = (1ull << lane) - 1;
 
   // ---- vertices
-  if (cnt.x) {
-    const Cross c = group_crossings(S, d, gid);
-    const uint64_t lt = (1ull << lane) - 1;
-    uint32_t vid = off.x + (uint32_t)(__popcll(c.bx & lt) + __popcll(c.by & lt) + __popcll(c.bz & lt));
-    const float f0 = valid ? f[n] : 0.0f;
-    const float p0[3] = {node_coord(g.o[0], g.s[0], i), node_coord(g.o[1], g.s[1], j),
-                         node_coord(g.o[2], g.s[2], k)};
-    float g0[3] = {0.0f, 0.0f, 0.0f};
-    const bool any = ((c.bx | c.by | c.bz) >> lane) & 1ull;
-    if (any && nrm) node_gradient(f, d, g, i, j, k, g0);
-    const uint64_t bits[3] = {c.bx, c.by, c.bz};
-    const int64_t step[3] = {1, d.nx, d.nxy};
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      if (!((bits[a] >> lane) & 1ull)) continue;
-      const float f1 = f[n + step[a]];
-      const float t = (iso - f0) / (f1 - f0);
+  if (cnt_v) {
+    const Cross cc = s_nb[wave][0];
+    uint32_t vid = cc.voff + (uint32_t)(__popcll(cc.bx & lt) + __popcll(cc.by & lt) + __popcll(cc.bz & lt));
+    const uint64_t bits[3] = {cc.bx, cc.by, cc.bz};
+    const bool any = ((cc.bx | cc.by | cc.bz) >> lane) & 1ull;
+    if (any) {
+      const float f0 = f[n];
+      const float p0[3] = {node_coord(g.o[0], g.s[0], i), node_coord(g.o[1], g.s[1], j),
+                           node_coord(g.o[2], g.s[2], k)};
+      float g0[3] = {0.0f, 0.0f, 0.0f};
+      if (nrm) node_gradient(f, d, g, i, j, k, g0);
+      const int64_t step[3] = {1, d.nx, d.nxy};
       const int ijk[3] = {i, j, k};
-      const float p1a = node_coord(g.o[a], g.s[a], ijk[a] + 1);
-      float p[3] = {p0[0], p0[1], p0[2]};
-      p[a] = p0[a] + t * (p1a - p0[a]);
-      pos[3 * (size_t)vid + 0] = p[0];
-      pos[3 * (size_t)vid + 1] = p[1];
-      pos[3 * (size_t)vid + 2] = p[2];
-      if (nrm) {
-        float g1[3];
-        node_gradient(f, d, g, i + (a == 0), j + (a == 1), k + (a == 2), g1);
-        const float gx = g0[0] + t * (g1[0] - g0[0]);
-        const float gy = g0[1] + t * (g1[1] - g0[1]);
-        const float gz = g0[2] + t * (g1[2] - g0[2]);
-        const float len = __builtin_sqrtf(gx * gx + gy * gy + gz * gz);
-        const float inv = len > 0.0f ? 1.0f / len : 0.0f;
-        nrm[3 * (size_t)vid + 0] = gx * inv;
-        nrm[3 * (size_t)vid + 1] = gy * inv;
-        nrm[3 * (size_t)vid + 2] = gz * inv;
+#pragma unroll
+      for (int ax = 0; ax < 3; ax++) {
+        if (!((bits[ax] >> lane) & 1ull)) continue;
+        const float f1 = f[n + step[ax]];
+        const float t = (iso - f0) / (f1 - f0);
+        const float p1a = node_coord(g.o[ax], g.s[ax], ijk[ax] + 1);
+        float p[3] = {p0[0], p0[1], p0[2]};
+        p[ax] = p0[ax] + t * (p1a - p0[ax]);
+        pos[3 * (size_t)vid + 0] = p[0];
+        pos[3 * (size_t)vid + 1] = p[1];
+        pos[3 * (size_t)vid + 2] = p[2];
+        if (nrm) {
+          float g1[3];
+          node_gradient(f, d, g, i + (ax == 0), j + (ax == 1), k + (ax == 2), g1);
+          const float gx = g0[0] + t * (g1[0] - g0[0]);
+          const float gy = g0[1] + t * (g1[1] - g0[1]);
+          const float gz = g0[2] + t * (g1[2] - g0[2]);
+          const float len = __builtin_sqrtf(gx * gx + gy * gy + gz * gz);
+          const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+          nrm[3 * (size_t)vid + 0] = gx * inv;
+          nrm[3 * (size_t)vid + 1] = gy * inv;
+          nrm[3 * (size_t)vid + 2] = gz * inv;
+        }
+        vid++;
       }
-      vid++;
     }
   }
 
   // ---- triangles
-  if (cnt.y) {
+  if (cnt_t) {
     const bool cell = valid && i + 1 < d.nx && j + 1 < d.ny && k + 1 < d.nz;
-    const int code = cell ? case_code(f, n, d.nx, d.nxy, iso) : 0;
-    const int nt = c_ntris[code];
+    int code = 0;
+    if (cell) {
+      const float c0 = f[n], c1 = f[n + 1], c3 = f[n + d.nx], c2 = f[n + d.nx + 1];
+      const float c4 = f[n + d.nxy], c5 = f[n + d.nxy + 1], c7 = f[n + d.nxy + d.nx], c6 = f[n + d.nxy + d.nx + 1];
+      code = (c0 < iso) | ((c1 < iso) << 1) | ((c2 < iso) << 2) | ((c3 < iso) << 3) | ((c4 < iso) << 4) |
+             ((c5 < iso) << 5) | ((c6 < iso) << 6) | ((c7 < iso) << 7);
+    }
+    const int nt = upf_mc_ntris[code];
     int inc = nt;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-      const int a = __shfl_up(inc, o, 64);
-      if (lane >= o) inc += a;
+      const int v = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += v;
     }
-    uint32_t tid = off.y + (uint32_t)(inc - nt);
+    uint32_t tid = toff + (uint32_t)(inc - nt);
     for (int t = 0; t < nt; t++, tid++) {
 #pragma unroll
       for (int v = 0; v < 3; v++) {
-        const int e = c_tris[code][3 * t + v];
-        const int ox = i + c_edge_owner[e][0], oy = j + c_edge_owner[e][1], oz = k + c_edge_owner[e][2];
+        const int e = upf_mc_tris[code][3 * t + v];
+        const int ol = lane + c_edge_owner[e][0];
+        const int slot = (ol >> 6) | (c_edge_owner[e][1] << 1) | (c_edge_owner[e][2] << 2);
         const int axis = c_edge_owner[e][3];
-        const int64_t og = ((int64_t)oz * d.ny + oy) * d.gpr + (ox >> 6);
-        const int ol = ox & 63;
-        const Cross c = group_crossings(S, d, og);
-        const uint64_t lt = (1ull << ol) - 1;
-        uint32_t id = offs[og].x + (uint32_t)(__popcll(c.bx & lt) + __popcll(c.by & lt) + __popcll(c.bz & lt));
-        if (axis >= 1) id += (uint32_t)((c.bx >> ol) & 1ull);
-        if (axis == 2) id += (uint32_t)((c.by >> ol) & 1ull);
+        const Cross cc = s_nb[wave][slot];
+        const int l = ol & 63;
+        const uint64_t m = (1ull << l) - 1;
+        uint32_t id = cc.voff + (uint32_t)(__popcll(cc.bx & m) + __popcll(cc.by & m) + __popcll(cc.bz & m));
+        if (axis >= 1) id += (uint32_t)((cc.bx >> l) & 1ull);
+        if (axis == 2) id += (uint32_t)((cc.by >> l) & 1ull);
         idx[3 * (size_t)tid + v] = id;
       }
     }
@@ -305,17 +350,14 @@ __global__ __launch_bounds__(256) void k_mc_codes(const float* __restrict__ f, c
   const int k = blockIdx.z;
   if (i >= d.nx - 1 || j >= d.ny - 1) return;
   const int64_t n = (int64_t)k * d.nxy + (int64_t)j * d.nx + i;
-  codes[((int64_t)k * (d.ny - 1) + j) * (d.nx - 1) + i] = (uint8_t)case_code(f, n, d.nx, d.nxy, iso);
+  const float c0 = f[n], c1 = f[n + 1], c3 = f[n + d.nx], c2 = f[n + d.nx + 1];
+  const float c4 = f[n + d.nxy], c5 = f[n + d.nxy + 1], c7 = f[n + d.nxy + d.nx], c6 = f[n + d.nxy + d.nx + 1];
+  codes[((int64_t)k * (d.ny - 1) + j) * (d.nx - 1) + i] =
+      (uint8_t)((c0 < iso) | ((c1 < iso) << 1) | ((c2 < iso) << 2) | ((c3 < iso) << 3) | ((c4 < iso) << 4) |
+                ((c5 < iso) << 5) | ((c6 < iso) << 6) | ((c7 < iso) << 7));
 }
 
-namespace {
-
-struct McLayout {
-  McDims d;
-  int64_t tiles;
-  size_t off_S, off_counts, off_offs, off_tiles, off_total, bytes;
-};
-
+// ------------------------------------------------------------------ host side
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 McLayout mc_layout(const DGrid& g) {
@@ -329,15 +371,52 @@ McLayout mc_layout(const DGrid& g) {
   L.tiles = (L.d.groups + kScanTile - 1) / kScanTile;
   size_t o = 0;
   L.off_S = o; o = align256(o + sizeof(uint64_t) * L.d.groups);
-  L.off_counts = o; o = align256(o + sizeof(uint2) * L.d.groups);
-  L.off_offs = o; o = align256(o + sizeof(uint2) * L.d.groups);
-  L.off_tiles = o; o = align256(o + sizeof(uint2) * L.tiles);
-  L.off_total = o; o = align256(o + sizeof(uint2));
+  L.off_counts = o; o = align256(o + sizeof(uint32_t) * L.d.groups);
+  L.off_voff = o; o = align256(o + sizeof(uint32_t) * L.d.groups);
+  L.off_act = o; o = align256(o + sizeof(uint2) * L.d.groups);
+  L.off_tiles = o; o = align256(o + sizeof(uint4) * L.tiles);
+  L.off_total = o; o = align256(o + sizeof(uint4));
   L.bytes = o;
   return L;
 }
 
-}  // namespace
+McPtrs mc_ptrs(const McLayout& L, void* ws) {
+  char* w = static_cast<char*>(ws);
+  return {reinterpret_cast<uint64_t*>(w + L.off_S), reinterpret_cast<uint32_t*>(w + L.off_counts),
+          reinterpret_cast<uint32_t*>(w + L.off_voff), reinterpret_cast<uint2*>(w + L.off_act),
+          reinterpret_cast<uint4*>(w + L.off_tiles), reinterpret_cast<uint4*>(w + L.off_total)};
+}
+
+int mc_grid_ok(const DGrid& g) {
+  // group ids and vertex ids are 32-bit
+  const int64_t groups = (int64_t)((g.n[0] + 63) / 64) * g.n[1] * g.n[2];
+  return groups < ((int64_t)1 << 31) ? UPF_OK : UPF_ERR_BAD_GRID;
+}
+
+int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws, size_t ws_bytes, hipStream_t s) {
+  const McLayout L = mc_layout(g);
+  if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
+  const McPtrs P = mc_ptrs(L, d_ws);
+  const dim3 cgrid(L.d.gpr, (L.d.ny + 3) / 4, (L.d.nz + kMcClassZ - 1) / kMcClassZ);
+  hipLaunchKernelGGL(k_mc_classify, cgrid, dim3(256), 0, s, d_field, L.d, iso, P.S, P.counts);
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)L.tiles), dim3(kScanThreads), 0, s, P.counts, L.d.groups, P.tiles);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, P.tiles, L.tiles, P.total);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.tiles), dim3(kScanThreads), 0, s, P.counts, L.d.groups, P.tiles,
+                     P.voff, P.act);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
+                   float* nrm, uint32_t* idx, hipStream_t s) {
+  const McLayout L = mc_layout(g);
+  const McPtrs P = mc_ptrs(L, d_ws);
+  if (max_active <= 0) return UPF_OK;
+  hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((max_active + 3) / 4)), dim3(256), 0, s, d_field, L.d, g, iso, P.S,
+                     P.counts, P.voff, P.act, P.total, pos, nrm, idx);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
 
 }  // namespace upf
 
@@ -347,7 +426,7 @@ extern "C" {
 
 size_t upf_mc_workspace_bytes(const upf_grid* grid) {
   DGrid g;
-  if (to_device(grid, &g)) return 0;
+  if (to_device(grid, &g) || mc_grid_ok(g)) return 0;
   return mc_layout(g).bytes;
 }
 
@@ -356,26 +435,13 @@ int upf_mc_count(const float* d_field, const upf_grid* grid, float iso, void* d_
   if (!d_field || !n_vertices || !n_triangles) return UPF_ERR_NULL;
   if (!std::isfinite(iso)) return UPF_ERR_INVALID_VALUE;
   DGrid g;
-  const int st = to_device(grid, &g);
-  if (st) return st;
-  const McLayout L = mc_layout(g);
-  if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
+  int st = to_device(grid, &g);
+  if (st || (st = mc_grid_ok(g))) return st;
   hipStream_t s = as_stream(stream);
-  char* ws = static_cast<char*>(d_ws);
-  uint64_t* S = reinterpret_cast<uint64_t*>(ws + L.off_S);
-  uint2* counts = reinterpret_cast<uint2*>(ws + L.off_counts);
-  uint2* offs = reinterpret_cast<uint2*>(ws + L.off_offs);
-  uint2* tiles = reinterpret_cast<uint2*>(ws + L.off_tiles);
-  uint2* total = reinterpret_cast<uint2*>(ws + L.off_total);
-  const unsigned nblk = (unsigned)((L.d.groups + 3) / 4);
-  hipLaunchKernelGGL(k_mc_classify, dim3(nblk), dim3(256), 0, s, d_field, L.d, iso, S, counts);
-  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)L.tiles), dim3(kScanThreads), 0, s, counts, L.d.groups, tiles);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tiles, L.tiles, total);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.tiles), dim3(kScanThreads), 0, s, counts, L.d.groups, tiles,
-                     offs);
-  UPF_LAUNCH_CHECK();
-  uint2 h_total;
-  UPF_HIP_CHECK(hipMemcpyAsync(&h_total, total, sizeof(uint2), hipMemcpyDeviceToHost, s));
+  if ((st = mc_classify_scan(d_field, g, iso, d_ws, ws_bytes, s))) return st;
+  const McPtrs P = mc_ptrs(mc_layout(g), d_ws);
+  uint4 h_total;
+  UPF_HIP_CHECK(hipMemcpyAsync(&h_total, P.total, sizeof(uint4), hipMemcpyDeviceToHost, s));
   UPF_HIP_CHECK(hipStreamSynchronize(s));
   *n_vertices = h_total.x;
   *n_triangles = h_total.y;
@@ -387,22 +453,18 @@ int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_
   if (!d_field || !out) return UPF_ERR_NULL;
   if (!std::isfinite(iso)) return UPF_ERR_INVALID_VALUE;
   DGrid g;
-  const int st = to_device(grid, &g);
-  if (st) return st;
+  int st = to_device(grid, &g);
+  if (st || (st = mc_grid_ok(g))) return st;
   const McLayout L = mc_layout(g);
   if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
   if (out->n_vertices > out->cap_vertices || out->n_triangles > out->cap_triangles) return UPF_ERR_CAPACITY;
   if (out->n_vertices == 0 && out->n_triangles == 0) return UPF_OK;
   if (!out->positions || !out->indices) return UPF_ERR_NULL;
-  char* ws = static_cast<char*>(d_ws);
-  const uint64_t* S = reinterpret_cast<const uint64_t*>(ws + L.off_S);
-  const uint2* counts = reinterpret_cast<const uint2*>(ws + L.off_counts);
-  const uint2* offs = reinterpret_cast<const uint2*>(ws + L.off_offs);
-  const unsigned nblk = (unsigned)((L.d.groups + 3) / 4);
-  hipLaunchKernelGGL(k_mc_emit, dim3(nblk), dim3(256), 0, as_stream(stream), d_field, L.d, g, iso, S, counts, offs,
-                     out->positions, out->normals, out->indices);
-  UPF_LAUNCH_CHECK();
-  return UPF_OK;
+  // The kernel reads the number of active groups from the workspace; launch enough
+  // waves for every group that can own a vertex or a triangle (at most one per vertex
+  // or triangle, at most one per group) and let the surplus exit.
+  const int64_t bound = std::min<int64_t>(L.d.groups, out->n_vertices + out->n_triangles);
+  return mc_emit_launch(d_field, g, iso, d_ws, bound, out->positions, out->normals, out->indices, as_stream(stream));
 }
 
 int upf_mc_extract(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out, void* d_ws,

@@ -1,0 +1,44 @@
+// upf_mc.hpp -- marching-cubes workspace layout and launch helpers shared by the
+// single-grid entry points and the batched path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "upf_device.hpp"
+
+namespace upf {
+
+constexpr int kMcClassZ = 32;  // z planes swept by one classify workgroup
+
+struct McDims {
+  int nx, ny, nz, gpr;  // gpr = 64-node groups per row
+  int64_t nxy;
+  int64_t groups;
+};
+
+struct McLayout {
+  McDims d;
+  int64_t tiles;
+  size_t off_S, off_counts, off_voff, off_act, off_tiles, off_total, bytes;
+};
+
+struct McPtrs {
+  uint64_t* S;        // per group: inside bits
+  uint32_t* counts;   // per group: packed vertex / triangle counts
+  uint32_t* voff;     // per group: first vertex id
+  uint2* act;         // active groups: (group id, first triangle id)
+  uint4* tiles;       // scan tile sums
+  uint4* total;       // (vertices, triangles, active groups)
+};
+
+McLayout mc_layout(const DGrid& g);
+McPtrs mc_ptrs(const McLayout& L, void* ws);
+int mc_grid_ok(const DGrid& g);
+// classify + scan, asynchronous; totals stay in the workspace
+int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws, size_t ws_bytes, hipStream_t s);
+// emit over at most max_active active groups (the kernel reads the true count)
+int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
+                   float* nrm, uint32_t* idx, hipStream_t s);
+
+}  // namespace upf
