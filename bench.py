@@ -82,6 +82,70 @@ def cpu_baseline(spec, grid, seconds: float):
     }
 
 
+def bench_batch(upf, S, torch):
+    """The uneven 64-chunk batch (committed manifest, seed 0): per-chunk launches
+    (field, count with its host read-back, emit) against one autotuned, graph-captured
+    batched run.  Buffers are allocated before either timed region."""
+    import json
+
+    from unnatural_planets_amd.batch import Batch
+    man = json.load(open(os.path.join(ROOT, "tests", "golden", "batch_manifest.json")))["chunks"]
+    chunks = S.batch_chunks(man)
+    samples = sum(m["n"] ** 3 for m in man)
+    L = upf.lib()
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        sp = ctypes.c_void_p(stream.cuda_stream)
+        # per-chunk path: a workspace per grid size, output buffers sized for the largest chunk
+        b = Batch(chunks)
+        nv_all, nt_all = b.size_outputs()
+        fields = [torch.empty(m["n"] ** 3, dtype=torch.float32, device="cuda") for m in man]
+        ws = {m["n"]: upf.McWorkspace(chunks[i].grid) for i, m in enumerate(man)}
+        pos = torch.empty((nv_all, 3), dtype=torch.float32, device="cuda")
+        nrm = torch.empty_like(pos)
+        idx = torch.empty((nt_all, 3), dtype=torch.int32, device="cuda")
+        nv, nt = ctypes.c_int64(), ctypes.c_int64()
+
+        def per_chunk():
+            for i, m in enumerate(man):
+                g = chunks[i].grid
+                upf.check(L.upf_field_eval_grid(ctypes.byref(chunks[i].spec), ctypes.byref(g),
+                                                ctypes.c_void_p(fields[i].data_ptr()), sp), "eval")
+                w = ws[m["n"]]
+                upf.check(L.upf_mc_count(ctypes.c_void_p(fields[i].data_ptr()), ctypes.byref(g), 0.0,
+                                         ctypes.c_void_p(w.buf.data_ptr()), w.bytes, ctypes.byref(nv),
+                                         ctypes.byref(nt), sp), "count")
+                mb = S.MeshBuffers()
+                mb.positions, mb.normals, mb.indices = pos.data_ptr(), nrm.data_ptr(), idx.data_ptr()
+                mb.cap_vertices, mb.cap_triangles = nv_all, nt_all
+                mb.n_vertices, mb.n_triangles = nv.value, nt.value
+                upf.check(L.upf_mc_emit(ctypes.c_void_p(fields[i].data_ptr()), ctypes.byref(g), 0.0, ctypes.byref(mb),
+                                        ctypes.c_void_p(w.buf.data_ptr()), w.bytes, sp), "emit")
+
+        def timed(fn, reps):
+            fn()
+            stream.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            stream.synchronize()
+            return (time.perf_counter() - t) / reps
+
+        t_seq = timed(per_chunk, 3)
+        best = b.autotune(stream)
+        b.capture(stream)
+        t_graph = timed(lambda: b.replay(stream), 5)
+        t_eager = timed(lambda: b.run(stream), 5)
+    return {
+        "chunks": len(man), "samples": samples, "vertices": nv_all, "triangles": nt_all,
+        "per_chunk_ms": round(t_seq * 1e3, 3), "batched_graph_ms": round(t_graph * 1e3, 3),
+        "batched_eager_ms": round(t_eager * 1e3, 3), "speedup_vs_per_chunk": round(t_seq / t_graph, 3),
+        "batched_gsamples_per_s": round(samples / t_graph / 1e9, 3),
+        "autotuned": {"field_variant": best.field_variant, "blocks_per_cu": best.blocks_per_cu,
+                      "classify_z": best.classify_z},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +155,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -233,6 +298,9 @@ def main() -> int:
             "vs_plain_time_ratio": round(t_adj / t_field, 3),
         }
         del dLdf, dfdx
+
+    if not args.no_batch:
+        out["batch"] = bench_batch(upf, S, torch)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec, grid, args.cpu_seconds)

@@ -275,6 +275,71 @@ int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_
 int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,
                       uint8_t* d_codes, void* stream);
 
+/* ---- batched chunks ---------------------------------------------------------------
+ * Many chunks (planets) of uneven grid sizes, shapes and seeds evaluated and meshed by
+ * one set of persistent launches: field rows, classify columns and active groups of
+ * all chunks are spread over a fixed grid of workgroups, and one scan numbers the
+ * vertices and triangles of all chunks (chunk after chunk).  upf_batch_run makes no
+ * allocation, no memset and no host synchronisation, so it can be captured into a
+ * HIP graph (upf_batch_capture / upf_batch_replay do that).  No counterpart in the
+ * reference, whose chunks are meshed one task each (sources/generator.cpp:159). */
+typedef struct upf_chunk {
+  upf_field_spec spec;
+  upf_grid grid;
+  float iso;
+  int32_t reserved[3];
+} upf_chunk;
+
+/* Launch knobs searched by upf_batch_autotune; every setting gives identical results. */
+typedef struct upf_batch_opts {
+  int32_t field_variant;  /* upf_eval_variant                                  */
+  int32_t blocks_per_cu;  /* persistent workgroups per compute unit (0 = 4)    */
+  int32_t classify_z;     /* z planes per classify item: 16, 32 or 64 (0 = 32) */
+  int32_t reserved;
+} upf_batch_opts;
+
+/* Per-chunk result, written on the device by upf_batch_run: the chunk's vertices are
+ * positions[vertex_offset .. vertex_offset + n_vertices), its triangles
+ * indices[triangle_offset ..), with vertex indices local to the chunk. */
+typedef struct upf_chunk_mesh {
+  int64_t vertex_offset;
+  int64_t n_vertices;
+  int64_t triangle_offset;
+  int64_t n_triangles;
+} upf_chunk_mesh;
+
+typedef struct upf_batch upf_batch; /* opaque plan: owns small device descriptors */
+
+int upf_batch_create(const upf_chunk* chunks, int32_t n_chunks, const upf_batch_opts* opts,
+                     upf_batch** out);
+void upf_batch_destroy(upf_batch* b);
+/* Field buffer size (floats, all chunks' grids back to back) and workspace size. */
+int64_t upf_batch_field_floats(const upf_batch* b);
+int64_t upf_batch_field_offset(const upf_batch* b, int32_t chunk);
+size_t upf_batch_workspace_bytes(const upf_batch* b);
+int upf_batch_set_opts(upf_batch* b, const upf_batch_opts* opts);
+int upf_batch_get_opts(const upf_batch* b, upf_batch_opts* opts);
+/* Evaluate + mesh every chunk.  Writes past cap_vertices / cap_triangles are dropped;
+ * the true totals are readable with upf_batch_totals. */
+int upf_batch_run(upf_batch* b, float* d_field, void* d_workspace, size_t workspace_bytes,
+                  float* d_positions, float* d_normals, uint32_t* d_indices, int64_t cap_vertices,
+                  int64_t cap_triangles, upf_chunk_mesh* d_meshes, void* stream);
+/* Reads the batch totals from the workspace (synchronises the stream). */
+int upf_batch_totals(const upf_batch* b, const void* d_workspace, int64_t* n_vertices,
+                     int64_t* n_triangles, void* stream);
+/* Capture upf_batch_run with these buffers into a HIP graph owned by the plan, and
+ * replay it. */
+int upf_batch_capture(upf_batch* b, float* d_field, void* d_workspace, size_t workspace_bytes,
+                      float* d_positions, float* d_normals, uint32_t* d_indices, int64_t cap_vertices,
+                      int64_t cap_triangles, upf_chunk_mesh* d_meshes, void* stream);
+int upf_batch_replay(upf_batch* b, void* stream);
+/* Time every knob setting on these buffers (HIP events), keep the fastest in the plan
+ * and return it in *best (nullable). */
+int upf_batch_autotune(upf_batch* b, float* d_field, void* d_workspace, size_t workspace_bytes,
+                       float* d_positions, float* d_normals, uint32_t* d_indices, int64_t cap_vertices,
+                       int64_t cap_triangles, upf_chunk_mesh* d_meshes, void* stream,
+                       upf_batch_opts* best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,100 @@
+"""Batched persistent launches against per-chunk launches: identical fields (bitwise)
+and identical meshes per chunk, for the eager run, for hipGraph replays interleaved with
+other eager work, and for every launch setting the autotuner tries."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from unnatural_planets_amd import spec as S
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _manifest(n=None):
+    m = json.load(open(os.path.join(HERE, "golden", "batch_manifest.json")))["chunks"]
+    return m[:n] if n else m
+
+
+def _reference(upf, chunks, variant=0):
+    out = []
+    for c in range(len(chunks)):
+        f = upf.field_eval_grid(chunks[c].spec, chunks[c].grid, variant=variant)
+        m = upf.mc_extract(f, chunks[c].grid, 0.0)
+        out.append((f, m))
+    return out
+
+
+def _check(upf, b, ref, table=None):
+    import torch
+    table = table or b.mesh_table()
+    for c, (f, m) in enumerate(ref):
+        assert torch.equal(b.field_of(c).view(torch.int32), f.view(torch.int32)), c
+        bm = b.mesh_of(c, table)
+        assert bm.n_vertices == m.n_vertices and bm.n_triangles == m.n_triangles, c
+        assert torch.equal(bm.indices, m.indices), c
+        assert torch.equal(bm.positions.view(torch.int32), m.positions.view(torch.int32)), c
+        assert torch.equal(bm.normals.view(torch.int32), m.normals.view(torch.int32)), c
+
+
+def test_batch_matches_per_chunk(upf):
+    from unnatural_planets_amd.batch import Batch
+    man = _manifest(24)  # a prefix keeps the reference loop short; same sizes mix
+    chunks = S.batch_chunks(man)
+    print("case: batch of", len(man), flush=True)
+    b = Batch(chunks)
+    nv, nt = b.size_outputs()
+    b.run()
+    ref = _reference(upf, chunks)
+    assert nv == sum(m.n_vertices for _, m in ref) and nt == sum(m.n_triangles for _, m in ref)
+    _check(upf, b, ref)
+
+
+def test_batch_graph_replays_and_autotune(upf):
+    import torch
+
+    from unnatural_planets_amd.batch import Batch
+    man = _manifest(12)
+    chunks = S.batch_chunks(man)
+    b = Batch(chunks)
+    b.size_outputs()
+    ref = _reference(upf, chunks)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        b.capture(stream)
+        for r in range(3):
+            b.field.fill_(float("nan"))  # eager work between replays
+            b.idx.fill_(-1)
+            b.replay(stream)
+            stream.synchronize()
+            _check(upf, b, ref)
+    best = b.autotune()
+    print("autotuned:", best.field_variant, best.blocks_per_cu, best.classify_z)
+    assert b.get_opts().field_variant == best.field_variant
+    b.idx.fill_(-1)
+    b.run()
+    torch.cuda.synchronize()
+    _check(upf, b, _reference(upf, chunks, best.field_variant))
+    # every launch setting gives the same result as the per-chunk path of its variant
+    for variant in (upf.EVAL_ROWS, upf.EVAL_PER_LANE):
+        ref_v = _reference(upf, chunks, variant)
+        for bpc, kz in ((2, 16), (8, 64)):
+            b.set_opts(S.BatchOpts(variant, bpc, kz, 0))
+            b.idx.fill_(-1)
+            b.run()
+            torch.cuda.synchronize()
+            _check(upf, b, ref_v)
+
+
+def test_batch_capacity_drops_writes_and_reports_totals(upf):
+    from unnatural_planets_amd.batch import Batch
+    chunks = S.batch_chunks(_manifest(4))
+    b = Batch(chunks)
+    b.allocate(10, 10)
+    b.run()
+    nv, nt = b.totals()
+    table = b.mesh_table()
+    assert nv > 10 and nt > 10
+    assert sum(t.n_vertices for t in table) == nv and sum(t.n_triangles for t in table) == nt

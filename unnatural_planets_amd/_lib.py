@@ -8,12 +8,17 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .spec import FieldSpec, Grid, Layer, MeshBuffers
+from .spec import BatchOpts, Chunk, FieldSpec, Grid, Layer, MeshBuffers
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libupf.so")
 
 _lib = None
+
+# (plan, field, workspace, workspace bytes, positions, normals, indices, cap vertices,
+#  cap triangles, chunk meshes, stream)
+_BATCH_RUN_ARGS = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -46,6 +51,20 @@ SIGNATURES = {
                                     ctypes.c_void_p]),
     "upf_mc_emit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float,
                                    ctypes.POINTER(MeshBuffers), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "upf_batch_create": (ctypes.c_int, [ctypes.POINTER(Chunk), ctypes.c_int32, ctypes.POINTER(BatchOpts),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "upf_batch_destroy": (None, [ctypes.c_void_p]),
+    "upf_batch_field_floats": (ctypes.c_int64, [ctypes.c_void_p]),
+    "upf_batch_field_offset": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int32]),
+    "upf_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.c_void_p]),
+    "upf_batch_set_opts": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchOpts)]),
+    "upf_batch_get_opts": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchOpts)]),
+    "upf_batch_run": (ctypes.c_int, _BATCH_RUN_ARGS),
+    "upf_batch_totals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                        ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]),
+    "upf_batch_capture": (ctypes.c_int, _BATCH_RUN_ARGS),
+    "upf_batch_replay": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "upf_batch_autotune": (ctypes.c_int, _BATCH_RUN_ARGS + [ctypes.POINTER(BatchOpts)]),
     "upf_mc_case_codes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_void_p,
                                          ctypes.c_void_p]),
 }

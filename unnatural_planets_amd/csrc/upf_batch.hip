@@ -1,0 +1,395 @@
+// upf_batch.hip -- many chunks of uneven sizes evaluated and meshed by one set of
+// persistent launches (no counterpart in the reference, which meshes one chunk per
+// task, sources/generator.cpp:159).
+//
+// Work items (a wavefront each), enumerated chunk after chunk:
+//   field     one grid row segment of 64 x nodes          (chunk, xb, j, k)
+//   classify  one group column over classify_z planes      (chunk, xb, j, kz)
+//   emit      one active group of the concatenated scan
+// A fixed grid of workgroups (blocks_per_cu per compute unit) strides over each list,
+// so small chunks share the machine with large ones and nothing waits on the host.
+// The marching-cubes groups of all chunks are concatenated, so a single scan numbers
+// vertices and triangles batch-wide; per-chunk ranges come out of the same scan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "upf_device_row.hpp"
+#include "upf_internal.hpp"
+#include "upf_mc.hpp"
+
+namespace upf {
+
+#include "upf_mc_device.hpp"
+
+struct BChunk {
+  DSpec spec;
+  DGrid grid;
+  McDims md;
+  float iso;
+  int32_t gx;          // 64-node row segments per row
+  int64_t node_off;    // first field value of the chunk
+  int64_t group_off;   // first marching-cubes group of the chunk
+  int64_t row_off;     // first field item
+  int64_t col_off;     // first classify item
+};
+
+// chunk holding item r of a list whose per-chunk starts are at member `off`
+template <int64_t BChunk::*OFF>
+__device__ __forceinline__ int find_chunk(const BChunk* __restrict__ ch, int n, int64_t r) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ch[mid].*OFF <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int VARIANT>
+__global__ __launch_bounds__(256) void k_batch_field(const BChunk* __restrict__ ch, int n_chunks, int64_t n_items,
+                                                     float* __restrict__ field) {
+  __shared__ float4 s_slice[4][kRowSlice];
+  __shared__ float4 s_gtab[16];
+  if (threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
+  __syncthreads();
+  const int wave = uniform(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, lane);
+  const int64_t stride = (int64_t)gridDim.x * 4;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < n_items; r += stride) {
+    const int c = uniform(find_chunk<&BChunk::row_off>(ch, n_chunks, r));
+    const BChunk& C = ch[c];
+    const int64_t local = r - C.row_off;
+    const int xb = (int)(local % C.gx);
+    const int64_t rest = local / C.gx;
+    const int j = (int)(rest % C.grid.n[1]);
+    const int k = (int)(rest / C.grid.n[1]);
+    const int i = xb * 64 + lane;
+    const int ic = min(i, C.grid.n[0] - 1);
+    const float x = node_coord(C.grid.o[0], C.grid.s[0], ic);
+    const float y = node_coord(C.grid.o[1], C.grid.s[1], j);
+    const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
+    const float v = VARIANT == UPF_EVAL_PER_LANE ? field_eval(C.spec, x, y, z) : row_field(ctx, C.spec, x, y, z);
+    if (i < C.grid.n[0]) field[C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + i] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_classify(const BChunk* __restrict__ ch, int n_chunks, int64_t n_items,
+                                                        int kz, const float* __restrict__ field,
+                                                        uint64_t* __restrict__ S, uint32_t* __restrict__ counts) {
+  __shared__ uint8_t s_ntris[256];
+  s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
+  __syncthreads();
+  const int wave = uniform(threadIdx.x >> 6);
+  const int64_t stride = (int64_t)gridDim.x * 4;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < n_items; r += stride) {
+    const int c = uniform(find_chunk<&BChunk::col_off>(ch, n_chunks, r));
+    const BChunk& C = ch[c];
+    const int64_t local = r - C.col_off;
+    const int xb = (int)(local % C.md.gpr);
+    const int64_t rest = local / C.md.gpr;
+    const int j = (int)(rest % C.md.ny);
+    const int k0 = (int)(rest / C.md.ny) * kz;
+    mc_classify_column(field + C.node_off, C.md, C.iso, xb, j, k0, min(k0 + kz, C.md.nz), s_ntris,
+                       S + C.group_off, counts + C.group_off);
+  }
+}
+
+// Per-chunk vertex / triangle ranges from the dense exclusive offsets.
+__global__ void k_batch_meshes(const BChunk* __restrict__ ch, int n_chunks, const uint32_t* __restrict__ voff,
+                               const uint32_t* __restrict__ toff, const uint4* __restrict__ total,
+                               upf_chunk_mesh* __restrict__ meshes) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_chunks) return;
+  const uint32_t v0 = voff[ch[c].group_off], t0 = toff[ch[c].group_off];
+  const uint32_t v1 = c + 1 < n_chunks ? voff[ch[c + 1].group_off] : total->x;
+  const uint32_t t1 = c + 1 < n_chunks ? toff[ch[c + 1].group_off] : total->y;
+  meshes[c].vertex_offset = v0;
+  meshes[c].n_vertices = (int64_t)v1 - v0;
+  meshes[c].triangle_offset = t0;
+  meshes[c].n_triangles = (int64_t)t1 - t0;
+}
+
+__global__ __launch_bounds__(256) void k_batch_emit(const BChunk* __restrict__ ch, int n_chunks,
+                                                    const float* __restrict__ field, const uint64_t* __restrict__ S,
+                                                    const uint32_t* __restrict__ counts,
+                                                    const uint32_t* __restrict__ voff, const uint2* __restrict__ act,
+                                                    const uint4* __restrict__ total, float* __restrict__ pos,
+                                                    float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
+                                                    int64_t cap_t) {
+  __shared__ Cross s_nb[4][8];
+  const int wave = uniform(threadIdx.x >> 6);
+  const int64_t n_active = (int64_t)total->z;
+  const int64_t stride = (int64_t)gridDim.x * 4;
+  for (int64_t a = (int64_t)blockIdx.x * 4 + wave; a < n_active; a += stride) {
+    const uint2 entry = act[a];
+    const int64_t g = (int64_t)(uint32_t)uniform((int)entry.x);
+    const int c = uniform(find_chunk<&BChunk::group_off>(ch, n_chunks, g));
+    const BChunk& C = ch[c];
+    const uint32_t vbase = voff[C.group_off];
+    mc_emit_group(field + C.node_off, C.md, C.grid, C.iso, S + C.group_off, counts + C.group_off,
+                  voff + C.group_off, (uint32_t)(g - C.group_off), (uint32_t)uniform((int)entry.y), vbase,
+                  s_nb[wave], pos, nrm, idx, cap_v, cap_t);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// ------------------------------------------------------------------ host plan
+namespace {
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+}  // namespace upf
+
+struct upf_batch {
+  std::vector<upf::BChunk> host;
+  upf::BChunk* dev = nullptr;
+  int device = 0;
+  int n = 0;
+  int cus = 256;
+  int64_t nodes = 0, groups = 0, rows = 0;
+  upf_batch_opts opts{};
+  size_t off_S = 0, off_counts = 0, off_voff = 0, off_toff = 0, off_act = 0, off_tiles = 0, off_total = 0,
+         ws_bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace upf {
+namespace {
+
+int64_t class_items(const upf_batch* b, int kz) {
+  int64_t t = 0;
+  for (const BChunk& c : b->host) t += (int64_t)c.md.gpr * c.md.ny * ((c.md.nz + kz - 1) / kz);
+  return t;
+}
+
+void set_col_offsets(upf_batch* b, int kz) {
+  int64_t t = 0;
+  for (BChunk& c : b->host) {
+    c.col_off = t;
+    t += (int64_t)c.md.gpr * c.md.ny * ((c.md.nz + kz - 1) / kz);
+  }
+}
+
+int normalize_opts(const upf_batch_opts* in, upf_batch_opts* out) {
+  upf_batch_opts o{};
+  if (in) o = *in;
+  if (o.field_variant < 0 || o.field_variant > UPF_EVAL_PER_LANE) return UPF_ERR_BAD_SPEC;
+  if (o.blocks_per_cu == 0) o.blocks_per_cu = 4;
+  if (o.blocks_per_cu < 1 || o.blocks_per_cu > 32) return UPF_ERR_BAD_SPEC;
+  if (o.classify_z == 0) o.classify_z = 32;
+  if (o.classify_z != 16 && o.classify_z != 32 && o.classify_z != 64) return UPF_ERR_BAD_SPEC;
+  o.reserved = 0;
+  *out = o;
+  return UPF_OK;
+}
+
+int upload(upf_batch* b) {
+  set_col_offsets(b, b->opts.classify_z);
+  UPF_HIP_CHECK(hipMemcpy(b->dev, b->host.data(), sizeof(BChunk) * b->n, hipMemcpyHostToDevice));
+  return UPF_OK;
+}
+
+int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float* nrm, uint32_t* idx, int64_t cap_v,
+        int64_t cap_t, upf_chunk_mesh* meshes, hipStream_t s) {
+  if (!field || !ws || !meshes || (cap_v > 0 && !pos) || (cap_t > 0 && !idx)) return UPF_ERR_NULL;
+  if (ws_bytes < b->ws_bytes) return UPF_ERR_WORKSPACE;
+  char* w = static_cast<char*>(ws);
+  uint64_t* S = reinterpret_cast<uint64_t*>(w + b->off_S);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(w + b->off_counts);
+  uint32_t* voff = reinterpret_cast<uint32_t*>(w + b->off_voff);
+  uint32_t* toff = reinterpret_cast<uint32_t*>(w + b->off_toff);
+  uint2* act = reinterpret_cast<uint2*>(w + b->off_act);
+  uint4* tiles = reinterpret_cast<uint4*>(w + b->off_tiles);
+  uint4* total = reinterpret_cast<uint4*>(w + b->off_total);
+  const unsigned grid = (unsigned)(b->cus * b->opts.blocks_per_cu);
+  if (b->opts.field_variant == UPF_EVAL_PER_LANE)
+    hipLaunchKernelGGL(k_batch_field<UPF_EVAL_PER_LANE>, dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows, field);
+  else
+    hipLaunchKernelGGL(k_batch_field<UPF_EVAL_ROWS>, dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows, field);
+  hipLaunchKernelGGL(k_batch_classify, dim3(grid), dim3(256), 0, s, b->dev, b->n, class_items(b, b->opts.classify_z),
+                     b->opts.classify_z, field, S, counts);
+  UPF_LAUNCH_CHECK();
+  int st = mc_scan_launch(counts, b->groups, tiles, total, voff, act, toff, s);
+  if (st) return st;
+  hipLaunchKernelGGL(k_batch_meshes, dim3((b->n + 63) / 64), dim3(64), 0, s, b->dev, b->n, voff, toff, total, meshes);
+  hipLaunchKernelGGL(k_batch_emit, dim3(grid), dim3(256), 0, s, b->dev, b->n, field, S, counts, voff, act, total, pos,
+                     nrm, idx, cap_v, cap_t);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+}  // namespace
+}  // namespace upf
+
+using namespace upf;
+
+extern "C" {
+
+int upf_batch_create(const upf_chunk* chunks, int32_t n_chunks, const upf_batch_opts* opts, upf_batch** out) {
+  if (!out || (!chunks && n_chunks > 0)) return UPF_ERR_NULL;
+  *out = nullptr;
+  if (n_chunks < 1 || n_chunks > 65536) return UPF_ERR_BAD_SPEC;
+  upf_batch* b = new (std::nothrow) upf_batch();
+  if (!b) return UPF_ERR_HIP;
+  int st = normalize_opts(opts, &b->opts);
+  if (st) { delete b; return st; }
+  b->n = n_chunks;
+  b->host.resize(n_chunks);
+  for (int c = 0; c < n_chunks; c++) {
+    BChunk& C = b->host[c];
+    std::memset(&C, 0, sizeof(C));
+    if ((st = to_device(&chunks[c].spec, &C.spec)) || (st = to_device(&chunks[c].grid, &C.grid)) ||
+        (st = mc_grid_ok(C.grid))) {
+      delete b;
+      return st;
+    }
+    if (!std::isfinite(chunks[c].iso)) { delete b; return UPF_ERR_INVALID_VALUE; }
+    C.md = mc_layout(C.grid).d;
+    C.iso = chunks[c].iso;
+    C.gx = (C.grid.n[0] + 63) / 64;
+    C.node_off = b->nodes;
+    C.group_off = b->groups;
+    C.row_off = b->rows;
+    b->nodes += (int64_t)C.grid.n[0] * C.grid.n[1] * C.grid.n[2];
+    b->groups += C.md.groups;
+    b->rows += (int64_t)C.gx * C.grid.n[1] * C.grid.n[2];
+  }
+  if (b->groups >= ((int64_t)1 << 31)) { delete b; return UPF_ERR_BAD_GRID; }
+  size_t o = 0;
+  b->off_S = o; o = al256(o + sizeof(uint64_t) * b->groups);
+  b->off_counts = o; o = al256(o + sizeof(uint32_t) * b->groups);
+  b->off_voff = o; o = al256(o + sizeof(uint32_t) * b->groups);
+  b->off_toff = o; o = al256(o + sizeof(uint32_t) * b->groups);
+  b->off_act = o; o = al256(o + sizeof(uint2) * b->groups);
+  b->off_tiles = o; o = al256(o + sizeof(uint4) * mc_scan_tiles(b->groups));
+  b->off_total = o; o = al256(o + sizeof(uint4));
+  b->ws_bytes = o;
+  if (hipGetDevice(&b->device) != hipSuccess ||
+      hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
+      hipMalloc(&b->dev, sizeof(BChunk) * n_chunks) != hipSuccess) {
+    delete b;
+    return UPF_ERR_HIP;
+  }
+  if ((st = upload(b))) { upf_batch_destroy(b); return st; }
+  *out = b;
+  return UPF_OK;
+}
+
+void upf_batch_destroy(upf_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->dev) (void)hipFree(b->dev);
+  delete b;
+}
+
+int64_t upf_batch_field_floats(const upf_batch* b) { return b ? b->nodes : 0; }
+int64_t upf_batch_field_offset(const upf_batch* b, int32_t chunk) {
+  return (b && chunk >= 0 && chunk < b->n) ? b->host[chunk].node_off : -1;
+}
+size_t upf_batch_workspace_bytes(const upf_batch* b) { return b ? b->ws_bytes : 0; }
+
+int upf_batch_set_opts(upf_batch* b, const upf_batch_opts* opts) {
+  if (!b) return UPF_ERR_NULL;
+  upf_batch_opts o;
+  const int st = normalize_opts(opts, &o);
+  if (st) return st;
+  b->opts = o;
+  return upload(b);
+}
+
+int upf_batch_get_opts(const upf_batch* b, upf_batch_opts* opts) {
+  if (!b || !opts) return UPF_ERR_NULL;
+  *opts = b->opts;
+  return UPF_OK;
+}
+
+int upf_batch_run(upf_batch* b, float* d_field, void* d_ws, size_t ws_bytes, float* d_pos, float* d_nrm,
+                  uint32_t* d_idx, int64_t cap_v, int64_t cap_t, upf_chunk_mesh* d_meshes, void* stream) {
+  if (!b) return UPF_ERR_NULL;
+  return run(b, d_field, d_ws, ws_bytes, d_pos, d_nrm, d_idx, cap_v, cap_t, d_meshes, as_stream(stream));
+}
+
+int upf_batch_totals(const upf_batch* b, const void* d_ws, int64_t* n_vertices, int64_t* n_triangles, void* stream) {
+  if (!b || !d_ws || !n_vertices || !n_triangles) return UPF_ERR_NULL;
+  uint4 t;
+  hipStream_t s = as_stream(stream);
+  UPF_HIP_CHECK(hipMemcpyAsync(&t, static_cast<const char*>(d_ws) + b->off_total, sizeof(t), hipMemcpyDeviceToHost, s));
+  UPF_HIP_CHECK(hipStreamSynchronize(s));
+  *n_vertices = t.x;
+  *n_triangles = t.y;
+  return UPF_OK;
+}
+
+int upf_batch_capture(upf_batch* b, float* d_field, void* d_ws, size_t ws_bytes, float* d_pos, float* d_nrm,
+                      uint32_t* d_idx, int64_t cap_v, int64_t cap_t, upf_chunk_mesh* d_meshes, void* stream) {
+  if (!b) return UPF_ERR_NULL;
+  hipStream_t s = as_stream(stream);
+  if (!s) return UPF_ERR_NULL;  // capture needs a non-default stream
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  UPF_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int st = run(b, d_field, d_ws, ws_bytes, d_pos, d_nrm, d_idx, cap_v, cap_t, d_meshes, s);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+  if (e != hipSuccess) return UPF_ERR_HIP;
+  b->graph = g;
+  UPF_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return UPF_OK;
+}
+
+int upf_batch_replay(upf_batch* b, void* stream) {
+  if (!b || !b->exec) return UPF_ERR_NULL;
+  UPF_HIP_CHECK(hipGraphLaunch(b->exec, as_stream(stream)));
+  return UPF_OK;
+}
+
+int upf_batch_autotune(upf_batch* b, float* d_field, void* d_ws, size_t ws_bytes, float* d_pos, float* d_nrm,
+                       uint32_t* d_idx, int64_t cap_v, int64_t cap_t, upf_chunk_mesh* d_meshes, void* stream,
+                       upf_batch_opts* best) {
+  if (!b) return UPF_ERR_NULL;
+  hipStream_t s = as_stream(stream);
+  hipEvent_t e0, e1;
+  UPF_HIP_CHECK(hipEventCreate(&e0));
+  UPF_HIP_CHECK(hipEventCreate(&e1));
+  const upf_batch_opts start = b->opts;
+  upf_batch_opts best_o = start;
+  float best_ms = 1e30f;
+  int st = UPF_OK;
+  const int variants[2] = {UPF_EVAL_ROWS, UPF_EVAL_PER_LANE};
+  const int bpcs[3] = {2, 4, 8};
+  const int kzs[3] = {16, 32, 64};
+  for (int cand = 0; cand < 18 && !st; cand++) {
+    const upf_batch_opts o{variants[cand / 9], bpcs[(cand / 3) % 3], kzs[cand % 3], 0};
+    if ((st = upf_batch_set_opts(b, &o))) break;
+    if ((st = run(b, d_field, d_ws, ws_bytes, d_pos, d_nrm, d_idx, cap_v, cap_t, d_meshes, s))) break;  // warm-up
+    if (hipEventRecord(e0, s) != hipSuccess) { st = UPF_ERR_HIP; break; }
+    for (int r = 0; r < 2 && !st; r++) st = run(b, d_field, d_ws, ws_bytes, d_pos, d_nrm, d_idx, cap_v, cap_t, d_meshes, s);
+    if (st) break;
+    float ms = 0.0f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      st = UPF_ERR_HIP;
+      break;
+    }
+    if (ms < best_ms) { best_ms = ms; best_o = o; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const int st2 = upf_batch_set_opts(b, st ? &start : &best_o);
+  if (st) return st;
+  if (best) *best = best_o;
+  return st2;
+}
+
+}  // extern "C"

@@ -6,10 +6,11 @@
 // corners (or 27 Voronoi cells), the lanes of the wave hash each lattice point once,
 // write its gradient vector / lattice value / feature point into the wave's slice of
 // LDS, and every lane then gathers its corners with immediate-offset LDS reads.
-// Results are bit-identical to the per-lane functions of upf_device.hpp: the same
-// hashes, and dot products with {-1,0,1} gradient vectors round exactly like the
-// per-lane signed sums.  Rows whose lattice span does not fit the slice fall back to
-// the per-lane functions (a wave-uniform branch).
+// The lattice (floors, hashes, gradient vectors, feature points) is exactly the
+// per-lane one of upf_device.hpp; the interpolation sums are regrouped (see "Collapsed
+// columns" below) and agree with the per-lane results to a few ulps.  Rows whose
+// lattice span does not fit the slice fall back to the per-lane functions (a
+// wave-uniform branch).
 #pragma once
 
 #include "upf_device.hpp"
@@ -17,6 +18,7 @@
 namespace upf {
 
 constexpr int kRowSlice = 256;  // float4 entries of LDS per wave
+constexpr int kRowSlice2 = 2 * kRowSlice;  // the same slice as float2 entries
 
 // Gradient vectors of Perlin's 16-entry set, indexed by h >> 28 (see gdot()).
 __device__ __forceinline__ float4 gradient_entry(int g) {
@@ -39,7 +41,22 @@ struct RowCtx {
   float4* slice;        // this wave's LDS slice (kRowSlice entries)
   const float4* gtab;   // 16 gradient vectors in LDS
   int lane;
+  uint32_t col_px;      // (lane / 4) * PX: the lane's column offset in the lattice hash
+  int cell_lx, cell_c9, cell_dy, cell_dz;  // the lane's feature point in the cellular writer
 };
+
+__device__ __forceinline__ RowCtx make_row_ctx(float4* slice, const float4* gtab, int lane) {
+  RowCtx c;
+  c.slice = slice;
+  c.gtab = gtab;
+  c.lane = lane;
+  c.col_px = (uint32_t)(lane >> 2) * 501125321u;
+  c.cell_lx = lane % 7;
+  c.cell_c9 = lane / 7;
+  c.cell_dy = c.cell_c9 % 3;
+  c.cell_dz = c.cell_c9 / 3;
+  return c;
+}
 
 // Range of an integer that is monotone across the lanes of the wave.
 __device__ __forceinline__ void lane_range(int v, int& lo, int& hi) {
@@ -49,142 +66,172 @@ __device__ __forceinline__ void lane_range(int v, int& lo, int& hi) {
   hi = max(a, b);
 }
 
-// Per-octave lattice data of one layer lives in the wave's slice in regions packed one
-// after another; a batch is as many consecutive octaves as fit.  Writers and readers
-// recompute the same (ixmin, nl) from lanes 0 and 63, so the region bases agree.
-//   gradient: region of 4*nl entries, entry base + combo*nl + xo holds the gradient
-//             vector of lattice point (ixmin + xo, iy + dy, iz + dz), combo = dy + 2 dz
-//             (consecutive x in consecutive 16-byte entries: conflict-free reads);
-//   value:    region of nl entries, entry base + xo holds the 4 combos' lattice values;
-//   cellular: region of 72 entries, entry base + lx + 8*(dy + 3 dz) holds the feature
-//             point of cell (cxmin - 1 + lx, cy - 1 + dy, cz - 1 + dz), lx < 7.
-struct RowOct {
-  int i0, lo, n;  // lane's lattice index, wave minimum, entries of x span
-};
+// Collapsed columns.  In a row y and z are the same in every lane, so the y and z part
+// of the trilinear interpolation is the same for every sample that uses a given lattice
+// column X (the 4 lattice points (X, iy + dy, iz + dz)).  The writers fold it once per
+// column:
+//   gradient: A(X) = fold(g_x),  B(X) = fold(g_y (ty - dy) + g_z (tz - dz))
+//   value:    A(X) = fold(v),    B(X) = 0
+//   fold(a)   = lerp(lerp(a_00, a_10, uy), lerp(a_01, a_11, uy), uz)   (index dy dz)
+// and a lane's octave becomes lerp(A(ix) tx + B(ix), A(ix+1) (tx - 1) + B(ix+1), ux):
+// the same trilinear sum with the x interpolation done last.  The lattice (floors,
+// hashes, gradients) is exactly the per-lane one; the sums round in another order, a
+// few ulps apart.  Cellular: the writers store per feature point (p_x, e) with
+// e = (p_y - q_y)^2 + (p_z - q_z)^2, so a lane's squared distance is one fma.
+//
+// Layout: the wave's slice as float2 entries, a region per octave of the layer packed
+// one after the other (a layer whose regions do not all fit is evaluated per lane):
+//   gradient, value: nl entries, entry xo = column ixmin + xo;
+//   cellular:        72 entries, entry lx + 8 (dy + 3 dz) = cell (cxmin-1+lx, cy-1+dy, cz-1+dz).
 
-__device__ __forceinline__ RowOct row_span(float q, bool cell) {
-  RowOct o;
-  o.i0 = (int)(cell ? rintf(q) : floorf(q));
-  int hi;
-  lane_range(o.i0, o.lo, hi);
-  o.n = hi - o.lo + (cell ? 3 : 2);
-  return o;
+// quad-permute (DPP) partners inside groups of 4 lanes: lane ^ 1 and lane ^ 2
+__device__ __forceinline__ float quad_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float quad_xor2(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
 }
 
-__device__ __forceinline__ int region_size(int noise, int n) {
-  return noise == UPF_NOISE_GRADIENT ? 4 * n : (noise == UPF_NOISE_VALUE ? n : (n <= 7 ? 72 : kRowSlice + 1));
+__device__ __forceinline__ float fractal_acc(const DLayer& L, int k, float n, float acc) {
+  return L.fractal == UPF_FRACTAL_NONE ? n
+         : (L.fractal == UPF_FRACTAL_RIDGED ? fmaf(L.amp[k], fmaf(-2.0f, fabsf(n), 1.0f), acc)
+                                            : fmaf(L.amp[k], n, acc));
 }
 
-__device__ __forceinline__ void row_write(const RowCtx& c, const DLayer& L, uint32_t seed, float qx, float qy,
-                                          float qz, const RowOct& o, int base) {
-  if (L.noise == UPF_NOISE_CELLULAR) {
-    const int cy = (int)rintf(qy), cz = (int)rintf(qz);
-    const int lx = c.lane % 7, c9 = c.lane / 7;  // lane 63: c9 = 9, idle
-    const int dy = c9 % 3, dz = c9 / 3;
-    const int kx = o.lo - 1 + lx, ky = cy - 1 + dy, kz = cz - 1 + dz;
-    const uint32_t h = (seed ^ ((uint32_t)kx * kPX) ^ ((uint32_t)ky * kPY) ^ ((uint32_t)kz * kPZ)) * kHM;
-    const uint32_t m = h ^ (h >> 15);
-    const float px = fmaf((float)(m & 1023u), L.js, (float)kx - L.j0);
-    const float py = fmaf((float)((m >> 10) & 1023u), L.js, (float)ky - L.j0);
-    const float pz = fmaf((float)((m >> 20) & 1023u), L.js, (float)kz - L.j0);
-    if (c9 < 9) c.slice[base + lx + 8 * c9] = make_float4(px, py, pz, 0.0f);
-    return;
+// Gradient / value columns of one octave.  Lane = 4 * column + combo (combo = dy + 2 dz),
+// 16 columns per round; the y and z folds take the partner lanes lane ^ 1 and lane ^ 2.
+template <int NOISE>
+__device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float qy, float qz, int lo, int n,
+                                          float2* region) {
+  const float fy = floorf(qy), fz = floorf(qz);
+  const float ty = qy - fy, tz = qz - fz;
+  const float uy = fade(ty), uz = fade(tz);
+  const int iy = __builtin_amdgcn_readfirstlane((int)fy), iz = __builtin_amdgcn_readfirstlane((int)fz);
+  const uint32_t y0 = (uint32_t)iy * kPY, z0 = (uint32_t)iz * kPZ;
+  const int combo = c.lane & 3;
+  const uint32_t s = seed ^ ((combo & 1) ? y0 + kPY : y0) ^ ((combo & 2) ? z0 + kPZ : z0);
+  const float ay = (combo & 1) ? ty - 1.0f : ty, az = (combo & 2) ? tz - 1.0f : tz;
+  const uint32_t xbase = (uint32_t)lo * kPX + c.col_px;
+  for (int r = 0; r * 16 < n; r++) {
+    const uint32_t h = (s ^ (xbase + (uint32_t)(16 * r) * kPX)) * kHM;
+    float a, b;
+    if (NOISE == UPF_NOISE_GRADIENT) {
+      const float4 g = c.gtab[h >> 28];
+      a = g.x;
+      b = fmaf(g.z, az, g.y * ay);
+    } else {
+      a = lattice_value(h);
+      b = 0.0f;
+    }
+    const float ya = lerpf(a, quad_xor1(a), uy);
+    const float za = lerpf(ya, quad_xor2(ya), uz);
+    float zb = 0.0f;
+    if (NOISE == UPF_NOISE_GRADIENT) {
+      const float yb = lerpf(b, quad_xor1(b), uy);
+      zb = lerpf(yb, quad_xor2(yb), uz);
+    }
+    const int xo = (c.lane >> 2) + 16 * r;
+    if (combo == 0 && xo < n) region[xo] = make_float2(za, zb);
   }
-  const int iy = (int)floorf(qy), iz = (int)floorf(qz);
-  const uint32_t y0 = (uint32_t)iy * kPY, y1 = y0 + kPY;
-  const uint32_t z0 = (uint32_t)iz * kPZ, z1 = z0 + kPZ;
-  const int combo = c.lane >> 4;  // dy + 2 dz
-  const uint32_t s = ((combo & 1) ? y1 : y0) ^ ((combo & 2) ? z1 : z0) ^ seed;
-  const uint32_t xbase = (uint32_t)o.lo * kPX;
-  for (int r = 0; r * 16 < o.n; r++) {
-    const int xo = (c.lane & 15) + 16 * r;
-    if (xo < o.n) {
-      const uint32_t h = (s ^ (xbase + (uint32_t)xo * kPX)) * kHM;
-      if (L.noise == UPF_NOISE_GRADIENT) c.slice[base + combo * o.n + xo] = c.gtab[h >> 28];
-      else reinterpret_cast<float*>(c.slice + base + xo)[combo] = lattice_value(h);
+}
+
+template <int NOISE>
+__device__ __forceinline__ float col_read(float qx, int lo, const float2* region) {
+  const float fx = floorf(qx);
+  const float tx = qx - fx;
+  const float ux = fade(tx);
+  const float2* e = region + ((int)fx - lo);
+  const float2 e0 = e[0], e1 = e[1];
+  if (NOISE == UPF_NOISE_VALUE) return lerpf(e0.x, e1.x, ux);
+  const float d0 = fmaf(e0.x, tx, e0.y);
+  const float d1 = fmaf(e1.x, tx - 1.0f, e1.y);
+  return lerpf(d0, d1, ux);
+}
+
+// Cellular: 63 lanes generate one feature point each, (lx, c9) = (lane % 7, lane / 7).
+__device__ __forceinline__ void cell_write(const RowCtx& c, const DLayer& L, uint32_t seed, float qy, float qz,
+                                           int lo, float2* region) {
+  const int cy = __builtin_amdgcn_readfirstlane((int)rintf(qy));
+  const int cz = __builtin_amdgcn_readfirstlane((int)rintf(qz));
+  const int lx = c.cell_lx, c9 = c.cell_c9;
+  const int kx = lo - 1 + lx, ky = cy - 1 + c.cell_dy, kz = cz - 1 + c.cell_dz;
+  const uint32_t h = (seed ^ ((uint32_t)kx * kPX) ^ ((uint32_t)ky * kPY) ^ ((uint32_t)kz * kPZ)) * kHM;
+  const uint32_t m = h ^ (h >> 15);
+  const float px = fmaf((float)(m & 1023u), L.js, (float)kx - L.j0);
+  const float py = fmaf((float)((m >> 10) & 1023u), L.js, (float)ky - L.j0);
+  const float pz = fmaf((float)((m >> 20) & 1023u), L.js, (float)kz - L.j0);
+  const float ey = py - qy, ez = pz - qz;
+  if (c9 < 9) region[lx + 8 * c9] = make_float2(px, fmaf(ez, ez, ey * ey));
+}
+
+__device__ __forceinline__ float cell_read(float qx, int lo, const float2* region) {
+  const float2* p = region + ((int)rintf(qx) - lo);  // cell (cx-1+dx, cy-1+dy, cz-1+dz) at p[dx + 8(dy+3dz)]
+  float best = __builtin_inff();
+#pragma unroll
+  for (int dz = 0; dz < 3; dz++)
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+      for (int dx = 0; dx < 3; dx++) {
+        const float2 v = p[dx + 8 * (dy + 3 * dz)];
+        const float d = v.x - qx;
+        best = fminf(best, fmaf(d, d, v.y));
+      }
+  return __builtin_sqrtf(best);
+}
+
+// One layer over the row.  The octave loops are unrolled, so each octave's span (lo, n)
+// and region base are wave-uniform scalars computed once.  When the layer's regions do
+// not fit the slice the layer is evaluated per lane.
+template <int NOISE>
+__device__ __forceinline__ float row_layer_t(const RowCtx& c, const DLayer& L, float x, float y, float z) {
+  constexpr bool CELL = NOISE == UPF_NOISE_CELLULAR;
+  const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
+  float2* slice = reinterpret_cast<float2*>(c.slice);
+  int lo[UPF_MAX_OCTAVES], base[UPF_MAX_OCTAVES];
+  int total = 0;
+  bool fits = true;
+#pragma unroll
+  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+    if (k < oct) {
+      const float q = x * L.freq[k];
+      const int i0 = CELL ? (int)rintf(q) : (int)floorf(q);
+      int hi;
+      lane_range(i0, lo[k], hi);
+      const int n = hi - lo[k] + (CELL ? 3 : 2);
+      base[k] = total;
+      total += CELL ? 72 : n;
+      fits = fits && (!CELL || n <= 7);
     }
   }
-}
-
-__device__ __forceinline__ float row_read(const RowCtx& c, const DLayer& L, float qx, float qy, float qz,
-                                          const RowOct& o, int base) {
-  if (L.noise == UPF_NOISE_CELLULAR) {
-    const float4* p = c.slice + base + (o.i0 - o.lo);  // cell (cx-1+dx, cy-1+dy, cz-1+dz) at p[dx + 8(dy + 3dz)]
-    float best = __builtin_inff();
+  if (!fits || total > kRowSlice2) return layer_value(L, x, y, z);
 #pragma unroll
-    for (int dz = 0; dz < 3; dz++)
-#pragma unroll
-      for (int dy = 0; dy < 3; dy++)
-#pragma unroll
-        for (int dx = 0; dx < 3; dx++) {
-          const float4 q = p[dx + 8 * (dy + 3 * dz)];
-          const float ex = q.x - qx, ey = q.y - qy, ez = q.z - qz;
-          best = fminf(best, fmaf(ez, ez, fmaf(ey, ey, ex * ex)));
-        }
-    return __builtin_sqrtf(best);
+  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+    if (k < oct) {
+      const float f = L.freq[k];
+      if (CELL) cell_write(c, L, L.seed + (uint32_t)k, y * f, z * f, lo[k], slice + base[k]);
+      else col_write<NOISE>(c, L.seed + (uint32_t)k, y * f, z * f, lo[k],
+                            (k + 1 < oct ? base[k + 1] : total) - base[k], slice + base[k]);
+    }
   }
-  const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
-  const float tx = qx - fx, ty = qy - fy, tz = qz - fz;
-  const float ux = fade(tx), uy = fade(ty), uz = fade(tz);
-  const float4* e = c.slice + base + (o.i0 - o.lo);
-  if (L.noise == UPF_NOISE_GRADIENT) {
-    const float sx = tx - 1.0f, sy = ty - 1.0f, sz = tz - 1.0f;
-    const int n = o.n;
-    // e[combo * n + dx]
-    const float x00 = lerpf(dot_g(e[0], tx, ty, tz), dot_g(e[1], sx, ty, tz), ux);
-    const float x10 = lerpf(dot_g(e[n], tx, sy, tz), dot_g(e[n + 1], sx, sy, tz), ux);
-    const float x01 = lerpf(dot_g(e[2 * n], tx, ty, sz), dot_g(e[2 * n + 1], sx, ty, sz), ux);
-    const float x11 = lerpf(dot_g(e[3 * n], tx, sy, sz), dot_g(e[3 * n + 1], sx, sy, sz), ux);
-    return lerpf(lerpf(x00, x10, uy), lerpf(x01, x11, uy), uz);
+  wave_sync();
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+    if (k < oct) {
+      const float q = x * L.freq[k];
+      const float n = CELL ? cell_read(q, lo[k], slice + base[k]) : col_read<NOISE>(q, lo[k], slice + base[k]);
+      acc = fractal_acc(L, k, n, acc);
+    }
   }
-  const float4 v0 = e[0], v1 = e[1];  // .x .y .z .w = combos (dy, dz) = 00 10 01 11
-  const float x00 = lerpf(v0.x, v1.x, ux), x10 = lerpf(v0.y, v1.y, ux);
-  const float x01 = lerpf(v0.z, v1.z, ux), x11 = lerpf(v0.w, v1.w, ux);
-  return lerpf(lerpf(x00, x10, uy), lerpf(x01, x11, uy), uz);
+  wave_sync();
+  return acc;
 }
 
 __device__ __forceinline__ float row_layer_value(const RowCtx& c, const DLayer& L, float x, float y, float z) {
-  const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
-  const bool cell = L.noise == UPF_NOISE_CELLULAR;
-  float acc = 0.0f;
-  int k = 0;
-  while (k < oct) {
-    // writers: as many octaves as fit in the slice
-    const int k0 = k;
-    int base = 0;
-    while (k < oct) {
-      const float f = L.freq[k];
-      const float qx = x * f, qy = y * f, qz = z * f;
-      const RowOct o = row_span(qx, cell);
-      const int sz = region_size(L.noise, o.n);
-      if (base + sz > kRowSlice) break;
-      row_write(c, L, L.seed + (uint32_t)k, qx, qy, qz, o, base);
-      base += sz;
-      k++;
-    }
-    if (k == k0) {  // one octave too wide for the slice: per-lane evaluation
-      const float n = octave_noise(L, k, x, y, z);
-      acc = L.fractal == UPF_FRACTAL_NONE ? n
-            : (L.fractal == UPF_FRACTAL_RIDGED ? fmaf(L.amp[k], fmaf(-2.0f, fabsf(n), 1.0f), acc)
-                                               : fmaf(L.amp[k], n, acc));
-      k++;
-      continue;
-    }
-    wave_sync();
-    base = 0;
-    for (int kk = k0; kk < k; kk++) {
-      const float f = L.freq[kk];
-      const float qx = x * f, qy = y * f, qz = z * f;
-      const RowOct o = row_span(qx, cell);
-      const float n = row_read(c, L, qx, qy, qz, o, base);
-      base += region_size(L.noise, o.n);
-      acc = L.fractal == UPF_FRACTAL_NONE ? n
-            : (L.fractal == UPF_FRACTAL_RIDGED ? fmaf(L.amp[kk], fmaf(-2.0f, fabsf(n), 1.0f), acc)
-                                               : fmaf(L.amp[kk], n, acc));
-    }
-    wave_sync();
-  }
-  return acc;
+  if (L.noise == UPF_NOISE_GRADIENT) return row_layer_t<UPF_NOISE_GRADIENT>(c, L, x, y, z);
+  if (L.noise == UPF_NOISE_VALUE) return row_layer_t<UPF_NOISE_VALUE>(c, L, x, y, z);
+  return row_layer_t<UPF_NOISE_CELLULAR>(c, L, x, y, z);
 }
 
 __device__ __forceinline__ float row_elevation(const RowCtx& c, const DSpec& s, float x, float y, float z) {

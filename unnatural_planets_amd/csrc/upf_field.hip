@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_eval_grid_row(const DSpec s, const DGri
   const float x = node_coord(g.o[0], g.s[0], ic);
   const float y = node_coord(g.o[1], g.s[1], j);
   const float z = node_coord(g.o[2], g.s[2], k);
-  const RowCtx c{s_slice[threadIdx.y], s_gtab, lane};
+  const RowCtx c = make_row_ctx(s_slice[threadIdx.y], s_gtab, lane);
   const float v = row_field(c, s, x, y, z);
   if (i < g.n[0]) out[((size_t)k * g.n[1] + j) * g.n[0] + i] = v;
 }

@@ -39,6 +39,11 @@ int mc_grid_ok(const DGrid& g);
 int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws, size_t ws_bytes, hipStream_t s);
 // emit over at most max_active active groups (the kernel reads the true count)
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
-                   float* nrm, uint32_t* idx, hipStream_t s);
+                   float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s);
+// exclusive scan of packed group counts (the three sums of classify) into voff/act/total
+// (and the dense triangle offsets toff when non-null)
+int mc_scan_launch(const uint32_t* counts, int64_t groups, uint4* tiles, uint4* total, uint32_t* voff, uint2* act,
+                   uint32_t* toff, hipStream_t s);
+int64_t mc_scan_tiles(int64_t groups);
 
 }  // namespace upf

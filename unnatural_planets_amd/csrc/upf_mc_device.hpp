@@ -1,0 +1,222 @@
+// upf_mc_device.hpp -- wave-level marching-cubes bodies shared by the single-grid
+// kernels (upf_mc.hip) and the batched kernels (upf_batch.hip).  Included inside
+// namespace upf.
+#pragma once
+
+#define UPF_MC_QUAL __constant__
+#include "mc_table.inc"
+#undef UPF_MC_QUAL
+
+// edge -> (owner corner offset x, y, z, axis)
+__constant__ int8_t c_edge_owner[12][4] = {
+    {0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
+    {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// packed per-group counts: vertices (bits 0-7, <= 192), triangles (8-16, <= 320), active (17)
+__device__ __forceinline__ uint32_t pack_counts(int nv, int nt) {
+  return (uint32_t)nv | ((uint32_t)nt << 8) | ((nv | nt) ? (1u << 17) : 0u);
+}
+
+// ------------------------------------------------------------------ classify
+// One wave: group column xb of row j, planes k0 <= k < k1.  The current z plane stays in
+// registers; each step loads the next plane's rows j and j+1 at x and x+1 (clamped, so
+// every load is unconditional and the loads of a step issue back to back).
+__device__ __forceinline__ void mc_classify_column(const float* __restrict__ f, const McDims& d, float iso, int xb,
+                                                   int j, int k0, int k1, const uint8_t* s_ntris,
+                                                   uint64_t* __restrict__ S, uint32_t* __restrict__ counts) {
+  const int lane = threadIdx.x & 63;
+  const int x = xb * 64 + lane;
+  const bool valid = x < d.nx;
+  const int xc = min(x, d.nx - 1), x1 = min(x + 1, d.nx - 1);
+  const int j1 = min(j + 1, d.ny - 1);
+  const bool hx = valid && x + 1 < d.nx, hy = valid && j + 1 < d.ny;
+  const int64_t r0 = (int64_t)j * d.nx, r1 = (int64_t)j1 * d.nx;
+  // current plane: a0 = (x, j), a1 = (x+1, j), a2 = (x+1, j+1), a3 = (x, j+1)
+  const float* p = f + (int64_t)k0 * d.nxy;
+  float a0 = p[r0 + xc], a1 = p[r0 + x1], a2 = p[r1 + x1], a3 = p[r1 + xc];
+  for (int k = k0; k < k1; k++) {
+    const bool hz = valid && k + 1 < d.nz;
+    const float* q = f + (int64_t)min(k + 1, d.nz - 1) * d.nxy;
+    const float b0 = q[r0 + xc], b1 = q[r0 + x1], b2 = q[r1 + x1], b3 = q[r1 + xc];
+    const bool s0 = valid && a0 < iso;
+    const bool cx = hx && ((a1 < iso) != s0);
+    const bool cy = hy && ((a3 < iso) != s0);
+    const bool cz = hz && ((b0 < iso) != s0);
+    int nt = 0;
+    if (hx && hy && hz) {
+      const int code = (a0 < iso) | ((a1 < iso) << 1) | ((a2 < iso) << 2) | ((a3 < iso) << 3) | ((b0 < iso) << 4) |
+                       ((b1 < iso) << 5) | ((b2 < iso) << 6) | ((b3 < iso) << 7);
+      nt = s_ntris[code];
+    }
+    const uint64_t sm = ballot64(s0);
+    const int nv = __popcll(ballot64(cx)) + __popcll(ballot64(cy)) + __popcll(ballot64(cz));
+    // triangle total from bit-sliced ballots (nt <= 5)
+    const int ntot = __popcll(ballot64(nt & 1)) + 2 * __popcll(ballot64(nt & 2)) + 4 * __popcll(ballot64(nt & 4));
+    if (lane == 0) {
+      const int64_t g = ((int64_t)k * d.ny + j) * d.gpr + xb;
+      S[g] = sm;
+      counts[g] = pack_counts(nv, ntot);
+    }
+    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+  }
+}
+
+// ------------------------------------------------------------------ emit
+struct Cross {
+  uint64_t bx, by, bz;
+  uint32_t voff, pad;
+};
+
+// Crossing masks of group (xb, j, k) (row-aligned, so its +y and +z neighbour groups
+// share its lane alignment).  Lanes past nx hold 0 sign bits and are masked.
+__device__ __forceinline__ Cross group_crossings(const uint64_t* __restrict__ S, const uint32_t* __restrict__ voff,
+                                                 const McDims& d, int xb, int j, int k) {
+  const int64_t gg = ((int64_t)k * d.ny + j) * d.gpr + xb;
+  const int first = xb * 64;
+  const int nval = min(64, d.nx - first);     // lanes inside the row
+  const int nvx = min(64, d.nx - 1 - first);  // lanes with an x neighbour
+  const uint64_t vmask = nval >= 64 ? ~0ull : ((1ull << nval) - 1);
+  const uint64_t xmask = nvx >= 64 ? ~0ull : (nvx > 0 ? ((1ull << nvx) - 1) : 0ull);
+  const uint64_t s = S[gg];
+  const uint64_t snext = (xb + 1 < d.gpr) ? S[gg + 1] : 0ull;
+  Cross c;
+  c.bx = (s ^ ((s >> 1) | (snext << 63))) & xmask;
+  c.by = (j + 1 < d.ny) ? ((s ^ S[gg + d.gpr]) & vmask) : 0ull;
+  c.bz = (k + 1 < d.nz) ? ((s ^ S[gg + (int64_t)d.gpr * d.ny]) & vmask) : 0ull;
+  c.voff = voff[gg];
+  c.pad = 0;
+  return c;
+}
+
+__device__ __forceinline__ float node_gradient_axis(const float* __restrict__ f, int64_t n, int idx, int nmax,
+                                                    int64_t step, float sp) {
+  if (idx == 0) return (f[n + step] - f[n]) / sp;
+  if (idx == nmax - 1) return (f[n] - f[n - step]) / sp;
+  return (f[n + step] - f[n - step]) / (2.0f * sp);
+}
+
+__device__ __forceinline__ void node_gradient(const float* __restrict__ f, const McDims& d, const DGrid& g, int i,
+                                              int j, int k, float out[3]) {
+  const int64_t n = (int64_t)k * d.nxy + (int64_t)j * d.nx + i;
+  out[0] = node_gradient_axis(f, n, i, d.nx, 1, g.s[0]);
+  out[1] = node_gradient_axis(f, n, j, d.ny, d.nx, g.s[1]);
+  out[2] = node_gradient_axis(f, n, k, d.nz, d.nxy, g.s[2]);
+}
+
+// One wave emits active group gid (local to its grid): vertices at voff (global ids,
+// stored at pos/nrm[id]) and triangles from toff (global), with index values made local
+// by subtracting vbase.  Writes at or past cap_v / cap_t are dropped (the caller sized
+// the buffers from the counts, or checks the counts afterwards).  s_nb: 8 LDS entries
+// private to this wave.
+__device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
+                                              const uint64_t* __restrict__ S, const uint32_t* __restrict__ counts,
+                                              const uint32_t* __restrict__ voff, uint32_t gid, uint32_t toff,
+                                              uint32_t vbase, Cross* s_nb, float* __restrict__ pos,
+                                              float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
+                                              int64_t cap_t) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t row = gid / (uint32_t)d.gpr;
+  const int xb = (int)(gid - row * (uint32_t)d.gpr);
+  const int k = (int)(row / (uint32_t)d.ny);
+  const int j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
+  const uint32_t c = counts[gid];
+  const int cnt_v = c & 0xFF, cnt_t = (c >> 8) & 0x1FF;
+
+  // crossing masks of the (up to) 8 groups a cell of this group takes edges from:
+  // slot = gx | dy << 1 | dz << 2, gx = 1 for the next group along x
+  if (lane < 8) {
+    const int gx = lane & 1, dy = (lane >> 1) & 1, dz = lane >> 2;
+    if (xb + gx < d.gpr && j + dy < d.ny && k + dz < d.nz) s_nb[lane] = group_crossings(S, voff, d, xb + gx, j + dy, k + dz);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  const int i = xb * 64 + lane;
+  const bool valid = i < d.nx;
+  const int64_t n = (int64_t)k * d.nxy + (int64_t)j * d.nx + i;
+  const uint64_t lt = (1ull << lane) - 1;
+
+  // ---- vertices
+  if (cnt_v) {
+    const Cross cc = s_nb[0];
+    uint32_t vid = cc.voff + (uint32_t)(__popcll(cc.bx & lt) + __popcll(cc.by & lt) + __popcll(cc.bz & lt));
+    const uint64_t bits[3] = {cc.bx, cc.by, cc.bz};
+    const bool any = ((cc.bx | cc.by | cc.bz) >> lane) & 1ull;
+    if (any) {
+      const float f0 = f[n];
+      const float p0[3] = {node_coord(g.o[0], g.s[0], i), node_coord(g.o[1], g.s[1], j),
+                           node_coord(g.o[2], g.s[2], k)};
+      float g0[3] = {0.0f, 0.0f, 0.0f};
+      if (nrm) node_gradient(f, d, g, i, j, k, g0);
+      const int64_t step[3] = {1, d.nx, d.nxy};
+      const int ijk[3] = {i, j, k};
+#pragma unroll
+      for (int ax = 0; ax < 3; ax++) {
+        if (!((bits[ax] >> lane) & 1ull)) continue;
+        const float f1 = f[n + step[ax]];
+        const float t = (iso - f0) / (f1 - f0);
+        const float p1a = node_coord(g.o[ax], g.s[ax], ijk[ax] + 1);
+        float p[3] = {p0[0], p0[1], p0[2]};
+        p[ax] = p0[ax] + t * (p1a - p0[ax]);
+        if ((int64_t)vid < cap_v) {
+          pos[3 * (size_t)vid + 0] = p[0];
+          pos[3 * (size_t)vid + 1] = p[1];
+          pos[3 * (size_t)vid + 2] = p[2];
+          if (nrm) {
+            float g1[3];
+            node_gradient(f, d, g, i + (ax == 0), j + (ax == 1), k + (ax == 2), g1);
+            const float gx = g0[0] + t * (g1[0] - g0[0]);
+            const float gy = g0[1] + t * (g1[1] - g0[1]);
+            const float gz = g0[2] + t * (g1[2] - g0[2]);
+            const float len = __builtin_sqrtf(gx * gx + gy * gy + gz * gz);
+            const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+            nrm[3 * (size_t)vid + 0] = gx * inv;
+            nrm[3 * (size_t)vid + 1] = gy * inv;
+            nrm[3 * (size_t)vid + 2] = gz * inv;
+          }
+        }
+        vid++;
+      }
+    }
+  }
+
+  // ---- triangles
+  if (cnt_t) {
+    const bool cell = valid && i + 1 < d.nx && j + 1 < d.ny && k + 1 < d.nz;
+    int code = 0;
+    if (cell) {
+      const float c0 = f[n], c1 = f[n + 1], c3 = f[n + d.nx], c2 = f[n + d.nx + 1];
+      const float c4 = f[n + d.nxy], c5 = f[n + d.nxy + 1], c7 = f[n + d.nxy + d.nx], c6 = f[n + d.nxy + d.nx + 1];
+      code = (c0 < iso) | ((c1 < iso) << 1) | ((c2 < iso) << 2) | ((c3 < iso) << 3) | ((c4 < iso) << 4) |
+             ((c5 < iso) << 5) | ((c6 < iso) << 6) | ((c7 < iso) << 7);
+    }
+    const int nt = upf_mc_ntris[code];
+    int inc = nt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += v;
+    }
+    uint32_t tid = toff + (uint32_t)(inc - nt);
+    for (int t = 0; t < nt; t++, tid++) {
+#pragma unroll
+      for (int v = 0; v < 3; v++) {
+        const int e = upf_mc_tris[code][3 * t + v];
+        const int ol = lane + c_edge_owner[e][0];
+        const int slot = (ol >> 6) | (c_edge_owner[e][1] << 1) | (c_edge_owner[e][2] << 2);
+        const int axis = c_edge_owner[e][3];
+        const Cross cc = s_nb[slot];
+        const int l = ol & 63;
+        const uint64_t m = (1ull << l) - 1;
+        uint32_t id = cc.voff + (uint32_t)(__popcll(cc.bx & m) + __popcll(cc.by & m) + __popcll(cc.bz & m));
+        if (axis >= 1) id += (uint32_t)((cc.bx >> l) & 1ull);
+        if (axis == 2) id += (uint32_t)((cc.by >> l) & 1ull);
+        if ((int64_t)tid < cap_t) idx[3 * (size_t)tid + v] = id - vbase;
+      }
+    }
+  }
+}

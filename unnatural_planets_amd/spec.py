@@ -181,3 +181,51 @@ def spec_torus_fbm_voronoi(seed: int = 7) -> FieldSpec:
         layer("gradient", "fbm", 4, 2.0, 2.0, 0.5, 0.05, seed),
         layer("cellular", "none", 1, 4.0, 2.0, 0.5, 0.03, seed + 101, jitter=0.9),
     ])
+
+
+# ---------------------------------------------------------------------------- batched chunks
+class Chunk(ctypes.Structure):
+    _fields_ = [("spec", FieldSpec), ("grid", Grid), ("iso", ctypes.c_float), ("reserved", ctypes.c_int32 * 3)]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("field_variant", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+                ("classify_z", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class ChunkMesh(ctypes.Structure):
+    _fields_ = [("vertex_offset", ctypes.c_int64), ("n_vertices", ctypes.c_int64),
+                ("triangle_offset", ctypes.c_int64), ("n_triangles", ctypes.c_int64)]
+
+
+BATCH_SIZES = (32, 48, 64, 96, 128, 192)
+BATCH_SHAPES = ("sphere", "box", "torus", "capsule")
+BATCH_SHAPE_PARAMS = {"sphere": (1.0,), "box": (0.7, 0.7, 0.7, 0.2), "torus": (1.0, 0.4), "capsule": (1.2, 0.5)}
+
+
+def batch_manifest(seed: int = 0, n: int = 64):
+    """The uneven batch: n chunks, grid size ~ U{32,48,64,96,128,192}^3 over [-1.6,1.6]^3,
+    shape ~ U{sphere, rounded cube, torus, capsule}, a gradient fBm layer with
+    octaves ~ U{3..6} (f0 2.0, lacunarity 2.0, gain 0.5, weight 0.05) and a per-chunk
+    seed, all drawn from numpy's default_rng(seed)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        size = int(rng.choice(BATCH_SIZES))
+        shape = BATCH_SHAPES[int(rng.integers(0, 4))]
+        octaves = int(rng.integers(3, 7))
+        cseed = int(rng.integers(0, 2**31))
+        out.append({"n": size, "shape": shape, "octaves": octaves, "seed": cseed})
+    return out
+
+
+def batch_chunks(manifest):
+    """ctypes Chunk array for a manifest."""
+    arr = (Chunk * len(manifest))()
+    for i, m in enumerate(manifest):
+        arr[i].spec = field_spec(m["shape"], BATCH_SHAPE_PARAMS[m["shape"]],
+                                 [layer("gradient", "fbm", m["octaves"], 2.0, 2.0, 0.5, 0.05, m["seed"])])
+        arr[i].grid = uniform_grid(-1.6, 1.6, m["n"])
+        arr[i].iso = 0.0
+    return arr
