@@ -10,8 +10,8 @@ marching cubes reads back to size its output.
 
 Prints one JSON line.  `value` = grid samples per second of the whole step (field +
 meshing) in Gsamples/s.  Also reported: the field kernel alone, marching cubes in
-cells/s, the fused forward+adjoint kernel, rooflines, and the CPU oracle timed on a
-bounded slab of the same grid (the reference binary cannot be built: its cage engine
+cells/s, the fused forward+adjoint kernel, the 64-chunk batch, the 1024^2 texel bake,
+rooflines, and the CPU oracle timed on the same grid (the reference binary cannot be built: its cage engine
 is absent).
 """
 from __future__ import annotations
@@ -41,8 +41,9 @@ def cpu_model() -> str:
 
 
 def cpu_baseline(spec, grid, seconds: float):
-    """The CPU oracle (FP32, -O3, OpenMP over z-slabs) on consecutive z-slabs of the
-    same grid until about `seconds` of work; returns Gsamples/s and the sample used."""
+    """The CPU oracle (FP32, -O3, OpenMP over z-slabs) on the same grid: the whole grid,
+    repeated to about `seconds` of work when one pass is shorter, else a centred block
+    of z-slabs of about `seconds`; plus a short single-thread run for the per-core rate."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -54,31 +55,99 @@ def cpu_baseline(spec, grid, seconds: float):
     threads = int(o.L.upfo_num_threads())
     nx, ny, nz = S.grid_dims(grid)
 
-    def run(k0, nslab):
+    def run(k0, nslab, reps=1):
         sub = S.Grid()
         for a in range(3):
             sub.origin[a], sub.spacing[a], sub.dims[a] = grid.origin[a], grid.spacing[a], grid.dims[a]
-        # slab k0..k0+nslab: shift the origin by whole nodes (same node coordinates as
-        # long as origin + k*spacing is recomputed; the slab is a timing sample)
+        # slabs k0..k0+nslab of the grid (the slab origin is rounded once more than the
+        # grid's own node coordinates; the slab is a timing sample, not a parity case)
         sub.origin[2] = np.float32(grid.origin[2]) + np.float32(k0) * np.float32(grid.spacing[2])
         sub.dims[2] = nslab
         t = time.perf_counter()
-        o.eval_grid(spec, sub)
+        for _ in range(reps):
+            o.eval_grid(spec, sub)
         return time.perf_counter() - t
 
+    nprobe = min(nz, 4 * threads)
+    per_slab = run(nz // 2 - nprobe // 2, nprobe) / nprobe
+    if per_slab * nz <= seconds:
+        reps = max(1, int(seconds / (per_slab * nz)))
+        t = run(0, nz, reps)
+        nslab = nz
+        what = f"the whole {nx}x{ny}x{nz} grid x{reps}"
+    else:
+        reps = 1
+        nslab = int(max(nprobe, min(nz, seconds / per_slab)))
+        t = run(max(0, nz // 2 - nslab // 2), nslab)
+        what = f"{nslab} central z-slabs of the {nx}x{ny}x{nz} grid"
+    samples = nx * ny * nslab * reps
+    # single thread, two slabs
+    o.L.upfo_set_num_threads(1)
     t1 = run(nz // 2, 2)
-    per_slab = t1 / 2
-    nslab = int(max(2, min(nz, seconds / max(per_slab, 1e-6))))
-    k0 = max(0, nz // 2 - nslab // 2)
-    t = run(k0, nslab)
-    samples = nx * ny * nslab
+    o.L.upfo_set_num_threads(threads)
     return {
         "value": samples / t / 1e9,
         "unit": "Gsamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{nslab} z-slabs of the {nx}x{ny}x{nz} grid ({samples} samples, {t:.1f} s) on {threads} "
-                  f"threads of '{cpu_model()}'",
+        "sample": f"{what} ({samples} samples, {t:.1f} s) on {threads} threads of '{cpu_model()}'",
+        "single_thread_value": 2 * nx * ny / t1 / 1e9,
+    }
+
+
+def bench_bake(upf, S, torch, reps: int):
+    """128^3 sphere + 4-octave gradient fBm (seed 1234) meshed, then baked into a 1024^2
+    atlas: atlas + rasterize + material, timed with HIP events on the launch stream."""
+    L = upf.lib()
+    s = S.spec_sphere_fbm(1234)
+    g = S.grid_sphere_fbm(128)
+    mesh = upf.mesh_generate_generic(s, g)
+    w = h = 1024
+    b = S.bake_spec(s)
+    T, V = mesh.n_triangles, mesh.n_vertices
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    uv = torch.empty((T, 3, 2), dtype=torch.float32, device="cuda")
+    tri = torch.empty((h, w), dtype=torch.int32, device="cuda")
+    albedo = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
+    bump = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    rough = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    P = [ctypes.c_void_p(t.data_ptr()) for t in (mesh.positions, mesh.normals, mesh.indices, uv, tri, albedo, bump,
+                                                 rough)]
+
+    def atlas():
+        upf.check(L.upf_atlas_per_triangle(T, w, h, P[3], sp), "atlas")
+
+    def raster():
+        upf.check(L.upf_bake_rasterize(P[3], T, w, h, P[4], sp), "rasterize")
+
+    def material():
+        upf.check(L.upf_bake(ctypes.byref(b), P[0], P[1], V, P[2], T, P[3], P[4], w, h, P[5], P[6], P[7], sp), "bake")
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    t_atlas, t_raster, t_mat = timed(atlas), timed(raster), timed(material)
+    covered = int((tri >= 0).sum())
+    # algorithmic bytes of k_bake: triangle id in + 20 B out per texel, the triangle's UV
+    # and indices, and the position and normal of its vertices
+    mbytes = w * h * (4 + 20) + T * (24 + 12) + V * 24
+    gbs = mbytes / t_mat / 1e9
+    return {
+        "workload": "128^3 sphere r=1 + 4-oct gradient fBm (seed 1234) -> marching cubes -> 1024^2 atlas",
+        "triangles": T, "vertices": V, "texels": w * h, "covered_texels": covered,
+        "atlas_us": round(t_atlas * 1e6, 2), "rasterize_us": round(t_raster * 1e6, 2),
+        "material_us": round(t_mat * 1e6, 2),
+        "gtexels_per_s": round(w * h / (t_atlas + t_raster + t_mat) / 1e9, 3),
+        "roofline": {"bound": "hbm", "kernel": "k_bake", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes": mbytes},
     }
 
 
@@ -156,7 +225,23 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-bake", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 3 times (the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = 256 * 1024 * 1024  # 1 GiB of float32
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_launch": n * 4, "launches": 3}), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the field and "
@@ -301,6 +386,9 @@ def main() -> int:
 
     if not args.no_batch:
         out["batch"] = bench_batch(upf, S, torch)
+
+    if not args.no_bake:
+        out["bake"] = bench_bake(upf, S, torch, max(5, args.steps))
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec, grid, args.cpu_seconds)

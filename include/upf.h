@@ -24,8 +24,8 @@
  *    allocates or frees caller memory.  Scratch comes in as an explicit workspace
  *    (upf_*_workspace_bytes tells how much).
  *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).  Calls are
- *    stream-ordered.  Calls that must return counts (upf_mc_extract, upf_bake_atlas)
- *    synchronise the stream before they return.
+ *    stream-ordered.  Calls that must return counts (upf_mc_extract, upf_mc_count,
+ *    upf_batch_totals) synchronise the stream before they return.
  *  - Errors are int status codes; nothing throws across the ABI.
  *  - No global mutable state: seeds live in the spec (the reference draws them from a
  *    process-global RNG, sources/math.cpp:73-77).
@@ -274,6 +274,57 @@ int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_
 /* Case code of every cell -> d_codes[dims[2]-1][dims[1]-1][dims[0]-1] (uint8). */
 int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,
                       uint8_t* d_codes, void* stream);
+
+/* ---- texel bake ---------------------------------------------------------------------
+ * Replaces meshGenerateTexture(mesh, {w, h, generator = Generator::pixel})
+ * (sources/textures.cpp:65-71): texels are rasterized in UV space (a texel belongs to
+ * the triangle whose UV triangle holds its centre ((x + 0.5)/w, (y + 0.5)/h); where UV
+ * triangles overlap the larger triangle id wins), then each covered texel interpolates
+ * position and normal (positionAt / normalAt, textures.cpp:27-28) and evaluates the
+ * material; uncovered texels stay NaN (textures.cpp:47-63).
+ *
+ * Material (a reduced, deterministic form of coloringDefault, coloringModes.cpp:905-930):
+ *   elev  = shape(p) / elevation_scale           (terrainSdfElevation, coloringModes.cpp:51-57)
+ *   slope = 1 - dot(n, p/|p|)                      (0 flat .. 2 overhang; the reference
+ *                                                  measures slope with 8 probes, :122-151)
+ *   d     = detail.weight * (the detail layer's fractal value at p)
+ *   albedo: sand -> grass -> rock -> snow by smoothstep bands of elev, cliff colour by a
+ *           smoothstep of slope, times (1 + detail_albedo * d); saturated (:895-902)
+ *   roughness = saturate(r0 + r_slope * slope + r_detail * d + r_snow * snow)
+ *   bump      = saturate(0.5 + b_detail * d + b_elev * elev)   (the reference's height map)
+ */
+typedef struct upf_bake_spec {
+  upf_field_spec field; /* shape and elevation_scale give elev */
+  upf_layer detail;     /* detail noise: weight * fractal value (blend ignored)             */
+  float bands[3];       /* elev of the sand|grass, grass|rock, rock|snow transitions        */
+  float band_width;     /* half width of each transition                                    */
+  float colors[5][3];   /* sand, grass, rock, snow, cliff (linear RGB)                      */
+  float cliff[2];       /* slope where the cliff colour starts and is full                  */
+  float detail_albedo;  /* albedo modulation by d                                           */
+  float roughness[4];   /* r0, r_slope, r_detail, r_snow                                    */
+  float bump[2];        /* b_detail, b_elev                                                 */
+  float reserved[2];
+} upf_bake_spec;
+
+/* Deterministic per-triangle atlas: triangles 2c and 2c+1 share square cell c (cells in
+ * row-major order, side = the largest integer number of texels that fits every cell in
+ * width x height); triangle 2c takes the lower-left half (0,0) (s,0) (0,s) of the cell's
+ * diagonal split, 2c+1 the upper-right half (s,s) (0,s) (s,0), corners on texel
+ * boundaries (texel centres on the diagonal go to 2c+1).  d_uv: float[n_triangles][3][2], UV per triangle corner in [0,1].
+ * UPF_ERR_CAPACITY if a cell would be smaller than 2 x 2 texels. */
+int upf_atlas_per_triangle(int64_t n_triangles, int32_t width, int32_t height, float* d_uv,
+                           void* stream);
+/* UV rasterization: d_tri[height][width] = covering triangle id or -1. */
+int upf_bake_rasterize(const float* d_uv, int64_t n_triangles, int32_t width, int32_t height,
+                       int32_t* d_tri, void* stream);
+/* Per-texel material.  Mesh: positions/normals float[V][3], indices uint32[T][3]; d_uv
+ * and d_tri from above.  Outputs: albedo float[h][w][3], bump and roughness float[h][w].
+ * A texel whose triangle id is outside [0, T) or whose triangle names a vertex outside
+ * [0, V) is written as uncovered (NaN). */
+int upf_bake(const upf_bake_spec* spec, const float* d_positions, const float* d_normals,
+             int64_t n_vertices, const uint32_t* d_indices, int64_t n_triangles, const float* d_uv,
+             const int32_t* d_tri, int32_t width, int32_t height, float* d_albedo, float* d_bump,
+             float* d_roughness, void* stream);
 
 /* ---- batched chunks ---------------------------------------------------------------
  * Many chunks (planets) of uneven grid sizes, shapes and seeds evaluated and meshed by

@@ -5,6 +5,7 @@
 
 #include "upf_oracle.hpp"
 #include "upf_oracle_mc.hpp"
+#include "upf_oracle_bake.hpp"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -183,6 +184,15 @@ void upfo_mc_copy(void* h, float* pos, float* nrm, uint32_t* idx) {
   if (idx) std::memcpy(idx, m->indices.data(), m->indices.size() * sizeof(uint32_t));
 }
 void upfo_mc_free(void* h) { delete static_cast<McMesh*>(h); }
+
+// ---- texel bake
+int upfo_atlas_side(int64_t n, int w, int h) { return atlas_side(n, w, h); }
+void upfo_atlas(int64_t n, int w, int h, float* uv) { atlas(n, w, h, uv); }
+void upfo_rasterize(const float* uv, int64_t n, int w, int h, int32_t* tri) { rasterize(uv, n, w, h, tri); }
+void upfo_bake(const upf_bake_spec* b, const float* P, const float* N, const uint32_t* I, const float* uv,
+               const int32_t* tri, int w, int h, float* albedo, float* bump, float* rough) {
+  bake(*b, P, N, I, uv, tri, w, h, albedo, bump, rough);
+}
 
 // ---- sources/math.cpp helpers (FP64), for closed-form checks
 double upfo_rescale(double v, double ia, double ib, double oa, double ob) { return rescale<double>(v, ia, ib, oa, ob); }

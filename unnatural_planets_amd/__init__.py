@@ -11,6 +11,7 @@ Reference functions mirrored (paths relative to the reference checkout):
   mesh_generate_base_land        sources/meshGeneration.cpp:47-55
   mesh_generate_base_water       sources/meshGeneration.cpp:57-98 (without the vertex mask)
   mesh_generate_base_navigation  sources/meshGeneration.cpp:100-107
+  generate_textures_land         sources/textures.cpp:45-71, 86-90 (rasterize + per-texel material)
 """
 from __future__ import annotations
 
@@ -19,8 +20,8 @@ from dataclasses import dataclass
 
 from . import spec as spec_mod
 from ._lib import UpfError, check, lib, status_string  # noqa: F401
-from .spec import (FieldSpec, Grid, Layer, MeshBuffers, copy_spec, field_spec, grid_dims,  # noqa: F401
-                   layer, uniform_grid)
+from .spec import (BakeSpec, FieldSpec, Grid, Layer, MeshBuffers, bake_spec, copy_spec,  # noqa: F401
+                   field_spec, grid_dims, layer, uniform_grid)
 
 __all__ = [
     "FieldSpec", "Grid", "Layer", "Mesh", "UpfError", "field_spec", "layer", "uniform_grid",
@@ -28,7 +29,8 @@ __all__ = [
     "mc_case_codes", "terrain_sdf_land", "terrain_sdf_water", "terrain_sdf_navigation",
     "terrain_sdf_elevation", "terrain_sdf_elevation_raw", "mesh_generate_generic",
     "mesh_generate_base_land", "mesh_generate_base_water", "mesh_generate_base_navigation",
-    "reference_grid", "flops_per_sample", "adjoint_flops_per_sample",
+    "reference_grid", "flops_per_sample", "adjoint_flops_per_sample", "BakeSpec", "bake_spec",
+    "Textures", "atlas_per_triangle", "bake_rasterize", "bake", "generate_textures_land",
 ]
 
 
@@ -264,3 +266,67 @@ def mesh_generate_base_navigation(s: FieldSpec, g: Grid | None = None) -> Mesh:
     if m.n_triangles == 0:
         raise UpfError(spec_mod.ERR_EMPTY_MESH, "generated empty base navigation mesh")
     return m
+
+
+# ---------------------------------------------------------------------------- texel bake
+@dataclass
+class Textures:
+    albedo: object     # float32 [h, w, 3]; NaN where no triangle covers the texel
+    bump: object       # float32 [h, w]
+    roughness: object  # float32 [h, w]
+    uv: object         # float32 [T, 3, 2]
+    tri: object        # int32 [h, w], covering triangle or -1
+
+
+def atlas_per_triangle(n_triangles: int, width: int, height: int, device=None, stream=None):
+    """Per-triangle UV atlas (upf_atlas_per_triangle) -> float32 [T, 3, 2]."""
+    torch = _torch()
+    uv = torch.empty((n_triangles, 3, 2), dtype=torch.float32, device=_dev(device))
+    check(lib().upf_atlas_per_triangle(n_triangles, width, height, _ptr(uv) if n_triangles else None,
+                                       _stream(stream)), "upf_atlas_per_triangle")
+    return uv
+
+
+def bake_rasterize(uv, width: int, height: int, stream=None):
+    """Covering triangle per texel (upf_bake_rasterize) -> int32 [h, w], -1 where uncovered."""
+    torch = _torch()
+    _check_f32(uv, "uv")
+    if uv.dim() != 3 or tuple(uv.shape[1:]) != (3, 2):
+        raise ValueError("uv must have shape [T, 3, 2]")
+    tri = torch.empty((height, width), dtype=torch.int32, device=uv.device)
+    n = int(uv.shape[0])
+    check(lib().upf_bake_rasterize(_ptr(uv) if n else None, n, width, height, _ptr(tri), _stream(stream)),
+          "upf_bake_rasterize")
+    return tri
+
+
+def bake(b: BakeSpec, mesh: Mesh, uv, tri, stream=None):
+    """Per-texel material (upf_bake) -> (albedo [h, w, 3], bump [h, w], roughness [h, w])."""
+    torch = _torch()
+    h, w = (int(x) for x in tri.shape)
+    n = mesh.n_triangles
+    _check_f32(mesh.positions, "positions")
+    if mesh.normals is None:
+        raise ValueError("the bake needs vertex normals")
+    _check_f32(mesh.normals, "normals", tuple(mesh.positions.shape))
+    _check_f32(uv, "uv", (n, 3, 2))
+    if tri.dtype != torch.int32 or not tri.is_contiguous() or not tri.is_cuda:
+        raise ValueError("tri must be a contiguous int32 CUDA tensor")
+    dev = tri.device
+    albedo = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
+    bump = torch.empty((h, w), dtype=torch.float32, device=dev)
+    rough = torch.empty((h, w), dtype=torch.float32, device=dev)
+    nz = n > 0
+    check(lib().upf_bake(ctypes.byref(b), _ptr(mesh.positions) if nz else None, _ptr(mesh.normals) if nz else None,
+                         mesh.n_vertices, _ptr(mesh.indices) if nz else None, n, _ptr(uv) if nz else None, _ptr(tri), w, h,
+                         _ptr(albedo), _ptr(bump), _ptr(rough), _stream(stream)), "upf_bake")
+    return albedo, bump, rough
+
+
+def generate_textures_land(b: BakeSpec, mesh: Mesh, width: int, height: int, stream=None) -> Textures:
+    """Atlas + rasterize + per-texel material for a land mesh (sources/textures.cpp:45-71,
+    86-90): the reference's Generator<false>::generate up to its dilation and U8 steps."""
+    uv = atlas_per_triangle(mesh.n_triangles, width, height, mesh.positions.device, stream)
+    tri = bake_rasterize(uv, width, height, stream)
+    albedo, bump, rough = bake(b, mesh, uv, tri, stream)
+    return Textures(albedo, bump, rough, uv, tri)

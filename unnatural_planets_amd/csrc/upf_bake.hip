@@ -1,0 +1,234 @@
+// upf_bake.hip -- UV atlas, UV rasterization and the per-texel material bake.
+//
+// Replaces meshGenerateTexture(mesh, {w, h, generator = Generator::pixel}) and the
+// pixel callback (sources/textures.cpp:24-43, 65-71).  The per-texel material is the
+// reduced, deterministic form of coloringDefault documented at upf_bake_spec
+// (sources/coloringModes.cpp:905-930).
+//
+// The bake is HBM-bound per texel (triangle id 4 B + output 20 B, plus the mesh rows it
+// touches, which stay in cache since neighbouring texels share triangles) once the
+// material is light; the detail noise adds about 60 flops per octave.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "upf_device.hpp"
+#include "upf_internal.hpp"
+
+namespace upf {
+
+struct DBake {
+  DSpec field;
+  DLayer detail;
+  float bands[3], band_width;
+  float colors[5][3];
+  float cliff[2];
+  float detail_albedo;
+  float roughness[4];
+  float bump[2];
+};
+
+// Edge function without fused operations, so every rounding matches the CPU restatement.
+__device__ __forceinline__ float edge_fn(float ax, float ay, float bx, float by, float px, float py) {
+  const float u = (bx - ax) * (py - ay);
+  const float v = (by - ay) * (px - ax);
+  return u - v;
+}
+
+__global__ __launch_bounds__(256) void k_atlas(int64_t n, int side, int cols, float inv_w, float inv_h,
+                                               float* __restrict__ uv) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const int64_t c = t >> 1;
+  const float x0 = (float)((int)(c % cols) * side), y0 = (float)((int)(c / cols) * side);
+  const float s = (float)side;
+  float px[3], py[3];
+  if ((t & 1) == 0) {  // lower-left half: (0,0) (s,0) (0,s)
+    px[0] = x0; py[0] = y0; px[1] = x0 + s; py[1] = y0; px[2] = x0; py[2] = y0 + s;
+  } else {             // upper-right half: (s,s) (0,s) (s,0)
+    px[0] = x0 + s; py[0] = y0 + s; px[1] = x0; py[1] = y0 + s; px[2] = x0 + s; py[2] = y0;
+  }
+  for (int v = 0; v < 3; v++) {
+    uv[6 * t + 2 * v] = px[v] * inv_w;
+    uv[6 * t + 2 * v + 1] = py[v] * inv_h;
+  }
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// One thread per triangle: every texel centre of its bounding box inside (or on) the UV
+// triangle gets max(current, t).
+__global__ __launch_bounds__(256) void k_rasterize(const float* __restrict__ uv, int64_t n, int w, int h,
+                                                   int32_t* __restrict__ tri) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float ax = uv[6 * t] * (float)w, ay = uv[6 * t + 1] * (float)h;
+  const float bx = uv[6 * t + 2] * (float)w, by = uv[6 * t + 3] * (float)h;
+  const float cx = uv[6 * t + 4] * (float)w, cy = uv[6 * t + 5] * (float)h;
+  const float area = edge_fn(ax, ay, bx, by, cx, cy);
+  if (!(area != 0.0f) || !isfinite(area)) return;
+  const float sgn = area > 0.0f ? 1.0f : -1.0f;
+  const float lx = fminf(ax, fminf(bx, cx)), hx = fmaxf(ax, fmaxf(bx, cx));
+  const float ly = fminf(ay, fminf(by, cy)), hy = fmaxf(ay, fmaxf(by, cy));
+  const int x0 = max(0, (int)floorf(lx - 0.5f)), x1 = min(w - 1, (int)ceilf(hx - 0.5f));
+  const int y0 = max(0, (int)floorf(ly - 0.5f)), y1 = min(h - 1, (int)ceilf(hy - 0.5f));
+  for (int y = y0; y <= y1; y++)
+    for (int x = x0; x <= x1; x++) {
+      const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+      // texel centres outside the bounding box are outside, whatever the edge functions round to
+      if (px < lx || px > hx || py < ly || py > hy) continue;
+      const float w0 = sgn * edge_fn(bx, by, cx, cy, px, py);
+      const float w1 = sgn * edge_fn(cx, cy, ax, ay, px, py);
+      const float w2 = sgn * edge_fn(ax, ay, bx, by, px, py);
+      if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f) atomicMax(tri + (int64_t)y * w + x, (int32_t)t);
+    }
+}
+
+__device__ __forceinline__ float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+__device__ __forceinline__ float smoothstep_(float e0, float e1, float x) {
+  const float t = sat((x - e0) / (e1 - e0));
+  return t * t * (3.0f - 2.0f * t);
+}
+__device__ __forceinline__ float mix_(float a, float b, float t) { return a + t * (b - a); }
+
+__global__ __launch_bounds__(256) void k_bake(const DBake b, const float* __restrict__ P, const float* __restrict__ N,
+                                              int64_t nv, const uint32_t* __restrict__ I, int64_t nt,
+                                              const float* __restrict__ uv, const int32_t* __restrict__ tri, int w,
+                                              int h, float* __restrict__ albedo, float* __restrict__ bump,
+                                              float* __restrict__ rough) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= w || y >= h) return;
+  const int64_t o = (int64_t)y * w + x;
+  const int32_t t = tri[o];
+  uint32_t i0 = 0, i1 = 0, i2 = 0;
+  const bool ok = t >= 0 && t < nt;
+  if (ok) {
+    i0 = I[3 * (int64_t)t];
+    i1 = I[3 * (int64_t)t + 1];
+    i2 = I[3 * (int64_t)t + 2];
+  }
+  if (!ok || i0 >= nv || i1 >= nv || i2 >= nv) {
+    const float nan = __builtin_nanf("");
+    albedo[3 * o] = albedo[3 * o + 1] = albedo[3 * o + 2] = nan;
+    bump[o] = nan;
+    rough[o] = nan;
+    return;
+  }
+  // barycentrics of the texel centre in the UV triangle
+  const float ax = uv[6 * (int64_t)t] * (float)w, ay = uv[6 * (int64_t)t + 1] * (float)h;
+  const float bx = uv[6 * (int64_t)t + 2] * (float)w, by = uv[6 * (int64_t)t + 3] * (float)h;
+  const float cx = uv[6 * (int64_t)t + 4] * (float)w, cy = uv[6 * (int64_t)t + 5] * (float)h;
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  const float area = edge_fn(ax, ay, bx, by, cx, cy);
+  const float w0 = edge_fn(bx, by, cx, cy, px, py) / area;
+  const float w1 = edge_fn(cx, cy, ax, ay, px, py) / area;
+  const float w2 = edge_fn(ax, ay, bx, by, px, py) / area;
+  float p[3], n[3];
+  for (int a = 0; a < 3; a++) {
+    p[a] = w0 * P[3 * (int64_t)i0 + a] + w1 * P[3 * (int64_t)i1 + a] + w2 * P[3 * (int64_t)i2 + a];
+    n[a] = w0 * N[3 * (int64_t)i0 + a] + w1 * N[3 * (int64_t)i1 + a] + w2 * N[3 * (int64_t)i2 + a];
+  }
+  const float nl = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  const float pl = sqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+  float cosang = 1.0f;
+  if (nl > 0.0f && pl > 0.0f) cosang = (n[0] * p[0] + n[1] * p[1] + n[2] * p[2]) / (nl * pl);
+  const float slope = 1.0f - cosang;
+  const float elev = shape_eval(b.field, p[0], p[1], p[2]) / b.field.escale;
+  const float d = b.detail.weight * layer_value(b.detail, p[0], p[1], p[2]);
+  const float s1 = smoothstep_(b.bands[0] - b.band_width, b.bands[0] + b.band_width, elev);
+  const float s2 = smoothstep_(b.bands[1] - b.band_width, b.bands[1] + b.band_width, elev);
+  const float s3 = smoothstep_(b.bands[2] - b.band_width, b.bands[2] + b.band_width, elev);
+  const float cl = smoothstep_(b.cliff[0], b.cliff[1], slope);
+  const float m = 1.0f + b.detail_albedo * d;
+  for (int a = 0; a < 3; a++) {
+    float c = mix_(b.colors[0][a], b.colors[1][a], s1);
+    c = mix_(c, b.colors[2][a], s2);
+    c = mix_(c, b.colors[3][a], s3);
+    c = mix_(c, b.colors[4][a], cl);
+    albedo[3 * o + a] = sat(c * m);
+  }
+  rough[o] = sat(b.roughness[0] + b.roughness[1] * slope + b.roughness[2] * d + b.roughness[3] * s3);
+  bump[o] = sat(0.5f + b.bump[0] * d + b.bump[1] * elev);
+}
+
+int atlas_side(int64_t n, int w, int h) {
+  const int64_t cells = (n + 1) / 2;
+  int best = 0;
+  for (int s = 2; s <= (w < h ? w : h); s++)
+    if ((int64_t)(w / s) * (h / s) >= cells) best = s;
+  return best;
+}
+
+}  // namespace upf
+
+using namespace upf;
+
+extern "C" {
+
+int upf_atlas_per_triangle(int64_t n, int32_t w, int32_t h, float* d_uv, void* stream) {
+  if (n < 0 || w < 2 || h < 2) return UPF_ERR_BAD_GRID;
+  if (n == 0) return UPF_OK;
+  if (!d_uv) return UPF_ERR_NULL;
+  const int side = atlas_side(n, w, h);
+  if (side < 2) return UPF_ERR_CAPACITY;
+  hipLaunchKernelGGL(k_atlas, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), n, side, w / side,
+                     1.0f / (float)w, 1.0f / (float)h, d_uv);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_bake_rasterize(const float* d_uv, int64_t n, int32_t w, int32_t h, int32_t* d_tri, void* stream) {
+  if (n < 0 || w < 1 || h < 1) return UPF_ERR_BAD_GRID;
+  if (!d_tri || (n > 0 && !d_uv)) return UPF_ERR_NULL;
+  if (n >= ((int64_t)1 << 31)) return UPF_ERR_BAD_GRID;
+  const int64_t texels = (int64_t)w * h;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((texels + 255) / 256)), dim3(256), 0, s, d_tri, texels, -1);
+  if (n > 0) hipLaunchKernelGGL(k_rasterize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_uv, n, w, h, d_tri);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_bake(const upf_bake_spec* spec, const float* d_P, const float* d_N, int64_t nv, const uint32_t* d_I,
+             int64_t n, const float* d_uv, const int32_t* d_tri, int32_t w, int32_t h, float* d_albedo,
+             float* d_bump, float* d_rough, void* stream) {
+  if (!spec || !d_tri || !d_albedo || !d_bump || !d_rough) return UPF_ERR_NULL;
+  if (n > 0 && (!d_P || !d_N || !d_I || !d_uv)) return UPF_ERR_NULL;
+  if (w < 1 || h < 1 || n < 0 || nv < 0) return UPF_ERR_BAD_GRID;
+  DBake b;
+  int st = to_device(&spec->field, &b.field);
+  if (st) return st;
+  if (spec->field.elevation_scale == 0.0f) return UPF_ERR_INVALID_VALUE;
+  upf_field_spec tmp = spec->field;  // validate the detail layer as a one-layer spec
+  tmp.n_layers = 1;
+  tmp.layers[0] = spec->detail;
+  DSpec dd;
+  if ((st = to_device(&tmp, &dd))) return st;
+  b.detail = dd.L[0];
+  const float* fl[] = {spec->bands, &spec->band_width, &spec->colors[0][0], spec->cliff, &spec->detail_albedo,
+                       spec->roughness, spec->bump};
+  const int nf[] = {3, 1, 15, 2, 1, 4, 2};
+  for (int i = 0; i < 7; i++)
+    for (int j = 0; j < nf[i]; j++)
+      if (!std::isfinite(fl[i][j])) return UPF_ERR_INVALID_VALUE;
+  for (int i = 0; i < 3; i++) b.bands[i] = spec->bands[i];
+  b.band_width = spec->band_width;
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 3; j++) b.colors[i][j] = spec->colors[i][j];
+  b.cliff[0] = spec->cliff[0];
+  b.cliff[1] = spec->cliff[1];
+  b.detail_albedo = spec->detail_albedo;
+  for (int i = 0; i < 4; i++) b.roughness[i] = spec->roughness[i];
+  b.bump[0] = spec->bump[0];
+  b.bump[1] = spec->bump[1];
+  hipLaunchKernelGGL(k_bake, dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, as_stream(stream), b, d_P, d_N, nv, d_I, n,
+                     d_uv, d_tri, w, h, d_albedo, d_bump, d_rough);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+}  // extern "C"

@@ -229,3 +229,42 @@ def batch_chunks(manifest):
         arr[i].grid = uniform_grid(-1.6, 1.6, m["n"])
         arr[i].iso = 0.0
     return arr
+
+
+# ---------------------------------------------------------------------------- texel bake
+class BakeSpec(ctypes.Structure):
+    _fields_ = [("field", FieldSpec), ("detail", Layer), ("bands", ctypes.c_float * 3),
+                ("band_width", ctypes.c_float), ("colors", (ctypes.c_float * 3) * 5),
+                ("cliff", ctypes.c_float * 2), ("detail_albedo", ctypes.c_float),
+                ("roughness", ctypes.c_float * 4), ("bump", ctypes.c_float * 2),
+                ("reserved", ctypes.c_float * 2)]
+
+
+assert ctypes.sizeof(BakeSpec) == ctypes.sizeof(FieldSpec) + 96 + 4 * (3 + 1 + 15 + 2 + 1 + 4 + 2 + 2)
+
+# sand, grass, rock, snow, cliff: linear RGB
+DEFAULT_COLORS = ((0.76, 0.70, 0.50), (0.25, 0.45, 0.15), (0.45, 0.40, 0.35), (0.95, 0.95, 0.97),
+                  (0.35, 0.30, 0.28))
+
+
+def bake_spec(field: FieldSpec, detail: Layer | None = None, bands=(-0.01, 0.005, 0.025), band_width=0.005,
+              colors=DEFAULT_COLORS, cliff=(0.05, 0.25), detail_albedo=0.3, roughness=(0.6, 0.5, 0.2, -0.3),
+              bump=(0.2, 5.0)) -> BakeSpec:
+    """Material of the texel bake.  The default bands suit a land field whose surface sits
+    within about +-0.05 of the base shape (elev = shape(p)/elevation_scale on the surface
+    is the elevation stack's value there)."""
+    b = BakeSpec()
+    b.field = field
+    b.detail = detail if detail is not None else layer("gradient", "fbm", 3, 16.0, 2.0, 0.5, 1.0, 977)
+    for i in range(3):
+        b.bands[i] = bands[i]
+    b.band_width = band_width
+    for i in range(5):
+        for j in range(3):
+            b.colors[i][j] = colors[i][j]
+    b.cliff[0], b.cliff[1] = cliff
+    b.detail_albedo = detail_albedo
+    for i in range(4):
+        b.roughness[i] = roughness[i]
+    b.bump[0], b.bump[1] = bump
+    return b
