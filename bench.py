@@ -215,6 +215,122 @@ def bench_batch(upf, S, torch):
     }
 
 
+def nb_planets():
+    """Eight planets of mixed shapes and seeds: a 5-octave gradient fBm (f0 2.0,
+    lacunarity 2.0, gain 0.5, weight 0.05) on sphere, rounded cube, torus and capsule."""
+    from unnatural_planets_amd import spec as S
+    out = []
+    for p in range(8):
+        shape = S.BATCH_SHAPES[p % 4]
+        out.append((f"{shape}/seed {1000 + p}",
+                    S.field_spec(shape, S.BATCH_SHAPE_PARAMS[shape],
+                                 [S.layer("gradient", "fbm", 5, 2.0, 2.0, 0.5, 0.05, 1000 + p)])))
+    return out
+
+
+def bench_nb(upf, S, torch, reps: int, n: int = 1025, tex: int = 4096):
+    """Narrow band: per planet a coarse pass over the 8-cell block corners of an n^3 fine
+    grid over [-1.6, 1.6]^3, fine evaluation and meshing of the active blocks (analytic
+    normals), then a tex^2 per-triangle atlas bake.  Buffers are sized by a first pass;
+    the timed passes then run all planets back to back."""
+    import numpy as np
+    L = upf.lib()
+    g = S.uniform_grid(-1.6, 1.6, n)
+    diag = S.NB_BLOCK * float(np.linalg.norm([g.spacing[0], g.spacing[1], g.spacing[2]]))
+    band = 2.0 * diag / 2  # |grad f| < 2; a block's points lie within diag/2 of a corner
+    planets = nb_planets()
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    # sizing pass
+    max_act = max_v = max_t = 0
+    ws = None
+    for _, s in planets:
+        m, _, st = upf.nb_extract(s, g, band, workspace=ws)
+        max_act, max_v, max_t = max(max_act, st.active_blocks), max(max_v, st.n_vertices), max(max_t, st.n_triangles)
+        ws = upf.NbWorkspace(g, max_act)
+        del m
+    pos = torch.empty((max_v, 3), dtype=torch.float32, device="cuda")
+    nrm = torch.empty_like(pos)
+    idx = torch.empty((max_t, 3), dtype=torch.int32, device="cuda")
+    uv = torch.empty((max_t, 3, 2), dtype=torch.float32, device="cuda")
+    tri = torch.empty((tex, tex), dtype=torch.int32, device="cuda")
+    albedo = torch.empty((tex, tex, 3), dtype=torch.float32, device="cuda")
+    bump = torch.empty((tex, tex), dtype=torch.float32, device="cuda")
+    rough = torch.empty((tex, tex), dtype=torch.float32, device="cuda")
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    bakes = [S.bake_spec(s) for _, s in planets]
+    stats = [S.NbStats() for _ in planets]
+
+    def mesh_one(i):
+        mb = S.MeshBuffers()
+        mb.positions, mb.normals, mb.indices = pos.data_ptr(), nrm.data_ptr(), idx.data_ptr()
+        mb.cap_vertices, mb.cap_triangles = max_v, max_t
+        upf.check(L.upf_nb_extract(ctypes.byref(planets[i][1]), ctypes.byref(g), 0.0, band, ctypes.byref(mb), None,
+                                   vp(ws.buf), ws.bytes, ctypes.byref(stats[i]), sp), "nb_extract")
+
+    def bake_one(i):
+        T, V = int(stats[i].n_triangles), int(stats[i].n_vertices)
+        upf.check(L.upf_atlas_per_triangle(T, tex, tex, vp(uv), sp), "atlas")
+        upf.check(L.upf_bake_rasterize(vp(uv), T, tex, tex, vp(tri), sp), "rasterize")
+        upf.check(L.upf_bake(ctypes.byref(bakes[i]), vp(pos), vp(nrm), V, vp(idx), T, vp(uv), vp(tri), tex, tex,
+                             vp(albedo), vp(bump), vp(rough), sp), "bake")
+
+    for i in range(len(planets)):  # warm-up
+        mesh_one(i)
+        bake_one(i)
+    torch.cuda.synchronize()
+    t_mesh = t_bake = 0.0
+    for _ in range(reps):
+        for i in range(len(planets)):
+            t0 = time.perf_counter()
+            mesh_one(i)  # synchronises twice inside (count read-backs)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            bake_one(i)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            t_mesh += t1 - t0
+            t_bake += t2 - t1
+    npl = len(planets) * reps
+    t_mesh, t_bake = t_mesh / npl, t_bake / npl
+    act = [int(st.active_blocks) for st in stats]
+    blocks = int(stats[0].blocks)
+    del pos, nrm, idx, uv, tri, albedo, bump, rough, ws
+    torch.cuda.empty_cache()
+
+    # the dense sweep of the same fine grid for planet 0, for comparison
+    s0 = planets[0][1]
+    field = torch.empty((n, n, n), dtype=torch.float32, device="cuda")
+    dws = upf.McWorkspace(g)
+    nv, nt = ctypes.c_int64(), ctypes.c_int64()
+
+    def dense():
+        upf.check(L.upf_field_eval_grid(ctypes.byref(s0), ctypes.byref(g), vp(field), sp), "eval")
+        upf.check(L.upf_mc_count(vp(field), ctypes.byref(g), 0.0, vp(dws.buf), dws.bytes, ctypes.byref(nv),
+                                 ctypes.byref(nt), sp), "count")
+    dense()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dense()
+    torch.cuda.synchronize()
+    t_dense = (time.perf_counter() - t0) / reps
+    del field, dws
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"8 planets (sphere, rounded cube, torus, capsule x2; 5-oct gradient fBm, seeds 1000-1007), "
+                    f"{n}^3 fine grid over [-1.6,1.6]^3 in 8^3-cell blocks, band {band:.4f}, {tex}^2 bake",
+        "fine_nodes": n ** 3, "blocks": blocks, "active_blocks_mean": round(float(np.mean(act)), 1),
+        "active_fraction_mean": round(float(np.mean(act)) / blocks, 5),
+        "triangles_mean": round(float(np.mean([st.n_triangles for st in stats])), 1),
+        "mesh_ms_per_planet": round(t_mesh * 1e3, 3), "bake_ms_per_planet": round(t_bake * 1e3, 3),
+        "latency_ms_per_planet": round((t_mesh + t_bake) * 1e3, 3),
+        "effective_gsamples_per_s": round(n ** 3 / t_mesh / 1e9, 2),
+        "dense_field_count_ms_planet0": round(t_dense * 1e3, 3),
+        "speedup_mesh_vs_dense_sweep": round(t_dense / t_mesh, 2),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +342,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-bake", action="store_true")
+    ap.add_argument("--no-nb", action="store_true", help="skip the narrow-band 8-planet workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -389,6 +506,9 @@ def main() -> int:
 
     if not args.no_bake:
         out["bake"] = bench_bake(upf, S, torch, max(5, args.steps))
+
+    if not args.no_nb:
+        out["narrow_band"] = bench_nb(upf, S, torch, 2)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec, grid, args.cpu_seconds)

@@ -71,8 +71,9 @@ enum upf_status {
   UPF_ERR_BAD_GRID = 5,      /* dims < 2 or too large, spacing not finite */
   UPF_ERR_HIP = 6,           /* a HIP runtime call failed */
   UPF_ERR_NULL = 7,          /* a required pointer is NULL */
-  UPF_ERR_EMPTY_MESH = 8     /* used by host mirrors: meshGenerateBase* on an empty mesh
-                                (meshGeneration.cpp:250-251, 303-304) */
+  UPF_ERR_EMPTY_MESH = 8,    /* used by host mirrors: meshGenerateBase* on an empty mesh
+                                (meshGeneration.cpp:50-51, 104-105) */
+  UPF_ERR_BAND = 9           /* narrow band: the surface leaves the active blocks     */
 };
 
 /* ---- spec enums ---------------------------------------------------------------- */
@@ -274,6 +275,46 @@ int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_
 /* Case code of every cell -> d_codes[dims[2]-1][dims[1]-1][dims[0]-1] (uint8). */
 int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,
                       uint8_t* d_codes, void* stream);
+
+/* ---- narrow band ----------------------------------------------------------------------
+ * Marching cubes of a fine grid evaluated only near the surface (no counterpart in the
+ * reference, which sweeps the dense grid, sources/meshGeneration.cpp:35-39).  The fine
+ * grid's cells are cut into blocks of UPF_NB_BLOCK^3 cells ((dims[a] - 1) must be a
+ * multiple of UPF_NB_BLOCK).  The field is first evaluated at the block corners (the fine
+ * nodes whose indices are multiples of UPF_NB_BLOCK); a block is active when one of its
+ * 8 corner values lies within `band` of iso (|f - iso| < band) or its corners straddle
+ * iso.  Active blocks are evaluated at every fine node and meshed.
+ *
+ * The mesh is the dense marching-cubes mesh of the fine grid (same vertex positions, same
+ * triangles, per-lane field values) whenever every block the surface passes through is
+ * active -- guaranteed when band >= L * (block diagonal) / 2 for a Lipschitz bound L of
+ * f (every point of a block lies within half its diagonal of one of its corners).
+ * Order differs from upf_mc_extract: vertices by (active block, node within the block's
+ * 9^3 brick, axis x<y<z), triangles by (active block, cell).  Normals are the normalized
+ * analytic gradient of f at each vertex (the fused forward + adjoint evaluation), not the
+ * interpolated central differences of upf_mc_emit.
+ *
+ * d_vertex_keys (optional): per vertex the edge it lies on, (node linear index) * 3 + axis
+ * in the fine grid -- a canonical id, equal for the same vertex in any extraction.
+ * Errors: UPF_ERR_WORKSPACE (more active blocks than the workspace holds; stats->
+ * active_blocks written), UPF_ERR_CAPACITY (counts written to stats and to out),
+ * UPF_ERR_BAND (a crossing edge on an active block's face belongs to an inactive block:
+ * band too narrow; stats->unresolved written).  Synchronises the stream twice (16 B
+ * and 32 B read-backs). */
+#define UPF_NB_BLOCK 8
+typedef struct upf_nb_stats {
+  int64_t blocks;        /* all blocks of the fine grid                         */
+  int64_t active_blocks; /* blocks evaluated at fine resolution                  */
+  int64_t n_vertices;
+  int64_t n_triangles;
+  int64_t unresolved;    /* crossing face edges whose owning block is inactive   */
+  int64_t reserved[3];
+} upf_nb_stats;
+
+size_t upf_nb_workspace_bytes(const upf_grid* fine, int64_t max_active_blocks);
+int upf_nb_extract(const upf_field_spec* spec, const upf_grid* fine, float iso, float band,
+                   upf_mesh_buffers* out, uint64_t* d_vertex_keys, void* d_workspace,
+                   size_t workspace_bytes, upf_nb_stats* stats, void* stream);
 
 /* ---- texel bake ---------------------------------------------------------------------
  * Replaces meshGenerateTexture(mesh, {w, h, generator = Generator::pixel})

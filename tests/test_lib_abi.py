@@ -92,6 +92,27 @@ def test_mc_workspace_size(upf):
     assert L.upf_mc_workspace_bytes(ctypes.byref(bad)) == 0
 
 
+def test_nb_workspace_size(upf):
+    L = upf.lib()
+    g = S.uniform_grid(-1, 1, 129)  # 16^3 blocks
+    small, big = L.upf_nb_workspace_bytes(ctypes.byref(g), 10), L.upf_nb_workspace_bytes(ctypes.byref(g), 110)
+    # per active block: a 9^3 FP32 brick, a 9^3 uint16 node table and three words (each
+    # array rounded up to 256 B)
+    assert abs((big - small) - 100 * (729 * 6 + 12)) < 5 * 256
+    assert L.upf_nb_workspace_bytes(ctypes.byref(g), 10 ** 9) == L.upf_nb_workspace_bytes(ctypes.byref(g), 4096)
+    bad = S.uniform_grid(-1, 1, 130)  # 129 cells: not a whole number of 8-cell blocks
+    assert L.upf_nb_workspace_bytes(ctypes.byref(bad), 10) == 0
+    assert L.upf_nb_workspace_bytes(ctypes.byref(g), -1) == 0
+    mb, st = S.MeshBuffers(), S.NbStats()
+    s = S.spec_sphere_fbm()
+    assert L.upf_nb_extract(ctypes.byref(s), ctypes.byref(bad), 0.0, 0.1, ctypes.byref(mb), None, None, 0,
+                            ctypes.byref(st), None) == S.ERR_BAD_GRID
+    assert L.upf_nb_extract(ctypes.byref(s), ctypes.byref(g), 0.0, -1.0, ctypes.byref(mb), None, None, 0,
+                            ctypes.byref(st), None) == S.ERR_INVALID_VALUE
+    assert L.upf_nb_extract(ctypes.byref(s), ctypes.byref(g), 0.0, 0.1, ctypes.byref(mb), None, None, 0,
+                            ctypes.byref(st), None) == S.ERR_WORKSPACE
+
+
 def test_null_and_grid_errors_without_gpu(upf):
     L = upf.lib()
     s = S.spec_sphere_fbm()

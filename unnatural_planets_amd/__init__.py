@@ -31,6 +31,7 @@ __all__ = [
     "mesh_generate_base_land", "mesh_generate_base_water", "mesh_generate_base_navigation",
     "reference_grid", "flops_per_sample", "adjoint_flops_per_sample", "BakeSpec", "bake_spec",
     "Textures", "atlas_per_triangle", "bake_rasterize", "bake", "generate_textures_land",
+    "NbWorkspace", "nb_extract",
 ]
 
 
@@ -266,6 +267,55 @@ def mesh_generate_base_navigation(s: FieldSpec, g: Grid | None = None) -> Mesh:
     if m.n_triangles == 0:
         raise UpfError(spec_mod.ERR_EMPTY_MESH, "generated empty base navigation mesh")
     return m
+
+
+# ---------------------------------------------------------------------------- narrow band
+class NbWorkspace:
+    """Narrow-band workspace for one fine grid shape, holding up to `max_active` blocks."""
+
+    def __init__(self, g: Grid, max_active: int, device=None):
+        torch = _torch()
+        self.max_active = int(max_active)
+        self.bytes = int(lib().upf_nb_workspace_bytes(ctypes.byref(g), self.max_active))
+        if self.bytes == 0:
+            raise UpfError(spec_mod.ERR_BAD_GRID, "upf_nb_workspace_bytes")
+        self.buf = torch.empty((self.bytes,), dtype=torch.uint8, device=_dev(device))
+
+
+def nb_extract(s: FieldSpec, g: Grid, band: float, iso: float = 0.0, normals: bool = True, keys: bool = False,
+               workspace: NbWorkspace | None = None, device=None, stream=None):
+    """Narrow-band marching cubes of the fine grid g (upf_nb_extract).  Grows the
+    workspace and the mesh buffers as the library reports (each retry reruns the
+    extraction).  Returns (Mesh, keys or None, NbStats)."""
+    torch = _torch()
+    dev = _dev(device)
+    ws = workspace or NbWorkspace(g, 1024, dev)
+    stats = spec_mod.NbStats()
+    cap_v = cap_t = 0
+    st = _stream(stream)
+    for _ in range(4):
+        pos = torch.empty((cap_v, 3), dtype=torch.float32, device=dev)
+        nrm = torch.empty((cap_v, 3), dtype=torch.float32, device=dev) if normals else None
+        idx = torch.empty((cap_t, 3), dtype=torch.int32, device=dev)
+        kk = torch.empty((cap_v,), dtype=torch.int64, device=dev) if keys else None
+        mb = MeshBuffers()
+        mb.positions = pos.data_ptr() if cap_v else None
+        mb.normals = nrm.data_ptr() if (normals and cap_v) else None
+        mb.indices = idx.data_ptr() if cap_t else None
+        mb.cap_vertices, mb.cap_triangles = cap_v, cap_t
+        r = lib().upf_nb_extract(ctypes.byref(s), ctypes.byref(g), iso, band, ctypes.byref(mb),
+                                 _ptr(kk) if (keys and cap_v) else None, _ptr(ws.buf), ws.bytes,
+                                 ctypes.byref(stats), st)
+        if r == spec_mod.ERR_WORKSPACE and stats.active_blocks > ws.max_active:
+            ws = NbWorkspace(g, stats.active_blocks, dev)
+            continue
+        if r == spec_mod.ERR_CAPACITY:
+            cap_v, cap_t = int(stats.n_vertices), int(stats.n_triangles)
+            continue
+        check(r, "upf_nb_extract")
+        n_v, n_t = int(stats.n_vertices), int(stats.n_triangles)
+        return Mesh(pos[:n_v], nrm[:n_v] if normals else None, idx[:n_t]), (kk[:n_v] if keys else None), stats
+    raise UpfError(spec_mod.ERR_CAPACITY, "upf_nb_extract did not converge")
 
 
 # ---------------------------------------------------------------------------- texel bake

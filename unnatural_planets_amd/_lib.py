@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .spec import BakeSpec, BatchOpts, Chunk, FieldSpec, Grid, Layer, MeshBuffers
+from .spec import BakeSpec, BatchOpts, Chunk, FieldSpec, Grid, Layer, MeshBuffers, NbStats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libupf.so")
@@ -67,6 +67,10 @@ SIGNATURES = {
     "upf_batch_autotune": (ctypes.c_int, _BATCH_RUN_ARGS + [ctypes.POINTER(BatchOpts)]),
     "upf_mc_case_codes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_void_p,
                                          ctypes.c_void_p]),
+    "upf_nb_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(Grid), ctypes.c_int64]),
+    "upf_nb_extract": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_float, ctypes.c_float,
+                                      ctypes.POINTER(MeshBuffers), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.POINTER(NbStats), ctypes.c_void_p]),
     "upf_atlas_per_triangle": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                                               ctypes.c_void_p]),
     "upf_bake_rasterize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,

@@ -203,6 +203,7 @@ const char* upf_status_string(int status) {
     case UPF_ERR_HIP: return "HIP runtime error";
     case UPF_ERR_NULL: return "null pointer";
     case UPF_ERR_EMPTY_MESH: return "empty mesh";
+    case UPF_ERR_BAND: return "narrow band too narrow: the surface leaves the active blocks";
     default: return "unknown status";
   }
 }

@@ -65,8 +65,13 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
 __device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, 0); }
+// Count words: kScanMc = marching-cubes groups (vertices bits 0-7, triangles 8-16,
+// active bit 17); kScanBlocks = narrow-band blocks (vertices 0-11, triangles 12-23,
+// active bit 24).
+template <int FMT>
 __device__ __forceinline__ uint4 unpack_counts(uint32_t c) {
-  return make_uint4(c & 0xFFu, (c >> 8) & 0x1FFu, (c >> 17) & 1u, 0);
+  if (FMT == kScanMc) return make_uint4(c & 0xFFu, (c >> 8) & 0x1FFu, (c >> 17) & 1u, 0);
+  return make_uint4(c & 0xFFFu, (c >> 12) & 0xFFFu, (c >> 24) & 1u, 0);
 }
 
 // Block-wide exclusive scan of one uint4 per thread; the block total goes to *total.
@@ -91,13 +96,14 @@ __device__ __forceinline__ uint4 block_exclusive_scan(uint4 v, uint4* total) {
   return make_uint4(before.x + inc.x - v.x, before.y + inc.y - v.y, before.z + inc.z - v.z, 0);
 }
 
+template <int FMT>
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint32_t* __restrict__ counts, int64_t n,
                                                              uint4* __restrict__ tile_sums) {
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   uint4 s = make_uint4(0, 0, 0, 0);
 #pragma unroll
   for (int t = 0; t < kScanItems; t++)
-    if (base + t < n) s = add4(s, unpack_counts(counts[base + t]));
+    if (base + t < n) s = add4(s, unpack_counts<FMT>(counts[base + t]));
   uint4 total;
   (void)block_exclusive_scan(s, &total);
   if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
@@ -128,6 +134,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(uint4* __restrict__ s
   if (threadIdx.x == 0) *total = carry;
 }
 
+template <int FMT>
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ counts, int64_t n,
                                                              const uint4* __restrict__ tile_off,
                                                              uint32_t* __restrict__ voff, uint2* __restrict__ act,
@@ -137,7 +144,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __r
   uint4 s = make_uint4(0, 0, 0, 0);
 #pragma unroll
   for (int t = 0; t < kScanItems; t++) {
-    v[t] = (base + t < n) ? unpack_counts(counts[base + t]) : make_uint4(0, 0, 0, 0);
+    v[t] = (base + t < n) ? unpack_counts<FMT>(counts[base + t]) : make_uint4(0, 0, 0, 0);
     s = add4(s, v[t]);
   }
   uint4 total;
@@ -145,9 +152,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __r
 #pragma unroll
   for (int t = 0; t < kScanItems; t++) {
     if (base + t < n) {
-      voff[base + t] = ex.x;
+      if (voff) voff[base + t] = ex.x;
       if (toff) toff[base + t] = ex.y;
-      if (v[t].z) act[ex.z] = make_uint2((uint32_t)(base + t), ex.y);
+      if (act && v[t].z) act[ex.z] = make_uint2((uint32_t)(base + t), ex.y);
     }
     ex = add4(ex, v[t]);
   }
@@ -214,12 +221,21 @@ int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws
 }
 
 int mc_scan_launch(const uint32_t* counts, int64_t groups, uint4* tiles, uint4* total, uint32_t* voff, uint2* act,
-                   uint32_t* toff, hipStream_t s) {
+                   uint32_t* toff, hipStream_t s, int fmt) {
   const int64_t ntiles = (groups + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups, tiles);
+  if (ntiles == 0) return UPF_ERR_BAD_GRID;
+  if (fmt == kScanMc)
+    hipLaunchKernelGGL(k_scan_tiles<kScanMc>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups, tiles);
+  else
+    hipLaunchKernelGGL(k_scan_tiles<kScanBlocks>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups,
+                       tiles);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tiles, ntiles, total);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups, tiles, voff, act,
-                     toff);
+  if (fmt == kScanMc)
+    hipLaunchKernelGGL(k_scan_apply<kScanMc>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups, tiles,
+                       voff, act, toff);
+  else
+    hipLaunchKernelGGL(k_scan_apply<kScanBlocks>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups,
+                       tiles, voff, act, toff);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }

@@ -40,10 +40,14 @@ int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws
 // emit over at most max_active active groups (the kernel reads the true count)
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
                    float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s);
-// exclusive scan of packed group counts (the three sums of classify) into voff/act/total
-// (and the dense triangle offsets toff when non-null)
+// count-word formats of mc_scan_launch
+constexpr int kScanMc = 0;      // vertices bits 0-7, triangles 8-16, active bit 17
+constexpr int kScanBlocks = 1;  // vertices bits 0-11, triangles 12-23, active bit 24
+// exclusive scan of packed counts (vertices, triangles, active) into voff / toff (dense
+// offsets, each optional), act (compacted (index, first triangle) of the active entries,
+// optional) and total
 int mc_scan_launch(const uint32_t* counts, int64_t groups, uint4* tiles, uint4* total, uint32_t* voff, uint2* act,
-                   uint32_t* toff, hipStream_t s);
+                   uint32_t* toff, hipStream_t s, int fmt = kScanMc);
 int64_t mc_scan_tiles(int64_t groups);
 
 }  // namespace upf

@@ -19,7 +19,8 @@ THETA_COUNT = 76
 
 # status
 OK, ERR_INVALID_VALUE, ERR_BAD_SPEC, ERR_CAPACITY, ERR_WORKSPACE, ERR_BAD_GRID, ERR_HIP, \
-    ERR_NULL, ERR_EMPTY_MESH = range(9)
+    ERR_NULL, ERR_EMPTY_MESH, ERR_BAND = range(10)
+NB_BLOCK = 8
 
 SHAPE = {"none": 0, "sphere": 1, "box": 2, "torus": 3, "capsule": 4}
 NOISE = {"gradient": 1, "value": 2, "cellular": 3}
@@ -191,6 +192,11 @@ class Chunk(ctypes.Structure):
 class BatchOpts(ctypes.Structure):
     _fields_ = [("field_variant", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
                 ("classify_z", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class NbStats(ctypes.Structure):
+    _fields_ = [("blocks", ctypes.c_int64), ("active_blocks", ctypes.c_int64), ("n_vertices", ctypes.c_int64),
+                ("n_triangles", ctypes.c_int64), ("unresolved", ctypes.c_int64), ("reserved", ctypes.c_int64 * 3)]
 
 
 class ChunkMesh(ctypes.Structure):
