@@ -28,6 +28,11 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, vendor figure (packed FP32)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+# HBM bytes per launch of k_eval_grid_row on the 512^3 workload, from rocprofv3 --pmc
+# FETCH_SIZE (164 KiB, x 2.0 by the 1 GiB copy calibration) and WRITE_SIZE (524288 KiB,
+# x 1.0): the kernel reads only its arguments and writes each node once
+# (profiles/field_torus_ridged_voronoi_512.txt).
+FIELD_TRAFFIC_512 = 2 * 164 * 1024 + 524288 * 1024
 
 
 def cpu_model() -> str:
@@ -473,7 +478,7 @@ def main() -> int:
         "mc_ms": round(t_mc * 1e3, 4),
         "roofline": {"bound": "valu", "kernel": "k_eval_grid_row", "achieved": round(field_tflops, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None},
+                     "traffic": FIELD_TRAFFIC_512 if nx == ny == nz == 512 else None},
         "roofline_mc": {"bound": "hbm", "kernel": "marching cubes (classify+scan+emit)", "achieved": round(mc_gbs, 1),
                         "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(mc_gbs / PEAK_HBM_GBS, 4),
                         "algorithmic_bytes": mc_bytes},

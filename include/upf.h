@@ -351,7 +351,8 @@ typedef struct upf_bake_spec {
  * row-major order, side = the largest integer number of texels that fits every cell in
  * width x height); triangle 2c takes the lower-left half (0,0) (s,0) (0,s) of the cell's
  * diagonal split, 2c+1 the upper-right half (s,s) (0,s) (s,0), corners on texel
- * boundaries (texel centres on the diagonal go to 2c+1).  d_uv: float[n_triangles][3][2], UV per triangle corner in [0,1].
+ * boundaries (texel centres on the diagonal go to 2c+1).
+ * d_uv: float[n_triangles][3][2], UV per triangle corner in [0,1].
  * UPF_ERR_CAPACITY if a cell would be smaller than 2 x 2 texels. */
 int upf_atlas_per_triangle(int64_t n_triangles, int32_t width, int32_t height, float* d_uv,
                            void* stream);

@@ -55,11 +55,12 @@ __global__ __launch_bounds__(256) void k_batch_field(const BChunk* __restrict__ 
                                                      float* __restrict__ field) {
   __shared__ float4 s_slice[4][kRowSlice];
   __shared__ float4 s_gtab[16];
+  __shared__ float4 s_oct[4][kRowOctTab];
   if (threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
   __syncthreads();
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, lane);
+  const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], lane);
   const int64_t stride = (int64_t)gridDim.x * 4;
   for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < n_items; r += stride) {
     const int c = uniform(find_chunk<&BChunk::row_off>(ch, n_chunks, r));

@@ -1,5 +1,6 @@
 // upf_common.cpp -- spec validation and conversion, layer expansion, the flop table
 // and status strings of libupf (host code).
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -65,6 +66,20 @@ int to_device(const upf_field_spec* in, DSpec* out) {
     if (a.blend != UPF_BLEND_ADD) out->all_add = 0;
   }
   return UPF_OK;
+}
+
+int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+  if (dev < 64) {
+    const int c = cache[dev].load(std::memory_order_relaxed);
+    if (c > 0) return c;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  if (dev < 64) cache[dev].store(cus, std::memory_order_relaxed);
+  return cus;
 }
 
 int to_device(const upf_grid* in, DGrid* out) {

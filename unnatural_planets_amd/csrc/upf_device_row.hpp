@@ -37,18 +37,22 @@ __device__ __forceinline__ float dot_g(const float4 g, float x, float y, float z
   return fmaf(g.z, z, fmaf(g.y, y, g.x * x));
 }
 
+constexpr int kRowOctTab = 3 * UPF_MAX_OCTAVES;  // float4 entries of the per-octave table
+
 struct RowCtx {
   float4* slice;        // this wave's LDS slice (kRowSlice entries)
   const float4* gtab;   // 16 gradient vectors in LDS
+  float4* oct;          // this wave's per-octave table (kRowOctTab entries)
   int lane;
   uint32_t col_px;      // (lane / 4) * PX: the lane's column offset in the lattice hash
   int cell_lx, cell_c9, cell_dy, cell_dz;  // the lane's feature point in the cellular writer
 };
 
-__device__ __forceinline__ RowCtx make_row_ctx(float4* slice, const float4* gtab, int lane) {
+__device__ __forceinline__ RowCtx make_row_ctx(float4* slice, const float4* gtab, float4* oct, int lane) {
   RowCtx c;
   c.slice = slice;
   c.gtab = gtab;
+  c.oct = oct;
   c.lane = lane;
   c.col_px = (uint32_t)(lane >> 2) * 501125321u;
   c.cell_lx = lane % 7;
@@ -98,8 +102,10 @@ __device__ __forceinline__ float fractal_acc(const DLayer& L, int k, float n, fl
                                             : fmaf(L.amp[k], n, acc));
 }
 
-// Gradient / value columns of one octave.  Lane = 4 * column + combo (combo = dy + 2 dz),
-// 16 columns per round; the y and z folds take the partner lanes lane ^ 1 and lane ^ 2.
+// Gradient / value columns of a single octave (one-octave layers, where the table set-up
+// of col_write_all would cost more than it saves).  Lane = 4 * column + combo (combo =
+// dy + 2 dz), 16 columns per round; the y and z folds take the partner lanes lane ^ 1 and
+// lane ^ 2, in the same order as col_write_all, so the entries are the same.
 template <int NOISE>
 __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float qy, float qz, int lo, int n,
                                           float2* region) {
@@ -132,6 +138,63 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
     }
     const int xo = (c.lane >> 2) + 16 * r;
     if (combo == 0 && xo < n) region[xo] = make_float2(za, zb);
+  }
+}
+
+// Columns of every octave of a gradient / value layer at once: column task t (region
+// entry t of the packed regions) goes to lane t % 64 of round t / 64, whatever its octave,
+// and that lane hashes and folds the column's 4 lattice points itself.  Lanes 0..oct-1
+// first put their octave's row constants into the wave's table:
+//   [3k] = (ty, tz, uy, uz),  [3k+1] = combo seeds s_c = (seed+k) ^ (iy+dy) PY ^ (iz+dz) PZ
+//   (c = dy + 2 dz, as bits),  [3k+2].x = lo_k - base_k (as bits).
+// The lattice points, hashes and gradient vectors are those of the per-lane noise; each
+// column folds y first (uy), then z (uz).  The rounds are the packed task count / 64, not
+// one or more per octave, and the y/z constants are computed once per octave.
+template <int NOISE>
+__device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, int oct, float y, float z,
+                                              const int* lo, const int* base, int total, float2* slice) {
+  if (c.lane < oct) {
+    float f = L.freq[0];
+    int off = lo[0] - base[0];
+#pragma unroll
+    for (int j = 1; j < UPF_MAX_OCTAVES; j++)
+      if (j < oct && c.lane == j) { f = L.freq[j]; off = lo[j] - base[j]; }
+    const float qy = y * f, qz = z * f;
+    const float fy = floorf(qy), fz = floorf(qz);
+    const float ty = qy - fy, tz = qz - fz;
+    const uint32_t y0 = (uint32_t)(int)fy * kPY, z0 = (uint32_t)(int)fz * kPZ;
+    const uint32_t sk = L.seed + (uint32_t)c.lane;
+    c.oct[3 * c.lane] = make_float4(ty, tz, fade(ty), fade(tz));
+    c.oct[3 * c.lane + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
+                                        __uint_as_float(sk ^ y0 ^ (z0 + kPZ)),
+                                        __uint_as_float(sk ^ (y0 + kPY) ^ (z0 + kPZ)));
+    c.oct[3 * c.lane + 2] = make_float4(__int_as_float(off), 0.0f, 0.0f, 0.0f);
+  }
+  wave_sync();
+  for (int r = 0; r * 64 < total; r++) {
+    const int t = 64 * r + c.lane;
+    int k = 0;
+#pragma unroll
+    for (int j = 1; j < UPF_MAX_OCTAVES; j++)
+      if (j < oct) k += t >= base[j];
+    const float4 p0 = c.oct[3 * k], p1 = c.oct[3 * k + 1];
+    const int off = __float_as_int(c.oct[3 * k + 2].x);
+    const uint32_t hx = (uint32_t)(t + off) * kPX;
+    const uint32_t h0 = (__float_as_uint(p1.x) ^ hx) * kHM, h1 = (__float_as_uint(p1.y) ^ hx) * kHM;
+    const uint32_t h2 = (__float_as_uint(p1.z) ^ hx) * kHM, h3 = (__float_as_uint(p1.w) ^ hx) * kHM;
+    const float ty = p0.x, tz = p0.y, uy = p0.z, uz = p0.w;
+    float A, B = 0.0f;
+    if (NOISE == UPF_NOISE_GRADIENT) {
+      const float4 g0 = c.gtab[h0 >> 28], g1 = c.gtab[h1 >> 28], g2 = c.gtab[h2 >> 28], g3 = c.gtab[h3 >> 28];
+      const float ty1 = ty - 1.0f, tz1 = tz - 1.0f;
+      A = lerpf(lerpf(g0.x, g1.x, uy), lerpf(g2.x, g3.x, uy), uz);
+      const float b0 = fmaf(g0.z, tz, g0.y * ty), b1 = fmaf(g1.z, tz, g1.y * ty1);
+      const float b2 = fmaf(g2.z, tz1, g2.y * ty), b3 = fmaf(g3.z, tz1, g3.y * ty1);
+      B = lerpf(lerpf(b0, b1, uy), lerpf(b2, b3, uy), uz);
+    } else {
+      A = lerpf(lerpf(lattice_value(h0), lattice_value(h1), uy), lerpf(lattice_value(h2), lattice_value(h3), uy), uz);
+    }
+    if (t < total) slice[t] = make_float2(A, B);
   }
 }
 
@@ -205,14 +268,18 @@ __device__ __forceinline__ float row_layer_t(const RowCtx& c, const DLayer& L, f
     }
   }
   if (!fits || total > kRowSlice2) return layer_value(L, x, y, z);
+  if (CELL) {
 #pragma unroll
-  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
-    if (k < oct) {
-      const float f = L.freq[k];
-      if (CELL) cell_write(c, L, L.seed + (uint32_t)k, y * f, z * f, lo[k], slice + base[k]);
-      else col_write<NOISE>(c, L.seed + (uint32_t)k, y * f, z * f, lo[k],
-                            (k + 1 < oct ? base[k + 1] : total) - base[k], slice + base[k]);
+    for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+      if (k < oct) {
+        const float f = L.freq[k];
+        cell_write(c, L, L.seed + (uint32_t)k, y * f, z * f, lo[k], slice + base[k]);
+      }
     }
+  } else if (oct == 1) {
+    col_write<NOISE>(c, L.seed, y * L.freq[0], z * L.freq[0], lo[0], total, slice);
+  } else {
+    col_write_all<NOISE>(c, L, oct, y, z, lo, base, total, slice);
   }
   wave_sync();
   float acc = 0.0f;

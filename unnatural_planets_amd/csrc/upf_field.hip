@@ -35,6 +35,7 @@ __global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g,
 __global__ __launch_bounds__(256) void k_eval_grid_row(const DSpec s, const DGrid g, float* __restrict__ out) {
   __shared__ float4 s_slice[kGridBY][kRowSlice];
   __shared__ float4 s_gtab[16];
+  __shared__ float4 s_oct[kGridBY][kRowOctTab];
   if (threadIdx.y == 0 && threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
   __syncthreads();
   const int lane = threadIdx.x;
@@ -46,7 +47,7 @@ __global__ __launch_bounds__(256) void k_eval_grid_row(const DSpec s, const DGri
   const float x = node_coord(g.o[0], g.s[0], ic);
   const float y = node_coord(g.o[1], g.s[1], j);
   const float z = node_coord(g.o[2], g.s[2], k);
-  const RowCtx c = make_row_ctx(s_slice[threadIdx.y], s_gtab, lane);
+  const RowCtx c = make_row_ctx(s_slice[threadIdx.y], s_gtab, s_oct[threadIdx.y], lane);
   const float v = row_field(c, s, x, y, z);
   if (i < g.n[0]) out[((size_t)k * g.n[1] + j) * g.n[0] + i] = v;
 }

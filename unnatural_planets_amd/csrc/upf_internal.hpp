@@ -16,6 +16,21 @@ int to_device(const upf_grid* in, DGrid* out);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Compute units of the current device (cached per device; 256 if the query fails).
+int device_cus();
+
+// Workgroups are dispatched round-robin over the XCDs (workgroup L runs on XCD L % 8),
+// and each XCD has its own L2.  Work item of workgroup L when n items are split into 8
+// contiguous ranges, one per XCD: neighbouring items (which share cache lines of their
+// inputs and outputs) then run on the same XCD.  Launch kXcds * ceil(n / kXcds)
+// workgroups; items >= n exit.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_item(int64_t L, int64_t n) {
+  const int64_t per = (n + kXcds - 1) / kXcds;
+  return (L % kXcds) * per + L / kXcds;
+}
+inline int64_t xcd_grid(int64_t n) { return kXcds * ((n + kXcds - 1) / kXcds); }
+
 #define UPF_HIP_CHECK(expr)                      \
   do {                                           \
     if ((expr) != hipSuccess) return UPF_ERR_HIP; \

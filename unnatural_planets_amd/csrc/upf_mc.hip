@@ -21,6 +21,7 @@
 // Scratch is 24 B per group (S, packed counts, voff, active entry), ~0.4 B per node.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "upf_internal.hpp"
@@ -31,15 +32,25 @@ namespace upf {
 #include "upf_mc_device.hpp"
 
 // ------------------------------------------------------------------ classify / emit kernels
+// Work item w = (z block, 4-row block, group column), column fastest, spread over the
+// XCDs in contiguous ranges: the groups of one row (one cache line of S and of counts)
+// and the row blocks that share a boundary row then meet in the same L2.
 __global__ __launch_bounds__(256) void k_mc_classify(const float* __restrict__ f, const McDims d, float iso,
-                                                     uint64_t* __restrict__ S, uint32_t* __restrict__ counts) {
+                                                     int64_t items, uint64_t* __restrict__ S,
+                                                     uint32_t* __restrict__ counts) {
   __shared__ uint8_t s_ntris[256];
   s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
   __syncthreads();
-  const int j = uniform(blockIdx.y * 4 + (threadIdx.x >> 6));
+  const int64_t w = xcd_item(blockIdx.x, items);
+  if (w >= items) return;
+  const int rbs = (d.ny + 3) / 4;
+  const int xb = (int)(w % d.gpr);
+  const int64_t r = w / d.gpr;
+  const int rb = (int)(r % rbs), zb = (int)(r / rbs);
+  const int j = uniform(rb * 4 + (threadIdx.x >> 6));
   if (j >= d.ny) return;
-  const int k0 = blockIdx.z * kMcClassZ;
-  mc_classify_column(f, d, iso, blockIdx.x, j, k0, min(k0 + kMcClassZ, d.nz), s_ntris, S, counts);
+  const int k0 = zb * kMcClassZ;
+  mc_classify_column(f, d, iso, xb, j, k0, min(k0 + kMcClassZ, d.nz), s_ntris, S, counts);
 }
 
 __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, const McDims d, const DGrid g,
@@ -214,8 +225,9 @@ int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws
   const McLayout L = mc_layout(g);
   if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
   const McPtrs P = mc_ptrs(L, d_ws);
-  const dim3 cgrid(L.d.gpr, (L.d.ny + 3) / 4, (L.d.nz + kMcClassZ - 1) / kMcClassZ);
-  hipLaunchKernelGGL(k_mc_classify, cgrid, dim3(256), 0, s, d_field, L.d, iso, P.S, P.counts);
+  const int64_t items = (int64_t)L.d.gpr * ((L.d.ny + 3) / 4) * ((L.d.nz + kMcClassZ - 1) / kMcClassZ);
+  hipLaunchKernelGGL(k_mc_classify, dim3((unsigned)xcd_grid(items)), dim3(256), 0, s, d_field, L.d, iso, items, P.S,
+                     P.counts);
   UPF_LAUNCH_CHECK();
   return mc_scan_launch(P.counts, L.d.groups, P.tiles, P.total, P.voff, P.act, nullptr, s);
 }

@@ -22,8 +22,9 @@ __device__ __forceinline__ uint32_t pack_counts(int nv, int nt) {
 
 // ------------------------------------------------------------------ classify
 // One wave: group column xb of row j, planes k0 <= k < k1.  The current z plane stays in
-// registers; each step loads the next plane's rows j and j+1 at x and x+1 (clamped, so
-// every load is unconditional and the loads of a step issue back to back).
+// registers; each step uses plane k+1 (loaded the step before) and loads plane k+2's rows
+// j and j+1 at x and x+1 (clamped, so every load is unconditional and the loads of a step
+// issue back to back).
 __device__ __forceinline__ void mc_classify_column(const float* __restrict__ f, const McDims& d, float iso, int xb,
                                                    int j, int k0, int k1, const uint8_t* s_ntris,
                                                    uint64_t* __restrict__ S, uint32_t* __restrict__ counts) {
@@ -37,10 +38,14 @@ __device__ __forceinline__ void mc_classify_column(const float* __restrict__ f, 
   // current plane: a0 = (x, j), a1 = (x+1, j), a2 = (x+1, j+1), a3 = (x, j+1)
   const float* p = f + (int64_t)k0 * d.nxy;
   float a0 = p[r0 + xc], a1 = p[r0 + x1], a2 = p[r1 + x1], a3 = p[r1 + xc];
+  // plane k+1 is loaded one step ahead, so two planes of loads are in flight per step
+  const float* q1 = f + (int64_t)min(k0 + 1, d.nz - 1) * d.nxy;
+  float c0 = q1[r0 + xc], c1 = q1[r0 + x1], c2 = q1[r1 + x1], c3 = q1[r1 + xc];
   for (int k = k0; k < k1; k++) {
     const bool hz = valid && k + 1 < d.nz;
-    const float* q = f + (int64_t)min(k + 1, d.nz - 1) * d.nxy;
-    const float b0 = q[r0 + xc], b1 = q[r0 + x1], b2 = q[r1 + x1], b3 = q[r1 + xc];
+    const float b0 = c0, b1 = c1, b2 = c2, b3 = c3;
+    const float* q = f + (int64_t)min(k + 2, d.nz - 1) * d.nxy;
+    c0 = q[r0 + xc]; c1 = q[r0 + x1]; c2 = q[r1 + x1]; c3 = q[r1 + xc];
     const bool s0 = valid && a0 < iso;
     const bool cx = hx && ((a1 < iso) != s0);
     const bool cy = hy && ((a3 < iso) != s0);
