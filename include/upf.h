@@ -227,7 +227,8 @@ typedef struct upf_eval_opts {
 } upf_eval_opts;
 
 /* upf_field_eval_grid with explicit launch options (NULL = defaults).  No counterpart
- * in the reference; same results for every variant, bit for bit. */
+ * in the reference.  Both variants use the same lattice points, hashes and gradients;
+ * UPF_EVAL_ROWS regroups the interpolation sums (a few ulps from UPF_EVAL_PER_LANE). */
 int upf_field_eval_grid_ex(const upf_field_spec* spec, const upf_grid* grid, float* d_field,
                            const upf_eval_opts* opts, void* stream);
 
@@ -271,6 +272,17 @@ int upf_mc_count(const float* d_field, const upf_grid* grid, float iso, void* d_
                  size_t workspace_bytes, int64_t* n_vertices, int64_t* n_triangles, void* stream);
 int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out,
                 void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Field evaluation fused with the counting half of marching cubes: evaluates the field
+ * into d_field (as upf_field_eval_grid_ex) while the field kernel stores each 64-node
+ * group's sign mask, then classifies from the sign masks alone (no second read of the
+ * field) and scans.  The counts and the workspace are those of upf_mc_count on the same
+ * field, so upf_mc_emit follows as after upf_mc_count.  Replaces the pair
+ * updateByPosition + makeMesh of meshGenerateGeneric (sources/meshGeneration.cpp:38-39)
+ * up to the emission.  Synchronises the stream (16 B read-back). */
+int upf_field_mesh_count(const upf_field_spec* spec, const upf_grid* grid, float iso, float* d_field,
+                         const upf_eval_opts* opts, void* d_workspace, size_t workspace_bytes,
+                         int64_t* n_vertices, int64_t* n_triangles, void* stream);
 
 /* Case code of every cell -> d_codes[dims[2]-1][dims[1]-1][dims[0]-1] (uint8). */
 int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,

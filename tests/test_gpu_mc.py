@@ -133,3 +133,28 @@ def test_mesh_generate_base_land_torus_512_properties(upf):
     cos = (gx * m.normals[:200000]).sum(1)
     assert float(cos.mean()) > 0.9
     assert torch.allclose(m.normals.norm(dim=1), torch.ones(m.n_vertices, device="cuda"), atol=1e-5)
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("dims,lo,hi", [((128, 128, 128), -1.3, 1.3), ((97, 70, 33), -1.3, 1.3),
+                                        ((200, 5, 9), -1.3, 1.3), ((2, 2, 2), -1.0, 1.0)])
+def test_fused_field_mesh_count_equals_separate_passes(upf, variant, dims, lo, hi):
+    """upf_field_mesh_count (sign masks from the field kernel, classification from the
+    masks) = upf_field_eval_grid_ex + upf_mc_count + upf_mc_emit, bit for bit."""
+    torch = _torch()
+    s = S.spec_torus_ridged_voronoi(42) if dims[0] != 128 else S.spec_sphere_fbm(1234)
+    g = S.Grid()
+    for a in range(3):
+        g.origin[a] = lo
+        g.spacing[a] = S.f32((hi - lo) / (dims[a] - 1))
+        g.dims[a] = dims[a]
+    f_ref = upf.field_eval_grid(s, g, variant=variant)
+    ref = upf.mc_extract(f_ref, g, 0.0)
+    f, ws, nv, nt = upf.field_mesh_count(s, g, 0.0, variant=variant)
+    assert torch.equal(f, f_ref)
+    assert (nv, nt) == (ref.n_vertices, ref.n_triangles)
+    m = upf.mc_emit_counted(f, g, 0.0, ws, nv, nt)
+    assert torch.equal(m.indices, ref.indices)
+    assert torch.equal(m.positions, ref.positions) and torch.equal(m.normals, ref.normals)
+    if dims[0] == 128:
+        assert nt > 10000

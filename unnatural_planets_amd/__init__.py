@@ -31,7 +31,7 @@ __all__ = [
     "mesh_generate_base_land", "mesh_generate_base_water", "mesh_generate_base_navigation",
     "reference_grid", "flops_per_sample", "adjoint_flops_per_sample", "BakeSpec", "bake_spec",
     "Textures", "atlas_per_triangle", "bake_rasterize", "bake", "generate_textures_land",
-    "NbWorkspace", "nb_extract",
+    "NbWorkspace", "nb_extract", "field_mesh_count", "mc_emit_counted",
 ]
 
 
@@ -240,12 +240,51 @@ def mc_case_codes(field, g: Grid, iso: float = 0.0, stream=None):
     return out
 
 
+def field_mesh_count(s: FieldSpec, g: Grid, iso: float = 0.0, field=None, workspace: McWorkspace | None = None,
+                     variant: int = EVAL_AUTO, stream=None):
+    """Field evaluation fused with marching-cubes classification (upf_field_mesh_count).
+    Returns (field [nz, ny, nx], workspace, n_vertices, n_triangles); mc_emit_counted
+    then writes the mesh."""
+    torch = _torch()
+    nx, ny, nz = grid_dims(g)
+    if field is None:
+        field = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+    _check_f32(field, "field", (nz, ny, nx))
+    ws = workspace or McWorkspace(g, field.device)
+    opts = EvalOpts()
+    opts.variant = variant
+    nv, nt = ctypes.c_int64(0), ctypes.c_int64(0)
+    check(lib().upf_field_mesh_count(ctypes.byref(s), ctypes.byref(g), iso, _ptr(field), ctypes.byref(opts),
+                                     _ptr(ws.buf), ws.bytes, ctypes.byref(nv), ctypes.byref(nt), _stream(stream)),
+          "upf_field_mesh_count")
+    return field, ws, int(nv.value), int(nt.value)
+
+
+def mc_emit_counted(field, g: Grid, iso: float, ws: McWorkspace, n_vertices: int, n_triangles: int,
+                    normals: bool = True, stream=None) -> Mesh:
+    """upf_mc_emit after upf_mc_count / upf_field_mesh_count left their state in ws."""
+    torch = _torch()
+    pos = torch.empty((n_vertices, 3), dtype=torch.float32, device=field.device)
+    nrm = torch.empty((n_vertices, 3), dtype=torch.float32, device=field.device) if normals else None
+    idx = torch.empty((n_triangles, 3), dtype=torch.int32, device=field.device)
+    mb = MeshBuffers()
+    mb.positions = pos.data_ptr() if n_vertices else None
+    mb.normals = nrm.data_ptr() if (normals and n_vertices) else None
+    mb.indices = idx.data_ptr() if n_triangles else None
+    mb.cap_vertices, mb.cap_triangles = n_vertices, n_triangles
+    mb.n_vertices, mb.n_triangles = n_vertices, n_triangles
+    check(lib().upf_mc_emit(_ptr(field), ctypes.byref(g), iso, ctypes.byref(mb), _ptr(ws.buf), ws.bytes,
+                            _stream(stream)), "upf_mc_emit")
+    return Mesh(pos, nrm, idx)
+
+
 def mesh_generate_generic(s: FieldSpec, g: Grid, iso: float = 0.0, normals: bool = True) -> Mesh:
-    """Grid sweep + marching cubes (sources/meshGeneration.cpp:32-45).  The reference then
-    runs meshRemoveDisconnected and meshFlipNormals; here normals already point
-    outward (towards increasing f), so no flip is needed."""
-    f = field_eval_grid(s, g)
-    return mc_extract(f, g, iso, normals)
+    """Grid sweep + marching cubes (sources/meshGeneration.cpp:32-45), as one fused
+    field + classification pass and the emission.  The reference then runs
+    meshRemoveDisconnected and meshFlipNormals; here normals already point outward
+    (towards increasing f), so no flip is needed."""
+    f, ws, nv, nt = field_mesh_count(s, g, iso)
+    return mc_emit_counted(f, g, iso, ws, nv, nt, normals)
 
 
 def mesh_generate_base_land(s: FieldSpec, g: Grid | None = None) -> Mesh:

@@ -75,7 +75,9 @@ __global__ __launch_bounds__(256) void k_batch_field(const BChunk* __restrict__ 
     const float x = node_coord(C.grid.o[0], C.grid.s[0], ic);
     const float y = node_coord(C.grid.o[1], C.grid.s[1], j);
     const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
-    const float v = VARIANT == UPF_EVAL_PER_LANE ? field_eval(C.spec, x, y, z) : row_field(ctx, C.spec, x, y, z);
+    float v;
+    if (VARIANT == UPF_EVAL_PER_LANE) v = field_eval(C.spec, x, y, z);
+    else row_field<1>(ctx, C.spec, x, &y, z, &v);
     if (i < C.grid.n[0]) field[C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + i] = v;
   }
 }

@@ -37,7 +37,8 @@ __device__ __forceinline__ float dot_g(const float4 g, float x, float y, float z
   return fmaf(g.z, z, fmaf(g.y, y, g.x * x));
 }
 
-constexpr int kRowOctTab = 3 * UPF_MAX_OCTAVES;  // float4 entries of the per-octave table
+constexpr int kRowMaxRows = 2;                                 // rows per wave at most
+constexpr int kRowOctTab = 3 * kRowMaxRows * UPF_MAX_OCTAVES;  // float4 entries of the per-octave table
 
 struct RowCtx {
   float4* slice;        // this wave's LDS slice (kRowSlice entries)
@@ -141,29 +142,35 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
   }
 }
 
-// Columns of every octave of a gradient / value layer at once: column task t (region
-// entry t of the packed regions) goes to lane t % 64 of round t / 64, whatever its octave,
-// and that lane hashes and folds the column's 4 lattice points itself.  Lanes 0..oct-1
-// first put their octave's row constants into the wave's table:
-//   [3k] = (ty, tz, uy, uz),  [3k+1] = combo seeds s_c = (seed+k) ^ (iy+dy) PY ^ (iz+dz) PZ
-//   (c = dy + 2 dz, as bits),  [3k+2].x = lo_k - base_k (as bits).
+// Columns of every octave of a gradient / value layer at once, for R rows that share x
+// (same column spans) and z and differ in y: column task t (region entry t of the packed
+// regions, row after row) goes to lane t % 64 of round t / 64, whatever its row and
+// octave, and that lane hashes and folds the column's 4 lattice points itself.  Lanes
+// 0..R*oct-1 (lane = r * oct + k) first put their row's octave constants into the wave's
+// table:
+//   [3e] = (ty, tz, uy, uz),  [3e+1] = combo seeds s_c = (seed+k) ^ (iy+dy) PY ^ (iz+dz) PZ
+//   (c = dy + 2 dz, as bits),  [3e+2].x = lo_k - base_k - r * total (as bits).
 // The lattice points, hashes and gradient vectors are those of the per-lane noise; each
 // column folds y first (uy), then z (uz).  The rounds are the packed task count / 64, not
-// one or more per octave, and the y/z constants are computed once per octave.
-template <int NOISE>
-__device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, int oct, float y, float z,
+// one or more per octave, and the y/z constants are computed once per octave and row.
+template <int NOISE, int R>
+__device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, int oct, const float* y, float z,
                                               const int* lo, const int* base, int total, float2* slice) {
-  if (c.lane < oct) {
+  if (c.lane < R * oct) {
+    const int r = (R == 2 && c.lane >= oct) ? 1 : 0;
+    const int k = c.lane - r * oct;
     float f = L.freq[0];
     int off = lo[0] - base[0];
 #pragma unroll
     for (int j = 1; j < UPF_MAX_OCTAVES; j++)
-      if (j < oct && c.lane == j) { f = L.freq[j]; off = lo[j] - base[j]; }
-    const float qy = y * f, qz = z * f;
+      if (j < oct && k == j) { f = L.freq[j]; off = lo[j] - base[j]; }
+    off -= r * total;
+    const float yr = (R == 2 && r) ? y[R - 1] : y[0];
+    const float qy = yr * f, qz = z * f;
     const float fy = floorf(qy), fz = floorf(qz);
     const float ty = qy - fy, tz = qz - fz;
     const uint32_t y0 = (uint32_t)(int)fy * kPY, z0 = (uint32_t)(int)fz * kPZ;
-    const uint32_t sk = L.seed + (uint32_t)c.lane;
+    const uint32_t sk = L.seed + (uint32_t)k;
     c.oct[3 * c.lane] = make_float4(ty, tz, fade(ty), fade(tz));
     c.oct[3 * c.lane + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
                                         __uint_as_float(sk ^ y0 ^ (z0 + kPZ)),
@@ -171,14 +178,17 @@ __device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, 
     c.oct[3 * c.lane + 2] = make_float4(__int_as_float(off), 0.0f, 0.0f, 0.0f);
   }
   wave_sync();
-  for (int r = 0; r * 64 < total; r++) {
-    const int t = 64 * r + c.lane;
+  for (int rd = 0; rd * 64 < R * total; rd++) {
+    const int t = 64 * rd + c.lane;
+    const int r = (R == 2 && t >= total) ? 1 : 0;
+    const int tt = t - r * total;
     int k = 0;
 #pragma unroll
     for (int j = 1; j < UPF_MAX_OCTAVES; j++)
-      if (j < oct) k += t >= base[j];
-    const float4 p0 = c.oct[3 * k], p1 = c.oct[3 * k + 1];
-    const int off = __float_as_int(c.oct[3 * k + 2].x);
+      if (j < oct) k += tt >= base[j];
+    const int e = r * oct + k;
+    const float4 p0 = c.oct[3 * e], p1 = c.oct[3 * e + 1];
+    const int off = __float_as_int(c.oct[3 * e + 2].x);
     const uint32_t hx = (uint32_t)(t + off) * kPX;
     const uint32_t h0 = (__float_as_uint(p1.x) ^ hx) * kHM, h1 = (__float_as_uint(p1.y) ^ hx) * kHM;
     const uint32_t h2 = (__float_as_uint(p1.z) ^ hx) * kHM, h3 = (__float_as_uint(p1.w) ^ hx) * kHM;
@@ -194,21 +204,31 @@ __device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, 
     } else {
       A = lerpf(lerpf(lattice_value(h0), lattice_value(h1), uy), lerpf(lattice_value(h2), lattice_value(h3), uy), uz);
     }
-    if (t < total) slice[t] = make_float2(A, B);
+    if (t < R * total) slice[t] = make_float2(A, B);
   }
 }
 
-template <int NOISE>
-__device__ __forceinline__ float col_read(float qx, int lo, const float2* region) {
+// A lane's octave from the columns of R rows: the x part (floor, t, fade, entry offset)
+// is computed once and each row reads its own region (the rows' regions are `stride`
+// entries apart).
+template <int NOISE, int R>
+__device__ __forceinline__ void col_read(float qx, int lo, const float2* region, int stride, float* out) {
   const float fx = floorf(qx);
   const float tx = qx - fx;
   const float ux = fade(tx);
-  const float2* e = region + ((int)fx - lo);
-  const float2 e0 = e[0], e1 = e[1];
-  if (NOISE == UPF_NOISE_VALUE) return lerpf(e0.x, e1.x, ux);
-  const float d0 = fmaf(e0.x, tx, e0.y);
-  const float d1 = fmaf(e1.x, tx - 1.0f, e1.y);
-  return lerpf(d0, d1, ux);
+  const int o = (int)fx - lo;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const float2* e = region + r * stride + o;
+    const float2 e0 = e[0], e1 = e[1];
+    if (NOISE == UPF_NOISE_VALUE) {
+      out[r] = lerpf(e0.x, e1.x, ux);
+    } else {
+      const float d0 = fmaf(e0.x, tx, e0.y);
+      const float d1 = fmaf(e1.x, tx - 1.0f, e1.y);
+      out[r] = lerpf(d0, d1, ux);
+    }
+  }
 }
 
 // Cellular: 63 lanes generate one feature point each, (lx, c9) = (lane % 7, lane / 7).
@@ -227,8 +247,8 @@ __device__ __forceinline__ void cell_write(const RowCtx& c, const DLayer& L, uin
   if (c9 < 9) region[lx + 8 * c9] = make_float2(px, fmaf(ez, ez, ey * ey));
 }
 
-__device__ __forceinline__ float cell_read(float qx, int lo, const float2* region) {
-  const float2* p = region + ((int)rintf(qx) - lo);  // cell (cx-1+dx, cy-1+dy, cz-1+dz) at p[dx + 8(dy+3dz)]
+__device__ __forceinline__ float cell_read(float qx, const float2* p) {
+  // cell (cx-1+dx, cy-1+dy, cz-1+dz) at p[dx + 8(dy+3dz)], p = region + rint(qx) - lo
   float best = __builtin_inff();
 #pragma unroll
   for (int dz = 0; dz < 3; dz++)
@@ -243,11 +263,13 @@ __device__ __forceinline__ float cell_read(float qx, int lo, const float2* regio
   return __builtin_sqrtf(best);
 }
 
-// One layer over the row.  The octave loops are unrolled, so each octave's span (lo, n)
-// and region base are wave-uniform scalars computed once.  When the layer's regions do
-// not fit the slice the layer is evaluated per lane.
-template <int NOISE>
-__device__ __forceinline__ float row_layer_t(const RowCtx& c, const DLayer& L, float x, float y, float z) {
+// One layer over R rows that share x and z (y[r] per row).  The octave loops are
+// unrolled, so each octave's span (lo, n) and region base are wave-uniform scalars
+// computed once; row r's regions follow row r-1's (total entries apart).  When the
+// layer's regions do not fit the slice the layer is evaluated per lane.
+template <int NOISE, int R>
+__device__ __forceinline__ void row_layer_t(const RowCtx& c, const DLayer& L, float x, const float* y, float z,
+                                            float* out) {
   constexpr bool CELL = NOISE == UPF_NOISE_CELLULAR;
   const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
   float2* slice = reinterpret_cast<float2*>(c.slice);
@@ -267,63 +289,106 @@ __device__ __forceinline__ float row_layer_t(const RowCtx& c, const DLayer& L, f
       fits = fits && (!CELL || n <= 7);
     }
   }
-  if (!fits || total > kRowSlice2) return layer_value(L, x, y, z);
+  if (!fits || total > kRowSlice2) {
+#pragma unroll
+    for (int r = 0; r < R; r++) out[r] = layer_value(L, x, y[r], z);
+    return;
+  }
+  if (R > 1 && R * total > kRowSlice2) {  // one row at a time: the same results as R = 1
+#pragma unroll
+    for (int r = 0; r < R; r++) row_layer_t<NOISE, 1>(c, L, x, y + r, z, out + r);
+    return;
+  }
   if (CELL) {
 #pragma unroll
-    for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
-      if (k < oct) {
-        const float f = L.freq[k];
-        cell_write(c, L, L.seed + (uint32_t)k, y * f, z * f, lo[k], slice + base[k]);
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+        if (k < oct) {
+          const float f = L.freq[k];
+          cell_write(c, L, L.seed + (uint32_t)k, y[r] * f, z * f, lo[k], slice + r * total + base[k]);
+        }
       }
-    }
   } else if (oct == 1) {
-    col_write<NOISE>(c, L.seed, y * L.freq[0], z * L.freq[0], lo[0], total, slice);
+#pragma unroll
+    for (int r = 0; r < R; r++) col_write<NOISE>(c, L.seed, y[r] * L.freq[0], z * L.freq[0], lo[0], total, slice + r * total);
   } else {
-    col_write_all<NOISE>(c, L, oct, y, z, lo, base, total, slice);
+    col_write_all<NOISE, R>(c, L, oct, y, z, lo, base, total, slice);
   }
   wave_sync();
-  float acc = 0.0f;
+  float acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = 0.0f;
 #pragma unroll
   for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
     if (k < oct) {
       const float q = x * L.freq[k];
-      const float n = CELL ? cell_read(q, lo[k], slice + base[k]) : col_read<NOISE>(q, lo[k], slice + base[k]);
-      acc = fractal_acc(L, k, n, acc);
+      float n[R];
+      if (CELL) {
+        const float2* p = slice + base[k] + ((int)rintf(q) - lo[k]);
+#pragma unroll
+        for (int r = 0; r < R; r++) n[r] = cell_read(q, p + r * total);
+      } else {
+        col_read<NOISE, R>(q, lo[k], slice + base[k], total, n);
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) acc[r] = fractal_acc(L, k, n[r], acc[r]);
     }
   }
   wave_sync();
-  return acc;
+#pragma unroll
+  for (int r = 0; r < R; r++) out[r] = acc[r];
 }
 
-__device__ __forceinline__ float row_layer_value(const RowCtx& c, const DLayer& L, float x, float y, float z) {
-  if (L.noise == UPF_NOISE_GRADIENT) return row_layer_t<UPF_NOISE_GRADIENT>(c, L, x, y, z);
-  if (L.noise == UPF_NOISE_VALUE) return row_layer_t<UPF_NOISE_VALUE>(c, L, x, y, z);
-  return row_layer_t<UPF_NOISE_CELLULAR>(c, L, x, y, z);
+template <int R>
+__device__ __forceinline__ void row_layer_value(const RowCtx& c, const DLayer& L, float x, const float* y, float z,
+                                                float* out) {
+  if (L.noise == UPF_NOISE_GRADIENT) row_layer_t<UPF_NOISE_GRADIENT, R>(c, L, x, y, z, out);
+  else if (L.noise == UPF_NOISE_VALUE) row_layer_t<UPF_NOISE_VALUE, R>(c, L, x, y, z, out);
+  else row_layer_t<UPF_NOISE_CELLULAR, R>(c, L, x, y, z, out);
 }
 
-__device__ __forceinline__ float row_elevation(const RowCtx& c, const DSpec& s, float x, float y, float z) {
-  float acc = 0.0f, m = 0.0f;
+template <int R>
+__device__ __forceinline__ void row_elevation(const RowCtx& c, const DSpec& s, float x, const float* y, float z,
+                                              float* out) {
+  float acc[R], m[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = m[r] = 0.0f;
   for (int l = 0; l < s.n_layers; l++) {
     const DLayer& L = s.L[l];
-    const float v = L.weight * row_layer_value(c, L, x, y, z);
-    if (L.blend == UPF_BLEND_ADD) acc = acc + v;
-    else if (L.blend == UPF_BLEND_MASK) m = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
-    else acc = fmaf(m, v - acc, acc);
+    float v[R];
+    row_layer_value<R>(c, L, x, y, z, v);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const float w = L.weight * v[r];
+      if (L.blend == UPF_BLEND_ADD) acc[r] = acc[r] + w;
+      else if (L.blend == UPF_BLEND_MASK) m[r] = __builtin_amdgcn_fmed3f(w + 0.5f, 0.0f, 1.0f);
+      else acc[r] = fmaf(m[r], w - acc[r], acc[r]);
+    }
   }
-  return acc;
+#pragma unroll
+  for (int r = 0; r < R; r++) out[r] = acc[r];
 }
 
-__device__ __forceinline__ float row_field(const RowCtx& c, const DSpec& s, float x, float y, float z) {
+// f at (x, y[r], z) for the R rows of the wave.
+template <int R>
+__device__ __forceinline__ void row_field(const RowCtx& c, const DSpec& s, float x, const float* y, float z,
+                                          float* out) {
   const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
                          s.compose == UPF_COMPOSE_ELEVATION_RAW;
-  const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval(s, x, y, z);
-  const float elev = need_elev ? row_elevation(c, s, x, y, z) : 0.0f;
-  switch (s.compose) {
-    case UPF_COMPOSE_LAND: return shape - s.escale * elev;
-    case UPF_COMPOSE_WATER: return shape;
-    case UPF_COMPOSE_NAVIGATION: return shape - s.escale * fmaxf(elev, 0.0f);
-    case UPF_COMPOSE_ELEVATION: return shape / s.escale;
-    default: return elev;
+  float elev[R];
+  if (need_elev) row_elevation<R>(c, s, x, y, z, elev);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval(s, x, y[r], z);
+    const float e = need_elev ? elev[r] : 0.0f;
+    switch (s.compose) {
+      case UPF_COMPOSE_LAND: out[r] = shape - s.escale * e; break;
+      case UPF_COMPOSE_WATER: out[r] = shape; break;
+      case UPF_COMPOSE_NAVIGATION: out[r] = shape - s.escale * fmaxf(e, 0.0f); break;
+      case UPF_COMPOSE_ELEVATION: out[r] = shape / s.escale; break;
+      default: out[r] = e; break;
+    }
   }
 }
 

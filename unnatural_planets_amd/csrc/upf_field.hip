@@ -18,38 +18,65 @@ namespace upf {
 constexpr int kGridBX = 64;
 constexpr int kGridBY = 4;
 
-__global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g, float* __restrict__ out) {
+// Optional fused output: the marching-cubes sign mask of the wave's 64-node group,
+// bit = (node inside the row and f < iso), at S[(k * ny + j) * gpr + xb].
+__device__ __forceinline__ void write_signs(uint64_t* __restrict__ S, const DGrid& g, int xb, int j, int k,
+                                            bool valid, float v, float iso) {
+  const uint64_t m = __ballot(valid && v < iso);
+  if ((threadIdx.x & 63) == 0) S[((int64_t)k * g.n[1] + j) * ((g.n[0] + 63) / 64) + xb] = m;
+}
+
+__global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g, float* __restrict__ out,
+                                                   uint64_t* __restrict__ S, float iso) {
   const int i = blockIdx.x * kGridBX + threadIdx.x;
   const int j = blockIdx.y * kGridBY + threadIdx.y;
   const int k = blockIdx.z;
-  if (i >= g.n[0] || j >= g.n[1]) return;
-  const float x = node_coord(g.o[0], g.s[0], i);
-  const float y = node_coord(g.o[1], g.s[1], j);
-  const float z = node_coord(g.o[2], g.s[2], k);
-  out[((size_t)k * g.n[1] + j) * g.n[0] + i] = field_eval(s, x, y, z);
-}
-
-// Wave-cooperative rows (upf_device_row.hpp): each wave evaluates 64 x nodes of one
-// row; lanes past the end of the row stand in for the last node so that all 64 lanes
-// take part in the shared lattice work, and only real nodes are stored.
-__global__ __launch_bounds__(256) void k_eval_grid_row(const DSpec s, const DGrid g, float* __restrict__ out) {
-  __shared__ float4 s_slice[kGridBY][kRowSlice];
-  __shared__ float4 s_gtab[16];
-  __shared__ float4 s_oct[kGridBY][kRowOctTab];
-  if (threadIdx.y == 0 && threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
-  __syncthreads();
-  const int lane = threadIdx.x;
-  const int i = blockIdx.x * kGridBX + lane;
-  const int j = __builtin_amdgcn_readfirstlane(blockIdx.y * kGridBY + threadIdx.y);
-  const int k = blockIdx.z;
-  if (j >= g.n[1]) return;
+  if (j >= g.n[1]) return;  // whole waves
+  const bool valid = i < g.n[0];
   const int ic = min(i, g.n[0] - 1);
   const float x = node_coord(g.o[0], g.s[0], ic);
   const float y = node_coord(g.o[1], g.s[1], j);
   const float z = node_coord(g.o[2], g.s[2], k);
+  const float v = field_eval(s, x, y, z);
+  if (valid) out[((size_t)k * g.n[1] + j) * g.n[0] + i] = v;
+  if (S) write_signs(S, g, blockIdx.x, j, k, valid, v, iso);
+}
+
+// Wave-cooperative rows (upf_device_row.hpp): each wave evaluates 64 x nodes of two
+// rows j, j+1 of one plane (they share the lattice column spans and the x part of the
+// interpolation); lanes past the end of the row stand in for the last node so that all
+// 64 lanes take part in the shared lattice work, and only real nodes are stored.  A
+// workgroup covers 64 x nodes of kRowPairs row pairs.
+constexpr int kRowPairs = 4;
+
+__global__ __launch_bounds__(256) void k_eval_grid_row(const DSpec s, const DGrid g, float* __restrict__ out,
+                                                       uint64_t* __restrict__ S, float iso) {
+  __shared__ float4 s_slice[kRowPairs][kRowSlice];
+  __shared__ float4 s_gtab[16];
+  __shared__ float4 s_oct[kRowPairs][kRowOctTab];
+  if (threadIdx.y == 0 && threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x * kGridBX + lane;
+  const int j = __builtin_amdgcn_readfirstlane((blockIdx.y * kRowPairs + threadIdx.y) * 2);
+  const int k = blockIdx.z;
+  if (j >= g.n[1]) return;
+  const bool two = j + 1 < g.n[1];
+  const int ic = min(i, g.n[0] - 1);
+  const float x = node_coord(g.o[0], g.s[0], ic);
+  const float y[2] = {node_coord(g.o[1], g.s[1], j), node_coord(g.o[1], g.s[1], two ? j + 1 : j)};
+  const float z = node_coord(g.o[2], g.s[2], k);
   const RowCtx c = make_row_ctx(s_slice[threadIdx.y], s_gtab, s_oct[threadIdx.y], lane);
-  const float v = row_field(c, s, x, y, z);
-  if (i < g.n[0]) out[((size_t)k * g.n[1] + j) * g.n[0] + i] = v;
+  float v[2];
+  row_field<2>(c, s, x, y, z, v);
+  const bool valid = i < g.n[0];
+  const size_t row0 = ((size_t)k * g.n[1] + j) * g.n[0];
+  if (valid) out[row0 + i] = v[0];
+  if (two && valid) out[row0 + g.n[0] + i] = v[1];
+  if (S) {
+    write_signs(S, g, blockIdx.x, j, k, valid, v[0], iso);
+    if (two) write_signs(S, g, blockIdx.x, j + 1, k, valid, v[1], iso);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_eval_points(const DSpec s, const float* __restrict__ xyz, int64_t n,
@@ -116,6 +143,20 @@ int adjoint_blocks(int64_t n) {
 }
 }  // namespace
 
+int field_grid_launch(const DSpec& s, const DGrid& g, float* d_field, int variant, uint64_t* S, float iso,
+                      hipStream_t stream) {
+  if (variant < 0 || variant > UPF_EVAL_PER_LANE) return UPF_ERR_BAD_SPEC;
+  const dim3 blocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + kGridBY - 1) / kGridBY, g.n[2]);
+  if (variant == UPF_EVAL_PER_LANE) {
+    hipLaunchKernelGGL(k_eval_grid, blocks, dim3(kGridBX, kGridBY, 1), 0, stream, s, g, d_field, S, iso);
+  } else {
+    const dim3 rblocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + 2 * kRowPairs - 1) / (2 * kRowPairs), g.n[2]);
+    hipLaunchKernelGGL(k_eval_grid_row, rblocks, dim3(kGridBX, kRowPairs, 1), 0, stream, s, g, d_field, S, iso);
+  }
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
 }  // namespace upf
 
 using namespace upf;
@@ -130,15 +171,7 @@ int upf_field_eval_grid_ex(const upf_field_spec* spec, const upf_grid* grid, flo
   int st = to_device(spec, &s);
   if (st) return st;
   if ((st = to_device(grid, &g))) return st;
-  const int variant = opts ? opts->variant : UPF_EVAL_AUTO;
-  if (variant < 0 || variant > UPF_EVAL_PER_LANE) return UPF_ERR_BAD_SPEC;
-  const dim3 blocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + kGridBY - 1) / kGridBY, g.n[2]);
-  if (variant == UPF_EVAL_PER_LANE)
-    hipLaunchKernelGGL(k_eval_grid, blocks, dim3(kGridBX, kGridBY, 1), 0, as_stream(stream), s, g, d_field);
-  else
-    hipLaunchKernelGGL(k_eval_grid_row, blocks, dim3(kGridBX, kGridBY, 1), 0, as_stream(stream), s, g, d_field);
-  UPF_LAUNCH_CHECK();
-  return UPF_OK;
+  return field_grid_launch(s, g, d_field, opts ? opts->variant : UPF_EVAL_AUTO, nullptr, 0.0f, as_stream(stream));
 }
 
 int upf_field_eval_grid(const upf_field_spec* spec, const upf_grid* grid, float* d_field, void* stream) {

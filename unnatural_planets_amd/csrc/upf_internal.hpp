@@ -16,6 +16,11 @@ int to_device(const upf_grid* in, DGrid* out);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Grid evaluation (variant = upf_eval_variant); with S non-null the kernel also writes
+// each 64-node group's marching-cubes sign mask (bit = f < iso) at S[(k*ny + j)*gpr + xb].
+int field_grid_launch(const DSpec& s, const DGrid& g, float* d_field, int variant, uint64_t* S, float iso,
+                      hipStream_t stream);
+
 // Compute units of the current device (cached per device; 256 if the query fails).
 int device_cus();
 

@@ -69,6 +69,73 @@ __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, co
                 s_nb[wave], pos, nrm, idx, cap_v, cap_t);
 }
 
+// Classification from the sign masks alone (the field kernel wrote them).  One lane per
+// group: its crossing masks come from S of the group, of its +x neighbour group and of
+// the +y / +z rows (8 words), so vertex counts are a few 64-bit operations per lane.
+// Triangle counts need the 64 cell codes: groups whose cells all have uniform corners
+// (the 8 words agree on every cell lane) have none; each remaining group of the wave is
+// then taken by the whole wave, one cell per lane, with the words broadcast from the
+// group's lane.
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __restrict__ S, const McDims d,
+                                                           uint32_t* __restrict__ counts) {
+  __shared__ uint8_t s_ntris[256];
+  s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
+  __syncthreads();
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool in = g < d.groups;
+  uint64_t a = 0, a1 = 0, ay = 0, ay1 = 0, b = 0, b1 = 0, by = 0, by1 = 0, cells = 0;
+  int nv = 0;
+  if (in) {
+    const uint32_t row = (uint32_t)(g / d.gpr);
+    const int xb = (int)(g - (int64_t)row * d.gpr);
+    const int k = (int)(row / (uint32_t)d.ny), j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
+    const bool nx1 = xb + 1 < d.gpr, hy = j + 1 < d.ny, hz = k + 1 < d.nz;
+    const int64_t plane = (int64_t)d.ny * d.gpr;
+    a = S[g];
+    const uint64_t ax = nx1 ? S[g + 1] : 0ull;
+    ay = hy ? S[g + d.gpr] : 0ull;
+    const uint64_t axy = (hy && nx1) ? S[g + d.gpr + 1] : 0ull;
+    b = hz ? S[g + plane] : 0ull;
+    const uint64_t bx = (hz && nx1) ? S[g + plane + 1] : 0ull;
+    by = (hz && hy) ? S[g + plane + d.gpr] : 0ull;
+    const uint64_t bxy = (hz && hy && nx1) ? S[g + plane + d.gpr + 1] : 0ull;
+    const int first = xb * 64;
+    const int nval = min(64, d.nx - first), nvx = min(64, d.nx - 1 - first);
+    const uint64_t vmask = nval >= 64 ? ~0ull : ((1ull << nval) - 1);
+    const uint64_t xmask = nvx >= 64 ? ~0ull : (nvx > 0 ? ((1ull << nvx) - 1) : 0ull);
+    // bit i of *1 = node i + 1 of the same row
+    a1 = (a >> 1) | (ax << 63); ay1 = (ay >> 1) | (axy << 63);
+    b1 = (b >> 1) | (bx << 63); by1 = (by >> 1) | (bxy << 63);
+    nv = __popcll((a ^ a1) & xmask) + (hy ? __popcll((a ^ ay) & vmask) : 0) + (hz ? __popcll((a ^ b) & vmask) : 0);
+    cells = (hy && hz) ? xmask : 0ull;
+  }
+  const uint64_t any = (a | a1 | ay | ay1 | b | b1 | by | by1) & cells;
+  const uint64_t all = (a & a1 & ay & ay1 & b & b1 & by & by1) & cells;
+  int ntot = 0;
+  uint64_t work = ballot64(in && any != 0ull && all != cells);
+  while (work) {
+    const int src = __ffsll((unsigned long long)work) - 1;
+    work &= work - 1;
+    const uint64_t A = readlane64(a, src), A1 = readlane64(a1, src), AY = readlane64(ay, src);
+    const uint64_t AY1 = readlane64(ay1, src), B = readlane64(b, src), B1 = readlane64(b1, src);
+    const uint64_t BY = readlane64(by, src), BY1 = readlane64(by1, src), C = readlane64(cells, src);
+    const int code = (int)((A >> lane) & 1) | (int)((A1 >> lane) & 1) << 1 | (int)((AY1 >> lane) & 1) << 2 |
+                     (int)((AY >> lane) & 1) << 3 | (int)((B >> lane) & 1) << 4 | (int)((B1 >> lane) & 1) << 5 |
+                     (int)((BY1 >> lane) & 1) << 6 | (int)((BY >> lane) & 1) << 7;
+    const int nt = ((C >> lane) & 1ull) ? s_ntris[code] : 0;
+    const int t = __popcll(ballot64(nt & 1)) + 2 * __popcll(ballot64(nt & 2)) + 4 * __popcll(ballot64(nt & 4));
+    if (lane == src) ntot = t;
+  }
+  if (in) counts[g] = pack_counts(nv, ntot);
+}
+
 // ------------------------------------------------------------------ scan
 // Three running sums at once: x = vertices, y = triangles, z = active groups.
 constexpr int kScanThreads = 256;
@@ -252,6 +319,16 @@ int mc_scan_launch(const uint32_t* counts, int64_t groups, uint4* tiles, uint4* 
   return UPF_OK;
 }
 
+int mc_classify_signs_scan(const DGrid& g, void* d_ws, size_t ws_bytes, hipStream_t s) {
+  const McLayout L = mc_layout(g);
+  if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
+  const McPtrs P = mc_ptrs(L, d_ws);
+  hipLaunchKernelGGL(k_mc_classify_signs, dim3((unsigned)((L.d.groups + 255) / 256)), dim3(256), 0, s, P.S, L.d,
+                     P.counts);
+  UPF_LAUNCH_CHECK();
+  return mc_scan_launch(P.counts, L.d.groups, P.tiles, P.total, P.voff, P.act, nullptr, s);
+}
+
 int64_t mc_scan_tiles(int64_t groups) { return (groups + kScanTile - 1) / kScanTile; }
 
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
@@ -287,6 +364,29 @@ int upf_mc_count(const float* d_field, const upf_grid* grid, float iso, void* d_
   hipStream_t s = as_stream(stream);
   if ((st = mc_classify_scan(d_field, g, iso, d_ws, ws_bytes, s))) return st;
   const McPtrs P = mc_ptrs(mc_layout(g), d_ws);
+  uint4 h_total;
+  UPF_HIP_CHECK(hipMemcpyAsync(&h_total, P.total, sizeof(uint4), hipMemcpyDeviceToHost, s));
+  UPF_HIP_CHECK(hipStreamSynchronize(s));
+  *n_vertices = h_total.x;
+  *n_triangles = h_total.y;
+  return UPF_OK;
+}
+
+int upf_field_mesh_count(const upf_field_spec* spec, const upf_grid* grid, float iso, float* d_field,
+                         const upf_eval_opts* opts, void* d_ws, size_t ws_bytes, int64_t* n_vertices,
+                         int64_t* n_triangles, void* stream) {
+  if (!d_field || !n_vertices || !n_triangles) return UPF_ERR_NULL;
+  if (!std::isfinite(iso)) return UPF_ERR_INVALID_VALUE;
+  DSpec sp;
+  DGrid g;
+  int st = to_device(spec, &sp);
+  if (st || (st = to_device(grid, &g)) || (st = mc_grid_ok(g))) return st;
+  const McLayout L = mc_layout(g);
+  if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
+  const McPtrs P = mc_ptrs(L, d_ws);
+  hipStream_t s = as_stream(stream);
+  if ((st = field_grid_launch(sp, g, d_field, opts ? opts->variant : UPF_EVAL_AUTO, P.S, iso, s))) return st;
+  if ((st = mc_classify_signs_scan(g, d_ws, ws_bytes, s))) return st;
   uint4 h_total;
   UPF_HIP_CHECK(hipMemcpyAsync(&h_total, P.total, sizeof(uint4), hipMemcpyDeviceToHost, s));
   UPF_HIP_CHECK(hipStreamSynchronize(s));

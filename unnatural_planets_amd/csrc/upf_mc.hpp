@@ -37,6 +37,8 @@ McPtrs mc_ptrs(const McLayout& L, void* ws);
 int mc_grid_ok(const DGrid& g);
 // classify + scan, asynchronous; totals stay in the workspace
 int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws, size_t ws_bytes, hipStream_t s);
+// classify from the sign masks already in the workspace (written by the field kernel) + scan
+int mc_classify_signs_scan(const DGrid& g, void* d_ws, size_t ws_bytes, hipStream_t s);
 // emit over at most max_active active groups (the kernel reads the true count)
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
                    float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s);

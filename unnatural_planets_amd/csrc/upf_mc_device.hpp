@@ -96,26 +96,22 @@ __device__ __forceinline__ Cross group_crossings(const uint64_t* __restrict__ S,
   return c;
 }
 
-__device__ __forceinline__ float node_gradient_axis(const float* __restrict__ f, int64_t n, int idx, int nmax,
-                                                    int64_t step, float sp) {
-  if (idx == 0) return (f[n + step] - f[n]) / sp;
-  if (idx == nmax - 1) return (f[n] - f[n - step]) / sp;
-  return (f[n + step] - f[n - step]) / (2.0f * sp);
-}
-
-__device__ __forceinline__ void node_gradient(const float* __restrict__ f, const McDims& d, const DGrid& g, int i,
-                                              int j, int k, float out[3]) {
-  const int64_t n = (int64_t)k * d.nxy + (int64_t)j * d.nx + i;
-  out[0] = node_gradient_axis(f, n, i, d.nx, 1, g.s[0]);
-  out[1] = node_gradient_axis(f, n, j, d.ny, d.nx, g.s[1]);
-  out[2] = node_gradient_axis(f, n, k, d.nz, d.nxy, g.s[2]);
+// One axis of the central-difference gradient at a node (one-sided on the grid faces),
+// given f one step up (fp), at the node (f0) and one step down (fm).
+__device__ __forceinline__ float grad_axis(int idx, int nmax, float fp, float f0, float fm, float sp) {
+  if (idx == 0) return (fp - f0) / sp;
+  if (idx == nmax - 1) return (f0 - fm) / sp;
+  return (fp - fm) / (2.0f * sp);
 }
 
 // One wave emits active group gid (local to its grid): vertices at voff (global ids,
 // stored at pos/nrm[id]) and triangles from toff (global), with index values made local
 // by subtracting vbase.  Writes at or past cap_v / cap_t are dropped (the caller sized
 // the buffers from the counts, or checks the counts afterwards).  s_nb: 8 LDS entries
-// private to this wave.
+// private to this wave.  Every field value a lane needs -- its cell's 8 corners and the
+// central-difference neighbours of its node and of its three edges' far ends -- is
+// loaded up front with clamped (in-bounds) indices, in the same round trip as the
+// counts and sign masks.
 __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
                                               const uint64_t* __restrict__ S, const uint32_t* __restrict__ counts,
                                               const uint32_t* __restrict__ voff, uint32_t gid, uint32_t toff,
@@ -127,6 +123,26 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
   const int xb = (int)(gid - row * (uint32_t)d.gpr);
   const int k = (int)(row / (uint32_t)d.ny);
   const int j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
+  const int i = xb * 64 + lane;
+  const bool valid = i < d.nx;
+
+  // field values (clamped coordinates; a clamped value is never used)
+  const int ic = min(i, d.nx - 1);
+  const int64_t sy = d.nx, sz = d.nxy;
+  const int xs[4] = {max(ic - 1, 0), ic, min(ic + 1, d.nx - 1), min(ic + 2, d.nx - 1)};  // i-1 .. i+2
+  const int ys[4] = {max(j - 1, 0), j, min(j + 1, d.ny - 1), min(j + 2, d.ny - 1)};
+  const int zs[4] = {max(k - 1, 0), k, min(k + 1, d.nz - 1), min(k + 2, d.nz - 1)};
+  auto at = [&](int xi, int yi, int zi) { return f[(int64_t)zs[zi] * sz + (int64_t)ys[yi] * sy + xs[xi]]; };
+  const float c0 = at(1, 1, 1), c1 = at(2, 1, 1), c2 = at(2, 2, 1), c3 = at(1, 2, 1);
+  const float c4 = at(1, 1, 2), c5 = at(2, 1, 2), c6 = at(2, 2, 2), c7 = at(1, 2, 2);
+  float xm = 0.f, ym = 0.f, zm = 0.f, x2 = 0.f, xym = 0.f, xzm = 0.f, y2 = 0.f, yxm = 0.f, yzm = 0.f;
+  float z2 = 0.f, zxm = 0.f, zym = 0.f;
+  if (nrm) {
+    xm = at(0, 1, 1); ym = at(1, 0, 1); zm = at(1, 1, 0);
+    x2 = at(3, 1, 1); xym = at(2, 0, 1); xzm = at(2, 1, 0);
+    y2 = at(1, 3, 1); yxm = at(0, 2, 1); yzm = at(1, 2, 0);
+    z2 = at(1, 1, 3); zxm = at(0, 1, 2); zym = at(1, 0, 2);
+  }
   const uint32_t c = counts[gid];
   const int cnt_v = c & 0xFF, cnt_t = (c >> 8) & 0x1FF;
 
@@ -140,9 +156,6 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-  const int i = xb * 64 + lane;
-  const bool valid = i < d.nx;
-  const int64_t n = (int64_t)k * d.nxy + (int64_t)j * d.nx + i;
   const uint64_t lt = (1ull << lane) - 1;
 
   // ---- vertices
@@ -152,17 +165,21 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
     const uint64_t bits[3] = {cc.bx, cc.by, cc.bz};
     const bool any = ((cc.bx | cc.by | cc.bz) >> lane) & 1ull;
     if (any) {
-      const float f0 = f[n];
+      const float f0 = c0;
       const float p0[3] = {node_coord(g.o[0], g.s[0], i), node_coord(g.o[1], g.s[1], j),
                            node_coord(g.o[2], g.s[2], k)};
-      float g0[3] = {0.0f, 0.0f, 0.0f};
-      if (nrm) node_gradient(f, d, g, i, j, k, g0);
-      const int64_t step[3] = {1, d.nx, d.nxy};
+      const float f1s[3] = {c1, c3, c4};
       const int ijk[3] = {i, j, k};
+      float g0[3] = {0.0f, 0.0f, 0.0f};
+      if (nrm) {
+        g0[0] = grad_axis(i, d.nx, c1, c0, xm, g.s[0]);
+        g0[1] = grad_axis(j, d.ny, c3, c0, ym, g.s[1]);
+        g0[2] = grad_axis(k, d.nz, c4, c0, zm, g.s[2]);
+      }
 #pragma unroll
       for (int ax = 0; ax < 3; ax++) {
         if (!((bits[ax] >> lane) & 1ull)) continue;
-        const float f1 = f[n + step[ax]];
+        const float f1 = f1s[ax];
         const float t = (iso - f0) / (f1 - f0);
         const float p1a = node_coord(g.o[ax], g.s[ax], ijk[ax] + 1);
         float p[3] = {p0[0], p0[1], p0[2]};
@@ -172,8 +189,21 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
           pos[3 * (size_t)vid + 1] = p[1];
           pos[3 * (size_t)vid + 2] = p[2];
           if (nrm) {
+            // gradient at the edge's far end
             float g1[3];
-            node_gradient(f, d, g, i + (ax == 0), j + (ax == 1), k + (ax == 2), g1);
+            if (ax == 0) {
+              g1[0] = grad_axis(i + 1, d.nx, x2, c1, c0, g.s[0]);
+              g1[1] = grad_axis(j, d.ny, c2, c1, xym, g.s[1]);
+              g1[2] = grad_axis(k, d.nz, c5, c1, xzm, g.s[2]);
+            } else if (ax == 1) {
+              g1[0] = grad_axis(i, d.nx, c2, c3, yxm, g.s[0]);
+              g1[1] = grad_axis(j + 1, d.ny, y2, c3, c0, g.s[1]);
+              g1[2] = grad_axis(k, d.nz, c7, c3, yzm, g.s[2]);
+            } else {
+              g1[0] = grad_axis(i, d.nx, c5, c4, zxm, g.s[0]);
+              g1[1] = grad_axis(j, d.ny, c7, c4, zym, g.s[1]);
+              g1[2] = grad_axis(k + 1, d.nz, z2, c4, c0, g.s[2]);
+            }
             const float gx = g0[0] + t * (g1[0] - g0[0]);
             const float gy = g0[1] + t * (g1[1] - g0[1]);
             const float gz = g0[2] + t * (g1[2] - g0[2]);
@@ -193,12 +223,9 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
   if (cnt_t) {
     const bool cell = valid && i + 1 < d.nx && j + 1 < d.ny && k + 1 < d.nz;
     int code = 0;
-    if (cell) {
-      const float c0 = f[n], c1 = f[n + 1], c3 = f[n + d.nx], c2 = f[n + d.nx + 1];
-      const float c4 = f[n + d.nxy], c5 = f[n + d.nxy + 1], c7 = f[n + d.nxy + d.nx], c6 = f[n + d.nxy + d.nx + 1];
+    if (cell)
       code = (c0 < iso) | ((c1 < iso) << 1) | ((c2 < iso) << 2) | ((c3 < iso) << 3) | ((c4 < iso) << 4) |
              ((c5 < iso) << 5) | ((c6 < iso) << 6) | ((c7 < iso) << 7);
-    }
     const int nt = upf_mc_ntris[code];
     int inc = nt;
 #pragma unroll
