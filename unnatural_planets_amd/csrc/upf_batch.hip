@@ -3,7 +3,7 @@
 // task, sources/generator.cpp:159).
 //
 // Work items (a wavefront each), enumerated chunk after chunk:
-//   field     one grid row segment of 64 x nodes          (chunk, xb, j, k)
+//   field     64 x nodes of a pair of grid rows            (chunk, xb, j / 2, k)
 //   classify  one group column over classify_z planes      (chunk, xb, j, kz)
 //   emit      one active group of the concatenated scan
 // A fixed grid of workgroups (blocks_per_cu per compute unit) strides over each list,
@@ -34,7 +34,7 @@ struct BChunk {
   int32_t gx;          // 64-node row segments per row
   int64_t node_off;    // first field value of the chunk
   int64_t group_off;   // first marching-cubes group of the chunk
-  int64_t row_off;     // first field item
+  int64_t row_off;     // first field item (row pairs)
   int64_t col_off;     // first classify item
 };
 
@@ -68,17 +68,28 @@ __global__ __launch_bounds__(256) void k_batch_field(const BChunk* __restrict__ 
     const int64_t local = r - C.row_off;
     const int xb = (int)(local % C.gx);
     const int64_t rest = local / C.gx;
-    const int j = (int)(rest % C.grid.n[1]);
-    const int k = (int)(rest / C.grid.n[1]);
+    const int pairs = (C.grid.n[1] + 1) / 2;
+    const int j = 2 * (int)(rest % pairs);
+    const int k = (int)(rest / pairs);
+    const bool two = j + 1 < C.grid.n[1];
     const int i = xb * 64 + lane;
     const int ic = min(i, C.grid.n[0] - 1);
     const float x = node_coord(C.grid.o[0], C.grid.s[0], ic);
-    const float y = node_coord(C.grid.o[1], C.grid.s[1], j);
+    const float y[2] = {node_coord(C.grid.o[1], C.grid.s[1], j),
+                        node_coord(C.grid.o[1], C.grid.s[1], two ? j + 1 : j)};
     const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
-    float v;
-    if (VARIANT == UPF_EVAL_PER_LANE) v = field_eval(C.spec, x, y, z);
-    else row_field<1>(ctx, C.spec, x, &y, z, &v);
-    if (i < C.grid.n[0]) field[C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + i] = v;
+    float v[2];
+    if (VARIANT == UPF_EVAL_PER_LANE) {
+      v[0] = field_eval(C.spec, x, y[0], z);
+      v[1] = two ? field_eval(C.spec, x, y[1], z) : v[0];
+    } else {
+      row_field<2>(ctx, C.spec, x, y, z, v);
+    }
+    if (i < C.grid.n[0]) {
+      const int64_t o = C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + i;
+      field[o] = v[0];
+      if (two) field[o + C.grid.n[0]] = v[1];
+    }
   }
 }
 
@@ -264,7 +275,7 @@ int upf_batch_create(const upf_chunk* chunks, int32_t n_chunks, const upf_batch_
     C.row_off = b->rows;
     b->nodes += (int64_t)C.grid.n[0] * C.grid.n[1] * C.grid.n[2];
     b->groups += C.md.groups;
-    b->rows += (int64_t)C.gx * C.grid.n[1] * C.grid.n[2];
+    b->rows += (int64_t)C.gx * ((C.grid.n[1] + 1) / 2) * C.grid.n[2];
   }
   if (b->groups >= ((int64_t)1 << 31)) { delete b; return UPF_ERR_BAD_GRID; }
   size_t o = 0;
