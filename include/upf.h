@@ -380,6 +380,21 @@ int upf_bake(const upf_bake_spec* spec, const float* d_positions, const float* d
              const int32_t* d_tri, int32_t width, int32_t height, float* d_albedo, float* d_bump,
              float* d_roughness, void* stream);
 
+/* ---- texture post-processing -----------------------------------------------------
+ * The steps after the bake in Generator::generate (sources/textures.cpp:73-81):
+ * imageDilation(image, 7, true) and imageConvert(image, U8).  cage's kernels are not in
+ * the reference checkout; the rules here are documented choices.
+ *
+ * Dilation: `rounds` passes; in each pass a texel whose first channel is NaN takes, per
+ * channel, the mean of its 8 neighbours whose first channel is not NaN (summed in the
+ * order dy = -1..1, dx = -1..1, then divided by their count), or stays NaN when it has
+ * none; other texels keep their value.  d_img: float[h][w][channels], in place;
+ * d_scratch: the same size. */
+int upf_image_dilate(float* d_img, float* d_scratch, int32_t width, int32_t height, int32_t channels,
+                     int32_t rounds, void* stream);
+/* U8 conversion: out = NaN ? 0 : (uint8)(saturate(v) * 255 + 0.5), n values. */
+int upf_image_to_u8(const float* d_img, int64_t n_values, uint8_t* d_out, void* stream);
+
 /* ---- batched chunks ---------------------------------------------------------------
  * Many chunks (planets) of uneven grid sizes, shapes and seeds evaluated and meshed by
  * one set of persistent launches: field rows, classify columns and active groups of

@@ -10,6 +10,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <vector>
 
 #include "upf_oracle.hpp"
@@ -136,6 +137,44 @@ inline void bake(const upf_bake_spec& b, const float* P, const float* N, const u
       }
       material(b, p, n, albedo + 3 * o, bump[o], rough[o]);
     }
+}
+
+// Dilation pass and U8 conversion as documented at upf_image_dilate / upf_image_to_u8
+// (the steps of sources/textures.cpp:73-81).
+inline void dilate(const float* in, float* out, int w, int h, int C) {
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      const int64_t o = ((int64_t)y * w + x) * C;
+      if (!std::isnan(in[o])) {
+        for (int c = 0; c < C; c++) out[o + c] = in[o + c];
+        continue;
+      }
+      float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      int cnt = 0;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int yy = y + dy, xx = x + dx;
+          if ((dx == 0 && dy == 0) || yy < 0 || yy >= h || xx < 0 || xx >= w) continue;
+          const int64_t q = ((int64_t)yy * w + xx) * C;
+          if (std::isnan(in[q])) continue;
+          for (int c = 0; c < C; c++) sum[c] = sum[c] + in[q + c];
+          cnt++;
+        }
+      for (int c = 0; c < C; c++) out[o + c] = cnt ? sum[c] / (float)cnt : NAN;
+    }
+}
+
+inline void dilate_rounds(float* img, int w, int h, int C, int rounds) {
+  std::vector<float> tmp((size_t)w * h * C);
+  for (int r = 0; r < rounds; r++) {
+    dilate(img, tmp.data(), w, h, C);
+    std::copy(tmp.begin(), tmp.end(), img);
+  }
+}
+
+inline uint8_t to_u8(float v) {
+  if (std::isnan(v)) return 0;
+  return (uint8_t)(std::min(std::max(v, 0.0f), 1.0f) * 255.0f + 0.5f);
 }
 
 }  // namespace upfo

@@ -194,6 +194,11 @@ void upfo_bake(const upf_bake_spec* b, const float* P, const float* N, const uin
   bake(*b, P, N, I, uv, tri, w, h, albedo, bump, rough);
 }
 
+void upfo_image_dilate(float* img, int w, int h, int channels, int rounds) { dilate_rounds(img, w, h, channels, rounds); }
+void upfo_image_to_u8(const float* img, int64_t n, uint8_t* out) {
+  for (int64_t i = 0; i < n; i++) out[i] = to_u8(img[i]);
+}
+
 // ---- sources/math.cpp helpers (FP64), for closed-form checks
 double upfo_rescale(double v, double ia, double ib, double oa, double ob) { return rescale<double>(v, ia, ib, oa, ob); }
 double upfo_sharp_edge(double v, double p) { return sharp_edge<double>(v, p); }

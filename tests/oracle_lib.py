@@ -50,6 +50,8 @@ class Oracle:
             "upfo_rasterize": (None, [_P, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _P]),
             "upfo_bake": (None, [ctypes.POINTER(BakeSpec), _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P,
                                  _P]),
+            "upfo_image_dilate": (None, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "upfo_image_to_u8": (None, [_P, ctypes.c_int64, _P]),
             "upfo_num_threads": (ctypes.c_int, []),
             "upfo_set_num_threads": (None, [ctypes.c_int]),
             "upfo_rescale": (ctypes.c_double, [ctypes.c_double] * 5),
@@ -170,6 +172,19 @@ class Oracle:
         self.L.upfo_bake(ctypes.byref(b), _f32p(pos), _f32p(nrm), _f32p(idx), _f32p(uv), _f32p(tri), w, h,
                          _f32p(albedo), _f32p(bump), _f32p(rough))
         return albedo, bump, rough
+
+    def image_dilate(self, img, rounds):
+        img = np.array(img, dtype=np.float32, order="C", copy=True)
+        h, w = img.shape[:2]
+        c = 1 if img.ndim == 2 else img.shape[2]
+        self.L.upfo_image_dilate(_f32p(img), w, h, c, rounds)
+        return img
+
+    def image_to_u8(self, img):
+        img = np.ascontiguousarray(img, dtype=np.float32)
+        out = np.empty(img.shape, dtype=np.uint8)
+        self.L.upfo_image_to_u8(_f32p(img), img.size, _f32p(out))
+        return out
 
 
 def mesh_edge_stats(idx: np.ndarray):

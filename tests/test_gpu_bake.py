@@ -100,3 +100,25 @@ def test_bake_empty_mesh_and_capacity(upf):
     with pytest.raises(upfm.UpfError) as e:
         upfm.bake(bad, m, uv, tri)
     assert e.value.status == S.ERR_INVALID_VALUE
+
+
+def test_texture_post_dilation_and_u8_bit_exact(upf, oracle):
+    """Dilation by 7 and U8 conversion of the sphere bake's images (the reference's
+    textures.cpp:73-81 steps), GPU against the oracle, bit for bit."""
+    s = S.spec_sphere_fbm(1234)
+    mesh = upfm.mesh_generate_generic(s, S.grid_sphere_fbm(96))
+    tex = upfm.generate_textures_land(S.bake_spec(s), mesh, 512, 384)
+    for img, rounds in ((tex.albedo, 7), (tex.bump, 7), (tex.roughness, 3), (tex.albedo, 8)):
+        ref = oracle.image_dilate(img.cpu().numpy(), rounds)
+        got = upfm.image_dilate(img.clone(), rounds).cpu().numpy()
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+        np.testing.assert_array_equal(got[~np.isnan(got)], ref[~np.isnan(ref)])
+        np.testing.assert_array_equal(upfm.image_to_u8(torch_from(got)).cpu().numpy(), oracle.image_to_u8(ref))
+    albedo, special, height = upfm.texture_post(tex)
+    assert albedo.shape == (384, 512, 3) and special.shape == (384, 512, 2) and height.shape == (384, 512)
+    assert int(special[..., 1].max()) == 0  # metallic
+
+
+def torch_from(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()

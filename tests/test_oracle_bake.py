@@ -8,7 +8,10 @@
   * barycentrics: affine reproduction -- with vertex positions an affine function of UV,
     the interpolated position is that function at the texel centre, recovered through
     the bump output; the normal is checked the same way through the slope term;
-  * material: flat colour zones of the elevation bands.
+  * material: flat colour zones of the elevation bands;
+  * dilation: a single dyadic seed fills exactly its Chebyshev ball of radius `rounds`;
+    valid texels never change; a gap between two seeds takes their mean;
+  * U8 conversion: hand-computed values, NaN to 0, saturation.
 """
 import numpy as np
 import pytest
@@ -160,3 +163,28 @@ def test_bake_band_colours(oracle):
     t = (elev > bands[1] - bw) & (elev < bands[1] + bw)
     lo, hi = np.minimum(cols[1], cols[2]), np.maximum(cols[1], cols[2])
     assert t.sum() > 10 and np.all(a[t] >= lo - 1e-6) and np.all(a[t] <= hi + 1e-6)
+
+
+def test_dilation_closed_forms(oracle):
+    w, h = 40, 30
+    img = np.full((h, w, 3), np.nan, dtype=np.float32)
+    img[10, 12] = (0.5, 0.25, 1.0)          # dyadic values: every mean of copies is exact
+    for r in (1, 3, 7):
+        out = oracle.image_dilate(img, r)
+        yy, xx = np.mgrid[0:h, 0:w]
+        inside = np.maximum(np.abs(yy - 10), np.abs(xx - 12)) <= r
+        assert np.all(out[inside] == np.array([0.5, 0.25, 1.0], dtype=np.float32))
+        assert np.all(np.isnan(out[~inside]))
+    # valid texels never change; a texel between two seeds takes their mean
+    two = np.full((5, 9), np.nan, dtype=np.float32)
+    two[2, 2], two[2, 4] = 0.25, 0.75
+    out = oracle.image_dilate(two, 1)
+    assert out[2, 2] == 0.25 and out[2, 4] == 0.75 and out[2, 3] == np.float32(0.5)
+    assert out[1, 1] == 0.25 and out[1, 5] == 0.75 and np.isnan(out[2, 6 + 1])
+    assert np.isnan(oracle.image_dilate(two, 0)[2, 3])  # 0 rounds: NaN stays
+
+
+def test_u8_conversion(oracle):
+    v = np.array([np.nan, -1.0, 0.0, 0.5, 1.0 / 255.0, 0.998, 1.0, 7.0, 0.2], dtype=np.float32)
+    got = oracle.image_to_u8(v)
+    np.testing.assert_array_equal(got, [0, 0, 0, 128, 1, 254, 255, 255, 51])

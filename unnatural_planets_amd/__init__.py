@@ -12,6 +12,7 @@ Reference functions mirrored (paths relative to the reference checkout):
   mesh_generate_base_water       sources/meshGeneration.cpp:57-98 (without the vertex mask)
   mesh_generate_base_navigation  sources/meshGeneration.cpp:100-107
   generate_textures_land         sources/textures.cpp:45-71, 86-90 (rasterize + per-texel material)
+  texture_post                   sources/textures.cpp:73-81 (dilation by 7, U8 conversion)
 """
 from __future__ import annotations
 
@@ -31,7 +32,8 @@ __all__ = [
     "mesh_generate_base_land", "mesh_generate_base_water", "mesh_generate_base_navigation",
     "reference_grid", "flops_per_sample", "adjoint_flops_per_sample", "BakeSpec", "bake_spec",
     "Textures", "atlas_per_triangle", "bake_rasterize", "bake", "generate_textures_land",
-    "NbWorkspace", "nb_extract", "field_mesh_count", "mc_emit_counted",
+    "NbWorkspace", "nb_extract", "field_mesh_count", "mc_emit_counted", "image_dilate", "image_to_u8",
+    "texture_post",
 ]
 
 
@@ -419,3 +421,40 @@ def generate_textures_land(b: BakeSpec, mesh: Mesh, width: int, height: int, str
     tri = bake_rasterize(uv, width, height, stream)
     albedo, bump, rough = bake(b, mesh, uv, tri, stream)
     return Textures(albedo, bump, rough, uv, tri)
+
+
+# ---------------------------------------------------------------------------- texture post
+def image_dilate(img, rounds: int = 7, stream=None):
+    """In-place dilation of an [h, w] or [h, w, c] float32 image (upf_image_dilate)."""
+    torch = _torch()
+    _check_f32(img, "img")
+    if img.dim() not in (2, 3):
+        raise ValueError("img must be [h, w] or [h, w, c]")
+    h, w = int(img.shape[0]), int(img.shape[1])
+    c = 1 if img.dim() == 2 else int(img.shape[2])
+    scratch = torch.empty_like(img)
+    check(lib().upf_image_dilate(_ptr(img), _ptr(scratch), w, h, c, rounds, _stream(stream)), "upf_image_dilate")
+    return img
+
+
+def image_to_u8(img, stream=None):
+    """NaN -> 0, else round(saturate(v) * 255) (upf_image_to_u8)."""
+    torch = _torch()
+    _check_f32(img, "img")
+    out = torch.empty(img.shape, dtype=torch.uint8, device=img.device)
+    check(lib().upf_image_to_u8(_ptr(img), img.numel(), _ptr(out), _stream(stream)), "upf_image_to_u8")
+    return out
+
+
+def texture_post(tex: Textures, rounds: int = 7, stream=None):
+    """The reference's images after the bake (sources/textures.cpp:73-81): albedo [h, w, 3],
+    special [h, w, 2] = (roughness, metallic = 0) and height [h, w], each dilated by
+    `rounds` texels and converted to U8.  Returns (albedo_u8, special_u8, height_u8)."""
+    torch = _torch()
+    albedo = tex.albedo.clone()
+    special = torch.stack([tex.roughness, torch.where(torch.isnan(tex.roughness), tex.roughness,
+                                                      torch.zeros_like(tex.roughness))], dim=2).contiguous()
+    height = tex.bump.clone()
+    for im in (albedo, special, height):
+        image_dilate(im, rounds, stream)
+    return image_to_u8(albedo, stream), image_to_u8(special, stream), image_to_u8(height, stream)

@@ -155,6 +155,53 @@ __global__ __launch_bounds__(256) void k_bake(const DBake b, const float* __rest
   bump[o] = sat(0.5f + b.bump[0] * d + b.bump[1] * elev);
 }
 
+// One dilation pass (layout [h][w][C], validity by channel 0).
+template <int C>
+__global__ __launch_bounds__(256) void k_dilate(const float* __restrict__ in, float* __restrict__ out, int w, int h) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= w || y >= h) return;
+  const int64_t o = ((int64_t)y * w + x) * C;
+  const float v0 = in[o];
+  if (!isnan(v0)) {
+#pragma unroll
+    for (int ch = 0; ch < C; ch++) out[o + ch] = in[o + ch];
+    return;
+  }
+  float sum[C];
+#pragma unroll
+  for (int ch = 0; ch < C; ch++) sum[ch] = 0.0f;
+  int cnt = 0;
+  for (int dy = -1; dy <= 1; dy++) {
+    const int yy = y + dy;
+    if (yy < 0 || yy >= h) continue;
+    for (int dx = -1; dx <= 1; dx++) {
+      const int xx = x + dx;
+      if ((dx == 0 && dy == 0) || xx < 0 || xx >= w) continue;
+      const int64_t q = ((int64_t)yy * w + xx) * C;
+      if (isnan(in[q])) continue;
+#pragma unroll
+      for (int ch = 0; ch < C; ch++) sum[ch] = sum[ch] + in[q + ch];
+      cnt++;
+    }
+  }
+  const float nan = __builtin_nanf("");
+#pragma unroll
+  for (int ch = 0; ch < C; ch++) out[o + ch] = cnt ? sum[ch] / (float)cnt : nan;
+}
+
+__global__ __launch_bounds__(256) void k_copy_f32(const float* __restrict__ in, float* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+__global__ __launch_bounds__(256) void k_to_u8(const float* __restrict__ in, int64_t n, uint8_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float v = in[i];
+  out[i] = isnan(v) ? (uint8_t)0 : (uint8_t)(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f + 0.5f);
+}
+
 int atlas_side(int64_t n, int w, int h) {
   const int64_t cells = (n + 1) / 2;
   int best = 0;
@@ -227,6 +274,46 @@ int upf_bake(const upf_bake_spec* spec, const float* d_P, const float* d_N, int6
   b.bump[1] = spec->bump[1];
   hipLaunchKernelGGL(k_bake, dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, as_stream(stream), b, d_P, d_N, nv, d_I, n,
                      d_uv, d_tri, w, h, d_albedo, d_bump, d_rough);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_image_dilate(float* d_img, float* d_scratch, int32_t w, int32_t h, int32_t channels, int32_t rounds,
+                     void* stream) {
+  if (w < 1 || h < 1 || rounds < 0) return UPF_ERR_BAD_GRID;
+  if (channels < 1 || channels > 4) return UPF_ERR_INVALID_VALUE;
+  if (rounds == 0) return UPF_OK;
+  if (!d_img || !d_scratch) return UPF_ERR_NULL;
+  hipStream_t s = as_stream(stream);
+  const int64_t n = (int64_t)w * h * channels;
+  float* a = d_img;
+  float* b = d_scratch;
+  if (rounds & 1) {  // an odd number of passes starts from the scratch copy and ends in d_img
+    hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_img, d_scratch, n);
+    a = d_scratch;
+    b = d_img;
+  }
+  const dim3 grid((w + 63) / 64, (h + 3) / 4);
+  for (int r = 0; r < rounds; r++) {
+    switch (channels) {
+      case 1: hipLaunchKernelGGL(k_dilate<1>, grid, dim3(256), 0, s, a, b, w, h); break;
+      case 2: hipLaunchKernelGGL(k_dilate<2>, grid, dim3(256), 0, s, a, b, w, h); break;
+      case 3: hipLaunchKernelGGL(k_dilate<3>, grid, dim3(256), 0, s, a, b, w, h); break;
+      default: hipLaunchKernelGGL(k_dilate<4>, grid, dim3(256), 0, s, a, b, w, h); break;
+    }
+    float* t = a;
+    a = b;
+    b = t;
+  }
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_image_to_u8(const float* d_img, int64_t n, uint8_t* d_out, void* stream) {
+  if (n < 0) return UPF_ERR_BAD_GRID;
+  if (n == 0) return UPF_OK;
+  if (!d_img || !d_out) return UPF_ERR_NULL;
+  hipLaunchKernelGGL(k_to_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), d_img, n, d_out);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
