@@ -288,6 +288,30 @@ int upf_field_mesh_count(const upf_field_spec* spec, const upf_grid* grid, float
 int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,
                       uint8_t* d_codes, void* stream);
 
+/* ---- mesh post-processing -------------------------------------------------------------
+ * The steps after marching cubes in meshGenerateGeneric / meshGenerateBaseWater
+ * (sources/meshGeneration.cpp:41-42, 62-95).  cage's meshRemoveDisconnected and
+ * meshRemoveInvalid are not in the reference checkout; the rules here are documented
+ * choices.  Both write an order-preserving compaction of `in` into `out` (kept vertex i
+ * keeps its order among the kept vertices, kept triangle t among the kept triangles;
+ * normals copied when both normal pointers are set), return UPF_ERR_CAPACITY with the
+ * counts in out when out is too small, and synchronise the stream.  meshFlipNormals
+ * (:42) has no counterpart: upf_mc_emit already writes outward normals and CCW-outward
+ * winding.  Triangles naming a vertex outside [0, in->n_vertices) are dropped. */
+size_t upf_mesh_post_workspace_bytes(int64_t n_vertices, int64_t n_triangles);
+/* meshRemoveDisconnected: keep the connected component (vertices joined by triangle
+ * edges) with the most triangles; ties go to the component holding the smaller vertex
+ * id.  Unreferenced vertices are removed. */
+int upf_mesh_remove_disconnected(const upf_mesh_buffers* in, upf_mesh_buffers* out, void* d_workspace,
+                                 size_t workspace_bytes, void* stream);
+/* Water-mesh vertex mask (meshGeneration.cpp:62-95): a vertex is valid when the raw
+ * elevation (spec with compose = ELEVATION_RAW, terrainSdfElevationRaw) at it is below
+ * `threshold` (the reference uses 0.1); two passes then make all three vertices of every
+ * triangle touching a valid vertex valid; invalid vertices and the triangles naming them
+ * are removed (meshRemoveInvalid). */
+int upf_mesh_water_mask(const upf_field_spec* spec, float threshold, const upf_mesh_buffers* in,
+                        upf_mesh_buffers* out, void* d_workspace, size_t workspace_bytes, void* stream);
+
 /* ---- narrow band ----------------------------------------------------------------------
  * Marching cubes of a fine grid evaluated only near the surface (no counterpart in the
  * reference, which sweeps the dense grid, sources/meshGeneration.cpp:35-39).  The fine

@@ -6,6 +6,7 @@
 #include "upf_oracle.hpp"
 #include "upf_oracle_mc.hpp"
 #include "upf_oracle_bake.hpp"
+#include "upf_oracle_post.hpp"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -197,6 +198,15 @@ void upfo_bake(const upf_bake_spec* b, const float* P, const float* N, const uin
 void upfo_image_dilate(float* img, int w, int h, int channels, int rounds) { dilate_rounds(img, w, h, channels, rounds); }
 void upfo_image_to_u8(const float* img, int64_t n, uint8_t* out) {
   for (int64_t i = 0; i < n; i++) out[i] = to_u8(img[i]);
+}
+
+// ---- mesh post-processing (keep flags)
+void upfo_mesh_remove_disconnected(int64_t nv, int64_t nt, const uint32_t* idx, uint8_t* keep_v, uint8_t* keep_t) {
+  remove_disconnected(nv, nt, idx, keep_v, keep_t);
+}
+void upfo_mesh_water_keep(int64_t nv, int64_t nt, const uint32_t* idx, const uint8_t* valid, uint8_t* keep_v,
+                          uint8_t* keep_t) {
+  water_keep(nv, nt, idx, valid, keep_v, keep_t);
 }
 
 // ---- sources/math.cpp helpers (FP64), for closed-form checks

@@ -52,6 +52,8 @@ class Oracle:
                                  _P]),
             "upfo_image_dilate": (None, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "upfo_image_to_u8": (None, [_P, ctypes.c_int64, _P]),
+            "upfo_mesh_remove_disconnected": (None, [ctypes.c_int64, ctypes.c_int64, _P, _P, _P]),
+            "upfo_mesh_water_keep": (None, [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P]),
             "upfo_num_threads": (ctypes.c_int, []),
             "upfo_set_num_threads": (None, [ctypes.c_int]),
             "upfo_rescale": (ctypes.c_double, [ctypes.c_double] * 5),
@@ -185,6 +187,29 @@ class Oracle:
         out = np.empty(img.shape, dtype=np.uint8)
         self.L.upfo_image_to_u8(_f32p(img), img.size, _f32p(out))
         return out
+
+    # ---- mesh post-processing: keep flags and the order-preserving compaction
+    def remove_disconnected_flags(self, nv, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        kv = np.zeros(nv, dtype=np.uint8)
+        kt = np.zeros(len(idx), dtype=np.uint8)
+        self.L.upfo_mesh_remove_disconnected(nv, len(idx), _f32p(idx), _f32p(kv), _f32p(kt))
+        return kv.astype(bool), kt.astype(bool)
+
+    def water_keep_flags(self, nv, idx, valid):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        valid = np.ascontiguousarray(valid, dtype=np.uint8)
+        kv = np.zeros(nv, dtype=np.uint8)
+        kt = np.zeros(len(idx), dtype=np.uint8)
+        self.L.upfo_mesh_water_keep(nv, len(idx), _f32p(idx), _f32p(valid), _f32p(kv), _f32p(kt))
+        return kv.astype(bool), kt.astype(bool)
+
+
+def compact_mesh(pos, nrm, idx, kv, kt):
+    """Order-preserving compaction by keep flags (the layout both libupf and the oracle use)."""
+    newid = np.cumsum(kv) - 1
+    out_idx = newid[np.asarray(idx, dtype=np.int64)[kt]].astype(np.uint32)
+    return pos[kv], (nrm[kv] if nrm is not None else None), out_idx
 
 
 def mesh_edge_stats(idx: np.ndarray):

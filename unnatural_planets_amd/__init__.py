@@ -9,7 +9,9 @@ Reference functions mirrored (paths relative to the reference checkout):
       sources/modesConfiguration.cpp:230-279 (point-wise field)
   mesh_generate_generic          sources/meshGeneration.cpp:32-45
   mesh_generate_base_land        sources/meshGeneration.cpp:47-55
-  mesh_generate_base_water       sources/meshGeneration.cpp:57-98 (without the vertex mask)
+  mesh_generate_base_water       sources/meshGeneration.cpp:57-98
+  mesh_remove_disconnected       sources/meshGeneration.cpp:41
+  mesh_water_mask                sources/meshGeneration.cpp:62-95
   mesh_generate_base_navigation  sources/meshGeneration.cpp:100-107
   generate_textures_land         sources/textures.cpp:45-71, 86-90 (rasterize + per-texel material)
   texture_post                   sources/textures.cpp:73-81 (dilation by 7, U8 conversion)
@@ -33,7 +35,7 @@ __all__ = [
     "reference_grid", "flops_per_sample", "adjoint_flops_per_sample", "BakeSpec", "bake_spec",
     "Textures", "atlas_per_triangle", "bake_rasterize", "bake", "generate_textures_land",
     "NbWorkspace", "nb_extract", "field_mesh_count", "mc_emit_counted", "image_dilate", "image_to_u8",
-    "texture_post",
+    "texture_post", "mesh_remove_disconnected", "mesh_water_mask",
 ]
 
 
@@ -280,13 +282,54 @@ def mc_emit_counted(field, g: Grid, iso: float, ws: McWorkspace, n_vertices: int
     return Mesh(pos, nrm, idx)
 
 
-def mesh_generate_generic(s: FieldSpec, g: Grid, iso: float = 0.0, normals: bool = True) -> Mesh:
+def _mesh_buffers(m: Mesh):
+    mb = MeshBuffers()
+    mb.positions = m.positions.data_ptr() if m.n_vertices else None
+    mb.normals = m.normals.data_ptr() if (m.normals is not None and m.n_vertices) else None
+    mb.indices = m.indices.data_ptr() if m.n_triangles else None
+    mb.cap_vertices = mb.n_vertices = m.n_vertices
+    mb.cap_triangles = mb.n_triangles = m.n_triangles
+    return mb
+
+
+def _mesh_post(fn, m: Mesh, *lead, stream=None) -> Mesh:
+    torch = _torch()
+    dev = m.positions.device
+    bytes_ = int(lib().upf_mesh_post_workspace_bytes(m.n_vertices, m.n_triangles))
+    if bytes_ == 0:
+        raise UpfError(spec_mod.ERR_BAD_GRID, "upf_mesh_post_workspace_bytes")
+    ws = torch.empty((bytes_,), dtype=torch.uint8, device=dev)
+    pos = torch.empty_like(m.positions)
+    nrm = torch.empty_like(m.normals) if m.normals is not None else None
+    idx = torch.empty_like(m.indices)
+    out = _mesh_buffers(Mesh(pos, nrm, idx))
+    check(fn(*lead, ctypes.byref(_mesh_buffers(m)), ctypes.byref(out), _ptr(ws), bytes_, _stream(stream)),
+          fn.__name__)
+    nv, nt = int(out.n_vertices), int(out.n_triangles)
+    return Mesh(pos[:nv], nrm[:nv] if nrm is not None else None, idx[:nt])
+
+
+def mesh_remove_disconnected(m: Mesh, stream=None) -> Mesh:
+    """meshRemoveDisconnected (sources/meshGeneration.cpp:41): the connected component with
+    the most triangles, order preserved (upf_mesh_remove_disconnected)."""
+    return _mesh_post(lib().upf_mesh_remove_disconnected, m, stream=stream)
+
+
+def mesh_water_mask(s: FieldSpec, m: Mesh, threshold: float = 0.1, stream=None) -> Mesh:
+    """The water-mesh vertex mask of sources/meshGeneration.cpp:62-95: vertices with raw
+    elevation below threshold, grown by two triangle rings (upf_mesh_water_mask)."""
+    return _mesh_post(lib().upf_mesh_water_mask, m, ctypes.byref(s), ctypes.c_float(threshold), stream=stream)
+
+
+def mesh_generate_generic(s: FieldSpec, g: Grid, iso: float = 0.0, normals: bool = True,
+                          remove_disconnected: bool = True) -> Mesh:
     """Grid sweep + marching cubes (sources/meshGeneration.cpp:32-45), as one fused
-    field + classification pass and the emission.  The reference then runs
-    meshRemoveDisconnected and meshFlipNormals; here normals already point outward
-    (towards increasing f), so no flip is needed."""
+    field + classification pass and the emission, then meshRemoveDisconnected (:41).
+    meshFlipNormals (:42) is not needed: normals and winding already point outward
+    (towards increasing f)."""
     f, ws, nv, nt = field_mesh_count(s, g, iso)
-    return mc_emit_counted(f, g, iso, ws, nv, nt, normals)
+    m = mc_emit_counted(f, g, iso, ws, nv, nt, normals)
+    return mesh_remove_disconnected(m) if remove_disconnected else m
 
 
 def mesh_generate_base_land(s: FieldSpec, g: Grid | None = None) -> Mesh:
@@ -297,9 +340,11 @@ def mesh_generate_base_land(s: FieldSpec, g: Grid | None = None) -> Mesh:
     return m
 
 
-def mesh_generate_base_water(s: FieldSpec, g: Grid | None = None) -> Mesh:
-    """sources/meshGeneration.cpp:57-60: water mesh (the vertex mask of :62-95 is not applied)."""
-    return mesh_generate_generic(_with_compose(s, "water"), g or reference_grid())
+def mesh_generate_base_water(s: FieldSpec, g: Grid | None = None, threshold: float = 0.1) -> Mesh:
+    """sources/meshGeneration.cpp:57-98: water mesh, then the vertex mask (raw elevation
+    below threshold, grown by two triangle rings) of :62-95."""
+    m = mesh_generate_generic(_with_compose(s, "water"), g or reference_grid())
+    return mesh_water_mask(s, m, threshold)
 
 
 def mesh_generate_base_navigation(s: FieldSpec, g: Grid | None = None) -> Mesh:
