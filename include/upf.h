@@ -84,6 +84,8 @@ enum upf_shape_id {
   UPF_SHAPE_TORUS = 3,   /* params: R, r   ring in the xz-plane, axis y (sdf.cpp:81-84)   */
   UPF_SHAPE_CAPSULE = 4, /* params: h, r   segment z in [-h/2, h/2] (sdf.cpp:25-28 swaps
                                             y and z before cage's y-axis capsule)          */
+  /* Extended shapes (the reference's compositions in sdf.cpp over textbook primitives).
+   * The adjoint reports their position gradient but no shape-parameter gradient. */
   UPF_SHAPE_CYLINDER = 5,/* params: h, r   axis z, |z| <= h/2   (sdf.cpp:30-38 swizzled)  */
   UPF_SHAPE_PLANE = 6,   /* params: nx, ny, nz (unit), d     dot(p,n) - d  (sdf.cpp:10-18)*/
   UPF_SHAPE_OCTAHEDRON = 7, /* params: s, rounding     (sdf.cpp:55-58)                    */

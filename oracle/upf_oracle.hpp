@@ -247,6 +247,85 @@ SpecT<T> spec_to(const upf_field_spec& s) {
 // ============================================================================ shapes
 // Closed forms (Quilez's distance-function catalogue) for the cage primitives the
 // reference wraps in sources/sdf.cpp.  Parameters as documented in upf.h.
+// ---- the extended shapes (sources/sdf.cpp, cited per case) over textbook primitives:
+// torus ring in the xz-plane (axis y), cylinder axis z with |z| <= h/2, box by half
+// extents, octahedron |x|+|y|+|z| = s (Quilez's exact form).
+template <class T> T hyp2(const T& a, const T& b) { return sqrt_(a * a + b * b); }
+template <class T> T hyp3(const T& a, const T& b, const T& c) { return sqrt_(a * a + b * b + c * c); }
+template <class T> T torus_xz(const T& x, const T& y, const T& z, const T& R, const T& r) {
+  return hyp2(hyp2(x, z) - R, y) - r;
+}
+template <class T> T cylinder_z(const T& x, const T& y, const T& z, const T& h, const T& r) {
+  const T radial = hyp2(x, y) - r;
+  const T axial = abs_(z) - h * T(0.5f);
+  const T zero(0.0f);
+  return min_(max_(radial, axial), zero) + hyp2(max_(radial, zero), max_(axial, zero));
+}
+template <class T> T box_half(const T& x, const T& y, const T& z, const T& e) {
+  const T zero(0.0f);
+  const T dx = abs_(x) - e, dy = abs_(y) - e, dz = abs_(z) - e;
+  return hyp3(max_(dx, zero), max_(dy, zero), max_(dz, zero)) + min_(max_(dx, max_(dy, dz)), zero);
+}
+template <class T> T octahedron(const T& x, const T& y, const T& z, const T& s) {
+  const T ax = abs_(x), ay = abs_(y), az = abs_(z);
+  const T m = ax + ay + az - s;
+  T q[3];
+  if (val(T(3.0f) * ax) < val(m)) { q[0] = ax; q[1] = ay; q[2] = az; }
+  else if (val(T(3.0f) * ay) < val(m)) { q[0] = ay; q[1] = az; q[2] = ax; }
+  else if (val(T(3.0f) * az) < val(m)) { q[0] = az; q[1] = ax; q[2] = ay; }
+  else return m * T(0.57735027f);
+  const T k = min_(max_((q[2] - q[1] + s) * T(0.5f), T(0.0f)), s);
+  return hyp3(q[0], q[1] - s + k, q[2] - k);
+}
+// smoothMin, sources/math.cpp:28-33 (interpolate(b, a, h) = b + (a - b) h)
+template <class T> T smin(const T& a, const T& b, const T& k) {
+  const T h = saturate_((b - a) / k * T(0.5f) + T(0.5f));
+  return b + (a - b) * h - k * h * (T(1.0f) - h);
+}
+
+template <class T> T shape_extended(int shape, const T* a, const T p[3]) {
+  const T x = p[0], y = p[1], z = p[2];
+  switch (shape) {
+    case UPF_SHAPE_CYLINDER: return cylinder_z(x, y, z, a[0], a[1]);      // sdf.cpp:30-38
+    case UPF_SHAPE_PLANE: return x * a[0] + y * a[1] + z * a[2] - a[3];   // sdf.cpp:10-18
+    case UPF_SHAPE_OCTAHEDRON: return octahedron(x, y, z, a[0]) - a[1];   // sdf.cpp:55-58
+    case UPF_SHAPE_DOUBLE_TORUS:                                         // sdf.cpp:292-299
+      return smin(torus_xz(y, z, x, a[0], a[1]), torus_xz(y, x, z, a[0], a[1]), a[2]);
+    case UPF_SHAPE_TORUS_CROSS:                                          // sdf.cpp:301-308
+      return smin(torus_xz(y - a[0], z, x, a[0], a[1]), torus_xz(y + a[0], x, z, a[0], a[1]), a[2]);
+    case UPF_SHAPE_H2O: {                                                // sdf.cpp:141-149
+      const T sx = x * T(0.75f), sy = y * T(0.75f), sz = z * T(0.75f);
+      const T hl = hyp3(sx + T(550.0f), sy - T(300.0f), sz) - T(450.0f);
+      const T hr = hyp3(sx - T(550.0f), sy - T(300.0f), sz) - T(450.0f);
+      const T ox = hyp3(sx, sy + T(100.0f), sz) - T(650.0f);
+      return smin(ox, min_(hl, hr), T(150.0f)) / T(0.75f);
+    }
+    case UPF_SHAPE_BOWL: return T(2000.0f) - hyp3(x, y, z + T(3000.0f)); // sdf.cpp:310-313
+    case UPF_SHAPE_INSIDE_CUBE:                                          // sdf.cpp:315-318
+      return -(box_half(x, y, z, T(1300.0f)) - T(200.0f));
+    case UPF_SHAPE_WORMHOLE: {                                           // sdf.cpp:399-404
+      const T u = min_(hyp2(x, z) - T(1400.0f), T(0.0f));
+      return T(700.0f) - hyp2(u, y);
+    }
+    case UPF_SHAPE_PIPE: {                                               // sdf.cpp:355-366
+      T q0 = x, q1 = y, q2 = z;
+      if (val(q0) < 0.0f) { q0 = -q0; const T t = q1; q1 = q2; q2 = t; }
+      q1 = abs_(q1) - T(800.0f);
+      return torus_xz(q0, q2, q1, T(800.0f), T(400.0f));
+    }
+    case UPF_SHAPE_HEMISPHERES: {                                        // sdf.cpp:383-397
+      const T c45(0.70710678f);
+      const T ux = abs_(x), uy = abs_(y);
+      const T q0 = ((ux - T(2000.0f)) - uy) * c45, q1 = ((ux - T(2000.0f)) + uy) * c45;
+      const T r0 = (ux - (uy - T(2000.0f))) * c45, r1 = (ux + (uy - T(2000.0f))) * c45;
+      const T tube = cylinder_z(q1, z, q0, T(10000.0f), T(200.0f));
+      const T s1 = hyp3(q0, q1, z) - T(1000.0f), s2 = hyp3(r0, r1, z) - T(1000.0f);
+      return smin(tube, min_(s1, s2), T(100.0f));
+    }
+    default: return T(NAN);
+  }
+}
+
 template <class T> T shape_eval(const SpecT<T>& s, const T p[3]) {
   const T* a = s.shape_params;
   switch (s.shape) {
@@ -269,7 +348,7 @@ template <class T> T shape_eval(const SpecT<T>& s, const T p[3]) {
       const T zc = p[2] - clamp_(p[2], -h2, h2);
       return sqrt_(p[0] * p[0] + p[1] * p[1] + zc * zc) - a[1];
     }
-    default: return T(NAN);
+    default: return shape_extended(s.shape, a, p);
   }
 }
 

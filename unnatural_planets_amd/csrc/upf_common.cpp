@@ -18,6 +18,13 @@ int shape_params_used(int shape) {
     case UPF_SHAPE_BOX: return 4;
     case UPF_SHAPE_TORUS: return 2;
     case UPF_SHAPE_CAPSULE: return 2;
+    case UPF_SHAPE_CYLINDER: return 2;
+    case UPF_SHAPE_PLANE: return 4;
+    case UPF_SHAPE_OCTAHEDRON: return 2;
+    case UPF_SHAPE_DOUBLE_TORUS: return 3;
+    case UPF_SHAPE_TORUS_CROSS: return 3;
+    case UPF_SHAPE_H2O: case UPF_SHAPE_BOWL: case UPF_SHAPE_INSIDE_CUBE: case UPF_SHAPE_WORMHOLE:
+    case UPF_SHAPE_PIPE: case UPF_SHAPE_HEMISPHERES: return 0;
     default: return -1;
   }
 }
@@ -118,13 +125,57 @@ double noise_flops(int noise) {
   }
 }
 
-double shape_flops(int shape) {
+// Host scalar that runs the extended-shape template (upf_shapes.hpp) and tallies, by the
+// counting rule, the flops of the float evaluation (fwd) and of the value-and-gradient
+// evaluation with three derivative directions (grad): + and - cost 1 / 4, * 1 / 10,
+// / 1 / 10, sqrt 1 / 5 (sqrt, 1/(2r), three products); sign, abs, min and max are free.
+struct OpCount {
+  float v;
+};
+struct OpTally {
+  double fwd = 0, grad = 0;
+};
+thread_local OpTally g_tally;
+inline OpCount op(float v, double f, double g) {
+  g_tally.fwd += f;
+  g_tally.grad += g;
+  return {v};
+}
+float value_of(const OpCount& a) { return a.v; }
+OpCount lift(const OpCount&, float v) { return {v}; }
+OpCount operator+(OpCount a, OpCount b) { return op(a.v + b.v, 1, 4); }
+OpCount operator-(OpCount a, OpCount b) { return op(a.v - b.v, 1, 4); }
+OpCount operator*(OpCount a, OpCount b) { return op(a.v * b.v, 1, 10); }
+OpCount operator/(OpCount a, OpCount b) { return op(a.v / b.v, 1, 10); }
+OpCount operator-(OpCount a) { return {-a.v}; }
+OpCount operator+(OpCount a, float b) { return op(a.v + b, 1, 1); }
+OpCount operator-(OpCount a, float b) { return op(a.v - b, 1, 1); }
+OpCount operator*(OpCount a, float b) { return op(a.v * b, 1, 4); }
+OpCount operator/(OpCount a, float b) { return op(a.v / b, 1, 4); }
+OpCount operator+(float a, OpCount b) { return op(a + b.v, 1, 1); }
+OpCount operator-(float a, OpCount b) { return op(a - b.v, 1, 1); }
+OpCount operator*(float a, OpCount b) { return op(a * b.v, 1, 4); }
+OpCount sq_root(OpCount a) { return op(std::sqrt(a.v), 1, 5); }
+
+// the point the flop count of a data-dependent shape is taken at (the oracle's count
+// uses the same point)
+constexpr float kCountPoint[3] = {0.3f, 0.2f, -0.4f};
+
+OpTally shape_ext_tally(int shape, const float* sp) {
+  g_tally = OpTally{};
+  (void)upf::shape_ext<OpCount>(shape, sp, OpCount{kCountPoint[0]}, OpCount{kCountPoint[1]}, OpCount{kCountPoint[2]},
+                           OpCount{0.0f});
+  return g_tally;
+}
+
+double shape_flops(int shape, const float* sp) {
   switch (shape) {
+    case UPF_SHAPE_NONE: return 0;
     case UPF_SHAPE_SPHERE: return 7;
     case UPF_SHAPE_BOX: return 11;
     case UPF_SHAPE_TORUS: return 10;
     case UPF_SHAPE_CAPSULE: return 9;
-    default: return 0;
+    default: return shape_ext_tally(shape, sp).fwd;
   }
 }
 
@@ -157,13 +208,14 @@ double noise_d_flops(int noise) {
   }
 }
 
-double shape_d_flops(int shape) {
+double shape_d_flops(int shape, const float* sp) {
   switch (shape) {
+    case UPF_SHAPE_NONE: return 0;
     case UPF_SHAPE_SPHERE: return 11;
     case UPF_SHAPE_BOX: return 18;
     case UPF_SHAPE_TORUS: return 18;
     case UPF_SHAPE_CAPSULE: return 15;
-    default: return 0;
+    default: return shape_ext_tally(shape, sp).grad;
   }
 }
 
@@ -188,7 +240,7 @@ double upf_adjoint_flops_per_sample(const upf_field_spec* s) {
   const bool has_elev = s->compose == UPF_COMPOSE_LAND || s->compose == UPF_COMPOSE_NAVIGATION ||
                         s->compose == UPF_COMPOSE_ELEVATION_RAW;
   double f = 0;
-  if (has_shape) f += shape_d_flops(s->shape) + 3 + 3 * shape_nparams(s->shape);
+  if (has_shape) f += shape_d_flops(s->shape, s->shape_params) + 3 + 3 * shape_nparams(s->shape);
   if (!has_elev || s->n_layers == 0) return f + 1;
   bool all_add = true;
   for (int l = 0; l < s->n_layers; l++) all_add &= s->layers[l].blend == UPF_BLEND_ADD;
@@ -264,7 +316,7 @@ int upf_spec_validate(const upf_field_spec* spec) {
 
 double upf_flops_per_sample(const upf_field_spec* s) {
   if (!s) return 0;
-  const double sh = shape_flops(s->shape);
+  const double sh = shape_flops(s->shape, s->shape_params);
   switch (s->compose) {
     case UPF_COMPOSE_LAND: return sh + elevation_flops(s) + 2;
     case UPF_COMPOSE_WATER: return sh;

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "../../include/upf.h"
+#include "upf_shapes.hpp"
 
 namespace upf {
 
@@ -194,8 +195,10 @@ __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, fl
       const float zc = z - fminf(fmaxf(z, -h2), h2);
       return __builtin_sqrtf(fmaf(zc, zc, fmaf(y, y, x * x))) - s.sp[1];
     }
-    default:
+    case UPF_SHAPE_NONE:
       return 0.0f;
+    default:
+      return shape_ext<float>(s.shape, s.sp, x, y, z, 0.0f);
   }
 }
 

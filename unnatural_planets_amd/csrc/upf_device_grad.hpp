@@ -149,8 +149,13 @@ __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, 
       dp[1] = -1.0f;
       return d - s.sp[1];
     }
-    default:
+    case UPF_SHAPE_NONE:
       return 0.0f;
+    default: {  // extended shapes: forward-mode position gradient, no parameter gradient
+      const Grad r = shape_ext<Grad>(s.shape, s.sp, gvar(x, 0), gvar(y, 1), gvar(z, 2), gc(0.0f));
+      g[0] = r.dx; g[1] = r.dy; g[2] = r.dz;
+      return r.v;
+    }
   }
 }
 

@@ -22,7 +22,13 @@ OK, ERR_INVALID_VALUE, ERR_BAD_SPEC, ERR_CAPACITY, ERR_WORKSPACE, ERR_BAD_GRID, 
     ERR_NULL, ERR_EMPTY_MESH, ERR_BAND = range(10)
 NB_BLOCK = 8
 
-SHAPE = {"none": 0, "sphere": 1, "box": 2, "torus": 3, "capsule": 4}
+SHAPE = {"none": 0, "sphere": 1, "box": 2, "torus": 3, "capsule": 4, "cylinder": 5, "plane": 6, "octahedron": 7,
+         "double_torus": 8, "torus_cross": 9, "h2o": 10, "bowl": 11, "inside_cube": 12, "wormhole": 13,
+         "pipe": 14, "hemispheres": 15}
+# the reference's parameters (sources/sdf.cpp, metres) for the parametric extended shapes
+SHAPE_REFERENCE_PARAMS = {"cylinder": (10000.0, 800.0), "octahedron": (800.0, 100.0),
+                          "double_torus": (900.0, 400.0, 200.0), "torus_cross": (750.0, 400.0, 200.0),
+                          "plane": (0.0, 0.0, -1.0, 0.0)}
 NOISE = {"gradient": 1, "value": 2, "cellular": 3}
 FRACTAL = {"none": 0, "fbm": 1, "ridged": 2}
 BLEND = {"add": 0, "mask": 1, "lerp": 2}
