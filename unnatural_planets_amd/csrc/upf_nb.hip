@@ -18,9 +18,9 @@
 //            triangle counts;
 //  scan      vertex and triangle bases per active block;
 //  emit      one workgroup per active block: vertices at base + prefix + axis rank,
-//            normals from the analytic gradient at the vertex, and triangles whose edge
-//            ids come from the owning block's node table (this block's, or a neighbour's
-//            through the dense map).
+//            and triangles whose edge ids come from the owning block's node table (this
+//            block's, or a neighbour's through the dense map);
+//  normals   one thread per vertex: the normalized analytic gradient at the vertex.
 // Vertex positions use the same interpolation as the dense path on the same per-lane
 // node values, so the mesh equals the dense one up to order when the band holds the
 // surface.
@@ -225,8 +225,7 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
   __syncthreads();
   const uint32_t vb = voff[a];
 
-  // ---- vertices of the owned nodes 3t .. 3t+2
-  ThetaAcc none{nullptr};
+  // ---- vertices of the owned nodes 3t .. 3t+2 (normals follow in k_nb_normals)
   for (int q = 0; q < 3; q++) {
     const int n = 3 * threadIdx.x + q;
     if (n >= kBrick) break;
@@ -254,15 +253,6 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
         pos[3 * (size_t)vid + 0] = p[0];
         pos[3 * (size_t)vid + 1] = p[1];
         pos[3 * (size_t)vid + 2] = p[2];
-        if (nrm) {
-          float gr[3];
-          (void)field_grad<false>(s, p[0], p[1], p[2], 0.0f, gr, none);
-          const float len = __builtin_sqrtf(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
-          const float inv = len > 0.0f ? 1.0f / len : 0.0f;
-          nrm[3 * (size_t)vid + 0] = gr[0] * inv;
-          nrm[3 * (size_t)vid + 1] = gr[1] * inv;
-          nrm[3 * (size_t)vid + 2] = gr[2] * inv;
-        }
         if (keys) keys[vid] = (uint64_t)node * 3u + (uint64_t)ax;
       }
       vid++;
@@ -312,6 +302,23 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
       }
     }
   }
+}
+
+// Normals: the normalized analytic gradient at each vertex, one thread per vertex (few
+// nodes of a brick own a vertex, so evaluating the gradient inside k_nb_emit would run
+// the whole gradient for mostly idle lanes).
+__global__ __launch_bounds__(256) void k_nb_normals(const DSpec s, const float* __restrict__ pos, int64_t n,
+                                                    float* __restrict__ nrm) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= n) return;
+  ThetaAcc none{nullptr};
+  float gr[3];
+  (void)field_grad<false>(s, pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 0.0f, gr, none);
+  const float len = __builtin_sqrtf(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
+  const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+  nrm[3 * v + 0] = gr[0] * inv;
+  nrm[3 * v + 1] = gr[1] * inv;
+  nrm[3 * v + 2] = gr[2] * inv;
 }
 
 // ------------------------------------------------------------------ host side
@@ -442,6 +449,9 @@ int upf_nb_extract(const upf_field_spec* spec, const upf_grid* grid, float iso, 
   hipLaunchKernelGGL(k_nb_emit, dim3((unsigned)nact), dim3(kNbThreads), 0, hs, s, g, d, iso, act, map, bricks, tabs,
                      voff, toff, out->positions, out->normals, out->indices, d_keys, out->cap_vertices,
                      out->cap_triangles);
+  if (out->normals && out->n_vertices)
+    hipLaunchKernelGGL(k_nb_normals, nblk(out->n_vertices), dim3(256), 0, hs, s, out->positions, out->n_vertices,
+                       out->normals);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
