@@ -7,8 +7,11 @@
 // (sources/coloringModes.cpp:51-57,122-151).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "upf_device_grad.hpp"
 #include "upf_device_row.hpp"
+#include "upf_device_row_grad.hpp"
 #include "upf_internal.hpp"
 
 namespace upf {
@@ -134,6 +137,55 @@ __global__ __launch_bounds__(256) void k_adjoint(const DSpec s, const DGrid g, c
   }
 }
 
+// Fused forward + adjoint over a grid with wave-cooperative rows: persistent waves
+// stride over 64-node row segments (so a block's θ partials are flushed once at the
+// end); lanes past the end of a row stand in for its last node with dLdf = 0.
+template <bool THETA>
+__global__ __launch_bounds__(256) void k_adjoint_rows(const DSpec s, const DGrid g, int64_t items,
+                                                      const float* __restrict__ dLdf, float* __restrict__ fout,
+                                                      float* __restrict__ dfdx, float* __restrict__ dLdtheta) {
+  __shared__ float4 s_slice[4][kRowSlice];
+  __shared__ float4 s_gtab[16];
+  __shared__ float4 s_oct[4][kRowOctTab];
+  __shared__ float th[4][UPF_THETA_COUNT];
+  if (threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
+  if (THETA)
+    for (int t = threadIdx.x; t < 4 * UPF_THETA_COUNT; t += 256) (&th[0][0])[t] = 0.0f;
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const RowCtx c = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], lane);
+  ThetaAcc acc{th[wave]};
+  const int gpr = (g.n[0] + 63) / 64;
+  const int64_t stride = (int64_t)gridDim.x * 4;
+  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < items; r += stride) {
+    const int xb = (int)(r % gpr);
+    const int64_t row = r / gpr;
+    const int j = (int)(row % g.n[1]), k = (int)(row / g.n[1]);
+    const int i = xb * 64 + lane;
+    const bool valid = i < g.n[0];
+    const int ic = min(i, g.n[0] - 1);
+    const int64_t node = ((int64_t)k * g.n[1] + j) * g.n[0] + ic;
+    const float x = node_coord(g.o[0], g.s[0], ic);
+    const float y = node_coord(g.o[1], g.s[1], j);
+    const float z = node_coord(g.o[2], g.s[2], k);
+    const float w = (THETA && valid) ? dLdf[node] : 0.0f;
+    float gr[3];
+    const float f = row_field_grad<THETA>(c, s, x, y, z, w, gr, acc);
+    if (valid) {
+      if (fout) fout[node] = f;
+      if (dfdx) { dfdx[3 * node] = gr[0]; dfdx[3 * node + 1] = gr[1]; dfdx[3 * node + 2] = gr[2]; }
+    }
+  }
+  if (THETA) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < UPF_THETA_COUNT; t += 256) {
+      const float v = th[0][t] + th[1][t] + th[2][t] + th[3][t];
+      if (v != 0.0f) atomicAdd(dLdtheta + t, v);
+    }
+  }
+}
+
 namespace {
 int adjoint_blocks(int64_t n) {
   // enough workgroups to fill 256 CUs several times; capped so the per-block θ
@@ -222,14 +274,14 @@ int upf_field_adjoint_grid(const upf_field_spec* spec, const upf_grid* grid, con
   int st = to_device(spec, &s);
   if (st) return st;
   if ((st = to_device(grid, &g))) return st;
-  const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
-  const int nb = adjoint_blocks(n);
+  const int64_t items = (int64_t)((g.n[0] + 63) / 64) * g.n[1] * g.n[2];
+  const int nb = (int)std::min<int64_t>((items + 3) / 4, (int64_t)device_cus() * 8);
   if (d_dLdtheta)
-    hipLaunchKernelGGL((k_adjoint<true, true>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, nullptr, n, d_dLdf,
-                       d_f, d_dfdx, d_dLdtheta);
+    hipLaunchKernelGGL((k_adjoint_rows<true>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, items, d_dLdf, d_f,
+                       d_dfdx, d_dLdtheta);
   else
-    hipLaunchKernelGGL((k_adjoint<false, true>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, nullptr, n,
-                       d_dLdf, d_f, d_dfdx, d_dLdtheta);
+    hipLaunchKernelGGL((k_adjoint_rows<false>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, items, d_dLdf,
+                       d_f, d_dfdx, d_dLdtheta);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
