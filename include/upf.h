@@ -421,6 +421,26 @@ int upf_image_dilate(float* d_img, float* d_scratch, int32_t width, int32_t heig
 /* U8 conversion: out = NaN ? 0 : (uint8)(saturate(v) * 255 + 0.5), n values. */
 int upf_image_to_u8(const float* d_img, int64_t n_values, uint8_t* d_out, void* stream);
 
+/* The two conversions a land chunk's U8 maps get before export
+ * (LandProcessor::chunkEntry, sources/generator.cpp:136-139).  cage's
+ * imageConvertSpecialToGltfPbr and imageConvertHeigthToNormal are not in the reference
+ * checkout; the rules here are documented choices, all in FP32.
+ *
+ * Height to normal, imageConvertHeigthToNormal(heightMap, strength): h = byte / 255 read
+ * with clamp-to-edge, Sobel derivatives along x (columns) and y (rows),
+ *   gx = (h[y-1][x+1] + 2 h[y][x+1] + h[y+1][x+1]) - (h[y-1][x-1] + 2 h[y][x-1] + h[y+1][x-1])
+ *   gy = (h[y+1][x-1] + 2 h[y+1][x] + h[y+1][x+1]) - (h[y-1][x-1] + 2 h[y-1][x] + h[y-1][x+1])
+ * summed left to right as written; v = (-strength * gx, -strength * gy, 1),
+ * n = v * (1 / sqrt(v.v)) and out = (uint8)((n * 0.5 + 0.5) * 255 + 0.5) per component.
+ * d_height: uint8[height][width]; d_out: uint8[height][width][3]. */
+int upf_image_height_to_normal(const uint8_t* d_height, int32_t width, int32_t height, float strength,
+                               uint8_t* d_out, void* stream);
+/* Special to glTF PBR, imageConvertSpecialToGltfPbr(special): the 2-channel
+ * (roughness, metallic) image becomes glTF's metallicRoughness layout, roughness in G
+ * and metallic in B; R (ambient occlusion in a packed occlusion-roughness-metallic map)
+ * is 255, no occlusion.  d_special: uint8[n][2]; d_out: uint8[n][3]. */
+int upf_image_special_to_gltf_pbr(const uint8_t* d_special, int64_t n_texels, uint8_t* d_out, void* stream);
+
 /* ---- batched chunks ---------------------------------------------------------------
  * Many chunks (planets) of uneven grid sizes, shapes and seeds evaluated and meshed by
  * one set of persistent launches: field rows, classify columns and active groups of

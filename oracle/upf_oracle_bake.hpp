@@ -177,4 +177,39 @@ inline uint8_t to_u8(float v) {
   return (uint8_t)(std::min(std::max(v, 0.0f), 1.0f) * 255.0f + 0.5f);
 }
 
+// The export conversions of a land chunk's U8 maps (sources/generator.cpp:136-139) as
+// documented at upf_image_height_to_normal / upf_image_special_to_gltf_pbr.  cage's
+// imageConvertHeigthToNormal and imageConvertSpecialToGltfPbr are not in the checkout.
+inline uint8_t unit_to_u8(float n) { return (uint8_t)((n * 0.5f + 0.5f) * 255.0f + 0.5f); }
+
+inline void height_to_normal(const uint8_t* in, int w, int h, float strength, uint8_t* out) {
+  auto at = [&](int x, int y) {
+    x = std::min(std::max(x, 0), w - 1);
+    y = std::min(std::max(y, 0), h - 1);
+    return (float)in[(int64_t)y * w + x] * (1.0f / 255.0f);
+  };
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      const float a = at(x - 1, y - 1), b = at(x, y - 1), c = at(x + 1, y - 1);
+      const float d = at(x - 1, y), f = at(x + 1, y);
+      const float g = at(x - 1, y + 1), hh = at(x, y + 1), i = at(x + 1, y + 1);
+      const float gx = (c + 2.0f * f + i) - (a + 2.0f * d + g);
+      const float gy = (g + 2.0f * hh + i) - (a + 2.0f * b + c);
+      const float vx = -strength * gx, vy = -strength * gy;
+      const float inv = 1.0f / std::sqrt(vx * vx + vy * vy + 1.0f);
+      uint8_t* o = out + ((int64_t)y * w + x) * 3;
+      o[0] = unit_to_u8(vx * inv);
+      o[1] = unit_to_u8(vy * inv);
+      o[2] = unit_to_u8(inv);
+    }
+}
+
+inline void special_to_pbr(const uint8_t* in, int64_t n, uint8_t* out) {
+  for (int64_t i = 0; i < n; i++) {
+    out[3 * i] = 255;
+    out[3 * i + 1] = in[2 * i];
+    out[3 * i + 2] = in[2 * i + 1];
+  }
+}
+
 }  // namespace upfo

@@ -200,6 +200,11 @@ void upfo_image_to_u8(const float* img, int64_t n, uint8_t* out) {
   for (int64_t i = 0; i < n; i++) out[i] = to_u8(img[i]);
 }
 
+void upfo_image_height_to_normal(const uint8_t* in, int w, int h, float strength, uint8_t* out) {
+  height_to_normal(in, w, h, strength, out);
+}
+void upfo_image_special_to_pbr(const uint8_t* in, int64_t n, uint8_t* out) { special_to_pbr(in, n, out); }
+
 // ---- mesh post-processing (keep flags)
 void upfo_mesh_remove_disconnected(int64_t nv, int64_t nt, const uint32_t* idx, uint8_t* keep_v, uint8_t* keep_t) {
   remove_disconnected(nv, nt, idx, keep_v, keep_t);

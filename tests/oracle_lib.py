@@ -52,6 +52,8 @@ class Oracle:
                                  _P]),
             "upfo_image_dilate": (None, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "upfo_image_to_u8": (None, [_P, ctypes.c_int64, _P]),
+            "upfo_image_height_to_normal": (None, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P]),
+            "upfo_image_special_to_pbr": (None, [_P, ctypes.c_int64, _P]),
             "upfo_mesh_remove_disconnected": (None, [ctypes.c_int64, ctypes.c_int64, _P, _P, _P]),
             "upfo_mesh_water_keep": (None, [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P]),
             "upfo_num_threads": (ctypes.c_int, []),
@@ -186,6 +188,20 @@ class Oracle:
         img = np.ascontiguousarray(img, dtype=np.float32)
         out = np.empty(img.shape, dtype=np.uint8)
         self.L.upfo_image_to_u8(_f32p(img), img.size, _f32p(out))
+        return out
+
+    def image_height_to_normal(self, height_u8, strength=1.0):
+        height_u8 = np.ascontiguousarray(height_u8, dtype=np.uint8)
+        h, w = height_u8.shape
+        out = np.empty((h, w, 3), dtype=np.uint8)
+        self.L.upfo_image_height_to_normal(_f32p(height_u8), w, h, strength, _f32p(out))
+        return out
+
+    def image_special_to_pbr(self, special_u8):
+        special_u8 = np.ascontiguousarray(special_u8, dtype=np.uint8)
+        n = special_u8.size // 2
+        out = np.empty(special_u8.shape[:-1] + (3,), dtype=np.uint8)
+        self.L.upfo_image_special_to_pbr(_f32p(special_u8), n, _f32p(out))
         return out
 
     # ---- mesh post-processing: keep flags and the order-preserving compaction

@@ -122,3 +122,31 @@ def test_texture_post_dilation_and_u8_bit_exact(upf, oracle):
 def torch_from(a):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 7), (9, 1), (65, 5), (1024, 768), (333, 200)])
+def test_height_to_normal_bit_exact(upf, oracle, w, h):
+    """Sobel height-to-normal of U8 height maps, GPU against the oracle bit for bit,
+    including the one-texel and one-row edge cases of the clamp-to-edge rule."""
+    rng = np.random.default_rng(w * 1000 + h)
+    hm = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
+    for strength in (1.0, 0.25, 3.0):
+        got = upfm.image_height_to_normal(torch_from(hm), strength).cpu().numpy()
+        np.testing.assert_array_equal(got, oracle.image_height_to_normal(hm, strength))
+
+
+def test_special_to_pbr_and_export_maps(upf, oracle):
+    """(roughness, metallic) -> (255, roughness, metallic), and the full export chain of a
+    baked land chunk (texture post, then the two conversions) against the oracle."""
+    rng = np.random.default_rng(5)
+    sp = rng.integers(0, 256, size=(37, 29, 2), dtype=np.uint8)
+    np.testing.assert_array_equal(upfm.image_special_to_gltf_pbr(torch_from(sp)).cpu().numpy(),
+                                  oracle.image_special_to_pbr(sp))
+    s = S.spec_sphere_fbm(1234)
+    mesh = upfm.mesh_generate_generic(s, S.grid_sphere_fbm(96))
+    tex = upfm.generate_textures_land(S.bake_spec(s), mesh, 256, 256)
+    albedo, special, height = upfm.texture_post(tex)
+    a2, pbr, nrm = upfm.export_maps(albedo, special, height)
+    assert a2 is albedo
+    np.testing.assert_array_equal(pbr.cpu().numpy(), oracle.image_special_to_pbr(special.cpu().numpy()))
+    np.testing.assert_array_equal(nrm.cpu().numpy(), oracle.image_height_to_normal(height.cpu().numpy(), 1.0))

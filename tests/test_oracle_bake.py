@@ -11,7 +11,9 @@
   * material: flat colour zones of the elevation bands;
   * dilation: a single dyadic seed fills exactly its Chebyshev ball of radius `rounds`;
     valid texels never change; a gap between two seeds takes their mean;
-  * U8 conversion: hand-computed values, NaN to 0, saturation.
+  * U8 conversion: hand-computed values, NaN to 0, saturation;
+  * height to normal: flat and ramp closed forms (interior and clamped edge columns),
+    exact transpose symmetry, hill orientation, unit length; special to glTF PBR layout.
 """
 import numpy as np
 import pytest
@@ -188,3 +190,56 @@ def test_u8_conversion(oracle):
     v = np.array([np.nan, -1.0, 0.0, 0.5, 1.0 / 255.0, 0.998, 1.0, 7.0, 0.2], dtype=np.float32)
     got = oracle.image_to_u8(v)
     np.testing.assert_array_equal(got, [0, 0, 0, 128, 1, 254, 255, 255, 51])
+
+
+# ---- export conversions (height -> normal, special -> glTF PBR)
+def _u8_of_unit(n):
+    return np.floor((np.asarray(n, dtype=np.float64) * 0.5 + 0.5) * 255 + 0.5).astype(np.uint8)
+
+
+def test_height_to_normal_flat_and_ramp_closed_forms(oracle):
+    """Flat height gives (0, 0, 1); a ramp of one byte per column gives the Sobel slope
+    8/255 inside and 4/255 on the clamped edge columns, in closed form."""
+    flat = np.full((6, 9), 77, dtype=np.uint8)
+    out = oracle.image_height_to_normal(flat, 1.0)
+    assert np.all(out[..., 0] == 128) and np.all(out[..., 1] == 128) and np.all(out[..., 2] == 255)
+    w, h = 200, 5
+    ramp = np.tile(np.arange(w, dtype=np.uint8), (h, 1))
+    for strength in (1.0, 2.5):
+        out = oracle.image_height_to_normal(ramp, strength)
+        for cols, g in ((slice(1, w - 1), 8.0 / 255), (slice(0, 1), 4.0 / 255), (slice(w - 1, w), 4.0 / 255)):
+            v = np.array([-strength * g, 0.0, 1.0])
+            n = v / np.linalg.norm(v)
+            assert np.all(out[:, cols] == _u8_of_unit(n)), (strength, cols)
+    # the same ramp along rows moves the y component
+    out = oracle.image_height_to_normal(np.ascontiguousarray(ramp[:, :5].T), 1.0)
+    assert np.all(out[1:-1, :, 1] == _u8_of_unit([-(8.0 / 255) / np.sqrt(1 + (8.0 / 255) ** 2)])[0])
+    assert np.all(out[1:-1, :, 0] == 128)
+
+
+def test_height_to_normal_symmetry_orientation_and_length(oracle):
+    """Transposing the height map swaps the x and y components exactly; a hill's normals
+    lean away from its top; decoded normals have unit length to U8 precision."""
+    rng = np.random.default_rng(3)
+    hm = rng.integers(0, 256, size=(23, 31), dtype=np.uint8)
+    a = oracle.image_height_to_normal(hm, 1.7)
+    b = oracle.image_height_to_normal(np.ascontiguousarray(hm.T), 1.7)
+    np.testing.assert_array_equal(a[..., 0], b[..., 1].T)
+    np.testing.assert_array_equal(a[..., 1], b[..., 0].T)
+    np.testing.assert_array_equal(a[..., 2], b[..., 2].T)
+    yy, xx = np.mgrid[0:41, 0:41]
+    hill = np.clip(250 - 3 * np.hypot(xx - 20, yy - 20), 0, 255).astype(np.uint8)
+    n = oracle.image_height_to_normal(hill, 1.0).astype(np.int32)
+    assert np.all(n[20, 23:35, 0] > 128) and np.all(n[20, 6:18, 0] < 128)
+    assert np.all(n[23:35, 20, 1] > 128) and np.all(n[6:18, 20, 1] < 128)
+    d = (a.astype(np.float64) / 255 - 0.5) * 2
+    assert np.all(np.abs(np.linalg.norm(d, axis=-1) - 1) < 2.0 * np.sqrt(3) / 255)
+
+
+def test_special_to_pbr_layout(oracle):
+    sp = np.array([[[0, 255], [17, 3]], [[200, 100], [255, 0]]], dtype=np.uint8)
+    out = oracle.image_special_to_pbr(sp)
+    assert out.shape == (2, 2, 3)
+    np.testing.assert_array_equal(out[..., 0], 255)
+    np.testing.assert_array_equal(out[..., 1], sp[..., 0])
+    np.testing.assert_array_equal(out[..., 2], sp[..., 1])

@@ -35,6 +35,7 @@ __all__ = [
     "reference_grid", "flops_per_sample", "adjoint_flops_per_sample", "BakeSpec", "bake_spec",
     "Textures", "atlas_per_triangle", "bake_rasterize", "bake", "generate_textures_land",
     "NbWorkspace", "nb_extract", "field_mesh_count", "mc_emit_counted", "image_dilate", "image_to_u8",
+    "image_height_to_normal", "image_special_to_gltf_pbr", "export_maps",
     "texture_post", "mesh_remove_disconnected", "mesh_water_mask",
 ]
 
@@ -503,3 +504,36 @@ def texture_post(tex: Textures, rounds: int = 7, stream=None):
     for im in (albedo, special, height):
         image_dilate(im, rounds, stream)
     return image_to_u8(albedo, stream), image_to_u8(special, stream), image_to_u8(height, stream)
+
+
+def image_height_to_normal(height_u8, strength: float = 1.0, stream=None):
+    """U8 height map [h, w] -> U8 normal map [h, w, 3] (upf_image_height_to_normal; the
+    reference's imageConvertHeigthToNormal(heightMap, 1), sources/generator.cpp:138)."""
+    torch = _torch()
+    if height_u8.dtype != torch.uint8 or height_u8.dim() != 2 or not height_u8.is_contiguous():
+        raise ValueError("height_u8 must be a contiguous [h, w] uint8 tensor")
+    h, w = int(height_u8.shape[0]), int(height_u8.shape[1])
+    out = torch.empty((h, w, 3), dtype=torch.uint8, device=height_u8.device)
+    check(lib().upf_image_height_to_normal(_ptr(height_u8), w, h, float(strength), _ptr(out), _stream(stream)),
+          "upf_image_height_to_normal")
+    return out
+
+
+def image_special_to_gltf_pbr(special_u8, stream=None):
+    """U8 (roughness, metallic) [..., 2] -> glTF metallicRoughness [..., 3] = (255, roughness,
+    metallic) (upf_image_special_to_gltf_pbr; imageConvertSpecialToGltfPbr,
+    sources/generator.cpp:136)."""
+    torch = _torch()
+    if special_u8.dtype != torch.uint8 or special_u8.shape[-1] != 2 or not special_u8.is_contiguous():
+        raise ValueError("special_u8 must be a contiguous [..., 2] uint8 tensor")
+    out = torch.empty(tuple(special_u8.shape[:-1]) + (3,), dtype=torch.uint8, device=special_u8.device)
+    check(lib().upf_image_special_to_gltf_pbr(_ptr(special_u8), special_u8.numel() // 2, _ptr(out), _stream(stream)),
+          "upf_image_special_to_gltf_pbr")
+    return out
+
+
+def export_maps(albedo_u8, special_u8, height_u8, stream=None):
+    """The conversions LandProcessor::chunkEntry applies before writing a chunk's images
+    (sources/generator.cpp:134-139): albedo unchanged, special -> glTF PBR, height ->
+    normal map with strength 1.  Returns (albedo, pbr, normal)."""
+    return albedo_u8, image_special_to_gltf_pbr(special_u8, stream), image_height_to_normal(height_u8, 1.0, stream)

@@ -78,6 +78,10 @@ SIGNATURES = {
     "upf_image_dilate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_int32, ctypes.c_void_p]),
     "upf_image_to_u8": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "upf_image_height_to_normal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                                  ctypes.c_void_p, ctypes.c_void_p]),
+    "upf_image_special_to_gltf_pbr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                                     ctypes.c_void_p]),
     "upf_mesh_post_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
     "upf_mesh_remove_disconnected": (ctypes.c_int, [ctypes.POINTER(MeshBuffers), ctypes.POINTER(MeshBuffers),
                                                     ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),

@@ -195,6 +195,45 @@ __global__ __launch_bounds__(256) void k_copy_f32(const float* __restrict__ in, 
   if (i < n) out[i] = in[i];
 }
 
+__device__ __forceinline__ uint8_t unit_to_u8(float n) { return (uint8_t)((n * 0.5f + 0.5f) * 255.0f + 0.5f); }
+
+// Sobel height-to-normal, one texel per lane over a 64 x 4 tile (rules at
+// upf_image_height_to_normal).  Neighbour bytes come through L1/L2: 1 B read and 3 B
+// written per texel from HBM.
+__global__ __launch_bounds__(256) void k_height_to_normal(const uint8_t* __restrict__ in, int w, int h, float strength,
+                                                          uint8_t* __restrict__ out) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= w || y >= h) return;
+  const int xm = x > 0 ? x - 1 : 0, xp = x + 1 < w ? x + 1 : w - 1;
+  const int ym = y > 0 ? y - 1 : 0, yp = y + 1 < h ? y + 1 : h - 1;
+  const uint8_t* rm = in + (int64_t)ym * w;
+  const uint8_t* r0 = in + (int64_t)y * w;
+  const uint8_t* rp = in + (int64_t)yp * w;
+  const float k = 1.0f / 255.0f;
+  const float a = rm[xm] * k, b = rm[x] * k, c = rm[xp] * k;
+  const float d = r0[xm] * k, f = r0[xp] * k;
+  const float g = rp[xm] * k, hh = rp[x] * k, i = rp[xp] * k;
+  const float gx = (c + 2.0f * f + i) - (a + 2.0f * d + g);
+  const float gy = (g + 2.0f * hh + i) - (a + 2.0f * b + c);
+  const float vx = -strength * gx, vy = -strength * gy;
+  const float inv = 1.0f / sqrtf(vx * vx + vy * vy + 1.0f);
+  uint8_t* o = out + ((int64_t)y * w + x) * 3;
+  o[0] = unit_to_u8(vx * inv);
+  o[1] = unit_to_u8(vy * inv);
+  o[2] = unit_to_u8(inv);
+}
+
+// (roughness, metallic) -> (255, roughness, metallic), one texel per lane.
+__global__ __launch_bounds__(256) void k_special_to_pbr(const uint8_t* __restrict__ in, int64_t n,
+                                                        uint8_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  out[3 * i] = 255;
+  out[3 * i + 1] = in[2 * i];
+  out[3 * i + 2] = in[2 * i + 1];
+}
+
 __global__ __launch_bounds__(256) void k_to_u8(const float* __restrict__ in, int64_t n, uint8_t* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -314,6 +353,27 @@ int upf_image_to_u8(const float* d_img, int64_t n, uint8_t* d_out, void* stream)
   if (n == 0) return UPF_OK;
   if (!d_img || !d_out) return UPF_ERR_NULL;
   hipLaunchKernelGGL(k_to_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), d_img, n, d_out);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_image_height_to_normal(const uint8_t* d_height, int32_t w, int32_t h, float strength, uint8_t* d_out,
+                               void* stream) {
+  if (w < 1 || h < 1) return UPF_ERR_BAD_GRID;
+  if (!(strength == strength) || isinf(strength)) return UPF_ERR_INVALID_VALUE;
+  if (!d_height || !d_out) return UPF_ERR_NULL;
+  hipLaunchKernelGGL(k_height_to_normal, dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, as_stream(stream), d_height,
+                     w, h, strength, d_out);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+int upf_image_special_to_gltf_pbr(const uint8_t* d_special, int64_t n, uint8_t* d_out, void* stream) {
+  if (n < 0) return UPF_ERR_BAD_GRID;
+  if (n == 0) return UPF_OK;
+  if (!d_special || !d_out) return UPF_ERR_NULL;
+  hipLaunchKernelGGL(k_special_to_pbr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), d_special,
+                     n, d_out);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
