@@ -104,7 +104,50 @@ enum upf_noise_id {
   UPF_NOISE_GRADIENT = 1, /* 3D lattice gradient noise, quintic fade, 12-edge gradients   */
   UPF_NOISE_VALUE = 2,    /* 3D lattice value noise, quintic fade                         */
   UPF_NOISE_CELLULAR = 3, /* F1 Euclidean distance to jittered feature points, 27 cells   */
-  UPF_NOISE_COUNT = 4
+  UPF_NOISE_SIMPLEX = 4,  /* 3D simplex noise (Perlin 2001): skew 1/3, unskew 1/6, the 4
+                             corners of the containing simplex, kernel (0.5 - r^2)^4 times
+                             the corner gradient's dot product, scaled by 72.  The radius
+                             0.5 keeps it continuous.  Gradients and hash are the lattice
+                             noise's; cage's Simplex (sources/elevationModes.cpp:19) is
+                             this noise's role in the reference                          */
+  UPF_NOISE_COUNT = 5
+};
+
+/* Elevation modes: the reference's fixed elevation functions
+ * (sources/elevationModes.cpp:9-198), selected with upf_field_spec.elevation_mode in
+ * place of the generic layer stack.  Each mode reads the noise layers it needs from
+ * layers[0..n) in the order listed (upf_spec_set_elevation_mode fills them with the
+ * reference's configurations); every layer's weight and blend are ignored.  Results are
+ * in metres, so elevation_scale = 1/10 gives the reference's meshElevationRatio
+ * (sources/modesConfiguration.cpp:227).
+ *   NONE     elev = 100                                                        (:9-12)
+ *   SIMPLE   1 layer: a = 0.3 - n0; a = (1.3 a - 0.35)^3 + 0.1; 100 - 1000 a   (:14-32)
+ *   LEGACY   2 layers: s = n0 * 0.0005 + 0.0015; a = n1(p * s) + 0.11;
+ *            a = a < 0 ? -(-a)^0.85 : a^1.7; 2500 a                             (:34-64)
+ *   LAKES    4 layers: land = 1 - saturate(0.5 n0 + 0.5)^1.24, then
+ *            l = 2 land - 1, land = 150 (l / (|l| + 0.17) + 0.15), then the mountains
+ *            of n1 (mask), n2 (ridges), n3 (terraces)                  (:68-123, 129-149)
+ *   ISLANDS  as LAKES with exponent 0.83                                   (:151-171)
+ *   CRATERS  2 layers: layers[0] is the impact Voronoi (sources/voronoi.cpp:10-80):
+ *            seed = its seed, octaves = points per cell (1..8), freq[0] = cell size in
+ *            metres; c = the nearest point, s = 100 + 50 n1(c), d = |p - c|,
+ *            x = saturate(d / s), h = 10 (x^4 - 0.9 x^5 - 0.1), elev = 200 + 190 h
+ *                                                                            (:173-198)
+ * The Voronoi's integer hash (cage's hash(), not in the reference checkout) is the
+ * murmur3 32-bit finaliser.  pow is the C library's powf.  The parameter gradient is
+ * defined for the generic layer stack only: with a mode, upf_field_adjoint with
+ * d_dLdtheta and upf_field_adjoint_grid return UPF_ERR_BAD_SPEC, while the position
+ * gradient (normals) is available through upf_field_eval_points.  Grids with a mode are
+ * evaluated per lane (UPF_EVAL_PER_LANE). */
+enum upf_elevation_mode_id {
+  UPF_ELEV_LAYERS = 0,  /* the generic layer stack (blend ops below)                    */
+  UPF_ELEV_NONE = 1,
+  UPF_ELEV_SIMPLE = 2,
+  UPF_ELEV_LEGACY = 3,
+  UPF_ELEV_LAKES = 4,
+  UPF_ELEV_ISLANDS = 5,
+  UPF_ELEV_CRATERS = 6,
+  UPF_ELEV_COUNT = 7
 };
 
 enum upf_fractal_id {
@@ -155,7 +198,7 @@ typedef struct upf_field_spec {
   int32_t shape;    /* upf_shape_id   */
   int32_t compose;  /* upf_compose_id */
   int32_t n_layers; /* 0..UPF_MAX_LAYERS */
-  int32_t flags;    /* reserved, 0 */
+  int32_t elevation_mode; /* upf_elevation_mode_id; 0 = the generic layer stack */
   float shape_params[UPF_MAX_SHAPE_PARAMS];
   float elevation_scale;
   float reserved[3];
@@ -201,6 +244,13 @@ const char* upf_status_string(int status);
 int upf_layer_init(upf_layer* layer, int32_t noise, int32_t fractal, int32_t octaves,
                    float frequency, float lacunarity, float gain, float weight,
                    uint32_t seed, float jitter, int32_t blend);
+
+/* Set spec->elevation_mode and fill spec->layers / n_layers with the noise
+ * configurations of that mode in the reference (sources/elevationModes.cpp:14-198), the
+ * k-th noise seeded with seed + k (the reference draws each from noiseSeed(),
+ * sources/math.cpp:73-77).  cage's unset NoiseFunctionCreateConfig fields are taken as
+ * lacunarity 2 and gain 0.5.  The shape, composition and elevation_scale are kept. */
+int upf_spec_set_elevation_mode(upf_field_spec* spec, int32_t mode, uint32_t seed);
 
 /* Checks ids, counts and that every parameter is finite.  Replaces the choose*Function
  * validation (sources/modesConfiguration.cpp:131-135, 190-194) and the NaN throws. */

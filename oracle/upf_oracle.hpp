@@ -93,6 +93,16 @@ inline float floor_(float a) { return std::floor(a); }
 inline double floor_(double a) { return std::floor(a); }
 inline float rint_(float a) { return std::nearbyint(a); }    // round half to even
 inline double rint_(double a) { return std::nearbyint(a); }
+// pow with a constant exponent (the elevation modes): the C library's powf / pow
+inline float pow_(float a, double e) { return std::pow(a, (float)e); }
+inline double pow_(double a, double e) { return std::pow(a, e); }
+template <int K> Dual<K> pow_(const Dual<K>& a, double e) {
+  Dual<K> r(std::pow(a.v, e));
+  const double s = a.v != 0 ? e * std::pow(a.v, e - 1) : 0.0;
+  for (int i = 0; i < K; i++) r.d[i] = a.d[i] * s;
+  return r;
+}
+inline CountedFloat pow_(CountedFloat a, double e) { CountedFloat::count()++; return std::pow(a.v, (float)e); }
 
 template <class T> T max_(const T& a, const T& b) { return val(a) < val(b) ? b : a; }
 template <class T> T min_(const T& a, const T& b) { return val(b) < val(a) ? b : a; }
@@ -211,6 +221,46 @@ template <class T> T noise_cellular(uint32_t seed, const T q[3], const T& jitter
   return sqrt_(best);
 }
 
+// 3D simplex noise (Perlin 2001, as described in Gustavson, "Simplex noise demystified",
+// 2005) with kernel radius^2 0.5 and output scale 72: the skewed cell of q, its four
+// simplex corners in the order of the unskewed offset's components, and per corner
+// (0.5 - |d|^2)^4 (g . d) where d = q - corner and g the lattice gradient of the corner's
+// hash.  Constants are formed in T (1/3, 1/6 and 2/6 round once in float).
+template <class T> T noise_simplex(uint32_t seed, const T q[3]) {
+  const T F3 = T(1.0f) / T(3.0f), G3 = T(1.0f) / T(6.0f), G3x2 = T(2.0f) * G3;
+  const T sk = (q[0] + q[1] + q[2]) * F3;
+  const T fi = floor_(q[0] + sk), fj = floor_(q[1] + sk), fk = floor_(q[2] + sk);
+  const int32_t i = (int32_t)val(fi), j = (int32_t)val(fj), k = (int32_t)val(fk);
+  const T t = (fi + fj + fk) * G3;
+  const T x0 = q[0] - (fi - t), y0 = q[1] - (fj - t), z0 = q[2] - (fk - t);
+  int a1[3], a2[3];  // corner offsets, Gustavson's rank table
+  const double X = val(x0), Y = val(y0), Z = val(z0);
+  if (X >= Y) {
+    if (Y >= Z) { a1[0] = 1; a1[1] = 0; a1[2] = 0; a2[0] = 1; a2[1] = 1; a2[2] = 0; }
+    else if (X >= Z) { a1[0] = 1; a1[1] = 0; a1[2] = 0; a2[0] = 1; a2[1] = 0; a2[2] = 1; }
+    else { a1[0] = 0; a1[1] = 0; a1[2] = 1; a2[0] = 1; a2[1] = 0; a2[2] = 1; }
+  } else {
+    if (Y < Z) { a1[0] = 0; a1[1] = 0; a1[2] = 1; a2[0] = 0; a2[1] = 1; a2[2] = 1; }
+    else if (X < Z) { a1[0] = 0; a1[1] = 1; a1[2] = 0; a2[0] = 0; a2[1] = 1; a2[2] = 1; }
+    else { a1[0] = 0; a1[1] = 1; a1[2] = 0; a2[0] = 1; a2[1] = 1; a2[2] = 0; }
+  }
+  const T d[4][3] = {
+      {x0, y0, z0},
+      {(x0 - T((float)a1[0])) + G3, (y0 - T((float)a1[1])) + G3, (z0 - T((float)a1[2])) + G3},
+      {(x0 - T((float)a2[0])) + G3x2, (y0 - T((float)a2[1])) + G3x2, (z0 - T((float)a2[2])) + G3x2},
+      {x0 - T(0.5f), y0 - T(0.5f), z0 - T(0.5f)}};
+  const int off[4][3] = {{0, 0, 0}, {a1[0], a1[1], a1[2]}, {a2[0], a2[1], a2[2]}, {1, 1, 1}};
+  T n = T(0.0f);
+  for (int c = 0; c < 4; c++) {
+    const T r = T(0.5f) - (d[c][0] * d[c][0] + d[c][1] * d[c][1] + d[c][2] * d[c][2]);
+    if (val(r) > 0) {
+      const T r2 = r * r;
+      n = n + r2 * r2 * grad_dot(hash3(seed, i + off[c][0], j + off[c][1], k + off[c][2]), d[c][0], d[c][1], d[c][2]);
+    }
+  }
+  return T(72.0f) * n;
+}
+
 // ============================================================================ spec in T
 template <class T>
 struct LayerT {
@@ -222,7 +272,7 @@ struct LayerT {
 
 template <class T>
 struct SpecT {
-  int shape = 0, compose = 0, n_layers = 0;
+  int shape = 0, compose = 0, n_layers = 0, elevation_mode = 0;
   T shape_params[UPF_MAX_SHAPE_PARAMS];
   T elevation_scale;
   LayerT<T> layers[UPF_MAX_LAYERS];
@@ -231,7 +281,7 @@ struct SpecT {
 template <class T>
 SpecT<T> spec_to(const upf_field_spec& s) {
   SpecT<T> r;
-  r.shape = s.shape; r.compose = s.compose; r.n_layers = s.n_layers;
+  r.shape = s.shape; r.compose = s.compose; r.n_layers = s.n_layers; r.elevation_mode = s.elevation_mode;
   for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++) r.shape_params[i] = T(s.shape_params[i]);
   r.elevation_scale = T(s.elevation_scale);
   for (int l = 0; l < UPF_MAX_LAYERS; l++) {
@@ -360,6 +410,7 @@ template <class T> T octave_noise(const LayerT<T>& L, int k, const T p[3]) {
     case UPF_NOISE_GRADIENT: return noise_gradient(seed, q);
     case UPF_NOISE_VALUE: return noise_value(seed, q);
     case UPF_NOISE_CELLULAR: return noise_cellular(seed, q, L.jitter);
+    case UPF_NOISE_SIMPLEX: return noise_simplex(seed, q);
     default: return T(NAN);
   }
 }
@@ -379,7 +430,120 @@ template <class T> T layer_value(const LayerT<T>& L, const T p[3]) {
   return acc;
 }
 
+// ============================================================================ elevation modes
+// sources/voronoi.cpp:10-80 with cage's hash() (absent from the checkout) taken as the
+// murmur3 finaliser hash1; all in FP32 as the reference's Real.  voronoi_nearest returns
+// the nearest of the 27 * ppc points (first in generation order on ties) -- points[0] of
+// evaluate() with a NaN normal (no plane projection, :61-66).
+// The ppc points of lattice cell c (genCell, :34-42) -> out[ppc][3].
+inline void voronoi_cell_points(uint32_t seed, int ppc, float cell_size, const int32_t c[3], float* out) {
+  const uint32_t sh[3] = {hash1(seed), hash1(hash1(seed)), hash1(hash1(hash1(seed)))};   // :17
+  auto mix = [&](uint32_t v[3]) {                                                         // :19-24
+    const uint32_t a[3] = {v[1], v[2], v[0]};
+    const uint32_t b[3] = {hash1(v[0]), hash1(v[1]), hash1(v[2])};
+    for (int i = 0; i < 3; i++) v[i] = sh[i] + v[i] + a[i] + b[i];
+  };
+  uint32_t v[3] = {(uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2]};
+  mix(v);                                                                                 // :36
+  for (int i = 0; i < ppc; i++) {
+    mix(v);                                                                               // :39
+    for (int a = 0; a < 3; a++)                                                           // :26, :40
+      out[3 * i + a] = ((float)((int32_t)v[a] % 65536) / 65535.0f + (float)c[a]) * cell_size;
+  }
+}
+
+inline void voronoi_nearest(uint32_t seed, int ppc, float cell_size, const float pos[3], float out[3]) {
+  const float frequency = 1.0f / cell_size;
+  const int32_t cell[3] = {(int32_t)(pos[0] * frequency), (int32_t)(pos[1] * frequency),
+                           (int32_t)(pos[2] * frequency)};                               // :51
+  float best = INFINITY;
+  std::vector<float> pts(3 * (size_t)ppc);
+  for (int z = -1; z < 2; z++)
+    for (int y = -1; y < 2; y++)
+      for (int x = -1; x < 2; x++) {                                                      // :52-55
+        const int32_t c[3] = {cell[0] + x, cell[1] + y, cell[2] + z};
+        voronoi_cell_points(seed, ppc, cell_size, c, pts.data());
+        for (int i = 0; i < ppc; i++) {
+          const float* p = &pts[3 * i];
+          const float dx = p[0] - pos[0], dy = p[1] - pos[1], dz = p[2] - pos[2];
+          const float d = dx * dx + dy * dy + dz * dz;                                    // :70
+          if (d < best) { best = d; out[0] = p[0]; out[1] = p[1]; out[2] = p[2]; }
+        }
+      }
+}
+
+// cage's one-argument smoothstep, x^2 (3 - 2x)
+template <class T> T smoothstep_(T x) { return x * x * (T(3.0f) - T(2.0f) * x); }
+template <class T> T terrace(T x, T steepness);
+template <class T> T smooth_max(T a, T b, T k);
+
+// commonElevationMountains, sources/elevationModes.cpp:68-123
+template <class T> T mountains(const SpecT<T>& s, const T p[3], T land) {
+  const T cover = T(1.0f) - saturate_(land * T(-0.1f));
+  if ((float)val(cover) < 1e-7f) return land;
+  const T mask = layer_value(s.layers[1], p);
+  const T rm = smoothstep_(saturate_(mask * T(7.0f) - T(0.3f)));
+  const T tm = smoothstep_(saturate_(mask * T(-7.0f) - T(1.5f)));
+  T ridge = layer_value(s.layers[2], p);
+  ridge = max_(ridge - T(0.1f), T(0.0f));
+  ridge = pow_(ridge, 1.6f);
+  ridge = ridge * (rm * cover);
+  ridge = ridge * T(1000.0f);
+  T terraces = layer_value(s.layers[3], p);
+  terraces = max_(terraces + T(0.1f), T(0.0f)) * T(2.5f);
+  terraces = terrace(terraces, T(4.0f));
+  terraces = terraces * (tm * cover);
+  terraces = terraces * T(250.0f);
+  return land + smooth_max(T(0.0f), max_(ridge, terraces), T(50.0f));
+}
+
+// The fixed elevation functions of sources/elevationModes.cpp, layer slots as documented
+// at upf_elevation_mode_id.
+template <class T> T elevation_mode(const SpecT<T>& s, const T p[3]) {
+  switch (s.elevation_mode) {
+    case UPF_ELEV_NONE: return T(100.0f);                                    // :9-12
+    case UPF_ELEV_SIMPLE: {                                                  // :14-32
+      T a = layer_value(s.layers[0], p);
+      a = -a + T(0.3f);
+      a = pow_(a * T(1.3f) - T(0.35f), 3.0f) + T(0.1f);
+      return T(100.0f) - a * T(1000.0f);
+    }
+    case UPF_ELEV_LEGACY: {                                                  // :34-64
+      const T scale = layer_value(s.layers[0], p) * T(0.0005f) + T(0.0015f);
+      const T ps[3] = {p[0] * scale, p[1] * scale, p[2] * scale};
+      T a = layer_value(s.layers[1], ps);
+      a = a + T(0.11f);
+      if (val(a) < 0) a = -pow_(-a, 0.85f);
+      else a = pow_(a, 1.7f);
+      return a * T(2500.0f);
+    }
+    case UPF_ELEV_LAKES:                                                     // :129-149
+    case UPF_ELEV_ISLANDS: {                                                 // :151-171
+      T land = layer_value(s.layers[0], p) * T(0.5f) + T(0.5f);
+      land = saturate_(land);
+      land = T(1.0f) - pow_(land, s.elevation_mode == UPF_ELEV_LAKES ? 1.24f : 0.83f);
+      land = land * T(2.0f) - T(1.0f);
+      land = land / (abs_(land) + T(0.17f)) + T(0.15f);
+      land = land * T(150.0f);
+      return mountains(s, p, land);
+    }
+    case UPF_ELEV_CRATERS: {                                                 // :173-198
+      const float pf[3] = {(float)val(p[0]), (float)val(p[1]), (float)val(p[2])};
+      float cf[3] = {0.0f, 0.0f, 0.0f};
+      voronoi_nearest(s.layers[0].seed, s.layers[0].octaves, (float)val(s.layers[0].freq[0]), pf, cf);
+      const T c[3] = {T(cf[0]), T(cf[1]), T(cf[2])};
+      const T sc = T(100.0f) + layer_value(s.layers[1], c) * T(50.0f);
+      const T d = sqrt_((p[0] - c[0]) * (p[0] - c[0]) + (p[1] - c[1]) * (p[1] - c[1]) + (p[2] - c[2]) * (p[2] - c[2]));
+      const T x = saturate_(d / sc);
+      const T h = (pow_(x, 4.0f) - T(0.9f) * pow_(x, 5.0f) - T(0.1f)) * T(10.0f);
+      return T(200.0f) + h * T(190.0f);
+    }
+    default: return T(NAN);
+  }
+}
+
 template <class T> T elevation_eval(const SpecT<T>& s, const T p[3]) {
+  if (s.elevation_mode != UPF_ELEV_LAYERS) return elevation_mode(s, p);
   T acc = T(0.0f), m = T(0.0f);
   for (int l = 0; l < s.n_layers; l++) {
     const LayerT<T>& L = s.layers[l];

@@ -55,6 +55,7 @@ double upfo_noise_f64(int noise, uint32_t seed, const double* q, double jitter) 
     case UPF_NOISE_GRADIENT: return noise_gradient<double>(seed, q);
     case UPF_NOISE_VALUE: return noise_value<double>(seed, q);
     case UPF_NOISE_CELLULAR: return noise_cellular<double>(seed, q, jitter);
+    case UPF_NOISE_SIMPLEX: return noise_simplex<double>(seed, q);
     default: return NAN;
   }
 }
@@ -63,6 +64,7 @@ float upfo_noise_f32(int noise, uint32_t seed, const float* q, float jitter) {
     case UPF_NOISE_GRADIENT: return noise_gradient<float>(seed, q);
     case UPF_NOISE_VALUE: return noise_value<float>(seed, q);
     case UPF_NOISE_CELLULAR: return noise_cellular<float>(seed, q, jitter);
+    case UPF_NOISE_SIMPLEX: return noise_simplex<float>(seed, q);
     default: return NAN;
   }
 }
@@ -204,6 +206,15 @@ void upfo_image_height_to_normal(const uint8_t* in, int w, int h, float strength
   height_to_normal(in, w, h, strength, out);
 }
 void upfo_image_special_to_pbr(const uint8_t* in, int64_t n, uint8_t* out) { special_to_pbr(in, n, out); }
+
+// ---- the craters' Voronoi: the nearest point to each of n positions
+void upfo_voronoi_nearest(uint32_t seed, int ppc, float cell_size, const float* pos, int64_t n, float* out) {
+  for (int64_t i = 0; i < n; i++) voronoi_nearest(seed, ppc, cell_size, pos + 3 * i, out + 3 * i);
+}
+
+void upfo_voronoi_cell_points(uint32_t seed, int ppc, float cell_size, const int32_t* cell, float* out) {
+  voronoi_cell_points(seed, ppc, cell_size, cell, out);
+}
 
 // ---- mesh post-processing (keep flags)
 void upfo_mesh_remove_disconnected(int64_t nv, int64_t nt, const uint32_t* idx, uint8_t* keep_v, uint8_t* keep_t) {

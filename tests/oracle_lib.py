@@ -52,6 +52,8 @@ class Oracle:
                                  _P]),
             "upfo_image_dilate": (None, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "upfo_image_to_u8": (None, [_P, ctypes.c_int64, _P]),
+            "upfo_voronoi_cell_points": (None, [ctypes.c_uint32, ctypes.c_int, ctypes.c_float, _P, _P]),
+            "upfo_voronoi_nearest": (None, [ctypes.c_uint32, ctypes.c_int, ctypes.c_float, _P, ctypes.c_int64, _P]),
             "upfo_image_height_to_normal": (None, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P]),
             "upfo_image_special_to_pbr": (None, [_P, ctypes.c_int64, _P]),
             "upfo_mesh_remove_disconnected": (None, [ctypes.c_int64, ctypes.c_int64, _P, _P, _P]),
@@ -188,6 +190,18 @@ class Oracle:
         img = np.ascontiguousarray(img, dtype=np.float32)
         out = np.empty(img.shape, dtype=np.uint8)
         self.L.upfo_image_to_u8(_f32p(img), img.size, _f32p(out))
+        return out
+
+    def voronoi_nearest(self, seed, ppc, cell_size, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        out = np.empty_like(pos)
+        self.L.upfo_voronoi_nearest(seed, ppc, cell_size, _f32p(pos), len(pos), _f32p(out))
+        return out
+
+    def voronoi_cell_points(self, seed, ppc, cell_size, cell):
+        c = np.ascontiguousarray(cell, dtype=np.int32)
+        out = np.empty((ppc, 3), dtype=np.float32)
+        self.L.upfo_voronoi_cell_points(seed, ppc, cell_size, _f32p(c), _f32p(out))
         return out
 
     def image_height_to_normal(self, height_u8, strength=1.0):

@@ -144,7 +144,7 @@ def test_special_to_pbr_and_export_maps(upf, oracle):
                                   oracle.image_special_to_pbr(sp))
     s = S.spec_sphere_fbm(1234)
     mesh = upfm.mesh_generate_generic(s, S.grid_sphere_fbm(96))
-    tex = upfm.generate_textures_land(S.bake_spec(s), mesh, 256, 256)
+    tex = upfm.generate_textures_land(S.bake_spec(s), mesh, 512, 384)
     albedo, special, height = upfm.texture_post(tex)
     a2, pbr, nrm = upfm.export_maps(albedo, special, height)
     assert a2 is albedo

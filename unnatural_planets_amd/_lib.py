@@ -27,6 +27,7 @@ SIGNATURES = {
     "upf_layer_init": (ctypes.c_int, [ctypes.POINTER(Layer), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_uint32, ctypes.c_float, ctypes.c_int32]),
+    "upf_spec_set_elevation_mode": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.c_int32, ctypes.c_uint32]),
     "upf_spec_validate": (ctypes.c_int, [ctypes.POINTER(FieldSpec)]),
     "upf_flops_per_sample": (ctypes.c_double, [ctypes.POINTER(FieldSpec)]),
     "upf_adjoint_flops_per_sample": (ctypes.c_double, [ctypes.POINTER(FieldSpec)]),

@@ -35,6 +35,9 @@ int to_device(const upf_field_spec* in, DSpec* out) {
   if (shape_params_used(in->shape) < 0) return UPF_ERR_BAD_SPEC;
   if (in->compose < 0 || in->compose >= UPF_COMPOSE_COUNT) return UPF_ERR_BAD_SPEC;
   if (in->n_layers < 0 || in->n_layers > UPF_MAX_LAYERS) return UPF_ERR_BAD_SPEC;
+  const int mode = in->elevation_mode;
+  if (mode < 0 || mode >= UPF_ELEV_COUNT) return UPF_ERR_BAD_SPEC;
+  if (mode != UPF_ELEV_LAYERS && in->n_layers != elevation_mode_layers(mode)) return UPF_ERR_BAD_SPEC;
   for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++)
     if (!finite(in->shape_params[i])) return UPF_ERR_INVALID_VALUE;
   if (!finite(in->elevation_scale)) return UPF_ERR_INVALID_VALUE;
@@ -45,10 +48,19 @@ int to_device(const upf_field_spec* in, DSpec* out) {
   out->n_layers = in->n_layers;
   for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++) out->sp[i] = in->shape_params[i];
   out->escale = in->elevation_scale;
+  out->mode = mode;
   out->all_add = 1;
   for (int l = 0; l < in->n_layers; l++) {
     const upf_layer& a = in->layers[l];
     DLayer& b = out->L[l];
+    if (mode == UPF_ELEV_CRATERS && l == 0) {  // the impact Voronoi: seed, points per cell, cell size
+      if (a.octaves < 1 || a.octaves > UPF_MAX_OCTAVES) return UPF_ERR_BAD_SPEC;
+      if (!finite(a.freq[0]) || !(a.freq[0] > 0.0f)) return UPF_ERR_INVALID_VALUE;
+      b.octaves = a.octaves;
+      b.seed = a.seed;
+      b.freq[0] = a.freq[0];
+      continue;
+    }
     if (a.noise < 1 || a.noise >= UPF_NOISE_COUNT) return UPF_ERR_BAD_SPEC;
     if (a.fractal < 0 || a.fractal >= UPF_FRACTAL_COUNT) return UPF_ERR_BAD_SPEC;
     if (a.blend < 0 || a.blend >= UPF_BLEND_COUNT) return UPF_ERR_BAD_SPEC;
@@ -121,6 +133,9 @@ double noise_flops(int noise) {
     // q (3) + js, j0 (2) + k - j0 per axis value (9) + 27 cells x (3 fma + 3 sub + 5 d2)
     // + sqrt (1)
     case UPF_NOISE_CELLULAR: return 3 + 2 + 9 + 27 * (6 + 3 + 5) + 1;
+    // q (3), skew (3), skewed point (3), unskew (3), corner 0 offset (6), corners 1-3
+    // (6 + 6 + 3), 4 corners x (r^2 5, t 1, t^4 2, dot 1, product 1, sum 1), scale 1
+    case UPF_NOISE_SIMPLEX: return 3 + 3 + 3 + 3 + 6 + 15 + 4 * 11 + 1;
     default: return 0;
   }
 }
@@ -156,6 +171,8 @@ OpCount operator+(float a, OpCount b) { return op(a + b.v, 1, 1); }
 OpCount operator-(float a, OpCount b) { return op(a - b.v, 1, 1); }
 OpCount operator*(float a, OpCount b) { return op(a * b.v, 1, 4); }
 OpCount sq_root(OpCount a) { return op(std::sqrt(a.v), 1, 5); }
+OpCount pow_v(OpCount a, float e) { return op(std::pow(a.v, e), 1, 5); }  // pow counted as 1
+OpCount floor_v(OpCount a) { return {std::floor(a.v)}; }
 
 // the point the flop count of a data-dependent shape is taken at (the oracle's count
 // uses the same point)
@@ -179,7 +196,31 @@ double shape_flops(int shape, const float* sp) {
   }
 }
 
+double layer_flops(const upf_layer& L) {
+  const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
+  return oct * (noise_flops(L.noise) + (L.fractal == UPF_FRACTAL_RIDGED ? 4 : (L.fractal == UPF_FRACTAL_FBM ? 2 : 0)));
+}
+
+// An elevation mode, tallied through its template (upf_modes.hpp) on the full path
+// (every layer value -0.6, which takes the lakes' mountains branch), each layer call
+// costing its layer's flops; the craters' Voronoi adds 17 flops per candidate point.
+double mode_flops(const upf_field_spec* s) {
+  g_tally = OpTally{};
+  const auto layer = [&](int l, OpCount, OpCount, OpCount) {
+    g_tally.fwd += layer_flops(s->layers[l]);
+    return OpCount{-0.6f};
+  };
+  const upf_layer& v = s->layers[0];
+  (void)upf::elevation_mode_v<OpCount>(s->elevation_mode, v.seed, v.octaves, v.freq[0] > 0.0f ? v.freq[0] : 1.0f,
+                                       s->n_layers, OpCount{kCountPoint[0]}, OpCount{kCountPoint[1]},
+                                       OpCount{kCountPoint[2]}, layer, OpCount{0.0f});
+  double f = g_tally.fwd;
+  if (s->elevation_mode == UPF_ELEV_CRATERS) f += 27.0 * v.octaves * 17 + 1;
+  return f;
+}
+
 double elevation_flops(const upf_field_spec* s) {
+  if (s->elevation_mode != UPF_ELEV_LAYERS) return mode_flops(s);
   double f = 0;
   for (int l = 0; l < s->n_layers; l++) {
     const upf_layer& L = s->layers[l];
@@ -204,6 +245,8 @@ double noise_d_flops(int noise) {
     case UPF_NOISE_VALUE: return 3 + 3 + 21 + 18 + 18 + 10 + 6 + 3;
     // forward cellular 393, 1/r, 3 products
     case UPF_NOISE_CELLULAR: return noise_flops(UPF_NOISE_CELLULAR) + 1 + 3;
+    // forward, then per corner t^3 1, -8 t^3 (g.d) 2, t^4 g 3, k d 3, sums 6; scale 3
+    case UPF_NOISE_SIMPLEX: return noise_flops(UPF_NOISE_SIMPLEX) + 4 * 15 + 3;
     default: return 0;
   }
 }
@@ -235,7 +278,7 @@ extern "C" {
 int upf_version(void) { return UPF_VERSION; }
 
 double upf_adjoint_flops_per_sample(const upf_field_spec* s) {
-  if (!s) return 0;
+  if (!s || s->elevation_mode != UPF_ELEV_LAYERS) return 0;
   const bool has_shape = s->compose != UPF_COMPOSE_ELEVATION_RAW;
   const bool has_elev = s->compose == UPF_COMPOSE_LAND || s->compose == UPF_COMPOSE_NAVIGATION ||
                         s->compose == UPF_COMPOSE_ELEVATION_RAW;
@@ -306,6 +349,44 @@ int upf_layer_init(upf_layer* layer, int32_t noise, int32_t fractal, int32_t oct
     f *= lacunarity;
     w *= gain;
   }
+  return UPF_OK;
+}
+
+int upf_spec_set_elevation_mode(upf_field_spec* s, int32_t mode, uint32_t seed) {
+  if (!s) return UPF_ERR_NULL;
+  if (mode < 0 || mode >= UPF_ELEV_COUNT) return UPF_ERR_BAD_SPEC;
+  s->elevation_mode = mode;
+  std::memset(s->layers, 0, sizeof(s->layers));
+  upf_layer* L = s->layers;
+  // (noise, fractal, octaves, frequency, lacunarity, gain) per sources/elevationModes.cpp
+  switch (mode) {
+    case UPF_ELEV_SIMPLE:  // :16-27
+      upf_layer_init(&L[0], UPF_NOISE_SIMPLEX, UPF_FRACTAL_RIDGED, 6, 0.0005f, 2.0f, 0.4f, 1.0f, seed, 0.0f, 0);
+      break;
+    case UPF_ELEV_LEGACY:  // :36-54
+      upf_layer_init(&L[0], UPF_NOISE_VALUE, UPF_FRACTAL_FBM, 4, 0.0005f, 2.0f, 0.5f, 1.0f, seed, 0.0f, 0);
+      upf_layer_init(&L[1], UPF_NOISE_VALUE, UPF_FRACTAL_FBM, 4, 1.0f, 2.0f, 0.5f, 1.0f, seed + 1, 0.0f, 0);
+      break;
+    case UPF_ELEV_LAKES:    // :131-141, then the mountains :70-100
+    case UPF_ELEV_ISLANDS:  // :153-163
+      upf_layer_init(&L[0], UPF_NOISE_VALUE, UPF_FRACTAL_FBM, 4, 0.0013f, 2.0f, 0.5f, 1.0f, seed, 0.0f, 0);
+      upf_layer_init(&L[1], UPF_NOISE_GRADIENT, UPF_FRACTAL_NONE, 1, 0.0015f, 2.0f, 0.5f, 1.0f, seed + 1, 0.0f, 0);
+      upf_layer_init(&L[2], UPF_NOISE_SIMPLEX, UPF_FRACTAL_RIDGED, 4, 0.001f, 1.5f, -0.4f, 1.0f, seed + 2, 0.0f, 0);
+      upf_layer_init(&L[3], UPF_NOISE_GRADIENT, UPF_FRACTAL_FBM, 3, 0.002f, 2.0f, 0.3f, 1.0f, seed + 3, 0.0f, 0);
+      break;
+    case UPF_ELEV_CRATERS:  // :175-190: Voronoi cell 300 m, 2 points per cell; value noise
+      L[0].noise = UPF_NOISE_CELLULAR;
+      L[0].octaves = 2;
+      L[0].seed = seed;
+      L[0].weight = 1.0f;
+      L[0].freq[0] = 300.0f;
+      L[0].amp[0] = 1.0f;
+      upf_layer_init(&L[1], UPF_NOISE_VALUE, UPF_FRACTAL_NONE, 1, 1.0f, 2.0f, 0.5f, 1.0f, seed + 1, 0.0f, 0);
+      break;
+    default:
+      break;
+  }
+  s->n_layers = upf::elevation_mode_layers(mode);
   return UPF_OK;
 }
 

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "../../include/upf.h"
+#include "upf_modes.hpp"
 #include "upf_shapes.hpp"
 
 namespace upf {
@@ -32,7 +33,8 @@ struct DSpec {
   int32_t shape, compose, n_layers, all_add;
   float sp[UPF_MAX_SHAPE_PARAMS];
   float escale;
-  float pad[3];
+  int32_t mode;  // upf_elevation_mode_id
+  float pad[2];
   DLayer L[UPF_MAX_LAYERS];
 };
 
@@ -174,6 +176,62 @@ __device__ __forceinline__ float noise_cellular(uint32_t seed, float qx, float q
   return __builtin_sqrtf(cellular_d2(seed, qx, qy, qz, js, j0, ex, ey, ez));
 }
 
+// 3D simplex noise (Perlin 2001, in the form of Gustavson's "Simplex noise demystified",
+// 2005): skew by (x+y+z)/3 to find the cell, unskew by (i+j+k)/6, walk to the four
+// corners of the containing simplex in the order of the offset components, and sum
+// (0.5 - r^2)^4 (g . d) over the corners with r^2 < 0.5, times 72.  Radius^2 = 0.5 is
+// the largest for which no other lattice point reaches into the simplex, so the noise is
+// continuous; the corner gradient g and the hash are the lattice noise's.
+struct SimplexCell {
+  int32_t i, j, k;
+  int32_t o1[3], o2[3];    // offsets of corners 1 and 2
+  float d[4][3];           // position minus corner
+};
+__device__ __forceinline__ void simplex_cell(float x, float y, float z, SimplexCell& c) {
+  const float F3 = 0.33333334f, G3 = 0.16666667f, G3x2 = 0.33333334f;
+  const float s = (x + y + z) * F3;
+  const float fi = floorf(x + s), fj = floorf(y + s), fk = floorf(z + s);
+  const float t = (fi + fj + fk) * G3;
+  const float x0 = x - (fi - t), y0 = y - (fj - t), z0 = z - (fk - t);
+  c.i = (int32_t)fi; c.j = (int32_t)fj; c.k = (int32_t)fk;
+  const bool xy = x0 >= y0, yz = y0 >= z0, xz = x0 >= z0;
+  // corner 1: the largest offset component's axis; corner 2: the two largest
+  c.o1[0] = (xy && xz) ? 1 : 0;
+  c.o1[1] = (!xy && yz) ? 1 : 0;
+  c.o1[2] = (!xz && !yz) ? 1 : 0;
+  c.o2[0] = (xy || xz) ? 1 : 0;
+  c.o2[1] = (!xy || yz) ? 1 : 0;
+  c.o2[2] = (!xz || !yz) ? 1 : 0;
+  c.d[0][0] = x0; c.d[0][1] = y0; c.d[0][2] = z0;
+  c.d[1][0] = (x0 - (float)c.o1[0]) + G3; c.d[1][1] = (y0 - (float)c.o1[1]) + G3;
+  c.d[1][2] = (z0 - (float)c.o1[2]) + G3;
+  c.d[2][0] = (x0 - (float)c.o2[0]) + G3x2; c.d[2][1] = (y0 - (float)c.o2[1]) + G3x2;
+  c.d[2][2] = (z0 - (float)c.o2[2]) + G3x2;
+  c.d[3][0] = x0 - 0.5f; c.d[3][1] = y0 - 0.5f; c.d[3][2] = z0 - 0.5f;
+}
+__device__ __forceinline__ uint32_t simplex_hash(uint32_t seed, const SimplexCell& c, int corner) {
+  const int32_t ox = corner == 0 ? 0 : (corner == 1 ? c.o1[0] : (corner == 2 ? c.o2[0] : 1));
+  const int32_t oy = corner == 0 ? 0 : (corner == 1 ? c.o1[1] : (corner == 2 ? c.o2[1] : 1));
+  const int32_t oz = corner == 0 ? 0 : (corner == 1 ? c.o1[2] : (corner == 2 ? c.o2[2] : 1));
+  const uint32_t h = seed ^ ((uint32_t)(c.i + ox) * kPX) ^ ((uint32_t)(c.j + oy) * kPY) ^ ((uint32_t)(c.k + oz) * kPZ);
+  return h * kHM;
+}
+__device__ __forceinline__ float noise_simplex(uint32_t seed, float x, float y, float z) {
+  SimplexCell c;
+  simplex_cell(x, y, z, c);
+  float n = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float dx = c.d[q][0], dy = c.d[q][1], dz = c.d[q][2];
+    const float t = 0.5f - (dx * dx + dy * dy + dz * dz);
+    if (t > 0.0f) {
+      const float t2 = t * t;
+      n = n + t2 * t2 * gdot(simplex_hash(seed, c, q), dx, dy, dz);
+    }
+  }
+  return 72.0f * n;
+}
+
 // ------------------------------------------------------------------ shapes
 __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, float z) {
   switch (s.shape) {
@@ -209,6 +267,7 @@ __device__ __forceinline__ float octave_noise(const DLayer& L, int k, float x, f
   const uint32_t seed = L.seed + (uint32_t)k;
   if (L.noise == UPF_NOISE_GRADIENT) return noise_gradient(seed, qx, qy, qz);
   if (L.noise == UPF_NOISE_VALUE) return noise_value(seed, qx, qy, qz);
+  if (L.noise == UPF_NOISE_SIMPLEX) return noise_simplex(seed, qx, qy, qz);
   return noise_cellular(seed, qx, qy, qz, L.js, L.j0);
 }
 
@@ -226,23 +285,40 @@ __device__ __forceinline__ float layer_value(const DLayer& L, float x, float y, 
   return acc;
 }
 
+// Elevation: the generic layer stack, or a fixed mode (upf_modes.hpp) whose layers go
+// through the same single layer_value call.  Layer 0 of the craters mode carries the
+// Voronoi parameters (seed, points per cell in octaves, cell size in freq[0]).
+// MODES = false compiles the modes out (kernels instantiated for the layer stack only).
+template <bool MODES = true>
 __device__ __forceinline__ float elevation_eval(const DSpec& s, float x, float y, float z) {
+  const int mode = MODES ? s.mode : UPF_ELEV_LAYERS;
+  ModeState<float> st;
+  if (mode != UPF_ELEV_LAYERS) mode_begin(mode, s.L[0].seed, s.L[0].octaves, s.L[0].freq[0], x, y, z, 0.0f, st);
   float acc = 0.0f, m = 0.0f;
   for (int l = 0; l < s.n_layers; l++) {
     const DLayer& L = s.L[l];
-    const float v = L.weight * layer_value(L, x, y, z);
+    float px = x, py = y, pz = z;
+    if (mode != UPF_ELEV_LAYERS) mode_position(mode, l, st, x, y, z, 0.0f, px, py, pz);
+    const float raw = layer_value(L, px, py, pz);
+    if (mode != UPF_ELEV_LAYERS) {
+      if (mode_after(mode, l, raw, 0.0f, st)) break;
+      continue;
+    }
+    const float v = L.weight * raw;
     if (L.blend == UPF_BLEND_ADD) acc = acc + v;
     else if (L.blend == UPF_BLEND_MASK) m = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
     else acc = fmaf(m, v - acc, acc);
   }
+  if (mode != UPF_ELEV_LAYERS) return mode_finish(mode, st, x, y, z, 0.0f);
   return acc;
 }
 
+template <bool MODES = true>
 __device__ __forceinline__ float field_eval(const DSpec& s, float x, float y, float z) {
   const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
                          s.compose == UPF_COMPOSE_ELEVATION_RAW;
   const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval(s, x, y, z);
-  const float elev = need_elev ? elevation_eval(s, x, y, z) : 0.0f;
+  const float elev = need_elev ? elevation_eval<MODES>(s, x, y, z) : 0.0f;
   switch (s.compose) {
     case UPF_COMPOSE_LAND: return shape - s.escale * elev;
     case UPF_COMPOSE_WATER: return shape;

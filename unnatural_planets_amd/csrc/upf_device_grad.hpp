@@ -83,6 +83,34 @@ __device__ __forceinline__ float noise_cellular_d(uint32_t seed, float qx, float
   return r;
 }
 
+// Simplex noise and its gradient: d/dq of t^4 (g . d) with t = 0.5 - |d|^2 is
+// t^4 g - 8 t^3 (g . d) d per corner.
+__device__ __forceinline__ float noise_simplex_d(uint32_t seed, float qx, float qy, float qz, float& dx, float& dy,
+                                                 float& dz) {
+  SimplexCell c;
+  simplex_cell(qx, qy, qz, c);
+  float n = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float ex = c.d[q][0], ey = c.d[q][1], ez = c.d[q][2];
+    const float t = 0.5f - (ex * ex + ey * ey + ez * ez);
+    if (t > 0.0f) {
+      const uint32_t h = simplex_hash(seed, c, q);
+      float vx, vy, vz;
+      gvec(h, vx, vy, vz);
+      const float dot = gdot(h, ex, ey, ez);
+      const float t2 = t * t, t4 = t2 * t2;
+      n = n + t4 * dot;
+      const float k = -8.0f * (t2 * t) * dot;
+      gx = gx + (t4 * vx + k * ex);
+      gy = gy + (t4 * vy + k * ey);
+      gz = gz + (t4 * vz + k * ez);
+    }
+  }
+  dx = 72.0f * gx; dy = 72.0f * gy; dz = 72.0f * gz;
+  return 72.0f * n;
+}
+
 __device__ __forceinline__ float octave_noise_d(const DLayer& L, int k, float x, float y, float z,
                                                 float& dx, float& dy, float& dz) {
   const float f = L.freq[k];
@@ -90,6 +118,7 @@ __device__ __forceinline__ float octave_noise_d(const DLayer& L, int k, float x,
   const uint32_t seed = L.seed + (uint32_t)k;
   if (L.noise == UPF_NOISE_GRADIENT) return noise_gradient_d(seed, qx, qy, qz, dx, dy, dz);
   if (L.noise == UPF_NOISE_VALUE) return noise_value_d(seed, qx, qy, qz, dx, dy, dz);
+  if (L.noise == UPF_NOISE_SIMPLEX) return noise_simplex_d(seed, qx, qy, qz, dx, dy, dz);
   return noise_cellular_d(seed, qx, qy, qz, L.js, L.j0, dx, dy, dz);
 }
 
@@ -216,8 +245,31 @@ __device__ __forceinline__ float layer_grad(const DLayer& L, int l, float x, flo
   return v;
 }
 
-// f, df/dp and (THETA) dLdf * df/dθ for one sample.
-template <bool THETA>
+// Elevation of a fixed mode with its position gradient (forward mode).  A layer sampled
+// at a position that is itself a function of the sample (the legacy mode's p * scale)
+// chains its lattice gradient through that position's derivatives.
+__device__ __forceinline__ Grad elevation_mode_grad(const DSpec& s, float x, float y, float z) {
+  const int mode = s.mode;
+  const Grad gx = gvar(x, 0), gy = gvar(y, 1), gz = gvar(z, 2), zero = gc(0.0f);
+  ModeState<Grad> st;
+  mode_begin(mode, s.L[0].seed, s.L[0].octaves, s.L[0].freq[0], gx, gy, gz, zero, st);
+  for (int l = 0; l < s.n_layers; l++) {
+    Grad px, py, pz;
+    mode_position(mode, l, st, gx, gy, gz, zero, px, py, pz);
+    float g[3] = {0.0f, 0.0f, 0.0f};
+    ThetaAcc none{nullptr};
+    const float v = layer_grad<false>(s.L[l], l, px.v, py.v, pz.v, 1.0f, 0.0f, g, none);
+    const Grad lv = {v, g[0] * px.dx + g[1] * py.dx + g[2] * pz.dx, g[0] * px.dy + g[1] * py.dy + g[2] * pz.dy,
+                     g[0] * px.dz + g[1] * py.dz + g[2] * pz.dz};
+    if (mode_after(mode, l, lv, zero, st)) break;
+  }
+  return mode_finish(mode, st, gx, gy, gz, zero);
+}
+
+// f, df/dp and (THETA) dLdf * df/dθ for one sample.  With an elevation mode (compiled
+// in with MODES) only the position gradient is formed: the parameter gradient is defined
+// for the layer stack.
+template <bool THETA, bool MODES = false>
 __device__ __forceinline__ float field_grad(const DSpec& s, float x, float y, float z, float dLdf, float g[3],
                                             ThetaAcc& acc) {
   float gs[3], dps[4];
@@ -237,6 +289,18 @@ __device__ __forceinline__ float field_grad(const DSpec& s, float x, float y, fl
   if (THETA) {
     const int np = shape_param_count(s.shape);
     for (int i = 0; i < np; i++) acc.add(kThShape + i, dLdf * dshape * dps[i]);
+  }
+  if (MODES && !THETA && need_elev && s.mode != UPF_ELEV_LAYERS) {
+    const Grad e = elevation_mode_grad(s, x, y, z);
+    if (s.compose == UPF_COMPOSE_ELEVATION_RAW) {
+      g[0] = e.dx; g[1] = e.dy; g[2] = e.dz;
+      return e.v;
+    }
+    const bool on = s.compose == UPF_COMPOSE_LAND || e.v > 0.0f;
+    if (on) {
+      g[0] = g[0] - s.escale * e.dx; g[1] = g[1] - s.escale * e.dy; g[2] = g[2] - s.escale * e.dz;
+    }
+    return s.compose == UPF_COMPOSE_LAND ? shape - s.escale * e.v : shape - s.escale * fmaxf(e.v, 0.0f);
   }
   if (!need_elev || s.n_layers == 0) {
     return s.compose == UPF_COMPOSE_ELEVATION ? shape / s.escale

@@ -29,7 +29,10 @@ SHAPE = {"none": 0, "sphere": 1, "box": 2, "torus": 3, "capsule": 4, "cylinder":
 SHAPE_REFERENCE_PARAMS = {"cylinder": (10000.0, 800.0), "octahedron": (800.0, 100.0),
                           "double_torus": (900.0, 400.0, 200.0), "torus_cross": (750.0, 400.0, 200.0),
                           "plane": (0.0, 0.0, -1.0, 0.0)}
-NOISE = {"gradient": 1, "value": 2, "cellular": 3}
+NOISE = {"gradient": 1, "value": 2, "cellular": 3, "simplex": 4}
+# elevation modes: the reference's fixed elevation functions (sources/elevationModes.cpp)
+ELEV = {"layers": 0, "none": 1, "simple": 2, "legacy": 3, "lakes": 4, "islands": 5, "craters": 6}
+ELEV_LAYERS = {0: None, 1: 0, 2: 1, 3: 2, 4: 4, 5: 4, 6: 2}
 FRACTAL = {"none": 0, "fbm": 1, "ridged": 2}
 BLEND = {"add": 0, "mask": 1, "lerp": 2}
 COMPOSE = {"land": 0, "water": 1, "navigation": 2, "elevation": 3, "elevation_raw": 4}
@@ -47,7 +50,7 @@ class Layer(ctypes.Structure):
 class FieldSpec(ctypes.Structure):
     _fields_ = [
         ("shape", ctypes.c_int32), ("compose", ctypes.c_int32), ("n_layers", ctypes.c_int32),
-        ("flags", ctypes.c_int32), ("shape_params", ctypes.c_float * MAX_SHAPE_PARAMS),
+        ("elevation_mode", ctypes.c_int32), ("shape_params", ctypes.c_float * MAX_SHAPE_PARAMS),
         ("elevation_scale", ctypes.c_float), ("reserved", ctypes.c_float * 3),
         ("layers", Layer * MAX_LAYERS),
     ]
@@ -121,6 +124,45 @@ def field_spec(shape="sphere", shape_params=(1.0,), layers=(), compose="land",
     for i, L in enumerate(layers):
         s.layers[i] = L
     return s
+
+
+def elevation_mode_layers(mode, seed: int = 0):
+    """The noise layers of an elevation mode, as upf_spec_set_elevation_mode fills them
+    (sources/elevationModes.cpp:14-198; the k-th noise seeded seed + k; cage's unset
+    lacunarity and gain taken as 2 and 0.5)."""
+    m = ELEV[mode] if isinstance(mode, str) else int(mode)
+    s0, s1, s2, s3 = (int(seed) + k for k in range(4))
+    if m == ELEV["simple"]:
+        return [layer("simplex", "ridged", 6, 0.0005, 2.0, 0.4, seed=s0, jitter=0.0)]
+    if m == ELEV["legacy"]:
+        return [layer("value", "fbm", 4, 0.0005, seed=s0, jitter=0.0), layer("value", "fbm", 4, 1.0, seed=s1, jitter=0.0)]
+    if m in (ELEV["lakes"], ELEV["islands"]):
+        return [layer("value", "fbm", 4, 0.0013, seed=s0, jitter=0.0),
+                layer("gradient", "none", 1, 0.0015, seed=s1, jitter=0.0),
+                layer("simplex", "ridged", 4, 0.001, 1.5, -0.4, seed=s2, jitter=0.0),
+                layer("gradient", "fbm", 3, 0.002, 2.0, 0.3, seed=s3, jitter=0.0)]
+    if m == ELEV["craters"]:
+        v = Layer()  # the impact Voronoi: seed, points per cell (octaves), cell size (freq[0])
+        v.noise, v.octaves, v.seed, v.weight = NOISE["cellular"], 2, s0 & 0xFFFFFFFF, 1.0
+        v.freq[0], v.amp[0] = 300.0, 1.0
+        return [v, layer("value", "none", 1, 1.0, seed=s1, jitter=0.0)]
+    return []
+
+
+def spec_elevation_mode(mode, seed: int = 1, shape="sphere", shape_params=(1100.0,), compose="land",
+                        elevation_scale=0.1) -> FieldSpec:
+    """A reference planet: base shape (metres, the sphere of sources/sdf.cpp:20-23 by
+    default) minus elevation / 10 (sources/modesConfiguration.cpp:227, 248-258), with the
+    elevation of a fixed mode."""
+    s = field_spec(shape, shape_params, elevation_mode_layers(mode, seed), compose, elevation_scale)
+    s.elevation_mode = ELEV[mode] if isinstance(mode, str) else int(mode)
+    return s
+
+
+def grid_reference(n: int = 128) -> Grid:
+    """The reference's sweep box [-2000, 2000]^3 at n nodes per axis
+    (sources/meshGeneration.cpp:19-28)."""
+    return uniform_grid(-2000.0, 2000.0, n)
 
 
 def uniform_grid(lo: float, hi: float, n: int) -> Grid:
