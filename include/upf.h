@@ -456,6 +456,88 @@ int upf_bake(const upf_bake_spec* spec, const float* d_positions, const float* d
              const int32_t* d_tri, int32_t width, int32_t height, float* d_albedo, float* d_bump,
              float* d_roughness, void* stream);
 
+/* ---- the full material stack ------------------------------------------------------
+ * terrainTile with the reference's coloring modes coloringDefault / coloringBarren /
+ * coloringDebug (sources/coloringModes.cpp:905-966, selected at
+ * sources/modesConfiguration.cpp:196-224): elevation, precipitation, temperature, slope
+ * from 8 elevation probes at radius 0.5 in the tangent plane, biome, type, and the
+ * layers bedrock, cliffs, mica, dirt, sand, grass, boulders, water, flowers, ice, snow.
+ *
+ * The reference's noise objects (about 40, each with its configuration in
+ * coloringModes.cpp) are evaluated with those configurations; object k is seeded
+ * seed + 16 k (the table is in upf_material.hpp) where the reference draws noiseSeed().
+ * cage's noise kinds are not in the reference's checkout; this library defines them as
+ *   Perlin          lattice gradient noise (UPF_NOISE_GRADIENT)
+ *   Value           lattice value noise (UPF_NOISE_VALUE)
+ *   Cubic           value noise interpolated by the cubic through the 4 lattice values
+ *                   around the sample per axis, t^3 p + t^2 ((a - b) - p) + t (c - a) + b
+ *                   with p = (d - c) - (a - b), scaled by 1 / 1.5^3
+ *   Simplex, SimplexReduced   simplex noise (UPF_NOISE_SIMPLEX)
+ *   Cellular        F1 / F2 over the 27 cells around the sample (one point per cell at
+ *                   k - 0.5 + r, r in [0, 1]^3), distance Euclidean (squared, rooted for
+ *                   the distance operations), Manhattan, or Hybrid (Manhattan +
+ *                   squared Euclidean); operation Cell = hash of the nearest cell as
+ *                   int32 * 2^-31, Subtract = F2 - F1 - 1, Multiply = F2 F1 / 2 - 1,
+ *                   Divide = F1 / F2 - 1
+ * with fractal amplitudes bounded by 1 / sum |gain|^k, lacunarity 2 and gain 0.5 where
+ * the reference leaves them unset.  The colour helpers: HSV is the hexcone model;
+ * colorHueShift is the reference's (sources/math.cpp:47-52); interpolateColor runs in
+ * HSV with the hue along the shorter arc; colorDeviation (sources/math.cpp:40-45, a
+ * global RNG per texel in the reference) moves the colour in HSV by a vector hashed
+ * from the position's float bits.  anyPerpendicular and the Voronoi are the reference's
+ * (sources/math.cpp:65-72, sources/voronoi.cpp with the plane projection of :58-66).
+ * The beach mask's 1-D noise of the elevation samples the 3-D noise at (e, 0, 0).
+ *
+ * The spec's field gives terrainSdfElevation = shape / elevation_scale (LAND purpose)
+ * and terrainSdfElevationRaw = the elevation stack (other purposes); its compose is
+ * ignored. */
+enum upf_coloring_id {
+  UPF_COLORING_DEFAULT = 0,
+  UPF_COLORING_BARREN = 1,
+  UPF_COLORING_DEBUG = 2,
+  UPF_COLORING_COUNT = 3
+};
+enum upf_purpose_id {  /* MeshPurposeEnum, sources/planets.h:50-56 */
+  UPF_PURPOSE_LAND = 1,
+  UPF_PURPOSE_WATER = 2,
+  UPF_PURPOSE_NAVIGATION = 3
+};
+typedef struct upf_material_spec {
+  upf_field_spec field;
+  int32_t coloring; /* upf_coloring_id */
+  int32_t purpose;  /* upf_purpose_id  */
+  int32_t poles;    /* config unnatural-planets/poles/enable   (sources/coloringModes.cpp:18) */
+  int32_t flowers;  /* config unnatural-planets/flowers/enable (:19): mica, boulders, flowers */
+  uint32_t seed;
+  int32_t reserved[3];
+} upf_material_spec;
+
+/* Tile (sources/planets.h:63-83): the fields the material stack produces.  slope in
+ * radians; biome and type in TerrainBiomeEnum / TerrainTypeEnum order (Bare = 0 ..
+ * Water = 11; Road = 0, Flat, Rough, Cliffs, Water). */
+typedef struct upf_tile {
+  float albedo[3];
+  float roughness, metallic, height, opacity;
+  float elevation, slope, temperature, precipitation;
+  int32_t biome, type;
+  int32_t reserved[3];
+} upf_tile;
+
+/* terrainTile at n points with unit normals (the navmesh loop of
+ * sources/tileProperties.cpp:258-273, purpose NAVIGATION there).  d_positions,
+ * d_normals: float[n][3]; d_tiles: upf_tile[n]. */
+int upf_material_tiles(const upf_material_spec* spec, const float* d_positions, const float* d_normals,
+                       int64_t n, upf_tile* d_tiles, void* stream);
+/* The reference's texel bake with the full material (Generator::pixel,
+ * sources/textures.cpp:24-43): the tile at the texel's barycentric position with the
+ * normalized barycentric normal.  d_albedo: float[h][w][3] for LAND, [h][w][4] (albedo,
+ * opacity) for WATER; d_special: float[h][w][2] (roughness, metallic); d_height:
+ * float[h][w].  Uncovered texels are NaN, as upf_bake. */
+int upf_bake_material(const upf_material_spec* spec, const float* d_positions, const float* d_normals,
+                      int64_t n_vertices, const uint32_t* d_indices, int64_t n_triangles, const float* d_uv,
+                      const int32_t* d_tri, int32_t width, int32_t height, float* d_albedo, float* d_special,
+                      float* d_height, void* stream);
+
 /* ---- texture post-processing -----------------------------------------------------
  * The steps after the bake in Generator::generate (sources/textures.cpp:73-81):
  * imageDilation(image, 7, true) and imageConvert(image, U8).  cage's kernels are not in

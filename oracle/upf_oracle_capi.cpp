@@ -7,6 +7,7 @@
 #include "upf_oracle_mc.hpp"
 #include "upf_oracle_bake.hpp"
 #include "upf_oracle_post.hpp"
+#include "upf_oracle_material.hpp"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -215,6 +216,75 @@ void upfo_voronoi_nearest(uint32_t seed, int ppc, float cell_size, const float* 
 void upfo_voronoi_cell_points(uint32_t seed, int ppc, float cell_size, const int32_t* cell, float* out) {
   voronoi_cell_points(seed, ppc, cell_size, cell, out);
 }
+
+// ---- the material stack: terrainTile at n points -> upf_tile[n]
+void upfo_material_tiles(const upf_material_spec* spec, const float* pos, const float* nrm, int64_t n, upf_tile* out) {
+  const mat::Spec s = mat::spec_from(*spec);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; i++) {
+    mat::Tile t;
+    for (int a = 0; a < 3; a++) { t.pos[a] = pos[3 * i + a]; t.nrm[a] = nrm[3 * i + a]; }
+    mat::tile(s, t);
+    upf_tile& o = out[i];
+    for (int a = 0; a < 3; a++) o.albedo[a] = t.albedo[a];
+    o.roughness = t.roughness; o.metallic = t.metallic; o.height = t.height; o.opacity = t.opacity;
+    o.elevation = t.elevation; o.slope = t.slope; o.temperature = t.temperature; o.precipitation = t.precipitation;
+    o.biome = t.biome; o.type = t.type;
+    o.reserved[0] = o.reserved[1] = o.reserved[2] = 0;
+  }
+}
+// Generator::pixel with the full material (sources/textures.cpp:24-43): the tile at the
+// texel's barycentric position with the normalized barycentric normal.
+void upfo_bake_material(const upf_material_spec* spec, const float* P, const float* N, const uint32_t* I,
+                        const float* uv, const int32_t* tri, int w, int h, float* albedo, float* special,
+                        float* height) {
+  const mat::Spec s = mat::spec_from(*spec);
+  const int ac = spec->purpose == UPF_PURPOSE_WATER ? 4 : 3;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      const int64_t o = (int64_t)y * w + x;
+      const int32_t t = tri[o];
+      if (t < 0) {
+        for (int c = 0; c < ac; c++) albedo[ac * o + c] = NAN;
+        special[2 * o] = special[2 * o + 1] = height[o] = NAN;
+        continue;
+      }
+      const float ax = uv[6 * (int64_t)t] * (float)w, ay = uv[6 * (int64_t)t + 1] * (float)h;
+      const float bx = uv[6 * (int64_t)t + 2] * (float)w, by = uv[6 * (int64_t)t + 3] * (float)h;
+      const float cx = uv[6 * (int64_t)t + 4] * (float)w, cy = uv[6 * (int64_t)t + 5] * (float)h;
+      const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+      const float area = edge_fn(ax, ay, bx, by, cx, cy);
+      const float wt[3] = {edge_fn(bx, by, cx, cy, px, py) / area, edge_fn(cx, cy, ax, ay, px, py) / area,
+                           edge_fn(ax, ay, bx, by, px, py) / area};
+      const uint32_t iv[3] = {I[3 * (int64_t)t], I[3 * (int64_t)t + 1], I[3 * (int64_t)t + 2]};
+      mat::Tile tl;
+      float n[3];
+      for (int a = 0; a < 3; a++) {
+        tl.pos[a] = wt[0] * P[3 * iv[0] + a] + wt[1] * P[3 * iv[1] + a] + wt[2] * P[3 * iv[2] + a];
+        n[a] = wt[0] * N[3 * iv[0] + a] + wt[1] * N[3 * iv[1] + a] + wt[2] * N[3 * iv[2] + a];
+      }
+      const float nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      if (nl > 0.0f) for (int a = 0; a < 3; a++) tl.nrm[a] = n[a] / nl;
+      else { tl.nrm[0] = 0.0f; tl.nrm[1] = 1.0f; tl.nrm[2] = 0.0f; }
+      mat::tile(s, tl);
+      for (int a = 0; a < 3; a++) albedo[ac * o + a] = tl.albedo[a];
+      if (ac == 4) albedo[ac * o + 3] = tl.opacity;
+      special[2 * o] = tl.roughness;
+      special[2 * o + 1] = tl.metallic;
+      height[o] = tl.height;
+    }
+}
+
+// single noise objects of the stack, for the pins: kind, fractal, octaves, frequency,
+// lacunarity, gain, distance, operation
+float upfo_material_noise(int kind, uint32_t seed, float freq, int fractal, int octaves, float lac, float gain, int dist,
+                          int ret, const float* p) {
+  mat::Cfg c{(mat::Kind)kind, freq, fractal, octaves, lac, gain, (mat::Dist)dist, (mat::Ret)ret};
+  return mat::noise(c, seed, p);
+}
+void upfo_rgb_to_hsv(const float* c, float* o) { mat::rgb_to_hsv(c, o); }
+void upfo_hsv_to_rgb(const float* c, float* o) { mat::hsv_to_rgb(c, o); }
 
 // ---- mesh post-processing (keep flags)
 void upfo_mesh_remove_disconnected(int64_t nv, int64_t nt, const uint32_t* idx, uint8_t* keep_v, uint8_t* keep_t) {

@@ -52,6 +52,13 @@ class Oracle:
                                  _P]),
             "upfo_image_dilate": (None, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "upfo_image_to_u8": (None, [_P, ctypes.c_int64, _P]),
+            "upfo_bake_material": (None, [ctypes.c_void_p, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P]),
+            "upfo_material_tiles": (None, [ctypes.c_void_p, _P, _P, ctypes.c_int64, _P]),
+            "upfo_material_noise": (ctypes.c_float, [ctypes.c_int, ctypes.c_uint32, ctypes.c_float, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                                     ctypes.c_int, _P]),
+            "upfo_rgb_to_hsv": (None, [_P, _P]),
+            "upfo_hsv_to_rgb": (None, [_P, _P]),
             "upfo_voronoi_cell_points": (None, [ctypes.c_uint32, ctypes.c_int, ctypes.c_float, _P, _P]),
             "upfo_voronoi_nearest": (None, [ctypes.c_uint32, ctypes.c_int, ctypes.c_float, _P, ctypes.c_int64, _P]),
             "upfo_image_height_to_normal": (None, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, _P]),
@@ -197,6 +204,44 @@ class Oracle:
         out = np.empty_like(pos)
         self.L.upfo_voronoi_nearest(seed, ppc, cell_size, _f32p(pos), len(pos), _f32p(out))
         return out
+
+    def material_tiles(self, m, pos, nrm):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros((len(pos), 16), dtype=np.float32)
+        self.L.upfo_material_tiles(ctypes.byref(m), _f32p(pos), _f32p(nrm), len(pos), _f32p(out))
+        return out
+
+    def bake_material(self, m, pos, nrm, idx, uv, tri):
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float32)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        uv = np.ascontiguousarray(uv, dtype=np.float32)
+        tri = np.ascontiguousarray(tri, dtype=np.int32)
+        h, w = tri.shape
+        ac = 4 if m.purpose == 2 else 3
+        albedo = np.empty((h, w, ac), dtype=np.float32)
+        special = np.empty((h, w, 2), dtype=np.float32)
+        height = np.empty((h, w), dtype=np.float32)
+        self.L.upfo_bake_material(ctypes.byref(m), _f32p(pos), _f32p(nrm), _f32p(idx), _f32p(uv), _f32p(tri), w, h,
+                                  _f32p(albedo), _f32p(special), _f32p(height))
+        return albedo, special, height
+
+    def material_noise(self, kind, seed, freq, p, fractal=0, octaves=3, lac=2.0, gain=0.5, dist=0, ret=0):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        return self.L.upfo_material_noise(kind, seed, freq, fractal, octaves, lac, gain, dist, ret, _f32p(a))
+
+    def rgb_to_hsv(self, c):
+        a = np.ascontiguousarray(c, dtype=np.float32)
+        o = np.empty(3, dtype=np.float32)
+        self.L.upfo_rgb_to_hsv(_f32p(a), _f32p(o))
+        return o
+
+    def hsv_to_rgb(self, c):
+        a = np.ascontiguousarray(c, dtype=np.float32)
+        o = np.empty(3, dtype=np.float32)
+        self.L.upfo_hsv_to_rgb(_f32p(a), _f32p(o))
+        return o
 
     def voronoi_cell_points(self, seed, ppc, cell_size, cell):
         c = np.ascontiguousarray(cell, dtype=np.int32)

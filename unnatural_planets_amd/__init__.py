@@ -23,8 +23,8 @@ from dataclasses import dataclass
 
 from . import spec as spec_mod
 from ._lib import UpfError, check, lib, status_string  # noqa: F401
-from .spec import (BakeSpec, FieldSpec, Grid, Layer, MeshBuffers, bake_spec, copy_spec,  # noqa: F401
-                   field_spec, grid_dims, layer, uniform_grid)
+from .spec import (BakeSpec, FieldSpec, Grid, Layer, MaterialSpec, MeshBuffers, bake_spec, copy_spec,  # noqa: F401
+                   field_spec, grid_dims, layer, material_spec, uniform_grid)
 
 __all__ = [
     "FieldSpec", "Grid", "Layer", "Mesh", "UpfError", "field_spec", "layer", "uniform_grid",
@@ -37,6 +37,7 @@ __all__ = [
     "NbWorkspace", "nb_extract", "field_mesh_count", "mc_emit_counted", "image_dilate", "image_to_u8",
     "image_height_to_normal", "image_special_to_gltf_pbr", "export_maps",
     "texture_post", "mesh_remove_disconnected", "mesh_water_mask",
+    "MaterialSpec", "material_spec", "material_tiles", "bake_material", "generate_textures", "texture_post_images",
 ]
 
 
@@ -537,3 +538,64 @@ def export_maps(albedo_u8, special_u8, height_u8, stream=None):
     (sources/generator.cpp:134-139): albedo unchanged, special -> glTF PBR, height ->
     normal map with strength 1.  Returns (albedo, pbr, normal)."""
     return albedo_u8, image_special_to_gltf_pbr(special_u8, stream), image_height_to_normal(height_u8, 1.0, stream)
+
+
+# ---------------------------------------------------------------------------- material stack
+def material_tiles(m: MaterialSpec, positions, normals, stream=None):
+    """terrainTile at points with unit normals (upf_material_tiles; the navmesh loop of
+    sources/tileProperties.cpp:258-273).  Returns a float32 [n, 16] tensor of upf_tile
+    rows (spec.tiles_fields names the columns)."""
+    torch = _torch()
+    _check_f32(positions, "positions")
+    n = int(positions.shape[0])
+    _check_f32(normals, "normals", (n, 3))
+    out = torch.empty((n, 16), dtype=torch.float32, device=positions.device)
+    check(lib().upf_material_tiles(ctypes.byref(m), _ptr(positions), _ptr(normals), n, _ptr(out), _stream(stream)),
+          "upf_material_tiles")
+    return out
+
+
+def bake_material(m: MaterialSpec, mesh: Mesh, uv, tri, stream=None):
+    """Per-texel full material (upf_bake_material; Generator::pixel, sources/textures.cpp:
+    24-43) -> (albedo [h, w, 3] or [h, w, 4] for water, special [h, w, 2], height [h, w])."""
+    torch = _torch()
+    h, w = (int(x) for x in tri.shape)
+    n = mesh.n_triangles
+    _check_f32(mesh.positions, "positions")
+    if mesh.normals is None:
+        raise ValueError("the bake needs vertex normals")
+    _check_f32(mesh.normals, "normals", tuple(mesh.positions.shape))
+    _check_f32(uv, "uv", (n, 3, 2))
+    if tri.dtype != torch.int32 or not tri.is_contiguous() or not tri.is_cuda:
+        raise ValueError("tri must be a contiguous int32 CUDA tensor")
+    dev = tri.device
+    ac = 4 if m.purpose == spec_mod.PURPOSE["water"] else 3
+    albedo = torch.empty((h, w, ac), dtype=torch.float32, device=dev)
+    special = torch.empty((h, w, 2), dtype=torch.float32, device=dev)
+    height = torch.empty((h, w), dtype=torch.float32, device=dev)
+    nz = n > 0
+    check(lib().upf_bake_material(ctypes.byref(m), _ptr(mesh.positions) if nz else None,
+                                  _ptr(mesh.normals) if nz else None, mesh.n_vertices,
+                                  _ptr(mesh.indices) if nz else None, n, _ptr(uv) if nz else None, _ptr(tri), w, h,
+                                  _ptr(albedo), _ptr(special), _ptr(height), _stream(stream)), "upf_bake_material")
+    return albedo, special, height
+
+
+def generate_textures(m: MaterialSpec, mesh: Mesh, width: int, height: int, stream=None):
+    """generateTexturesLand / generateTexturesWater (sources/textures.cpp:86-96) with the
+    full material: atlas, rasterization and per-texel tiles, then the dilation by 7 and
+    the U8 conversion (:73-81).  Returns (albedo_u8, special_u8, height_u8, uv)."""
+    uv = atlas_per_triangle(mesh.n_triangles, width, height, mesh.positions.device, stream)
+    tri = bake_rasterize(uv, width, height, stream)
+    albedo, special, hmap = bake_material(m, mesh, uv, tri, stream)
+    a, s, h = texture_post_images(albedo, special, hmap, 7, stream)
+    return a, s, h, uv
+
+
+def texture_post_images(albedo, special, height, rounds: int = 7, stream=None):
+    """Dilation and U8 conversion of the three images (sources/textures.cpp:73-81)."""
+    ims = [albedo.clone(), special.clone(), height.clone()]
+    for im in ims:
+        image_dilate(im, rounds, stream)
+    return tuple(image_to_u8(im, stream) for im in ims)
+

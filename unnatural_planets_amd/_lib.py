@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .spec import BakeSpec, BatchOpts, Chunk, FieldSpec, Grid, Layer, MeshBuffers, NbStats
+from .spec import BakeSpec, BatchOpts, Chunk, FieldSpec, Grid, Layer, MaterialSpec, MeshBuffers, NbStats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libupf.so")
@@ -79,6 +79,12 @@ SIGNATURES = {
     "upf_image_dilate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_int32, ctypes.c_void_p]),
     "upf_image_to_u8": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "upf_material_tiles": (ctypes.c_int, [ctypes.POINTER(MaterialSpec), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "upf_bake_material": (ctypes.c_int, [ctypes.POINTER(MaterialSpec), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
     "upf_image_height_to_normal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                                   ctypes.c_void_p, ctypes.c_void_p]),
     "upf_image_special_to_gltf_pbr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,

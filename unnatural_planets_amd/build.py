@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libupf.so")
 
-SOURCES = ["upf_common.cpp", "upf_field.hip", "upf_mc.hip", "upf_batch.hip", "upf_bake.hip", "upf_nb.hip", "upf_meshpost.hip"]
+SOURCES = ["upf_common.cpp", "upf_field.hip", "upf_mc.hip", "upf_batch.hip", "upf_bake.hip", "upf_nb.hip", "upf_meshpost.hip", "upf_material.hip"]
 HEADERS = ["upf_device.hpp", "upf_device_grad.hpp", "upf_device_row.hpp", "upf_internal.hpp", "upf_mc.hpp", "upf_mc_device.hpp", "mc_table.inc",
            os.path.join("..", "..", "include", "upf.h")]
 

@@ -322,3 +322,54 @@ def bake_spec(field: FieldSpec, detail: Layer | None = None, bands=(-0.01, 0.005
         b.roughness[i] = roughness[i]
     b.bump[0], b.bump[1] = bump
     return b
+
+
+# ---------------------------------------------------------------------------- material stack
+COLORING = {"default": 0, "barren": 1, "debug": 2}
+PURPOSE = {"land": 1, "water": 2, "navigation": 3}
+BIOMES = ("bare", "tundra", "taiga", "shrubland", "grassland", "temperate_seasonal_forest", "temperate_rain_forest",
+          "desert", "savanna", "tropical_seasonal_forest", "tropical_rain_forest", "water")
+TYPES = ("road", "flat", "rough", "cliffs", "water")
+
+
+class MaterialSpec(ctypes.Structure):
+    _fields_ = [("field", FieldSpec), ("coloring", ctypes.c_int32), ("purpose", ctypes.c_int32),
+                ("poles", ctypes.c_int32), ("flowers", ctypes.c_int32), ("seed", ctypes.c_uint32),
+                ("reserved", ctypes.c_int32 * 3)]
+
+
+class Tile(ctypes.Structure):
+    _fields_ = [("albedo", ctypes.c_float * 3), ("roughness", ctypes.c_float), ("metallic", ctypes.c_float),
+                ("height", ctypes.c_float), ("opacity", ctypes.c_float), ("elevation", ctypes.c_float),
+                ("slope", ctypes.c_float), ("temperature", ctypes.c_float), ("precipitation", ctypes.c_float),
+                ("biome", ctypes.c_int32), ("type", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+
+assert ctypes.sizeof(MaterialSpec) == ctypes.sizeof(FieldSpec) + 32
+assert ctypes.sizeof(Tile) == 64
+TILE_FLOATS = ("r", "g", "b", "roughness", "metallic", "height", "opacity", "elevation", "slope", "temperature",
+               "precipitation")
+
+
+def material_spec(field: FieldSpec, coloring="default", purpose="land", poles=True, flowers=True,
+                  seed: int = 1) -> MaterialSpec:
+    """A material stack over a planet spec (the reference's coloring mode and the
+    poles / flowers configuration switches)."""
+    m = MaterialSpec()
+    m.field = field
+    m.coloring = COLORING[coloring] if isinstance(coloring, str) else int(coloring)
+    m.purpose = PURPOSE[purpose] if isinstance(purpose, str) else int(purpose)
+    m.poles, m.flowers = int(bool(poles)), int(bool(flowers))
+    m.seed = int(seed) & 0xFFFFFFFF
+    return m
+
+
+def tiles_fields(raw):
+    """Named columns of a [n, 16] float32 tile array (numpy): floats per TILE_FLOATS, then
+    biome and type as int32."""
+    import numpy as np
+    raw = np.ascontiguousarray(raw, dtype=np.float32).reshape(-1, 16)
+    out = {k: raw[:, i] for i, k in enumerate(TILE_FLOATS)}
+    out["biome"] = raw[:, 11].view(np.int32)
+    out["type"] = raw[:, 12].view(np.int32)
+    return out
