@@ -13,8 +13,8 @@ marching cubes reads back to size its output.
 Prints one JSON line.  `value` = grid samples per second of the whole step (field +
 meshing) in Gsamples/s.  Also reported: the field kernel alone, marching cubes in
 cells/s, the fused forward+adjoint kernel, the 64-chunk batch, the 1024^2 texel bake,
-rooflines, and the CPU oracle timed on the same grid (the reference binary cannot be built: its cage engine
-is absent).
+the narrow-band planets, the full material stack, rooflines, and the CPU oracle timed
+on the same grid (the reference binary cannot be built: its cage engine is absent).
 """
 from __future__ import annotations
 
@@ -155,6 +155,62 @@ def bench_bake(upf, S, torch, reps: int):
         "gtexels_per_s": round(w * h / (t_atlas + t_raster + t_mat) / 1e9, 3),
         "roofline": {"bound": "hbm", "kernel": "k_bake", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes": mbytes},
+    }
+
+
+def bench_material(upf, S, torch, reps: int):
+    """The full material stack (coloringDefault, poles and flowers on) on a reference
+    planet: sphere r=1100 m + the lakes elevation mode (seed 3) meshed on the 128^3
+    reference box, its per-texel bake into a 1024^2 atlas, and terrainTile at 1M points on
+    the planet's shell.  Both kernels are VALU-bound; rates are reported."""
+    L = upf.lib()
+    f = S.spec_elevation_mode("lakes", 3)
+    mesh = upf.mesh_generate_generic(f, S.grid_reference(128))
+    m = S.material_spec(f, "default", "land", seed=7)
+    w = h = 1024
+    T, V = mesh.n_triangles, mesh.n_vertices
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    uv = upf.atlas_per_triangle(T, w, h)
+    tri = upf.bake_rasterize(uv, w, h)
+    albedo = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
+    special = torch.empty((h, w, 2), dtype=torch.float32, device="cuda")
+    height = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    n = 1 << 20
+    g = torch.Generator(device="cuda").manual_seed(5)
+    d = torch.randn((n, 3), device="cuda", generator=g)
+    d = d / d.norm(dim=1, keepdim=True)
+    r = 1080.0 + 60.0 * torch.rand((n, 1), device="cuda", generator=g)
+    pos = (d * r).contiguous()
+    nrm = d.contiguous()
+    tiles = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+
+    def bake():
+        upf.check(L.upf_bake_material(ctypes.byref(m), vp(mesh.positions), vp(mesh.normals), V, vp(mesh.indices), T,
+                                      vp(uv), vp(tri), w, h, vp(albedo), vp(special), vp(height), sp), "bake_material")
+
+    def points():
+        upf.check(L.upf_material_tiles(ctypes.byref(m), vp(pos), vp(nrm), n, vp(tiles), sp), "material_tiles")
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    t_bake, t_pts = timed(bake), timed(points)
+    covered = int((tri >= 0).sum())
+    return {
+        "workload": "sphere r=1100 m + lakes elevation (seed 3), 128^3 reference box -> marching cubes -> 1024^2 "
+                    "atlas, coloringDefault with poles and flowers (seed 7); terrainTile at 2^20 shell points",
+        "triangles": T, "covered_texels": covered,
+        "bake_ms": round(t_bake * 1e3, 3), "gtexels_per_s": round(w * h / t_bake / 1e9, 3),
+        "tiles_ms": round(t_pts * 1e3, 3), "mtiles_per_s": round(n / t_pts / 1e6, 1),
     }
 
 
@@ -350,6 +406,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-bake", action="store_true")
     ap.add_argument("--no-nb", action="store_true", help="skip the narrow-band 8-planet workload")
+    ap.add_argument("--no-material", action="store_true", help="skip the full material stack")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -523,6 +580,9 @@ def main() -> int:
 
     if not args.no_nb:
         out["narrow_band"] = bench_nb(upf, S, torch, 2)
+
+    if not args.no_material:
+        out["material"] = bench_material(upf, S, torch, max(3, args.steps // 2))
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec, grid, args.cpu_seconds)
