@@ -97,7 +97,29 @@ enum upf_shape_id {
   UPF_SHAPE_WORMHOLE = 13,     /* no params, metre scale  (sdf.cpp:399-404)                */
   UPF_SHAPE_PIPE = 14,         /* no params, metre scale  (sdf.cpp:355-366)                */
   UPF_SHAPE_HEMISPHERES = 15,  /* no params, metre scale  (sdf.cpp:383-397)                */
-  UPF_SHAPE_COUNT = 16
+  /* Further shapes of sdf.cpp (metre scale).  cage's atan2(a, b) is read as C's atan2(a, b),
+   * quaternion rolls as rotations about z; textbook primitives where cage supplies one. */
+  UPF_SHAPE_H3O = 16,          /* no params (sdf.cpp:151-163)                               */
+  UPF_SHAPE_H4O = 17,          /* no params (sdf.cpp:165-178)                               */
+  UPF_SHAPE_GEAR = 18,         /* no params (sdf.cpp:180-204)                               */
+  UPF_SHAPE_MANDELBULB = 19,   /* no params, power 3, 20 iterations (sdf.cpp:206-232)       */
+  UPF_SHAPE_MOBIUS = 20,       /* no params (sdf.cpp:94-120); on the axis, where the
+                                  reference normalizes a zero vector, the projection is 0   */
+  UPF_SHAPE_FIBERS = 21,       /* params: gyroid offset ox, oy, oz (the reference draws it
+                                  from randomRange3(-100, 100), sdf.cpp:122-139)            */
+  UPF_SHAPE_ASTEROID = 22,     /* params: frequency (the reference draws 0.65 .. 0.75), seed
+                                  (an integer < 2^24); simplex fBm 2 octaves, gain 0.3, of
+                                  the unit direction (sdf.cpp:333-353)                     */
+  UPF_SHAPE_TWISTED_PLANE = 23,/* no params (sdf.cpp:368-373)                               */
+  UPF_SHAPE_BELT = 24,         /* no params (sdf.cpp:375-381)                               */
+  UPF_SHAPE_TRIANGULAR_PRISM = 25, /* no params; exact point-triangle distance (Ericson)
+                                      to the reference's triangle (sdf.cpp:62-74)           */
+  UPF_SHAPE_HEXAGONAL_PRISM = 26,  /* no params; along y, apothem 400, length 300
+                                      (Quilez's hexagonal prism; sdf.cpp:76-79)             */
+  UPF_SHAPE_TWISTED_HEX_PRISM = 27,/* no params; apothem 650, length 900 (sdf.cpp:234-242) */
+  UPF_SHAPE_TETRAHEDRON = 28,  /* no params; regular, circumradius 600, exact distance,
+                                  minus 100 (sdf.cpp:50-53)                                 */
+  UPF_SHAPE_COUNT = 29
 };
 
 enum upf_noise_id {

@@ -103,6 +103,48 @@ template <int K> Dual<K> pow_(const Dual<K>& a, double e) {
   return r;
 }
 inline CountedFloat pow_(CountedFloat a, double e) { CountedFloat::count()++; return std::pow(a.v, (float)e); }
+// sin, cos, atan2, acos, log (the further shapes): C library in float / double, chain
+// rule for the duals, one operation each for the counter
+inline float sin_(float a) { return std::sin(a); }
+inline double sin_(double a) { return std::sin(a); }
+inline float cos_(float a) { return std::cos(a); }
+inline double cos_(double a) { return std::cos(a); }
+inline float atan2_(float y, float x) { return std::atan2(y, x); }
+inline double atan2_(double y, double x) { return std::atan2(y, x); }
+inline float acos_(float a) { return std::acos(a); }
+inline double acos_(double a) { return std::acos(a); }
+inline float log_(float a) { return std::log(a); }
+inline double log_(double a) { return std::log(a); }
+template <int K> Dual<K> sin_(const Dual<K>& a) {
+  Dual<K> r(std::sin(a.v)); const double c = std::cos(a.v);
+  for (int i = 0; i < K; i++) r.d[i] = a.d[i] * c;
+  return r;
+}
+template <int K> Dual<K> cos_(const Dual<K>& a) {
+  Dual<K> r(std::cos(a.v)); const double s = -std::sin(a.v);
+  for (int i = 0; i < K; i++) r.d[i] = a.d[i] * s;
+  return r;
+}
+template <int K> Dual<K> atan2_(const Dual<K>& y, const Dual<K>& x) {
+  Dual<K> r(std::atan2(y.v, x.v)); const double q = x.v * x.v + y.v * y.v;
+  for (int i = 0; i < K; i++) r.d[i] = q > 0 ? (x.v * y.d[i] - y.v * x.d[i]) / q : 0.0;
+  return r;
+}
+template <int K> Dual<K> acos_(const Dual<K>& a) {
+  Dual<K> r(std::acos(a.v)); const double t = 1 - a.v * a.v;
+  for (int i = 0; i < K; i++) r.d[i] = t > 0 ? -a.d[i] / std::sqrt(t) : 0.0;
+  return r;
+}
+template <int K> Dual<K> log_(const Dual<K>& a) {
+  Dual<K> r(std::log(a.v));
+  for (int i = 0; i < K; i++) r.d[i] = a.d[i] / a.v;
+  return r;
+}
+inline CountedFloat sin_(CountedFloat a) { CountedFloat::count()++; return std::sin(a.v); }
+inline CountedFloat cos_(CountedFloat a) { CountedFloat::count()++; return std::cos(a.v); }
+inline CountedFloat atan2_(CountedFloat y, CountedFloat x) { CountedFloat::count()++; return std::atan2(y.v, x.v); }
+inline CountedFloat acos_(CountedFloat a) { CountedFloat::count()++; return std::acos(a.v); }
+inline CountedFloat log_(CountedFloat a) { CountedFloat::count()++; return std::log(a.v); }
 
 template <class T> T max_(const T& a, const T& b) { return val(a) < val(b) ? b : a; }
 template <class T> T min_(const T& a, const T& b) { return val(b) < val(a) ? b : a; }
@@ -333,6 +375,163 @@ template <class T> T smin(const T& a, const T& b, const T& k) {
   return b + (a - b) * h - k * h * (T(1.0f) - h);
 }
 
+// ---- the further shapes of sources/sdf.cpp (cited per case).  cage's atan2(a, b) is
+// taken as C's atan2(a, b); its primitives are the textbook ones named per case.
+// Closest point on a triangle (Ericson, Real-Time Collision Detection, 5.1.5) -> distance.
+template <class T> T tri_dist(const T p[3], const float A[3], const float B[3], const float C[3]) {
+  T ab[3], ac[3], ap[3], bp[3], cp[3];
+  for (int i = 0; i < 3; i++) {
+    ab[i] = T(B[i]) - T(A[i]); ac[i] = T(C[i]) - T(A[i]);
+    ap[i] = p[i] - T(A[i]); bp[i] = p[i] - T(B[i]); cp[i] = p[i] - T(C[i]);
+  }
+  auto dot = [](const T* u, const T* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+  const T d1 = dot(ap, ab), d2 = dot(ap, ac), d3 = dot(bp, ab), d4 = dot(bp, ac), d5 = dot(cp, ab), d6 = dot(cp, ac);
+  const T va = d3 * d6 - d5 * d4, vb = d5 * d2 - d1 * d6, vc = d1 * d4 - d3 * d2;
+  T q[3];
+  if (val(d1) <= 0 && val(d2) <= 0) for (int i = 0; i < 3; i++) q[i] = T(A[i]);
+  else if (val(d3) >= 0 && val(d4) <= val(d3)) for (int i = 0; i < 3; i++) q[i] = T(B[i]);
+  else if (val(vc) <= 0 && val(d1) >= 0 && val(d3) <= 0) {
+    const T v = d1 / (d1 - d3);
+    for (int i = 0; i < 3; i++) q[i] = T(A[i]) + v * ab[i];
+  } else if (val(d6) >= 0 && val(d5) <= val(d6)) for (int i = 0; i < 3; i++) q[i] = T(C[i]);
+  else if (val(vb) <= 0 && val(d2) >= 0 && val(d6) <= 0) {
+    const T w = d2 / (d2 - d6);
+    for (int i = 0; i < 3; i++) q[i] = T(A[i]) + w * ac[i];
+  } else if (val(va) <= 0 && val(d4 - d3) >= 0 && val(d5 - d6) >= 0) {
+    const T w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    for (int i = 0; i < 3; i++) q[i] = T(B[i]) + w * (T(C[i]) - T(B[i]));
+  } else {
+    const T den = va + vb + vc;
+    const T v = vb / den, w = vc / den;
+    for (int i = 0; i < 3; i++) q[i] = T(A[i]) + ab[i] * v + ac[i] * w;
+  }
+  return hyp3(p[0] - q[0], p[1] - q[1], p[2] - q[2]);
+}
+// hexagonal prism along z, apothem hx, half length hz (Quilez)
+template <class T> T hex_prism(const T& x, const T& y, const T& z, float hx, float hz) {
+  const T kx(-0.8660254f), ky(0.5f), kz(0.57735027f), zero(0.0f);
+  T px = abs_(x), py = abs_(y);
+  const T pz = abs_(z);
+  const T dk = min_(px * kx + py * ky, zero) * T(2.0f);
+  px = px - dk * kx;
+  py = py - dk * ky;
+  const T cx = min_(max_(px, -kz * T(hx)), kz * T(hx));
+  const T ex = px - cx, ey = py - T(hx);
+  const T dx = val(ey) < 0 ? -hyp2(ex, ey) : hyp2(ex, ey);
+  const T dy = pz - T(hz);
+  return min_(max_(dx, dy), zero) + hyp2(max_(dx, zero), max_(dy, zero));
+}
+template <class T> T noise_simplex(uint32_t seed, const T q[3]);
+template <class T> T shape_further(int shape, const T* a, const T p[3]) {
+  const T x = p[0], y = p[1], z = p[2], zero(0.0f);
+  switch (shape) {
+    case UPF_SHAPE_H3O: case UPF_SHAPE_H4O: {                          // sdf.cpp:151-178
+      const T s[3] = {x * T(0.75f), y * T(0.75f), z * T(0.75f)};
+      const float c3[3][2] = {{680.0f, 0.0f}, {-340.0f, 588.89727f}, {-340.0f, -588.89727f}};
+      const float c4[4][3] = {{-550, 400, 0}, {550, 400, 0}, {0, -400, -550}, {0, -400, 550}};
+      T h(1e30f);
+      if (shape == UPF_SHAPE_H3O)
+        for (auto& c : c3) h = min_(h, hyp3(s[0] - T(c[0]), s[1] - T(c[1]), s[2]) - T(450.0f));
+      else
+        for (auto& c : c4) h = min_(h, hyp3(s[0] - T(c[0]), s[1] - T(c[1]), s[2] - T(c[2])) - T(450.0f));
+      return smin(hyp3(s[0], s[1], s[2]) - T(650.0f), h, T(150.0f)) / T(0.75f);
+    }
+    case UPF_SHAPE_GEAR: {                                             // sdf.cpp:180-204
+      const T gx = x * T(0.5f), gy = y * T(0.5f), gz = z * T(0.5f);  // pos = (x, z, y) / 2
+      const float ang = 6.2831853f / 9.0f;
+      const unsigned sector = (unsigned)(std::atan2((float)val(gx), (float)val(gy)) / ang + 9.0f + 0.5f) % 9u;
+      const float r = (float)sector * -ang;
+      const T sn(std::sin(r)), cs(std::cos(r));
+      const T qx = cs * gx - sn * gy, qy = sn * gx + cs * gy;
+      const T b = hyp2(max_(abs_(qx - T(700.0f)) - T(150.0f), zero), max_(abs_(qy) - T(70.0f), zero));
+      const T c = abs_(hyp2(gx, gy) - T(450.0f)) - T(150.0f);
+      const T h = abs_(gz) - T(30.0f);
+      return (-smin(-min_(b, c), -h, T(50.0f)) - T(50.0f)) * T(2.0f);  // smoothMax, sources/math.cpp:35-38
+    }
+    case UPF_SHAPE_MANDELBULB: {                                       // sdf.cpp:206-232
+      if ((float)(val(x) * val(x) + val(y) * val(y) + val(z) * val(z)) < 1e-3f) return zero;
+      const T q[3] = {x * T(0.0004f), y * T(0.0004f), z * T(0.0004f)};
+      T w[3] = {q[0], q[1], q[2]}, dr(1.0f), r(0.0f);
+      for (int i = 0; i < 20; i++) {
+        r = hyp3(w[0], w[1], w[2]);
+        if (val(r) > 4.0) break;
+        T theta = acos_(w[2] / r), phi = atan2_(w[1], w[0]);
+        dr = pow_(r, 2.0) * T(3.0f) * dr + T(1.0f);
+        const T zr = pow_(r, 3.0);
+        theta = theta * T(3.0f);
+        phi = phi * T(3.0f);
+        const T st = sin_(theta);
+        w[0] = zr * (st * cos_(phi)) + q[0];
+        w[1] = zr * (sin_(phi) * st) + q[1];
+        w[2] = zr * cos_(theta) + q[2];
+      }
+      const T value = T(0.5f) * log_(r) * r / dr;
+      return (value + T(0.1f)) * T(150.0f);
+    }
+    case UPF_SHAPE_MOBIUS: {                                           // sdf.cpp:94-120
+      const T l = hyp2(x, y);
+      const T proj = val(l) > 0 ? x * (x / l) + y * (y / l) : zero;
+      const T rot = atan2_(x, y) * T(0.5f);
+      const T u = proj - T(1000.0f), v = z;
+      const T ang = atan2_(u, v) + rot, lp = hyp2(u, v);
+      const T qx = cos_(ang) * lp, qy = sin_(ang) * lp;
+      const T dx = abs_(qx) - T(400.0f), dy = abs_(qy) - T(40.0f);
+      return hyp2(max_(dx, zero), max_(dy, zero)) + min_(max_(dx, dy), zero) - T(200.0f);
+    }
+    case UPF_SHAPE_FIBERS: {                                           // sdf.cpp:122-139
+      const T sc(0.0007f), g(3.23f);
+      const T px = (x * sc + a[0]) * g, py = (y * sc + a[1]) * g, pz = (z * sc + a[2]) * g;
+      const T d = sin_(px) * cos_(pz) + sin_(py) * cos_(px) + sin_(pz) * cos_(py);
+      const T gy = abs_(d + T(1.6f)) / g - T(0.2f);
+      return gy * T(0.7f) / sc + max_(hyp3(x, y, z) - T(1500.0f), zero) * T(0.5f);
+    }
+    case UPF_SHAPE_ASTEROID: {                                         // sdf.cpp:333-353
+      const T l = hyp3(x, y, z);
+      if (val(l) < 1.0) return T(-100.0f);
+      const T d[3] = {x / l, y / l, z / l};
+      const uint32_t seed = (uint32_t)val(a[1]);
+      const T f0 = a[0], f1 = a[0] * T(2.0f);
+      const T q0[3] = {d[0] * f0, d[1] * f0, d[2] * f0}, q1[3] = {d[0] * f1, d[1] * f1, d[2] * f1};
+      const T b(1.0f / 1.3f);  // fBm amplitudes bounded by 1 / (1 + gain 0.3)
+      const T n = noise_simplex<T>(seed, q0) * b + noise_simplex<T>(seed + 1u, q1) * (b * T(0.3f));
+      return l + n * T(600.0f) - T(1300.0f);
+    }
+    case UPF_SHAPE_TWISTED_PLANE: {                                    // sdf.cpp:368-373
+      const T m = min_(abs_(z) / T(1000.0f), T(1.0f));
+      const T rot = T(1.0f) - (T(1.0f) - m) * (T(1.0f) - m);
+      const float sg = val(z) > 0 ? 1.0f : (val(z) < 0 ? -1.0f : 0.0f);
+      const T ang = rot * T(sg * 35.0f * 0.017453292f);
+      return y * cos_(ang) + x * sin_(ang);
+    }
+    case UPF_SHAPE_BELT:                                               // sdf.cpp:375-381
+      if (val(x) > 0) return -cylinder_z(x, y, z, T(10000.0f), T(1800.0f));
+      return T(1800.0f) - abs_(z);
+    case UPF_SHAPE_TRIANGULAR_PRISM: {                                 // sdf.cpp:62-74
+      const float A[3] = {0.0f, 900.0f, 0.0f}, B[3] = {779.42286f, -450.0f, 0.0f}, C[3] = {-779.42286f, -450.0f, 0.0f};
+      const T q[3] = {x, z, max_(abs_(y) - T(150.0f), zero)};
+      return tri_dist(q, A, B, C) - T(100.0f);
+    }
+    case UPF_SHAPE_HEXAGONAL_PRISM:                                    // sdf.cpp:76-79
+      return hex_prism(x, z, y, 400.0f, 150.0f) - T(100.0f);
+    case UPF_SHAPE_TWISTED_HEX_PRISM: {                                // sdf.cpp:234-242
+      const T ang = y * T(0.001f), c = cos_(ang), s = sin_(ang);
+      return hex_prism(c * x - s * z, s * x + c * z, y, 650.0f, 450.0f) - T(200.0f);
+    }
+    case UPF_SHAPE_TETRAHEDRON: {                                      // sdf.cpp:50-53
+      const float k = 346.41016f;
+      const float v[4][3] = {{k, k, k}, {k, -k, -k}, {-k, k, -k}, {-k, -k, k}};
+      const int f[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+      T inside(-1e30f), dmin(1e30f);
+      for (int i = 0; i < 4; i++) {
+        inside = max_(inside, (x * T(v[i][0]) + y * T(v[i][1]) + z * T(v[i][2])) * T(-1.0f / 600.0f) - T(200.0f));
+        dmin = min_(dmin, tri_dist(p, v[f[i][0]], v[f[i][1]], v[f[i][2]]));
+      }
+      return (val(inside) > 0 ? dmin : inside) - T(100.0f);
+    }
+    default: return T(NAN);
+  }
+}
+
 template <class T> T shape_extended(int shape, const T* a, const T p[3]) {
   const T x = p[0], y = p[1], z = p[2];
   switch (shape) {
@@ -372,7 +571,7 @@ template <class T> T shape_extended(int shape, const T* a, const T p[3]) {
       const T s1 = hyp3(q0, q1, z) - T(1000.0f), s2 = hyp3(r0, r1, z) - T(1000.0f);
       return smin(tube, min_(s1, s2), T(100.0f));
     }
-    default: return T(NAN);
+    default: return shape_further(shape, a, p);
   }
 }
 

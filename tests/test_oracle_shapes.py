@@ -82,5 +82,102 @@ def test_flop_table_counts_extended_shapes(oracle, upf, name):
 
 def test_shape_ids_validated(upf):
     s = S.field_spec("sphere", (1.0,), [])
-    s.shape = 16
+    s.shape = 29
     assert upf.lib().upf_spec_validate(ctypes.byref(s)) == S.ERR_BAD_SPEC
+    for name in FURTHER:
+        assert upf.lib().upf_spec_validate(ctypes.byref(shape_spec(name))) == S.OK
+
+
+# ---- the further shapes (ids 16 .. 28)
+FURTHER = ["h3o", "h4o", "gear", "mandelbulb", "mobius", "fibers", "asteroid", "twisted_plane", "belt",
+           "triangular_prism", "hexagonal_prism", "twisted_hex_prism", "tetrahedron"]
+
+
+def test_further_closed_forms(oracle):
+    # H3O / H4O at an H sphere's centre: h = -450, o = |c| - 650 = 30 (|c| = 680), so the
+    # smooth min is h and f = -450 / 0.75
+    c3 = np.array([[680, 0, 0], [-340, 588.89727, 0]]) / 0.75
+    np.testing.assert_allclose(f64(oracle, "h3o", c3), -600, atol=2e-3)
+    c4 = np.array([[-550, 400, 0], [0, -400, 550]]) / 0.75
+    np.testing.assert_allclose(f64(oracle, "h4o", c4), -600, atol=2e-3)
+    # H3O along z: the O sphere wins by more than the blend width, f = |z| - 650 / 0.75
+    z = np.array([900.0, 1500.0, 2500.0])
+    np.testing.assert_allclose(f64(oracle, "h3o", np.stack([0 * z, 0 * z, z], 1)), z - 650 / 0.75, atol=2e-3)
+    # gear on its axis: the ring term 300 wins while |z| / 2 - 30 is far below; far above,
+    # f = |z| - 160
+    np.testing.assert_allclose(f64(oracle, "gear", [[0, 0, 0], [0, 0, 200]]), 500, atol=1e-9)
+    np.testing.assert_allclose(f64(oracle, "gear", [[0, 0, 1200], [0, 0, -3000]]), [1040, 2840], atol=1e-9)
+    # Mandelbulb: 0 near the origin; beyond |0.0004 p| = 4 the loop stops at once
+    p = np.array([[12000, 3000, -4000]], dtype=np.float32)
+    r = 0.0004 * np.linalg.norm(p.astype(np.float64))
+    np.testing.assert_allclose(f64(oracle, "mandelbulb", [[0.01, 0.01, 0.01]]), 0, atol=0)
+    np.testing.assert_allclose(f64(oracle, "mandelbulb", p), (0.5 * np.log(r) * r + 0.1) * 150, rtol=1e-6)
+    # Moebius strip: on its centre circle (radius 1000 about z) the rectangle test gives
+    # -minor axis: f = -40 - 200 at every angle
+    a = np.linspace(0, 2 * np.pi, 13)[:-1]
+    np.testing.assert_allclose(f64(oracle, "mobius", np.stack([1000 * np.cos(a), 1000 * np.sin(a), 0 * a], 1)),
+                               -240, atol=2e-3)
+    # asteroid: -100 inside |p| < 1; radially f(2 p) - f(p) = |p|
+    np.testing.assert_allclose(f64(oracle, "asteroid", [[0.1, 0.2, 0.3]]), -100, atol=0)
+    d = np.random.default_rng(5).standard_normal((50, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    p1 = (d * 700).astype(np.float32)
+    p2 = (2 * p1.astype(np.float64)).astype(np.float32)
+    np.testing.assert_allclose(f64(oracle, "asteroid", p2) - f64(oracle, "asteroid", p1),
+                               np.linalg.norm(p1.astype(np.float64), axis=1), atol=2e-3)
+    # twisted plane: untwisted at z = 0, 35 degrees beyond |z| = 1000
+    t = np.radians(35.0)
+    np.testing.assert_allclose(f64(oracle, "twisted_plane", [[10, 20, 0], [10, 20, 1500], [10, 20, -2000]]),
+                               [20, 20 * np.cos(t) + 10 * np.sin(t), 20 * np.cos(t) - 10 * np.sin(t)], atol=2e-4)
+    # belt: 1800 - |z| for x <= 0; inside the cylinder's length for x > 0, 1800 - |(x, y)|
+    np.testing.assert_allclose(f64(oracle, "belt", [[-5, 0, 300], [1000, 1000, 200], [3000, 0, -10]]),
+                               [1500, 1800 - np.hypot(1000, 1000), -1200], atol=2e-4)
+    # prisms: above the triangle's / hexagon's interior f = |y| - 150 - 100; the hexagon's
+    # apothem direction z at mid length f = z - 400 - 100
+    np.testing.assert_allclose(f64(oracle, "triangular_prism", [[0, 400, 0], [100, -1000, -100]]), [150, 750],
+                               atol=2e-4)
+    np.testing.assert_allclose(f64(oracle, "hexagonal_prism", [[0, 400, 0], [0, 0, 900]]), [150, 400], atol=2e-4)
+    np.testing.assert_allclose(f64(oracle, "twisted_hex_prism", [[0, 1000, 0], [0, 0, 1000]]), [350, 150],
+                               atol=2e-4)
+    # tetrahedron (circumradius 600): centre -200 - 100, vertex -100, beyond a vertex and
+    # beyond a face centre the distance grows one to one
+    k = 600 / np.sqrt(3)
+    np.testing.assert_allclose(f64(oracle, "tetrahedron", [[0, 0, 0], [k, k, k], [1.5 * k, 1.5 * k, 1.5 * k],
+                                                           [-250 / np.sqrt(3)] * 3]),
+                               [-300, -100, 300 - 100, 50 - 100], atol=2e-3)
+
+
+def test_fibers_offset_shift_identity(oracle):
+    # inside |p| < 1500 the falloff term is 0: f(p + D, off) = f(p, off + 0.0007 D)
+    rng = np.random.default_rng(6)
+    p = rng.uniform(-700, 700, size=(200, 3)).astype(np.float32)
+    D = np.float32([90.0, -60.0, 30.0])
+    off = np.float32([37.0, -12.5, 71.0])
+    a = f64(oracle, "fibers", p + D, tuple(off))
+    b = f64(oracle, "fibers", p, tuple(off + np.float32(0.0007) * D))
+    np.testing.assert_allclose(a, b, atol=2e-2)
+
+
+@pytest.mark.parametrize("name,lo,hi", [("tetrahedron", -1500, 1500), ("triangular_prism", -2000, 2000),
+                                        ("hexagonal_prism", -1500, 1500), ("belt", -4000, 4000)])
+def test_further_exact_distances_are_eikonal(oracle, name, lo, hi):
+    pts = np.random.default_rng(7).uniform(lo, hi, size=(4000, 3)).astype(np.float32)
+    _, g = oracle.grad_points(shape_spec(name), pts)
+    n = np.linalg.norm(g, axis=1)
+    assert np.mean(np.abs(n - 1) < 1e-6) > 0.9, np.quantile(np.abs(n - 1), [0.5, 0.9])
+
+
+@pytest.mark.parametrize("name", [n for n in FURTHER])
+def test_further_gradients_match_central_differences(oracle, name):
+    rng = np.random.default_rng(8)
+    pts = rng.uniform(-2200, 2200, size=(400, 3)).astype(np.float64)
+    s = shape_spec(name)
+    _, g = oracle.grad_points(s, pts.astype(np.float32))
+    h = 0.05
+    fd = np.zeros_like(g)
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = h
+        fd[:, a] = (f64(oracle, name, pts + e) - f64(oracle, name, pts - e)) / (2 * h)
+    err = np.linalg.norm(g - fd, axis=1) / np.maximum(np.linalg.norm(g, axis=1), 1e-3)
+    assert np.median(err) < 1e-3 and np.mean(err < 1e-2) > 0.9, (name, np.quantile(err, [0.5, 0.9]))

@@ -94,6 +94,7 @@ __device__ __forceinline__ float smoothstep_(float e0, float e1, float x) {
 }
 __device__ __forceinline__ float mix_(float a, float b, float t) { return a + t * (b - a); }
 
+template <bool EXT>
 __global__ __launch_bounds__(256) void k_bake(const DBake b, const float* __restrict__ P, const float* __restrict__ N,
                                               int64_t nv, const uint32_t* __restrict__ I, int64_t nt,
                                               const float* __restrict__ uv, const int32_t* __restrict__ tri, int w,
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(256) void k_bake(const DBake b, const float* __rest
   float cosang = 1.0f;
   if (nl > 0.0f && pl > 0.0f) cosang = (n[0] * p[0] + n[1] * p[1] + n[2] * p[2]) / (nl * pl);
   const float slope = 1.0f - cosang;
-  const float elev = shape_eval(b.field, p[0], p[1], p[2]) / b.field.escale;
+  const float elev = shape_eval<EXT>(b.field, p[0], p[1], p[2]) / b.field.escale;
   const float d = b.detail.weight * layer_value(b.detail, p[0], p[1], p[2]);
   const float s1 = smoothstep_(b.bands[0] - b.band_width, b.bands[0] + b.band_width, elev);
   const float s2 = smoothstep_(b.bands[1] - b.band_width, b.bands[1] + b.band_width, elev);
@@ -311,8 +312,13 @@ int upf_bake(const upf_bake_spec* spec, const float* d_P, const float* d_N, int6
   for (int i = 0; i < 4; i++) b.roughness[i] = spec->roughness[i];
   b.bump[0] = spec->bump[0];
   b.bump[1] = spec->bump[1];
-  hipLaunchKernelGGL(k_bake, dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, as_stream(stream), b, d_P, d_N, nv, d_I, n,
-                     d_uv, d_tri, w, h, d_albedo, d_bump, d_rough);
+  const dim3 grid((w + 63) / 64, (h + 3) / 4);
+  if (spec_ext(b.field))
+    hipLaunchKernelGGL(k_bake<true>, grid, dim3(256), 0, as_stream(stream), b, d_P, d_N, nv, d_I, n, d_uv, d_tri, w, h,
+                       d_albedo, d_bump, d_rough);
+  else
+    hipLaunchKernelGGL(k_bake<false>, grid, dim3(256), 0, as_stream(stream), b, d_P, d_N, nv, d_I, n, d_uv, d_tri, w,
+                       h, d_albedo, d_bump, d_rough);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }

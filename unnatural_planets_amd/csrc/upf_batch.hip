@@ -50,7 +50,7 @@ __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ ch, int n, 
   return lo;
 }
 
-template <int VARIANT, bool MODES>
+template <int VARIANT, bool EXT>
 __global__ __launch_bounds__(256) void k_batch_field(const BChunk* __restrict__ ch, int n_chunks, int64_t n_items,
                                                      float* __restrict__ field) {
   __shared__ float4 s_slice[4][kRowSlice];
@@ -80,8 +80,8 @@ __global__ __launch_bounds__(256) void k_batch_field(const BChunk* __restrict__ 
     const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
     float v[2];
     if (VARIANT == UPF_EVAL_PER_LANE) {
-      v[0] = field_eval<MODES>(C.spec, x, y[0], z);
-      v[1] = two ? field_eval<MODES>(C.spec, x, y[1], z) : v[0];
+      v[0] = field_eval<EXT>(C.spec, x, y[0], z);
+      v[1] = two ? field_eval<EXT>(C.spec, x, y[1], z) : v[0];
     } else {
       row_field<2>(ctx, C.spec, x, y, z, v);
     }
@@ -169,7 +169,7 @@ struct upf_batch {
   int n = 0;
   int cus = 256;
   int64_t nodes = 0, groups = 0, rows = 0;
-  bool any_mode = false;  // a chunk with an elevation mode: the field runs per lane
+  bool any_ext = false;  // a chunk with an elevation mode or a further shape: the field runs per lane
   upf_batch_opts opts{};
   size_t off_S = 0, off_counts = 0, off_voff = 0, off_toff = 0, off_act = 0, off_tiles = 0, off_total = 0,
          ws_bytes = 0;
@@ -226,7 +226,7 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
   uint4* tiles = reinterpret_cast<uint4*>(w + b->off_tiles);
   uint4* total = reinterpret_cast<uint4*>(w + b->off_total);
   const unsigned grid = (unsigned)(b->cus * b->opts.blocks_per_cu);
-  if (b->any_mode)
+  if (b->any_ext)
     hipLaunchKernelGGL((k_batch_field<UPF_EVAL_PER_LANE, true>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
                        field);
   else if (b->opts.field_variant == UPF_EVAL_PER_LANE)
@@ -275,7 +275,7 @@ int upf_batch_create(const upf_chunk* chunks, int32_t n_chunks, const upf_batch_
     if (!std::isfinite(chunks[c].iso)) { delete b; return UPF_ERR_INVALID_VALUE; }
     C.md = mc_layout(C.grid).d;
     C.iso = chunks[c].iso;
-    if (C.spec.mode != UPF_ELEV_LAYERS) b->any_mode = true;
+    if (spec_ext(C.spec)) b->any_ext = true;
     C.gx = (C.grid.n[0] + 63) / 64;
     C.node_off = b->nodes;
     C.group_off = b->groups;

@@ -25,6 +25,11 @@ int shape_params_used(int shape) {
     case UPF_SHAPE_TORUS_CROSS: return 3;
     case UPF_SHAPE_H2O: case UPF_SHAPE_BOWL: case UPF_SHAPE_INSIDE_CUBE: case UPF_SHAPE_WORMHOLE:
     case UPF_SHAPE_PIPE: case UPF_SHAPE_HEMISPHERES: return 0;
+    case UPF_SHAPE_H3O: case UPF_SHAPE_H4O: case UPF_SHAPE_GEAR: case UPF_SHAPE_MANDELBULB: case UPF_SHAPE_MOBIUS:
+    case UPF_SHAPE_TWISTED_PLANE: case UPF_SHAPE_BELT: case UPF_SHAPE_TRIANGULAR_PRISM:
+    case UPF_SHAPE_HEXAGONAL_PRISM: case UPF_SHAPE_TWISTED_HEX_PRISM: case UPF_SHAPE_TETRAHEDRON: return 0;
+    case UPF_SHAPE_FIBERS: return 3;
+    case UPF_SHAPE_ASTEROID: return 2;
     default: return -1;
   }
 }
@@ -171,15 +176,37 @@ OpCount operator+(float a, OpCount b) { return op(a + b.v, 1, 1); }
 OpCount operator-(float a, OpCount b) { return op(a - b.v, 1, 1); }
 OpCount operator*(float a, OpCount b) { return op(a * b.v, 1, 4); }
 OpCount sq_root(OpCount a) { return op(std::sqrt(a.v), 1, 5); }
-OpCount pow_v(OpCount a, float e) { return op(std::pow(a.v, e), 1, 5); }  // pow counted as 1
+// transcendental functions counted as 1 (and 5 for value and gradient)
+OpCount pow_v(OpCount a, float e) { return op(std::pow(a.v, e), 1, 5); }
 OpCount floor_v(OpCount a) { return {std::floor(a.v)}; }
+OpCount sin_v(OpCount a) { return op(std::sin(a.v), 1, 5); }
+OpCount cos_v(OpCount a) { return op(std::cos(a.v), 1, 5); }
+OpCount atan2_v(OpCount y, OpCount x) { return op(std::atan2(y.v, x.v), 1, 12); }
+OpCount acos_v(OpCount a) { return op(std::acos(a.v), 1, 7); }
+OpCount log_v(OpCount a) { return op(std::log(a.v), 1, 5); }
 
 // the point the flop count of a data-dependent shape is taken at (the oracle's count
 // uses the same point)
 constexpr float kCountPoint[3] = {0.3f, 0.2f, -0.4f};
 
+// metre-scale count point for the further shapes (inside the Mandelbulb's escape radius,
+// off every axis)
+constexpr float kCountPointM[3] = {300.0f, 200.0f, -400.0f};
+
 OpTally shape_ext_tally(int shape, const float* sp) {
   g_tally = OpTally{};
+  if (shape == UPF_SHAPE_ASTEROID) {
+    // |p| 5, direction 6, two simplex octaves (value: 75 each, gradient: 138 each), the
+    // octave sum 3, the radial composition 3
+    g_tally.fwd = 5 + 6 + 2 * (3 + 75) + 3 + 3;
+    g_tally.grad = 5 * 4 + 6 * 10 + 2 * (3 + 138) + 3 + 3 * 4 + 18;
+    return g_tally;
+  }
+  if (shape >= UPF_SHAPE_H3O) {
+    (void)upf::shape_more_t<OpCount>(shape, sp, OpCount{kCountPointM[0]}, OpCount{kCountPointM[1]},
+                                     OpCount{kCountPointM[2]}, OpCount{0.0f});
+    return g_tally;
+  }
   (void)upf::shape_ext<OpCount>(shape, sp, OpCount{kCountPoint[0]}, OpCount{kCountPoint[1]}, OpCount{kCountPoint[2]},
                            OpCount{0.0f});
   return g_tally;

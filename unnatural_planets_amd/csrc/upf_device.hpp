@@ -44,6 +44,11 @@ struct DGrid {
   int32_t pad;
 };
 
+// Whether a spec needs the EXT instantiations (an elevation mode or a further shape).
+__host__ __device__ inline bool spec_ext(const DSpec& s) {
+  return s.mode != UPF_ELEV_LAYERS || s.shape >= UPF_SHAPE_H3O;
+}
+
 // ------------------------------------------------------------------ hash and lattice
 constexpr uint32_t kPX = 501125321u;
 constexpr uint32_t kPY = 1136930381u;
@@ -233,6 +238,27 @@ __device__ __forceinline__ float noise_simplex(uint32_t seed, float x, float y, 
 }
 
 // ------------------------------------------------------------------ shapes
+// The further shapes (ids UPF_SHAPE_H3O and above): out of line, so that the field
+// kernels carry one call instead of their code; parameters pass by value.
+__device__ __noinline__ float shape_more(int shape, float p0, float p1, float p2, float p3, float x, float y,
+                                         float z) {
+  const float sp[4] = {p0, p1, p2, p3};
+  if (shape == UPF_SHAPE_ASTEROID) {  // sources/sdf.cpp:333-353
+    const float l = __builtin_sqrtf(x * x + y * y + z * z);
+    if (l < 1.0f) return -100.0f;
+    const float dx = x / l, dy = y / l, dz = z / l;
+    const uint32_t seed = (uint32_t)p1;
+    const float b = 1.0f / 1.3f;  // amplitudes bounded by 1 / (1 + 0.3)
+    const float n0 = noise_simplex(seed, dx * p0, dy * p0, dz * p0);
+    const float f1 = p0 * 2.0f;
+    const float n1 = noise_simplex(seed + 1u, dx * f1, dy * f1, dz * f1);
+    const float n = n0 * b + n1 * (b * 0.3f);
+    return l + n * 600.0f - 1300.0f;
+  }
+  return shape_more_t<float>(shape, sp, x, y, z, 0.0f);
+}
+
+template <bool EXT = true>
 __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, float z) {
   switch (s.shape) {
     case UPF_SHAPE_SPHERE:
@@ -256,6 +282,8 @@ __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, fl
     case UPF_SHAPE_NONE:
       return 0.0f;
     default:
+      if (EXT && s.shape >= UPF_SHAPE_H3O)
+        return shape_more(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], x, y, z);
       return shape_ext<float>(s.shape, s.sp, x, y, z, 0.0f);
   }
 }
@@ -288,10 +316,11 @@ __device__ __forceinline__ float layer_value(const DLayer& L, float x, float y, 
 // Elevation: the generic layer stack, or a fixed mode (upf_modes.hpp) whose layers go
 // through the same single layer_value call.  Layer 0 of the craters mode carries the
 // Voronoi parameters (seed, points per cell in octaves, cell size in freq[0]).
-// MODES = false compiles the modes out (kernels instantiated for the layer stack only).
-template <bool MODES = true>
+// EXT = false compiles the modes and the further shapes out (the kernels' common
+// instantiation); specs with either run the EXT = true instantiations.
+template <bool EXT = true>
 __device__ __forceinline__ float elevation_eval(const DSpec& s, float x, float y, float z) {
-  const int mode = MODES ? s.mode : UPF_ELEV_LAYERS;
+  const int mode = EXT ? s.mode : UPF_ELEV_LAYERS;
   ModeState<float> st;
   if (mode != UPF_ELEV_LAYERS) mode_begin(mode, s.L[0].seed, s.L[0].octaves, s.L[0].freq[0], x, y, z, 0.0f, st);
   float acc = 0.0f, m = 0.0f;
@@ -313,12 +342,12 @@ __device__ __forceinline__ float elevation_eval(const DSpec& s, float x, float y
   return acc;
 }
 
-template <bool MODES = true>
+template <bool EXT = true>
 __device__ __forceinline__ float field_eval(const DSpec& s, float x, float y, float z) {
   const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
                          s.compose == UPF_COMPOSE_ELEVATION_RAW;
-  const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval(s, x, y, z);
-  const float elev = need_elev ? elevation_eval<MODES>(s, x, y, z) : 0.0f;
+  const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<EXT>(s, x, y, z);
+  const float elev = need_elev ? elevation_eval<EXT>(s, x, y, z) : 0.0f;
   switch (s.compose) {
     case UPF_COMPOSE_LAND: return shape - s.escale * elev;
     case UPF_COMPOSE_WATER: return shape;

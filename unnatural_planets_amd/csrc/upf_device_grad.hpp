@@ -123,7 +123,34 @@ __device__ __forceinline__ float octave_noise_d(const DLayer& L, int k, float x,
 }
 
 // ------------------------------------------------------------------ shapes + gradient
+// The further shapes with their position gradient (forward mode), out of line.
+__device__ __noinline__ Grad shape_more_d(int shape, float p0, float p1, float p2, float p3, float x, float y,
+                                          float z) {
+  const float sp[4] = {p0, p1, p2, p3};
+  if (shape == UPF_SHAPE_ASTEROID) {
+    const float l = __builtin_sqrtf(x * x + y * y + z * z);
+    if (l < 1.0f) return gc(-100.0f);
+    const Grad gx = gvar(x, 0), gy = gvar(y, 1), gz = gvar(z, 2);
+    const Grad gl = sq_root(gx * gx + gy * gy + gz * gz);
+    const Grad dx = gx / gl, dy = gy / gl, dz = gz / gl;
+    const uint32_t seed = (uint32_t)p1;
+    const float b = 1.0f / 1.3f;
+    float a0, a1, a2, c0, c1, c2;
+    const float n0 = noise_simplex_d(seed, dx.v * p0, dy.v * p0, dz.v * p0, a0, a1, a2);
+    const float f1 = p0 * 2.0f;
+    const float n1 = noise_simplex_d(seed + 1u, dx.v * f1, dy.v * f1, dz.v * f1, c0, c1, c2);
+    // dn/dd = b (f a + 0.3 (2 f) c), chained through d = p / |p|
+    const float k0 = b * p0, k1 = b * 0.3f * f1;
+    const float ex = k0 * a0 + k1 * c0, ey = k0 * a1 + k1 * c1, ez = k0 * a2 + k1 * c2;
+    const Grad n = {n0 * b + n1 * (b * 0.3f), ex * dx.dx + ey * dy.dx + ez * dz.dx, ex * dx.dy + ey * dy.dy + ez * dz.dy,
+                    ex * dx.dz + ey * dy.dz + ez * dz.dz};
+    return gl + n * 600.0f - 1300.0f;
+  }
+  return shape_more_t<Grad>(shape, sp, gvar(x, 0), gvar(y, 1), gvar(z, 2), gc(0.0f));
+}
+
 // Returns shape(p); writes d shape/dp and d shape/d params[0..3].
+template <bool EXT = true>
 __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, float z, float g[3],
                                               float dp[4]) {
   g[0] = g[1] = g[2] = 0.0f;
@@ -181,7 +208,9 @@ __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, 
     case UPF_SHAPE_NONE:
       return 0.0f;
     default: {  // extended shapes: forward-mode position gradient, no parameter gradient
-      const Grad r = shape_ext<Grad>(s.shape, s.sp, gvar(x, 0), gvar(y, 1), gvar(z, 2), gc(0.0f));
+      const Grad r = EXT && s.shape >= UPF_SHAPE_H3O ? shape_more_d(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], x, y, z)
+                                              : shape_ext<Grad>(s.shape, s.sp, gvar(x, 0), gvar(y, 1), gvar(z, 2),
+                                                                gc(0.0f));
       g[0] = r.dx; g[1] = r.dy; g[2] = r.dz;
       return r.v;
     }
@@ -266,10 +295,10 @@ __device__ __forceinline__ Grad elevation_mode_grad(const DSpec& s, float x, flo
   return mode_finish(mode, st, gx, gy, gz, zero);
 }
 
-// f, df/dp and (THETA) dLdf * df/dθ for one sample.  With an elevation mode (compiled
-// in with MODES) only the position gradient is formed: the parameter gradient is defined
-// for the layer stack.
-template <bool THETA, bool MODES = false>
+// f, df/dp and (THETA) dLdf * df/dθ for one sample.  EXT compiles in the modes and the
+// further shapes; with an elevation mode only the position gradient is formed (the
+// parameter gradient is defined for the layer stack).
+template <bool THETA, bool EXT = false>
 __device__ __forceinline__ float field_grad(const DSpec& s, float x, float y, float z, float dLdf, float g[3],
                                             ThetaAcc& acc) {
   float gs[3], dps[4];
@@ -279,7 +308,7 @@ __device__ __forceinline__ float field_grad(const DSpec& s, float x, float y, fl
   float shape = 0.0f;
   float dshape = 0.0f;  // d f / d shape
   if (need_shape) {
-    shape = shape_eval_d(s, x, y, z, gs, dps);
+    shape = shape_eval_d<EXT>(s, x, y, z, gs, dps);
     dshape = s.compose == UPF_COMPOSE_ELEVATION ? 1.0f / s.escale : 1.0f;
   } else {
     gs[0] = gs[1] = gs[2] = 0.0f;
@@ -290,7 +319,7 @@ __device__ __forceinline__ float field_grad(const DSpec& s, float x, float y, fl
     const int np = shape_param_count(s.shape);
     for (int i = 0; i < np; i++) acc.add(kThShape + i, dLdf * dshape * dps[i]);
   }
-  if (MODES && !THETA && need_elev && s.mode != UPF_ELEV_LAYERS) {
+  if (EXT && !THETA && need_elev && s.mode != UPF_ELEV_LAYERS) {
     const Grad e = elevation_mode_grad(s, x, y, z);
     if (s.compose == UPF_COMPOSE_ELEVATION_RAW) {
       g[0] = e.dx; g[1] = e.dy; g[2] = e.dz;

@@ -380,7 +380,7 @@ __device__ __forceinline__ void row_field(const RowCtx& c, const DSpec& s, float
   if (need_elev) row_elevation<R>(c, s, x, y, z, elev);
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval(s, x, y[r], z);
+    const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<false>(s, x, y[r], z);
     const float e = need_elev ? elev[r] : 0.0f;
     switch (s.compose) {
       case UPF_COMPOSE_LAND: out[r] = shape - s.escale * e; break;

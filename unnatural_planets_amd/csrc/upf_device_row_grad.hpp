@@ -234,7 +234,7 @@ __device__ __forceinline__ float row_field_grad(const RowCtx& c, const DSpec& s,
                          s.compose == UPF_COMPOSE_ELEVATION_RAW;
   float shape = 0.0f, dshape = 0.0f;
   if (need_shape) {
-    shape = shape_eval_d(s, x, y, z, gs, dps);
+    shape = shape_eval_d<false>(s, x, y, z, gs, dps);
     dshape = s.compose == UPF_COMPOSE_ELEVATION ? 1.0f / s.escale : 1.0f;
   } else {
     gs[0] = gs[1] = gs[2] = 0.0f;

@@ -29,7 +29,7 @@ __device__ __forceinline__ void write_signs(uint64_t* __restrict__ S, const DGri
   if ((threadIdx.x & 63) == 0) S[((int64_t)k * g.n[1] + j) * ((g.n[0] + 63) / 64) + xb] = m;
 }
 
-template <bool MODES>
+template <bool EXT>
 __global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g, float* __restrict__ out,
                                                    uint64_t* __restrict__ S, float iso) {
   const int i = blockIdx.x * kGridBX + threadIdx.x;
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g,
   const float x = node_coord(g.o[0], g.s[0], ic);
   const float y = node_coord(g.o[1], g.s[1], j);
   const float z = node_coord(g.o[2], g.s[2], k);
-  const float v = field_eval<MODES>(s, x, y, z);
+  const float v = field_eval<EXT>(s, x, y, z);
   if (valid) out[((size_t)k * g.n[1] + j) * g.n[0] + i] = v;
   if (S) write_signs(S, g, blockIdx.x, j, k, valid, v, iso);
 }
@@ -83,17 +83,17 @@ __global__ __launch_bounds__(256) void k_eval_grid_row(const DSpec s, const DGri
   }
 }
 
-template <bool MODES>
+template <bool EXT>
 __global__ __launch_bounds__(256) void k_eval_points(const DSpec s, const float* __restrict__ xyz, int64_t n,
                                                      float* __restrict__ f) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  f[i] = field_eval<MODES>(s, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  f[i] = field_eval<EXT>(s, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
 }
 
 // Fused forward + adjoint over a grid-stride loop; every lane of a block runs the same
 // number of iterations so the wave reductions inside ThetaAcc see all 64 lanes.
-template <bool THETA, bool GRID, bool MODES>
+template <bool THETA, bool GRID, bool EXT>
 __global__ __launch_bounds__(256) void k_adjoint(const DSpec s, const DGrid g, const float* __restrict__ xyz,
                                                  int64_t n, const float* __restrict__ dLdf, float* __restrict__ fout,
                                                  float* __restrict__ dfdx, float* __restrict__ dLdtheta) {
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const DSpec s, const DGrid g, c
     }
     const float w = (THETA && valid) ? dLdf[i] : 0.0f;
     float gr[3];
-    const float f = field_grad<THETA, MODES>(s, x, y, z, w, gr, acc);
+    const float f = field_grad<THETA, EXT>(s, x, y, z, w, gr, acc);
     if (valid) {
       if (fout) fout[i] = f;
       if (dfdx) { dfdx[3 * i] = gr[0]; dfdx[3 * i + 1] = gr[1]; dfdx[3 * i + 2] = gr[2]; }
@@ -200,10 +200,11 @@ int adjoint_blocks(int64_t n) {
 int field_grid_launch(const DSpec& s, const DGrid& g, float* d_field, int variant, uint64_t* S, float iso,
                       hipStream_t stream) {
   if (variant < 0 || variant > UPF_EVAL_PER_LANE) return UPF_ERR_BAD_SPEC;
-  if (s.mode != UPF_ELEV_LAYERS) variant = UPF_EVAL_PER_LANE;  // the row form covers the layer stack
+  const bool ext = spec_ext(s);
+  if (ext) variant = UPF_EVAL_PER_LANE;  // the row form covers the layer stack and the common shapes
   const dim3 blocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + kGridBY - 1) / kGridBY, g.n[2]);
   if (variant == UPF_EVAL_PER_LANE) {
-    if (s.mode != UPF_ELEV_LAYERS)
+    if (ext)
       hipLaunchKernelGGL(k_eval_grid<true>, blocks, dim3(kGridBX, kGridBY, 1), 0, stream, s, g, d_field, S, iso);
     else
       hipLaunchKernelGGL(k_eval_grid<false>, blocks, dim3(kGridBX, kGridBY, 1), 0, stream, s, g, d_field, S, iso);
@@ -246,7 +247,7 @@ int upf_field_eval_points(const upf_field_spec* spec, const float* d_xyz, int64_
   if (st) return st;
   if (d_dfdx) return upf_field_adjoint(spec, d_xyz, n, nullptr, d_f, d_dfdx, nullptr, stream);
   const dim3 pb((unsigned)((n + 255) / 256));
-  if (s.mode != UPF_ELEV_LAYERS)
+  if (spec_ext(s))
     hipLaunchKernelGGL(k_eval_points<true>, pb, dim3(256), 0, as_stream(stream), s, d_xyz, n, d_f);
   else
     hipLaunchKernelGGL(k_eval_points<false>, pb, dim3(256), 0, as_stream(stream), s, d_xyz, n, d_f);
@@ -266,15 +267,17 @@ int upf_field_adjoint(const upf_field_spec* spec, const float* d_xyz, int64_t n,
   if (d_dLdtheta && s.mode != UPF_ELEV_LAYERS) return UPF_ERR_BAD_SPEC;
   DGrid g{};
   const int nb = adjoint_blocks(n);
-  if (d_dLdtheta)
-    hipLaunchKernelGGL((k_adjoint<true, false, false>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, d_xyz, n, d_dLdf,
-                       d_f, d_dfdx, d_dLdtheta);
-  else if (s.mode != UPF_ELEV_LAYERS)
-    hipLaunchKernelGGL((k_adjoint<false, false, true>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, d_xyz, n,
-                       d_dLdf, d_f, d_dfdx, d_dLdtheta);
+  const dim3 gb(nb), tb(256);
+  hipStream_t hs = as_stream(stream);
+  const bool ext = spec_ext(s);
+  if (d_dLdtheta && ext)
+    hipLaunchKernelGGL((k_adjoint<true, false, true>), gb, tb, 0, hs, s, g, d_xyz, n, d_dLdf, d_f, d_dfdx, d_dLdtheta);
+  else if (d_dLdtheta)
+    hipLaunchKernelGGL((k_adjoint<true, false, false>), gb, tb, 0, hs, s, g, d_xyz, n, d_dLdf, d_f, d_dfdx, d_dLdtheta);
+  else if (ext)
+    hipLaunchKernelGGL((k_adjoint<false, false, true>), gb, tb, 0, hs, s, g, d_xyz, n, d_dLdf, d_f, d_dfdx, d_dLdtheta);
   else
-    hipLaunchKernelGGL((k_adjoint<false, false, false>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, d_xyz, n,
-                       d_dLdf, d_f, d_dfdx, d_dLdtheta);
+    hipLaunchKernelGGL((k_adjoint<false, false, false>), gb, tb, 0, hs, s, g, d_xyz, n, d_dLdf, d_f, d_dfdx, d_dLdtheta);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
@@ -287,7 +290,20 @@ int upf_field_adjoint_grid(const upf_field_spec* spec, const upf_grid* grid, con
   int st = to_device(spec, &s);
   if (st) return st;
   if ((st = to_device(grid, &g))) return st;
-  if (s.mode != UPF_ELEV_LAYERS) return UPF_ERR_BAD_SPEC;  // the row adjoint covers the layer stack
+  if (s.mode != UPF_ELEV_LAYERS) return UPF_ERR_BAD_SPEC;  // the parameter adjoint covers the layer stack
+  if (spec_ext(s)) {  // a further shape: per lane over the nodes
+    const int64_t n = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const dim3 gb(adjoint_blocks(n)), tb(256);
+    hipStream_t hs = as_stream(stream);
+    if (d_dLdtheta)
+      hipLaunchKernelGGL((k_adjoint<true, true, true>), gb, tb, 0, hs, s, g, nullptr, n, d_dLdf, d_f, d_dfdx,
+                         d_dLdtheta);
+    else
+      hipLaunchKernelGGL((k_adjoint<false, true, true>), gb, tb, 0, hs, s, g, nullptr, n, d_dLdf, d_f, d_dfdx,
+                         d_dLdtheta);
+    UPF_LAUNCH_CHECK();
+    return UPF_OK;
+  }
   const int64_t items = (int64_t)((g.n[0] + 63) / 64) * g.n[1] * g.n[2];
   const int nb = (int)std::min<int64_t>((items + 3) / 4, (int64_t)device_cus() * 8);
   if (d_dLdtheta)

@@ -17,15 +17,6 @@
 namespace upf {
 
 // ------------------------------------------------------------------ scalar helpers
-UPF_HD float pow_v(float a, float e) { return powf(a, e); }
-UPF_HD Grad pow_v(const Grad& a, float e) {
-  const float v = powf(a.v, e);
-  const float d = a.v != 0.0f ? e * powf(a.v, e - 1.0f) : 0.0f;  // d/da a^e, 0 at a = 0
-  return {v, d * a.dx, d * a.dy, d * a.dz};
-}
-UPF_HD float floor_v(float a) { return floorf(a); }
-UPF_HD Grad floor_v(const Grad& a) { return gc(floorf(a.v)); }
-
 template <class T> UPF_HD T sat_v(T a, T zero) { return clamp01(a, zero, lift(zero, 1.0f)); }
 // cage's one-argument smoothstep / smootherstep: x^2 (3 - 2x) and x^3 (x (6x - 15) + 10)
 template <class T> UPF_HD T smoothstep1(T x) { return x * x * (3.0f - 2.0f * x); }

@@ -81,7 +81,7 @@ __device__ __forceinline__ int wg_exclusive_sum(int v, int* total) {
   return before + inc - v;
 }
 
-template <bool MODES>
+template <bool EXT>
 __global__ __launch_bounds__(256) void k_nb_coarse(const DSpec s, const DGrid g, const NbDims d,
                                                    float* __restrict__ coarse) {
   const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_nb_coarse(const DSpec s, const DGrid g,
   const int I = (int)(c % cx);
   const int64_t r = c / cx;
   const int J = (int)(r % cy), K = (int)(r / cy);
-  coarse[c] = field_eval<MODES>(s, node_coord(g.o[0], g.s[0], I * kB), node_coord(g.o[1], g.s[1], J * kB),
+  coarse[c] = field_eval<EXT>(s, node_coord(g.o[0], g.s[0], I * kB), node_coord(g.o[1], g.s[1], J * kB),
                          node_coord(g.o[2], g.s[2], K * kB));
 }
 
@@ -127,7 +127,7 @@ __global__ void k_nb_map(const uint2* __restrict__ act, int64_t nact, int32_t* _
   if (r < nact) map[act[r].x] = (int32_t)r;
 }
 
-template <bool MODES>
+template <bool EXT>
 __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGrid g, const NbDims d, float iso,
                                                         const uint2* __restrict__ act,
                                                         const int32_t* __restrict__ map, float* __restrict__ bricks,
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
   for (int n = threadIdx.x; n < kBrick; n += kNbThreads) {
     const int k = n / (kBN * kBN), r = n - k * kBN * kBN;
     const int j = r / kBN, i = r - j * kBN;
-    const float v = field_eval<MODES>(s, node_coord(g.o[0], g.s[0], gi0 + i), node_coord(g.o[1], g.s[1], gj0 + j),
+    const float v = field_eval<EXT>(s, node_coord(g.o[0], g.s[0], gi0 + i), node_coord(g.o[1], g.s[1], gj0 + j),
                                node_coord(g.o[2], g.s[2], gk0 + k));
     s_f[n] = v;
     brick[n] = v;
@@ -309,14 +309,14 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
 // Normals: the normalized analytic gradient at each vertex, one thread per vertex (few
 // nodes of a brick own a vertex, so evaluating the gradient inside k_nb_emit would run
 // the whole gradient for mostly idle lanes).
-template <bool MODES>
+template <bool EXT>
 __global__ __launch_bounds__(256) void k_nb_normals(const DSpec s, const float* __restrict__ pos, int64_t n,
                                                     float* __restrict__ nrm) {
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= n) return;
   ThetaAcc none{nullptr};
   float gr[3];
-  (void)field_grad<false, MODES>(s, pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 0.0f, gr, none);
+  (void)field_grad<false, EXT>(s, pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 0.0f, gr, none);
   const float len = __builtin_sqrtf(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
   const float inv = len > 0.0f ? 1.0f / len : 0.0f;
   nrm[3 * v + 0] = gr[0] * inv;
@@ -416,7 +416,7 @@ int upf_nb_extract(const upf_field_spec* spec, const upf_grid* grid, float iso, 
   hipStream_t hs = as_stream(stream);
 
   const auto nblk = [](int64_t n) { return dim3((unsigned)((n + 255) / 256)); };
-  const bool modes = s.mode != UPF_ELEV_LAYERS;
+  const bool modes = spec_ext(s);
   if (modes) hipLaunchKernelGGL(k_nb_coarse<true>, nblk(L.coarse), dim3(256), 0, hs, s, g, d, coarse);
   else hipLaunchKernelGGL(k_nb_coarse<false>, nblk(L.coarse), dim3(256), 0, hs, s, g, d, coarse);
   hipLaunchKernelGGL(k_nb_activity, nblk(d.blocks), dim3(256), 0, hs, coarse, d, iso, band, counts);

@@ -24,11 +24,15 @@ NB_BLOCK = 8
 
 SHAPE = {"none": 0, "sphere": 1, "box": 2, "torus": 3, "capsule": 4, "cylinder": 5, "plane": 6, "octahedron": 7,
          "double_torus": 8, "torus_cross": 9, "h2o": 10, "bowl": 11, "inside_cube": 12, "wormhole": 13,
-         "pipe": 14, "hemispheres": 15}
+         "pipe": 14, "hemispheres": 15, "h3o": 16, "h4o": 17, "gear": 18, "mandelbulb": 19, "mobius": 20,
+         "fibers": 21, "asteroid": 22, "twisted_plane": 23, "belt": 24, "triangular_prism": 25,
+         "hexagonal_prism": 26, "twisted_hex_prism": 27, "tetrahedron": 28}
 # the reference's parameters (sources/sdf.cpp, metres) for the parametric extended shapes
 SHAPE_REFERENCE_PARAMS = {"cylinder": (10000.0, 800.0), "octahedron": (800.0, 100.0),
                           "double_torus": (900.0, 400.0, 200.0), "torus_cross": (750.0, 400.0, 200.0),
-                          "plane": (0.0, 0.0, -1.0, 0.0)}
+                          "plane": (0.0, 0.0, -1.0, 0.0),
+                          # the reference draws these at run time: a fixed draw in its range
+                          "fibers": (37.0, -12.5, 71.0), "asteroid": (0.7, 1234.0)}
 NOISE = {"gradient": 1, "value": 2, "cellular": 3, "simplex": 4}
 # elevation modes: the reference's fixed elevation functions (sources/elevationModes.cpp)
 ELEV = {"layers": 0, "none": 1, "simple": 2, "legacy": 3, "lakes": 4, "islands": 5, "craters": 6}
