@@ -62,11 +62,15 @@ __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, co
                                                  int64_t cap_t) {
   __shared__ Cross s_nb[4][8];
   const int wave = uniform(threadIdx.x >> 6);
-  const int64_t a = (int64_t)blockIdx.x * 4 + wave;
-  if (a >= (int64_t)total->z) return;
-  const uint2 entry = act[a];
-  mc_emit_group(f, d, g, iso, S, counts, voff, (uint32_t)uniform((int)entry.x), (uint32_t)uniform((int)entry.y), 0u,
-                s_nb[wave], pos, nrm, idx, cap_v, cap_t);
+  // grid-stride over the active groups: the grid is sized to keep the CUs full, not to
+  // the (unknown on the host) number of active groups, so no wave launches only to exit
+  const int64_t nact = (int64_t)total->z;
+  for (int64_t a = (int64_t)blockIdx.x * 4 + wave; a < nact; a += (int64_t)gridDim.x * 4) {
+    const uint2 entry = act[a];
+    mc_emit_group(f, d, g, iso, S, counts, voff, (uint32_t)uniform((int)entry.x), (uint32_t)uniform((int)entry.y),
+                  0u, s_nb[wave], pos, nrm, idx, cap_v, cap_t);
+    __builtin_amdgcn_wave_barrier();  // this group's reads of s_nb before the next group's writes
+  }
 }
 
 // Classification from the sign masks alone (the field kernel wrote them).  One lane per
@@ -331,11 +335,15 @@ int mc_classify_signs_scan(const DGrid& g, void* d_ws, size_t ws_bytes, hipStrea
 
 int64_t mc_scan_tiles(int64_t groups) { return (groups + kScanTile - 1) / kScanTile; }
 
+constexpr int kEmitBlocksPerCu = 32;
+
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
                    float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s) {
   const McLayout L = mc_layout(g);
   const McPtrs P = mc_ptrs(L, d_ws);
   if (max_active <= 0) return UPF_OK;
+  // 32 blocks of 4 waves per CU (more than stay resident), each looping over the list
+  max_active = std::min<int64_t>(max_active, (int64_t)device_cus() * 4 * kEmitBlocksPerCu);
   hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((max_active + 3) / 4)), dim3(256), 0, s, d_field, L.d, g, iso, P.S,
                      P.counts, P.voff, P.act, P.total, pos, nrm, idx, cap_v, cap_t);
   UPF_LAUNCH_CHECK();

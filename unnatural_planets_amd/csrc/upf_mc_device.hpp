@@ -12,6 +12,36 @@ __constant__ int8_t c_edge_owner[12][4] = {
     {0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
     {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
 
+// The same tables as compile-time data for the emit: the triangulation with rows padded
+// to 16 bytes (one 16-byte load per cell), and the edge owners packed 5 bits per edge
+// (dx | dy << 1 | dz << 2 | axis << 3) into one 64-bit immediate.
+namespace mc_ct {
+#define UPF_MC_QUAL static constexpr
+#include "mc_table.inc"
+#undef UPF_MC_QUAL
+constexpr int8_t owner[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
+                                 {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+struct Tris16 {
+  int8_t v[256][16];
+};
+constexpr Tris16 pad_tris() {
+  Tris16 t{};
+  for (int c = 0; c < 256; c++) {
+    for (int q = 0; q < 15; q++) t.v[c][q] = upf_mc_tris[c][q];
+    t.v[c][15] = -1;
+  }
+  return t;
+}
+constexpr uint64_t pack_owner() {
+  uint64_t r = 0;
+  for (int e = 0; e < 12; e++)
+    r |= (uint64_t)(owner[e][0] | (owner[e][1] << 1) | (owner[e][2] << 2) | (owner[e][3] << 3)) << (5 * e);
+  return r;
+}
+}  // namespace mc_ct
+__constant__ __align__(16) mc_ct::Tris16 c_tris16 = mc_ct::pad_tris();
+constexpr uint64_t kEdgeOwner = mc_ct::pack_owner();
+
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -126,23 +156,14 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
   const int i = xb * 64 + lane;
   const bool valid = i < d.nx;
 
-  // field values (clamped coordinates; a clamped value is never used)
+  // field values (clamped coordinates; a clamped value is never used), loaded only by
+  // the lanes that own a vertex
   const int ic = min(i, d.nx - 1);
   const int64_t sy = d.nx, sz = d.nxy;
   const int xs[4] = {max(ic - 1, 0), ic, min(ic + 1, d.nx - 1), min(ic + 2, d.nx - 1)};  // i-1 .. i+2
   const int ys[4] = {max(j - 1, 0), j, min(j + 1, d.ny - 1), min(j + 2, d.ny - 1)};
   const int zs[4] = {max(k - 1, 0), k, min(k + 1, d.nz - 1), min(k + 2, d.nz - 1)};
   auto at = [&](int xi, int yi, int zi) { return f[(int64_t)zs[zi] * sz + (int64_t)ys[yi] * sy + xs[xi]]; };
-  const float c0 = at(1, 1, 1), c1 = at(2, 1, 1), c2 = at(2, 2, 1), c3 = at(1, 2, 1);
-  const float c4 = at(1, 1, 2), c5 = at(2, 1, 2), c6 = at(2, 2, 2), c7 = at(1, 2, 2);
-  float xm = 0.f, ym = 0.f, zm = 0.f, x2 = 0.f, xym = 0.f, xzm = 0.f, y2 = 0.f, yxm = 0.f, yzm = 0.f;
-  float z2 = 0.f, zxm = 0.f, zym = 0.f;
-  if (nrm) {
-    xm = at(0, 1, 1); ym = at(1, 0, 1); zm = at(1, 1, 0);
-    x2 = at(3, 1, 1); xym = at(2, 0, 1); xzm = at(2, 1, 0);
-    y2 = at(1, 3, 1); yxm = at(0, 2, 1); yzm = at(1, 2, 0);
-    z2 = at(1, 1, 3); zxm = at(0, 1, 2); zym = at(1, 0, 2);
-  }
   const uint32_t c = counts[gid];
   const int cnt_v = c & 0xFF, cnt_t = (c >> 8) & 0x1FF;
 
@@ -165,6 +186,17 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
     const uint64_t bits[3] = {cc.bx, cc.by, cc.bz};
     const bool any = ((cc.bx | cc.by | cc.bz) >> lane) & 1ull;
     if (any) {
+      const float c0 = at(1, 1, 1), c1 = at(2, 1, 1), c3 = at(1, 2, 1), c4 = at(1, 1, 2);
+      float c2 = 0.f, c5 = 0.f, c7 = 0.f;
+      float xm = 0.f, ym = 0.f, zm = 0.f, x2 = 0.f, xym = 0.f, xzm = 0.f, y2 = 0.f, yxm = 0.f, yzm = 0.f;
+      float z2 = 0.f, zxm = 0.f, zym = 0.f;
+      if (nrm) {
+        c2 = at(2, 2, 1); c5 = at(2, 1, 2); c7 = at(1, 2, 2);
+        xm = at(0, 1, 1); ym = at(1, 0, 1); zm = at(1, 1, 0);
+        x2 = at(3, 1, 1); xym = at(2, 0, 1); xzm = at(2, 1, 0);
+        y2 = at(1, 3, 1); yxm = at(0, 2, 1); yzm = at(1, 2, 0);
+        z2 = at(1, 1, 3); zxm = at(0, 1, 2); zym = at(1, 0, 2);
+      }
       const float f0 = c0;
       const float p0[3] = {node_coord(g.o[0], g.s[0], i), node_coord(g.o[1], g.s[1], j),
                            node_coord(g.o[2], g.s[2], k)};
@@ -222,10 +254,23 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
   // ---- triangles
   if (cnt_t) {
     const bool cell = valid && i + 1 < d.nx && j + 1 < d.ny && k + 1 < d.nz;
+    // the case code from the sign words (bit = f < iso) of the cell's 4 corner rows and
+    // their +x neighbour groups: wave-uniform 64-bit loads instead of 8 values per lane
+    const bool nx1 = xb + 1 < d.gpr, hy = j + 1 < d.ny, hz = k + 1 < d.nz;
+    const int64_t gr = (int64_t)gid, plane = (int64_t)d.ny * d.gpr;
+    const uint64_t wa = S[gr], wax = nx1 ? S[gr + 1] : 0ull;
+    const uint64_t way = hy ? S[gr + d.gpr] : 0ull, waxy = (hy && nx1) ? S[gr + d.gpr + 1] : 0ull;
+    const uint64_t wb = hz ? S[gr + plane] : 0ull, wbx = (hz && nx1) ? S[gr + plane + 1] : 0ull;
+    const uint64_t wby = (hz && hy) ? S[gr + plane + d.gpr] : 0ull;
+    const uint64_t wbxy = (hz && hy && nx1) ? S[gr + plane + d.gpr + 1] : 0ull;
+    const uint64_t wa1 = (wa >> 1) | (wax << 63), way1 = (way >> 1) | (waxy << 63);
+    const uint64_t wb1 = (wb >> 1) | (wbx << 63), wby1 = (wby >> 1) | (wbxy << 63);
     int code = 0;
     if (cell)
-      code = (c0 < iso) | ((c1 < iso) << 1) | ((c2 < iso) << 2) | ((c3 < iso) << 3) | ((c4 < iso) << 4) |
-             ((c5 < iso) << 5) | ((c6 < iso) << 6) | ((c7 < iso) << 7);
+      code = (int)((wa >> lane) & 1ull) | ((int)((wa1 >> lane) & 1ull) << 1) | ((int)((way1 >> lane) & 1ull) << 2) |
+             ((int)((way >> lane) & 1ull) << 3) | ((int)((wb >> lane) & 1ull) << 4) |
+             ((int)((wb1 >> lane) & 1ull) << 5) | ((int)((wby1 >> lane) & 1ull) << 6) |
+             ((int)((wby >> lane) & 1ull) << 7);
     const int nt = upf_mc_ntris[code];
     int inc = nt;
 #pragma unroll
@@ -233,14 +278,22 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
       const int v = __shfl_up(inc, o, 64);
       if (lane >= o) inc += v;
     }
-    uint32_t tid = toff + (uint32_t)(inc - nt);
-    for (int t = 0; t < nt; t++, tid++) {
+    const uint32_t tid0 = toff + (uint32_t)(inc - nt);
+    // the cell's triangulation row in one 16-byte load; edge owners from the immediate
+    const int4 row = *reinterpret_cast<const int4*>(&c_tris16.v[code][0]);
+    const uint32_t rw[4] = {(uint32_t)row.x, (uint32_t)row.y, (uint32_t)row.z, (uint32_t)row.w};
+#pragma unroll
+    for (int t = 0; t < UPF_MC_MAX_TRIS; t++) {
+      if (t >= nt) break;
+      const uint32_t tid = tid0 + (uint32_t)t;
 #pragma unroll
       for (int v = 0; v < 3; v++) {
-        const int e = upf_mc_tris[code][3 * t + v];
-        const int ol = lane + c_edge_owner[e][0];
-        const int slot = (ol >> 6) | (c_edge_owner[e][1] << 1) | (c_edge_owner[e][2] << 2);
-        const int axis = c_edge_owner[e][3];
+        const int q = 3 * t + v;
+        const int e = (int)(int8_t)(uint8_t)(rw[q >> 2] >> (8 * (q & 3)));
+        const int own = (int)((kEdgeOwner >> (5 * e)) & 31u);
+        const int ol = lane + (own & 1);
+        const int slot = (ol >> 6) | (((own >> 1) & 1) << 1) | (((own >> 2) & 1) << 2);
+        const int axis = own >> 3;
         const Cross cc = s_nb[slot];
         const int l = ol & 63;
         const uint64_t m = (1ull << l) - 1;
