@@ -358,6 +358,21 @@ int upf_field_mesh_count(const upf_field_spec* spec, const upf_grid* grid, float
                          const upf_eval_opts* opts, void* d_workspace, size_t workspace_bytes,
                          int64_t* n_vertices, int64_t* n_triangles, void* stream);
 
+/* upf_field_mesh_count followed by upf_mc_emit in one stream-ordered call, with no host
+ * synchronisation.  The mesh goes to `out` within its capacities (out->cap_vertices,
+ * out->cap_triangles; out->n_vertices / n_triangles are neither read nor written), and
+ * the required counts go to d_counts[0] (vertices) and d_counts[1] (triangles), device
+ * memory, in stream order.  Where a count exceeds its capacity the writes past it are
+ * dropped and the mesh is incomplete: the caller compares after its own synchronisation
+ * and calls again with larger buffers.  For loops that keep buffers sized by an earlier
+ * call (re-meshing a field whose parameters change a little), this removes the count
+ * read-back and the host round trip between classification and emission.  Mesh and
+ * counts are those of upf_field_mesh_count + upf_mc_emit bit for bit.  Replaces
+ * updateByPosition + makeMesh of meshGenerateGeneric (sources/meshGeneration.cpp:38-40). */
+int upf_field_mesh_async(const upf_field_spec* spec, const upf_grid* grid, float iso, float* d_field,
+                         const upf_eval_opts* opts, upf_mesh_buffers* out, int64_t* d_counts,
+                         void* d_workspace, size_t workspace_bytes, void* stream);
+
 /* Case code of every cell -> d_codes[dims[2]-1][dims[1]-1][dims[0]-1] (uint8). */
 int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,
                       uint8_t* d_codes, void* stream);

@@ -158,3 +158,60 @@ def test_fused_field_mesh_count_equals_separate_passes(upf, variant, dims, lo, h
     assert torch.equal(m.positions, ref.positions) and torch.equal(m.normals, ref.normals)
     if dims[0] == 128:
         assert nt > 10000
+
+
+def _grid(dims, lo, hi):
+    g = S.Grid()
+    for a in range(3):
+        g.origin[a] = lo
+        g.spacing[a] = S.f32((hi - lo) / (dims[a] - 1))
+        g.dims[a] = dims[a]
+    return g
+
+
+@pytest.mark.parametrize("dims", [(128, 128, 128), (97, 70, 33), (2, 2, 2)])
+def test_field_mesh_async_equals_two_calls(upf, dims):
+    """upf_field_mesh_async (no host synchronisation) = upf_field_mesh_count + upf_mc_emit
+    bit for bit, twice in a row on the same buffers; rows past the counts stay untouched."""
+    torch = _torch()
+    s = S.spec_sphere_fbm(1234) if dims[0] == 128 else S.spec_torus_ridged_voronoi(42)
+    g = _grid(dims, -1.3, 1.3)
+    f_ref, ws, nv, nt = upf.field_mesh_count(s, g, 0.0)
+    ref = upf.mc_emit_counted(f_ref, g, 0.0, ws, nv, nt)
+    cap_v, cap_t = nv + 100, nt + 100
+    mesh = upf.Mesh(torch.full((cap_v, 3), 7.0, device="cuda"), torch.full((cap_v, 3), 9.0, device="cuda"),
+                    torch.full((cap_t, 3), -5, dtype=torch.int32, device="cuda"))
+    field = torch.empty_like(f_ref)
+    for call in range(2):
+        counts = torch.full((2,), -1, dtype=torch.int64, device="cuda")
+        upf.field_mesh_async(s, g, field, mesh, counts)
+        torch.cuda.synchronize()
+        print(f"case: async {dims} call {call}: counts {counts.tolist()} vs {(nv, nt)}", flush=True)
+        assert counts.tolist() == [nv, nt]
+        assert torch.equal(field, f_ref)
+        assert torch.equal(mesh.positions[:nv], ref.positions) and torch.equal(mesh.normals[:nv], ref.normals)
+        assert torch.equal(mesh.indices[:nt], ref.indices)
+        assert bool((mesh.positions[nv:] == 7.0).all()) and bool((mesh.normals[nv:] == 9.0).all())
+        assert bool((mesh.indices[nt:] == -5).all())
+
+
+def test_field_mesh_async_capacity_overflow(upf):
+    """Capacities below the counts: the counts report the full mesh, the rows within the
+    capacities equal the full mesh's, and nothing past them is written."""
+    torch = _torch()
+    s = S.spec_sphere_fbm(1234)
+    g = _grid((96, 96, 96), -1.3, 1.3)
+    f_ref, ws, nv, nt = upf.field_mesh_count(s, g, 0.0)
+    ref = upf.mc_emit_counted(f_ref, g, 0.0, ws, nv, nt)
+    cap_v, cap_t = nv // 3, nt // 2
+    pos = torch.full((nv, 3), 7.0, device="cuda")
+    nrm = torch.full((nv, 3), 9.0, device="cuda")
+    idx = torch.full((nt, 3), -5, dtype=torch.int32, device="cuda")
+    counts = torch.full((2,), -1, dtype=torch.int64, device="cuda")
+    field = torch.empty_like(f_ref)
+    upf.field_mesh_async(s, g, field, upf.Mesh(pos[:cap_v], nrm[:cap_v], idx[:cap_t]), counts)
+    torch.cuda.synchronize()
+    assert counts.tolist() == [nv, nt]
+    assert torch.equal(pos[:cap_v], ref.positions[:cap_v]) and torch.equal(nrm[:cap_v], ref.normals[:cap_v])
+    assert torch.equal(idx[:cap_t], ref.indices[:cap_t])
+    assert bool((pos[cap_v:] == 7.0).all()) and bool((nrm[cap_v:] == 9.0).all()) and bool((idx[cap_t:] == -5).all())

@@ -284,6 +284,31 @@ def mc_emit_counted(field, g: Grid, iso: float, ws: McWorkspace, n_vertices: int
     return Mesh(pos, nrm, idx)
 
 
+def field_mesh_async(s: FieldSpec, g: Grid, field, mesh: Mesh, counts, iso: float = 0.0,
+                     workspace: McWorkspace | None = None, variant: int = EVAL_AUTO, stream=None) -> McWorkspace:
+    """Field + marching cubes in one stream-ordered call (upf_field_mesh_async): the mesh
+    goes into the preallocated `mesh` tensors (their row counts are the capacities) and
+    the required counts into `counts` (int64 [2] on the device).  Nothing synchronises;
+    the mesh is complete when counts <= the capacities.  Returns the workspace."""
+    nx, ny, nz = grid_dims(g)
+    _check_f32(field, "field", (nz, ny, nx))
+    if counts.dtype != _torch().int64 or counts.numel() < 2 or not counts.is_contiguous():
+        raise ValueError("counts must be a contiguous int64 tensor of 2 elements")
+    ws = workspace or McWorkspace(g, field.device)
+    opts = EvalOpts()
+    opts.variant = variant
+    mb = MeshBuffers()
+    cap_v, cap_t = int(mesh.positions.shape[0]), int(mesh.indices.shape[0])
+    mb.positions = mesh.positions.data_ptr() if cap_v else None
+    mb.normals = mesh.normals.data_ptr() if (mesh.normals is not None and cap_v) else None
+    mb.indices = mesh.indices.data_ptr() if cap_t else None
+    mb.cap_vertices, mb.cap_triangles = cap_v, cap_t
+    check(lib().upf_field_mesh_async(ctypes.byref(s), ctypes.byref(g), iso, _ptr(field), ctypes.byref(opts),
+                                      ctypes.byref(mb), _ptr(counts), _ptr(ws.buf), ws.bytes, _stream(stream)),
+          "upf_field_mesh_async")
+    return ws
+
+
 def _mesh_buffers(m: Mesh):
     mb = MeshBuffers()
     mb.positions = m.positions.data_ptr() if m.n_vertices else None

@@ -59,9 +59,11 @@ __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, co
                                                  const uint32_t* __restrict__ voff, const uint2* __restrict__ act,
                                                  const uint4* __restrict__ total, float* __restrict__ pos,
                                                  float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
-                                                 int64_t cap_t) {
+                                                 int64_t cap_t, int64_t* __restrict__ counts_out) {
   __shared__ Cross s_nb[4][8];
   const int wave = uniform(threadIdx.x >> 6);
+  if (counts_out && blockIdx.x == 0 && threadIdx.x < 2)
+    counts_out[threadIdx.x] = (int64_t)(threadIdx.x ? total->y : total->x);
   // grid-stride over the active groups: the grid is sized to keep the CUs full, not to
   // the (unknown on the host) number of active groups, so no wave launches only to exit
   const int64_t nact = (int64_t)total->z;
@@ -338,14 +340,14 @@ int64_t mc_scan_tiles(int64_t groups) { return (groups + kScanTile - 1) / kScanT
 constexpr int kEmitBlocksPerCu = 32;
 
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
-                   float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s) {
+                   float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s, int64_t* counts_out) {
   const McLayout L = mc_layout(g);
   const McPtrs P = mc_ptrs(L, d_ws);
   if (max_active <= 0) return UPF_OK;
   // 32 blocks of 4 waves per CU (more than stay resident), each looping over the list
   max_active = std::min<int64_t>(max_active, (int64_t)device_cus() * 4 * kEmitBlocksPerCu);
   hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((max_active + 3) / 4)), dim3(256), 0, s, d_field, L.d, g, iso, P.S,
-                     P.counts, P.voff, P.act, P.total, pos, nrm, idx, cap_v, cap_t);
+                     P.counts, P.voff, P.act, P.total, pos, nrm, idx, cap_v, cap_t, counts_out);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
@@ -401,6 +403,29 @@ int upf_field_mesh_count(const upf_field_spec* spec, const upf_grid* grid, float
   *n_vertices = h_total.x;
   *n_triangles = h_total.y;
   return UPF_OK;
+}
+
+int upf_field_mesh_async(const upf_field_spec* spec, const upf_grid* grid, float iso, float* d_field,
+                         const upf_eval_opts* opts, upf_mesh_buffers* out, int64_t* d_counts, void* d_ws,
+                         size_t ws_bytes, void* stream) {
+  if (!d_field || !out || !d_counts) return UPF_ERR_NULL;
+  if (!std::isfinite(iso)) return UPF_ERR_INVALID_VALUE;
+  if (out->cap_vertices < 0 || out->cap_triangles < 0) return UPF_ERR_CAPACITY;
+  if ((out->cap_vertices && !out->positions) || (out->cap_triangles && !out->indices)) return UPF_ERR_NULL;
+  DSpec sp;
+  DGrid g;
+  int st = to_device(spec, &sp);
+  if (st || (st = to_device(grid, &g)) || (st = mc_grid_ok(g))) return st;
+  const McLayout L = mc_layout(g);
+  if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
+  const McPtrs P = mc_ptrs(L, d_ws);
+  hipStream_t s = as_stream(stream);
+  if ((st = field_grid_launch(sp, g, d_field, opts ? opts->variant : UPF_EVAL_AUTO, P.S, iso, s))) return st;
+  if ((st = mc_classify_signs_scan(g, d_ws, ws_bytes, s))) return st;
+  // the number of active groups is known only on the device: launch the persistent grid
+  // for the largest possible list; the kernel also stores the totals in d_counts
+  return mc_emit_launch(d_field, g, iso, d_ws, L.d.groups, out->positions, out->normals, out->indices,
+                        out->cap_vertices, out->cap_triangles, s, d_counts);
 }
 
 int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out, void* d_ws,

@@ -41,7 +41,8 @@ int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws
 int mc_classify_signs_scan(const DGrid& g, void* d_ws, size_t ws_bytes, hipStream_t s);
 // emit over at most max_active active groups (the kernel reads the true count)
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
-                   float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s);
+                   float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s,
+                   int64_t* counts_out = nullptr);
 // count-word formats of mc_scan_launch
 constexpr int kScanMc = 0;      // vertices bits 0-7, triangles 8-16, active bit 17
 constexpr int kScanBlocks = 1;  // vertices bits 0-11, triangles 12-23, active bit 24

@@ -88,10 +88,25 @@ __global__ void k_pp_jump(int32_t* label, int64_t nv, uint32_t* changed) {
   }
 }
 
+// Triangles per component label.  A mesh is mostly one component, so nearly every
+// triangle adds to the same counter; one atomic per triangle would serialise on that
+// address.  Each wave instead adds once per distinct label among its lanes: the lowest
+// pending lane's label is broadcast, the lanes holding it are counted by ballot, and
+// its owner adds the count (one round for a wave inside one component).
 __global__ void k_pp_count(const uint32_t* __restrict__ idx, int64_t nt, int64_t nv,
                            const int32_t* __restrict__ label, uint32_t* __restrict__ cnt) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t < nt && tri_ok(idx, t, nv)) atomicAdd(cnt + label[idx[3 * t]], 1u);
+  int32_t l = -1;
+  if (t < nt && tri_ok(idx, t, nv)) l = label[idx[3 * t]];
+  const int lane = (int)(threadIdx.x & 63);
+  uint64_t todo = __ballot(l >= 0);
+  while (todo) {
+    const int lead = __builtin_ctzll(todo);
+    const int32_t key = __shfl(l, lead);
+    const uint64_t same = __ballot(l == key) & todo;
+    if (lane == lead) atomicAdd(cnt + key, (uint32_t)__popcll(same));
+    todo &= ~same;
+  }
 }
 
 __global__ void k_pp_best(const uint32_t* __restrict__ cnt, int64_t nv, unsigned long long* best) {
