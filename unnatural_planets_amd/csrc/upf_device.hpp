@@ -19,6 +19,13 @@
 
 namespace upf {
 
+// Square root of the hot field's distances (sphere, box, torus, capsule, cellular F1):
+// the hardware v_sqrt_f32 (1 ulp) instead of the correctly rounded expansion (denormal
+// scaling, two Newton corrections, class fix-ups: 16 instructions), which made up 8% of
+// the row field kernel's instructions on the torus + noise workload.  1 ulp is far
+// inside the field's 1e-5 tolerance; the compositions of upf_shapes.hpp keep sqrt_f.
+__device__ __forceinline__ float sqrt_hw(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 // Kernel-side copy of a layer: the spec's layer plus the cellular constants
 // js = jitter/1023 and j0 = jitter*0.5, precomputed on the host in FP32.
 struct DLayer {
@@ -178,7 +185,7 @@ __device__ __forceinline__ float cellular_d2(uint32_t seed, float qx, float qy, 
 
 __device__ __forceinline__ float noise_cellular(uint32_t seed, float qx, float qy, float qz, float js, float j0) {
   float ex, ey, ez;
-  return __builtin_sqrtf(cellular_d2(seed, qx, qy, qz, js, j0, ex, ey, ez));
+  return sqrt_hw(cellular_d2(seed, qx, qy, qz, js, j0, ex, ey, ez));
 }
 
 // 3D simplex noise (Perlin 2001, in the form of Gustavson's "Simplex noise demystified",
@@ -244,7 +251,7 @@ __device__ __noinline__ float shape_more(int shape, float p0, float p1, float p2
                                          float z) {
   const float sp[4] = {p0, p1, p2, p3};
   if (shape == UPF_SHAPE_ASTEROID) {  // sources/sdf.cpp:333-353
-    const float l = __builtin_sqrtf(x * x + y * y + z * z);
+    const float l = sqrt_f(x * x + y * y + z * z);
     if (l < 1.0f) return -100.0f;
     const float dx = x / l, dy = y / l, dz = z / l;
     const uint32_t seed = (uint32_t)p1;
@@ -262,22 +269,22 @@ template <bool EXT = true>
 __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, float z) {
   switch (s.shape) {
     case UPF_SHAPE_SPHERE:
-      return __builtin_sqrtf(fmaf(z, z, fmaf(y, y, x * x))) - s.sp[0];
+      return sqrt_hw(fmaf(z, z, fmaf(y, y, x * x))) - s.sp[0];
     case UPF_SHAPE_BOX: {
       const float qx = fabsf(x) - s.sp[0], qy = fabsf(y) - s.sp[1], qz = fabsf(z) - s.sp[2];
       const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
-      const float outside = __builtin_sqrtf(fmaf(mz, mz, fmaf(my, my, mx * mx)));
+      const float outside = sqrt_hw(fmaf(mz, mz, fmaf(my, my, mx * mx)));
       const float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
       return outside + inside - s.sp[3];
     }
     case UPF_SHAPE_TORUS: {
-      const float ring = __builtin_sqrtf(fmaf(z, z, x * x)) - s.sp[0];
-      return __builtin_sqrtf(fmaf(y, y, ring * ring)) - s.sp[1];
+      const float ring = sqrt_hw(fmaf(z, z, x * x)) - s.sp[0];
+      return sqrt_hw(fmaf(y, y, ring * ring)) - s.sp[1];
     }
     case UPF_SHAPE_CAPSULE: {
       const float h2 = s.sp[0] * 0.5f;
       const float zc = z - fminf(fmaxf(z, -h2), h2);
-      return __builtin_sqrtf(fmaf(zc, zc, fmaf(y, y, x * x))) - s.sp[1];
+      return sqrt_hw(fmaf(zc, zc, fmaf(y, y, x * x))) - s.sp[1];
     }
     case UPF_SHAPE_NONE:
       return 0.0f;

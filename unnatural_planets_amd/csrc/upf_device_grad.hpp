@@ -77,7 +77,7 @@ __device__ __forceinline__ float noise_cellular_d(uint32_t seed, float qx, float
                                                   float j0, float& dx, float& dy, float& dz) {
   float ex, ey, ez;
   const float d2 = cellular_d2(seed, qx, qy, qz, js, j0, ex, ey, ez);
-  const float r = __builtin_sqrtf(d2);
+  const float r = sqrt_hw(d2);
   const float inv = r > 0.0f ? 1.0f / r : 0.0f;
   dx = ex * inv; dy = ey * inv; dz = ez * inv;
   return r;
@@ -128,7 +128,7 @@ __device__ __noinline__ Grad shape_more_d(int shape, float p0, float p1, float p
                                           float z) {
   const float sp[4] = {p0, p1, p2, p3};
   if (shape == UPF_SHAPE_ASTEROID) {
-    const float l = __builtin_sqrtf(x * x + y * y + z * z);
+    const float l = sqrt_f(x * x + y * y + z * z);
     if (l < 1.0f) return gc(-100.0f);
     const Grad gx = gvar(x, 0), gy = gvar(y, 1), gz = gvar(z, 2);
     const Grad gl = sq_root(gx * gx + gy * gy + gz * gz);
@@ -157,7 +157,7 @@ __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, 
   dp[0] = dp[1] = dp[2] = dp[3] = 0.0f;
   switch (s.shape) {
     case UPF_SHAPE_SPHERE: {
-      const float r = __builtin_sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
+      const float r = sqrt_hw(fmaf(z, z, fmaf(y, y, x * x)));
       const float inv = r > 0.0f ? 1.0f / r : 0.0f;
       g[0] = x * inv; g[1] = y * inv; g[2] = z * inv;
       dp[0] = -1.0f;
@@ -167,7 +167,7 @@ __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, 
       const float q[3] = {fabsf(x) - s.sp[0], fabsf(y) - s.sp[1], fabsf(z) - s.sp[2]};
       const float sg[3] = {x < 0.0f ? -1.0f : 1.0f, y < 0.0f ? -1.0f : 1.0f, z < 0.0f ? -1.0f : 1.0f};
       const float m[3] = {fmaxf(q[0], 0.0f), fmaxf(q[1], 0.0f), fmaxf(q[2], 0.0f)};
-      const float outside = __builtin_sqrtf(fmaf(m[2], m[2], fmaf(m[1], m[1], m[0] * m[0])));
+      const float outside = sqrt_hw(fmaf(m[2], m[2], fmaf(m[1], m[1], m[0] * m[0])));
       const float qm = fmaxf(q[0], fmaxf(q[1], q[2]));
       const float inside = fminf(qm, 0.0f);
       float dq[3];
@@ -184,9 +184,9 @@ __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, 
       return outside + inside - s.sp[3];
     }
     case UPF_SHAPE_TORUS: {
-      const float rho = __builtin_sqrtf(fmaf(z, z, x * x));
+      const float rho = sqrt_hw(fmaf(z, z, x * x));
       const float ring = rho - s.sp[0];
-      const float d = __builtin_sqrtf(fmaf(y, y, ring * ring));
+      const float d = sqrt_hw(fmaf(y, y, ring * ring));
       const float invd = d > 0.0f ? 1.0f / d : 0.0f;
       const float invr = rho > 0.0f ? 1.0f / rho : 0.0f;
       const float a = ring * invd;
@@ -198,7 +198,7 @@ __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, 
     case UPF_SHAPE_CAPSULE: {
       const float h2 = s.sp[0] * 0.5f;
       const float zc = z - fminf(fmaxf(z, -h2), h2);
-      const float d = __builtin_sqrtf(fmaf(zc, zc, fmaf(y, y, x * x)));
+      const float d = sqrt_hw(fmaf(zc, zc, fmaf(y, y, x * x)));
       const float inv = d > 0.0f ? 1.0f / d : 0.0f;
       g[0] = x * inv; g[1] = y * inv; g[2] = zc * inv;
       dp[0] = z > h2 ? -0.5f * zc * inv : (z < -h2 ? 0.5f * zc * inv : 0.0f);

@@ -260,7 +260,7 @@ __device__ __forceinline__ float cell_read(float qx, const float2* p) {
         const float d = v.x - qx;
         best = fminf(best, fmaf(d, d, v.y));
       }
-  return __builtin_sqrtf(best);
+  return sqrt_hw(best);
 }
 
 // One layer over R rows that share x and z (y[r] per row).  The octave loops are

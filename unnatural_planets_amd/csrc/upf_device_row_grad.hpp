@@ -140,7 +140,7 @@ __device__ __forceinline__ float cell_read_d(float qx, const float4* p, float& d
         const float d2 = fmaf(d, d, v.y);
         if (d2 < best) { best = d2; bp = v; }
       }
-  const float r = __builtin_sqrtf(best);
+  const float r = sqrt_hw(best);
   const float inv = r > 0.0f ? 1.0f / r : 0.0f;
   dx = (qx - bp.x) * inv;
   dy = bp.z * inv;

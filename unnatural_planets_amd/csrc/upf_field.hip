@@ -53,7 +53,11 @@ __global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g,
 // workgroup covers 64 x nodes of kRowPairs row pairs.
 constexpr int kRowPairs = 4;
 
-__global__ __launch_bounds__(256) void k_eval_grid_row(const DSpec s, const DGrid g, float* __restrict__ out,
+// Six waves per SIMD (at most 80 VGPRs, from 87 unconstrained: a 16-byte spill on a cold
+// path): the kernel keeps the VALU busy ~93% of the time at five waves and the extra
+// wave fills the issue slots the others leave; 1.80 -> 1.63 ms on the 512^3 torus
+// workload (seven waves 1.65 ms, eight 1.85 ms with spills in the hot loop).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_eval_grid_row(const DSpec s, const DGrid g, float* __restrict__ out,
                                                        uint64_t* __restrict__ S, float iso) {
   __shared__ float4 s_slice[kRowPairs][kRowSlice];
   __shared__ float4 s_gtab[16];
@@ -141,9 +145,11 @@ __global__ __launch_bounds__(256) void k_adjoint(const DSpec s, const DGrid g, c
 
 // Fused forward + adjoint over a grid with wave-cooperative rows: persistent waves
 // stride over 64-node row segments (so a block's θ partials are flushed once at the
-// end); lanes past the end of a row stand in for its last node with dLdf = 0.
+// end); lanes past the end of a row stand in for its last node with dLdf = 0.  Five
+// waves per SIMD (at most 96 VGPRs, from 109): 9.94 -> 9.68 ms on the 512^3 torus
+// workload with dL/dtheta (six waves, 80 VGPRs, spill and gain nothing).
 template <bool THETA>
-__global__ __launch_bounds__(256) void k_adjoint_rows(const DSpec s, const DGrid g, int64_t items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_adjoint_rows(const DSpec s, const DGrid g, int64_t items,
                                                       const float* __restrict__ dLdf, float* __restrict__ fout,
                                                       float* __restrict__ dfdx, float* __restrict__ dLdtheta) {
   __shared__ float4 s_slice[4][kRowSlice];

@@ -49,6 +49,8 @@ UPF_HD Grad operator/(Grad a, float b) { return {a.v / b, a.dx / b, a.dy / b, a.
 UPF_HD Grad operator+(float a, Grad b) { return {a + b.v, b.dx, b.dy, b.dz}; }
 UPF_HD Grad operator-(float a, Grad b) { return {a - b.v, -b.dx, -b.dy, -b.dz}; }
 UPF_HD Grad operator*(float a, Grad b) { return {a * b.v, a * b.dx, a * b.dy, a * b.dz}; }
+// Correctly rounded square root (the compositions below rely on sqrt(x^2 + y^2 + z^2)
+// >= |z|, e.g. acos(z / r) in the Mandelbulb).
 UPF_HD float sqrt_f(float a) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_sqrtf(a);
