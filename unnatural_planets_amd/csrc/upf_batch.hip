@@ -133,25 +133,25 @@ __global__ void k_batch_meshes(const BChunk* __restrict__ ch, int n_chunks, cons
 
 __global__ __launch_bounds__(256) void k_batch_emit(const BChunk* __restrict__ ch, int n_chunks,
                                                     const float* __restrict__ field, const uint64_t* __restrict__ S,
-                                                    const uint32_t* __restrict__ counts,
                                                     const uint32_t* __restrict__ voff, const uint2* __restrict__ act,
                                                     const uint4* __restrict__ total, float* __restrict__ pos,
                                                     float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
                                                     int64_t cap_t) {
-  __shared__ Cross s_nb[4][8];
+  __shared__ EmitPass s_pass[4];
   const int wave = uniform(threadIdx.x >> 6);
-  const int64_t n_active = (int64_t)total->z;
-  const int64_t stride = (int64_t)gridDim.x * 4;
-  for (int64_t a = (int64_t)blockIdx.x * 4 + wave; a < n_active; a += stride) {
-    const uint2 entry = act[a];
-    const int64_t g = (int64_t)(uint32_t)uniform((int)entry.x);
+  // contiguous runs of the active-group list per wave, in passes of one chunk each
+  const int64_t n_active = (int64_t)total->z, nw = (int64_t)gridDim.x * 4;
+  const int64_t per = (n_active + nw - 1) / nw;
+  int64_t a = ((int64_t)blockIdx.x * 4 + wave) * per;
+  const int64_t a1 = min(a + per, n_active);
+  while (a < a1) {
+    const int64_t g = (int64_t)(uint32_t)uniform((int)act[a].x);
     const int c = uniform(find_chunk<&BChunk::group_off>(ch, n_chunks, g));
     const BChunk& C = ch[c];
     const uint32_t vbase = voff[C.group_off];
-    mc_emit_group(field + C.node_off, C.md, C.grid, C.iso, S + C.group_off, counts + C.group_off,
-                  voff + C.group_off, (uint32_t)(g - C.group_off), (uint32_t)uniform((int)entry.y), vbase,
-                  s_nb[wave], pos, nrm, idx, cap_v, cap_t);
-    __builtin_amdgcn_wave_barrier();
+    a += mc_emit_pass(field + C.node_off, C.md, C.grid, C.iso, S + C.group_off, voff + C.group_off, act, a,
+                      (int)min<int64_t>(kEmitPassGroups, a1 - a), (uint32_t)C.group_off,
+                      (uint32_t)(C.group_off + C.md.groups), vbase, s_pass[wave], pos, nrm, idx, cap_v, cap_t);
   }
 }
 
@@ -243,7 +243,7 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
   int st = mc_scan_launch(counts, b->groups, tiles, total, voff, act, toff, s);
   if (st) return st;
   hipLaunchKernelGGL(k_batch_meshes, dim3((b->n + 63) / 64), dim3(64), 0, s, b->dev, b->n, voff, toff, total, meshes);
-  hipLaunchKernelGGL(k_batch_emit, dim3(grid), dim3(256), 0, s, b->dev, b->n, field, S, counts, voff, act, total, pos,
+  hipLaunchKernelGGL(k_batch_emit, dim3(grid), dim3(256), 0, s, b->dev, b->n, field, S, voff, act, total, pos,
                      nrm, idx, cap_v, cap_t);
   UPF_LAUNCH_CHECK();
   return UPF_OK;

@@ -55,24 +55,24 @@ __global__ __launch_bounds__(256) void k_mc_classify(const float* __restrict__ f
 
 __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, const McDims d, const DGrid g,
                                                  float iso, const uint64_t* __restrict__ S,
-                                                 const uint32_t* __restrict__ counts,
                                                  const uint32_t* __restrict__ voff, const uint2* __restrict__ act,
                                                  const uint4* __restrict__ total, float* __restrict__ pos,
                                                  float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
                                                  int64_t cap_t, int64_t* __restrict__ counts_out) {
-  __shared__ Cross s_nb[4][8];
+  __shared__ EmitPass s_pass[4];
   const int wave = uniform(threadIdx.x >> 6);
   if (counts_out && blockIdx.x == 0 && threadIdx.x < 2)
     counts_out[threadIdx.x] = (int64_t)(threadIdx.x ? total->y : total->x);
-  // grid-stride over the active groups: the grid is sized to keep the CUs full, not to
-  // the (unknown on the host) number of active groups, so no wave launches only to exit
-  const int64_t nact = (int64_t)total->z;
-  for (int64_t a = (int64_t)blockIdx.x * 4 + wave; a < nact; a += (int64_t)gridDim.x * 4) {
-    const uint2 entry = act[a];
-    mc_emit_group(f, d, g, iso, S, counts, voff, (uint32_t)uniform((int)entry.x), (uint32_t)uniform((int)entry.y),
-                  0u, s_nb[wave], pos, nrm, idx, cap_v, cap_t);
-    __builtin_amdgcn_wave_barrier();  // this group's reads of s_nb before the next group's writes
-  }
+  // each wave takes a contiguous run of the active-group list (neighbouring groups share
+  // sign words and field values in cache) in passes of up to kEmitPassGroups groups; the
+  // grid is sized to keep the CUs full, the list's length is known only on the device
+  const int64_t nact = (int64_t)total->z, nw = (int64_t)gridDim.x * 4;
+  const int64_t per = (nact + nw - 1) / nw;
+  int64_t a = ((int64_t)blockIdx.x * 4 + wave) * per;
+  const int64_t a1 = min(a + per, nact);
+  while (a < a1)
+    a += mc_emit_pass(f, d, g, iso, S, voff, act, a, (int)min<int64_t>(kEmitPassGroups, a1 - a), 0u, 0xFFFFFFFFu, 0u,
+                      s_pass[wave], pos, nrm, idx, cap_v, cap_t);
 }
 
 // Classification from the sign masks alone (the field kernel wrote them).  One lane per
@@ -82,11 +82,6 @@ __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, co
 // (the 8 words agree on every cell lane) have none; each remaining group of the wave is
 // then taken by the whole wave, one cell per lane, with the words broadcast from the
 // group's lane.
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
-  return ((uint64_t)hi << 32) | lo;
-}
 
 __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __restrict__ S, const McDims d,
                                                            uint32_t* __restrict__ counts) {
@@ -347,7 +342,7 @@ int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, 
   // 32 blocks of 4 waves per CU (more than stay resident), each looping over the list
   max_active = std::min<int64_t>(max_active, (int64_t)device_cus() * 4 * kEmitBlocksPerCu);
   hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((max_active + 3) / 4)), dim3(256), 0, s, d_field, L.d, g, iso, P.S,
-                     P.counts, P.voff, P.act, P.total, pos, nrm, idx, cap_v, cap_t, counts_out);
+                     P.voff, P.act, P.total, pos, nrm, idx, cap_v, cap_t, counts_out);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }

@@ -44,6 +44,11 @@ constexpr uint64_t kEdgeOwner = mc_ct::pack_owner();
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
 
 // packed per-group counts: vertices (bits 0-7, <= 192), triangles (8-16, <= 320), active (17)
 __device__ __forceinline__ uint32_t pack_counts(int nv, int nt) {
@@ -101,7 +106,8 @@ __device__ __forceinline__ void mc_classify_column(const float* __restrict__ f, 
 
 // ------------------------------------------------------------------ emit
 struct Cross {
-  uint64_t bx, by, bz;
+  uint64_t bx, by, bz;  // crossing edges along x, y, z (bit = owning node of the row segment)
+  uint64_t w;           // the sign word itself (bit = f < iso)
   uint32_t voff, pad;
 };
 
@@ -121,6 +127,7 @@ __device__ __forceinline__ Cross group_crossings(const uint64_t* __restrict__ S,
   c.bx = (s ^ ((s >> 1) | (snext << 63))) & xmask;
   c.by = (j + 1 < d.ny) ? ((s ^ S[gg + d.gpr]) & vmask) : 0ull;
   c.bz = (k + 1 < d.nz) ? ((s ^ S[gg + (int64_t)d.gpr * d.ny]) & vmask) : 0ull;
+  c.w = s;
   c.voff = voff[gg];
   c.pad = 0;
   return c;
@@ -134,58 +141,137 @@ __device__ __forceinline__ float grad_axis(int idx, int nmax, float fp, float f0
   return (fp - fm) / (2.0f * sp);
 }
 
-// One wave emits active group gid (local to its grid): vertices at voff (global ids,
-// stored at pos/nrm[id]) and triangles from toff (global), with index values made local
-// by subtracting vbase.  Writes at or past cap_v / cap_t are dropped (the caller sized
-// the buffers from the counts, or checks the counts afterwards).  s_nb: 8 LDS entries
-// private to this wave.  Every field value a lane needs -- its cell's 8 corners and the
-// central-difference neighbours of its node and of its three edges' far ends -- is
-// loaded up front with clamped (in-bounds) indices, in the same round trip as the
-// counts and sign masks.
-__device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
-                                              const uint64_t* __restrict__ S, const uint32_t* __restrict__ counts,
-                                              const uint32_t* __restrict__ voff, uint32_t gid, uint32_t toff,
-                                              uint32_t vbase, Cross* s_nb, float* __restrict__ pos,
-                                              float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
-                                              int64_t cap_t) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t row = gid / (uint32_t)d.gpr;
-  const int xb = (int)(gid - row * (uint32_t)d.gpr);
-  const int k = (int)(row / (uint32_t)d.ny);
-  const int j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
-  const int i = xb * 64 + lane;
-  const bool valid = i < d.nx;
+// Emission, one pass of a wave: up to kEmitPassGroups consecutive entries of the
+// active-group list (act[a..], all of one grid: global group ids gbase + local id below
+// glimit) whose cells with work -- a vertex on one of the node's own edges, or a case
+// code other than 0 and 255 (exactly the codes with triangles) -- number at most 64;
+// each lane takes one such cell.  A group holds 64 cells of which about a tenth cross
+// the surface, so filling the lanes with cells that have work instead of the 64 cells
+// of one group cuts the instructions per group several-fold.
+//   1. lane 8 b + s loads the crossing masks and sign word of neighbour slot s of group
+//      b (slot = gx | dy << 1 | dz << 2: the groups a cell takes edges from);
+//   2. lane b counts its group's cells with work; the pass takes the leading groups
+//      whose counts sum to at most 64 (at least one: a group has 64 cells);
+//   3. the cells are listed in LDS in group order and, within a group, in cell order;
+//   4. lane t takes listed cell t.
+// Vertex ids (voff + the crossings of the lower nodes of the row segment, x < y < z per
+// node), triangle ids (toff + the triangles of the lower cells of the group, table
+// order) and every float operation are those of one group per wave, so the mesh does
+// not depend on how the groups are split into passes.  Writes at or past cap_v / cap_t
+// are dropped; index values are made local by subtracting vbase.  Every field value a
+// lane needs -- its node's central-difference neighbours and those of its three edges'
+// far ends -- is loaded with clamped (in-bounds) indices by the lanes that own a vertex.
+// Returns the number of list entries consumed.
+constexpr int kEmitPassGroups = 8;
 
-  // field values (clamped coordinates; a clamped value is never used), loaded only by
-  // the lanes that own a vertex
-  const int ic = min(i, d.nx - 1);
-  const int64_t sy = d.nx, sz = d.nxy;
-  const int xs[4] = {max(ic - 1, 0), ic, min(ic + 1, d.nx - 1), min(ic + 2, d.nx - 1)};  // i-1 .. i+2
-  const int ys[4] = {max(j - 1, 0), j, min(j + 1, d.ny - 1), min(j + 2, d.ny - 1)};
-  const int zs[4] = {max(k - 1, 0), k, min(k + 1, d.nz - 1), min(k + 2, d.nz - 1)};
-  auto at = [&](int xi, int yi, int zi) { return f[(int64_t)zs[zi] * sz + (int64_t)ys[yi] * sy + xs[xi]]; };
-  const uint32_t c = counts[gid];
-  const int cnt_v = c & 0xFF, cnt_t = (c >> 8) & 0x1FF;
+struct EmitPass {  // one wave's LDS
+  Cross nb[kEmitPassGroups][8];
+  uint32_t toff[kEmitPassGroups];
+  int xb[kEmitPassGroups], j[kEmitPassGroups], k[kEmitPassGroups], pre[kEmitPassGroups];
+  uint16_t item[64];  // (group << 6) | cell
+};
 
-  // crossing masks of the (up to) 8 groups a cell of this group takes edges from:
-  // slot = gx | dy << 1 | dz << 2, gx = 1 for the next group along x
-  if (lane < 8) {
-    const int gx = lane & 1, dy = (lane >> 1) & 1, dz = lane >> 2;
-    if (xb + gx < d.gpr && j + dy < d.ny && k + dz < d.nz) s_nb[lane] = group_crossings(S, voff, d, xb + gx, j + dy, k + dz);
-  }
+__device__ __forceinline__ void emit_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
-  const uint64_t lt = (1ull << lane) - 1;
+__device__ __forceinline__ int mc_emit_pass(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
+                                            const uint64_t* __restrict__ S, const uint32_t* __restrict__ voff,
+                                            const uint2* __restrict__ act, int64_t a, int ng, uint32_t gbase,
+                                            uint32_t glimit, uint32_t vbase, EmitPass& P, float* __restrict__ pos,
+                                            float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
+                                            int64_t cap_t) {
+  const int lane = threadIdx.x & 63;
+  // entries of another grid end the pass (the list is sorted, so a grid's entries are
+  // contiguous); entry a is the caller's grid, so every pass takes at least one group
+  {
+    const bool mine = lane < ng && act[a + lane].x < glimit;
+    ng = max(__popcll(ballot64(mine) & 0xFFull), 1);
+  }
+  // ---- 1. neighbour crossing records
+  {
+    const int b = lane >> 3, sl = lane & 7;
+    if (b < ng) {
+      const uint2 e = act[a + b];
+      const uint32_t gid = e.x - gbase;
+      const uint32_t row = gid / (uint32_t)d.gpr;
+      const int xb = (int)(gid - row * (uint32_t)d.gpr);
+      const int k = (int)(row / (uint32_t)d.ny);
+      const int j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
+      const int gx = sl & 1, dy = (sl >> 1) & 1, dz = sl >> 2;
+      Cross c;
+      c.bx = c.by = c.bz = c.w = 0ull;
+      c.voff = c.pad = 0u;
+      if (xb + gx < d.gpr && j + dy < d.ny && k + dz < d.nz) c = group_crossings(S, voff, d, xb + gx, j + dy, k + dz);
+      P.nb[b][sl] = c;
+      if (sl == 0) {
+        P.toff[b] = e.y;
+        P.xb[b] = xb;
+        P.j[b] = j;
+        P.k[b] = k;
+      }
+    }
+  }
+  emit_sync();
+  // ---- 2. cells with work per group (lane b), and how many groups this pass takes
+  uint64_t items = 0ull;
+  int nit = 0;
+  if (lane < ng) {
+    const Cross* n = P.nb[lane];
+    const int xb = P.xb[lane], j = P.j[lane], k = P.k[lane];
+    const uint64_t wa = n[0].w, wax = n[1].w, way = n[2].w, waxy = n[3].w;
+    const uint64_t wb = n[4].w, wbx = n[5].w, wby = n[6].w, wbxy = n[7].w;
+    const uint64_t wa1 = (wa >> 1) | (wax << 63), way1 = (way >> 1) | (waxy << 63);
+    const uint64_t wb1 = (wb >> 1) | (wbx << 63), wby1 = (wby >> 1) | (wbxy << 63);
+    const int nvx = min(64, d.nx - 1 - xb * 64);
+    const uint64_t xmask = nvx >= 64 ? ~0ull : (nvx > 0 ? ((1ull << nvx) - 1) : 0ull);
+    const uint64_t cells = (j + 1 < d.ny && k + 1 < d.nz) ? xmask : 0ull;
+    const uint64_t any = (wa | wa1 | way | way1 | wb | wb1 | wby | wby1) & cells;
+    const uint64_t all = (wa & wa1 & way & way1 & wb & wb1 & wby & wby1) & cells;
+    items = (any & ~all) | n[0].bx | n[0].by | n[0].bz;
+    nit = __popcll(items);
+  }
+  int incl = nit;
+#pragma unroll
+  for (int o = 1; o < kEmitPassGroups; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  const int pre = incl - nit;
+  const int npass = __popcll(ballot64(lane < ng && incl <= 64));  // >= 1
+  const int nitems = __shfl(incl, npass - 1, 64);
+  if (lane < npass) P.pre[lane] = pre;
+  // ---- 3. the list of cells
+  for (int b = 0; b < npass; b++) {
+    const uint64_t it = readlane64(items, b);
+    const int p = __builtin_amdgcn_readlane(pre, b);
+    if ((it >> lane) & 1ull) P.item[p + __popcll(it & ((1ull << lane) - 1))] = (uint16_t)((b << 6) | lane);
+  }
+  emit_sync();
+  // ---- 4. one listed cell per lane
+  const bool act_lane = lane < nitems;
+  const int e = act_lane ? (int)P.item[lane] : 0;
+  const int b = e >> 6, l = e & 63;
+  const Cross* nb = P.nb[b];
+  const int xb = P.xb[b], j = P.j[b], k = P.k[b];
+  const int i = xb * 64 + l;
+  const uint64_t lt = (1ull << l) - 1;
 
-  // ---- vertices
-  if (cnt_v) {
-    const Cross cc = s_nb[0];
-    uint32_t vid = cc.voff + (uint32_t)(__popcll(cc.bx & lt) + __popcll(cc.by & lt) + __popcll(cc.bz & lt));
-    const uint64_t bits[3] = {cc.bx, cc.by, cc.bz};
-    const bool any = ((cc.bx | cc.by | cc.bz) >> lane) & 1ull;
-    if (any) {
+  // vertices on the node's own edges
+  {
+    const Cross cc = nb[0];
+    const bool own = act_lane && (((cc.bx | cc.by | cc.bz) >> l) & 1ull);
+    if (own) {
+      uint32_t vid = cc.voff + (uint32_t)(__popcll(cc.bx & lt) + __popcll(cc.by & lt) + __popcll(cc.bz & lt));
+      const uint64_t bits[3] = {cc.bx, cc.by, cc.bz};
+      const int ic = min(i, d.nx - 1);
+      const int64_t sy = d.nx, sz = d.nxy;
+      const int xs[4] = {max(ic - 1, 0), ic, min(ic + 1, d.nx - 1), min(ic + 2, d.nx - 1)};  // i-1 .. i+2
+      const int ys[4] = {max(j - 1, 0), j, min(j + 1, d.ny - 1), min(j + 2, d.ny - 1)};
+      const int zs[4] = {max(k - 1, 0), k, min(k + 1, d.nz - 1), min(k + 2, d.nz - 1)};
+      auto at = [&](int xi, int yi, int zi) { return f[(int64_t)zs[zi] * sz + (int64_t)ys[yi] * sy + xs[xi]]; };
       const float c0 = at(1, 1, 1), c1 = at(2, 1, 1), c3 = at(1, 2, 1), c4 = at(1, 1, 2);
       float c2 = 0.f, c5 = 0.f, c7 = 0.f;
       float xm = 0.f, ym = 0.f, zm = 0.f, x2 = 0.f, xym = 0.f, xzm = 0.f, y2 = 0.f, yxm = 0.f, yzm = 0.f;
@@ -210,7 +296,7 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
       }
 #pragma unroll
       for (int ax = 0; ax < 3; ax++) {
-        if (!((bits[ax] >> lane) & 1ull)) continue;
+        if (!((bits[ax] >> l) & 1ull)) continue;
         const float f1 = f1s[ax];
         const float t = (iso - f0) / (f1 - f0);
         const float p1a = node_coord(g.o[ax], g.s[ax], ijk[ax] + 1);
@@ -251,34 +337,28 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
     }
   }
 
-  // ---- triangles
-  if (cnt_t) {
-    const bool cell = valid && i + 1 < d.nx && j + 1 < d.ny && k + 1 < d.nz;
-    // the case code from the sign words (bit = f < iso) of the cell's 4 corner rows and
-    // their +x neighbour groups: wave-uniform 64-bit loads instead of 8 values per lane
-    const bool nx1 = xb + 1 < d.gpr, hy = j + 1 < d.ny, hz = k + 1 < d.nz;
-    const int64_t gr = (int64_t)gid, plane = (int64_t)d.ny * d.gpr;
-    const uint64_t wa = S[gr], wax = nx1 ? S[gr + 1] : 0ull;
-    const uint64_t way = hy ? S[gr + d.gpr] : 0ull, waxy = (hy && nx1) ? S[gr + d.gpr + 1] : 0ull;
-    const uint64_t wb = hz ? S[gr + plane] : 0ull, wbx = (hz && nx1) ? S[gr + plane + 1] : 0ull;
-    const uint64_t wby = (hz && hy) ? S[gr + plane + d.gpr] : 0ull;
-    const uint64_t wbxy = (hz && hy && nx1) ? S[gr + plane + d.gpr + 1] : 0ull;
-    const uint64_t wa1 = (wa >> 1) | (wax << 63), way1 = (way >> 1) | (waxy << 63);
-    const uint64_t wb1 = (wb >> 1) | (wbx << 63), wby1 = (wby >> 1) | (wbxy << 63);
-    int code = 0;
-    if (cell)
-      code = (int)((wa >> lane) & 1ull) | ((int)((wa1 >> lane) & 1ull) << 1) | ((int)((way1 >> lane) & 1ull) << 2) |
-             ((int)((way >> lane) & 1ull) << 3) | ((int)((wb >> lane) & 1ull) << 4) |
-             ((int)((wb1 >> lane) & 1ull) << 5) | ((int)((wby1 >> lane) & 1ull) << 6) |
-             ((int)((wby >> lane) & 1ull) << 7);
-    const int nt = upf_mc_ntris[code];
-    int inc = nt;
+  // triangles of the cell: the case code from the sign words of its 4 corner rows and
+  // their +x neighbour groups; ids from the in-group prefix of the triangle counts
+  int code = 0;
+  if (act_lane && i + 1 < d.nx && j + 1 < d.ny && k + 1 < d.nz) {
+    // corner at x = i: bit l of the slot's word; at x = i + 1: bit l + 1, or bit 0 of the
+    // next group's word (slot + 1) for the last cell of the segment
+    auto bit = [&](int s0) { return (int)((nb[s0].w >> l) & 1ull); };
+    auto bit1 = [&](int s0) { return l < 63 ? (int)((nb[s0].w >> (l + 1)) & 1ull) : (int)(nb[s0 + 1].w & 1ull); };
+    code = bit(0) | (bit1(0) << 1) | (bit1(2) << 2) | (bit(2) << 3) | (bit(4) << 4) | (bit1(4) << 5) |
+           (bit1(6) << 6) | (bit(6) << 7);
+  }
+  const int nt = upf_mc_ntris[code];
+  int inc = nt;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int v = __shfl_up(inc, o, 64);
-      if (lane >= o) inc += v;
-    }
-    const uint32_t tid0 = toff + (uint32_t)(inc - nt);
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
+  }
+  const int exc = inc - nt;
+  const int seg = __shfl(exc, P.pre[b], 64);  // the triangles of the pass's earlier groups
+  if (nt) {
+    const uint32_t tid0 = P.toff[b] + (uint32_t)(exc - seg);
     // the cell's triangulation row in one 16-byte load; edge owners from the immediate
     const int4 row = *reinterpret_cast<const int4*>(&c_tris16.v[code][0]);
     const uint32_t rw[4] = {(uint32_t)row.x, (uint32_t)row.y, (uint32_t)row.z, (uint32_t)row.w};
@@ -289,19 +369,21 @@ __device__ __forceinline__ void mc_emit_group(const float* __restrict__ f, const
 #pragma unroll
       for (int v = 0; v < 3; v++) {
         const int q = 3 * t + v;
-        const int e = (int)(int8_t)(uint8_t)(rw[q >> 2] >> (8 * (q & 3)));
-        const int own = (int)((kEdgeOwner >> (5 * e)) & 31u);
-        const int ol = lane + (own & 1);
+        const int ed = (int)(int8_t)(uint8_t)(rw[q >> 2] >> (8 * (q & 3)));
+        const int own = (int)((kEdgeOwner >> (5 * ed)) & 31u);
+        const int ol = l + (own & 1);
         const int slot = (ol >> 6) | (((own >> 1) & 1) << 1) | (((own >> 2) & 1) << 2);
         const int axis = own >> 3;
-        const Cross cc = s_nb[slot];
-        const int l = ol & 63;
-        const uint64_t m = (1ull << l) - 1;
+        const Cross cc = nb[slot];
+        const int lo = ol & 63;
+        const uint64_t m = (1ull << lo) - 1;
         uint32_t id = cc.voff + (uint32_t)(__popcll(cc.bx & m) + __popcll(cc.by & m) + __popcll(cc.bz & m));
-        if (axis >= 1) id += (uint32_t)((cc.bx >> l) & 1ull);
-        if (axis == 2) id += (uint32_t)((cc.by >> l) & 1ull);
+        if (axis >= 1) id += (uint32_t)((cc.bx >> lo) & 1ull);
+        if (axis == 2) id += (uint32_t)((cc.by >> lo) & 1ull);
         if ((int64_t)tid < cap_t) idx[3 * (size_t)tid + v] = id - vbase;
       }
     }
   }
+  emit_sync();  // this pass's reads of P before the next pass's writes
+  return npass;
 }
