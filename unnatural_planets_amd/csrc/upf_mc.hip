@@ -78,14 +78,19 @@ __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, co
 // Classification from the sign masks alone (the field kernel wrote them).  One lane per
 // group: its crossing masks come from S of the group, of its +x neighbour group and of
 // the +y / +z rows (8 words), so vertex counts are a few 64-bit operations per lane.
-// Triangle counts need the 64 cell codes: groups whose cells all have uniform corners
-// (the 8 words agree on every cell lane) have none; each remaining group of the wave is
-// then taken by the whole wave, one cell per lane, with the words broadcast from the
-// group's lane.
+// Triangle counts need the codes of the mixed cells (corners not all on one side; the
+// other codes have no triangles).  Each lane lists its group's mixed cells in LDS; in
+// passes of at most 64 cells (a group's cells never split), lane t takes listed cell t,
+// builds its code from the group's 8 words (in LDS) and adds the code's triangle count
+// to the group's LDS counter.  A wave with ~6 active groups of ~7 mixed cells each needs
+// one pass, instead of one wave-wide step per active group.
 
 __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __restrict__ S, const McDims d,
                                                            uint32_t* __restrict__ counts) {
   __shared__ uint8_t s_ntris[256];
+  __shared__ uint64_t s_w[4][64][8];
+  __shared__ uint32_t s_nt[4][64];
+  __shared__ uint16_t s_item[4][64];
   s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
   __syncthreads();
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -119,22 +124,52 @@ __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __res
   }
   const uint64_t any = (a | a1 | ay | ay1 | b | b1 | by | by1) & cells;
   const uint64_t all = (a & a1 & ay & ay1 & b & b1 & by & by1) & cells;
-  int ntot = 0;
-  uint64_t work = ballot64(in && any != 0ull && all != cells);
-  while (work) {
-    const int src = __ffsll((unsigned long long)work) - 1;
-    work &= work - 1;
-    const uint64_t A = readlane64(a, src), A1 = readlane64(a1, src), AY = readlane64(ay, src);
-    const uint64_t AY1 = readlane64(ay1, src), B = readlane64(b, src), B1 = readlane64(b1, src);
-    const uint64_t BY = readlane64(by, src), BY1 = readlane64(by1, src), C = readlane64(cells, src);
-    const int code = (int)((A >> lane) & 1) | (int)((A1 >> lane) & 1) << 1 | (int)((AY1 >> lane) & 1) << 2 |
-                     (int)((AY >> lane) & 1) << 3 | (int)((B >> lane) & 1) << 4 | (int)((B1 >> lane) & 1) << 5 |
-                     (int)((BY1 >> lane) & 1) << 6 | (int)((BY >> lane) & 1) << 7;
-    const int nt = ((C >> lane) & 1ull) ? s_ntris[code] : 0;
-    const int t = __popcll(ballot64(nt & 1)) + 2 * __popcll(ballot64(nt & 2)) + 4 * __popcll(ballot64(nt & 4));
-    if (lane == src) ntot = t;
+  const uint64_t mixed = any & ~all;
+  const int nmix = __popcll(mixed);
+  const int wv = threadIdx.x >> 6;
+  // the words in code-bit order: bit q of cell c's code is bit c of w[q]
+  uint64_t* w = s_w[wv][lane];
+  w[0] = a; w[1] = a1; w[2] = ay1; w[3] = ay; w[4] = b; w[5] = b1; w[6] = by1; w[7] = by;
+  s_nt[wv][lane] = 0u;
+  int incl = nmix;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
   }
-  if (in) counts[g] = pack_counts(nv, ntot);
+  const int total = __shfl(incl, 63, 64);
+  for (int base = 0; base < total;) {
+    // the groups whose cells lie within [base, base + 64); the first group left starts at
+    // base and has at most 64 cells, so every pass takes one
+    const bool take = nmix > 0 && incl - nmix >= base && incl <= base + 64;
+    const uint64_t tk = ballot64(take);
+    const int end = __shfl(incl, 63 - __clzll((long long)tk), 64);
+    if (take) {
+      uint64_t m = mixed;
+      int p = incl - nmix - base;
+      while (m) {
+        s_item[wv][p++] = (uint16_t)((lane << 6) | (__ffsll((unsigned long long)m) - 1));
+        m &= m - 1;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < end - base) {
+      const int e = s_item[wv][lane];
+      const int grp = e >> 6, c = e & 63;
+      const uint64_t* gw = s_w[wv][grp];
+      int code = 0;
+#pragma unroll
+      for (int q = 0; q < 8; q++) code |= (int)((gw[q] >> c) & 1ull) << q;
+      atomicAdd(&s_nt[wv][grp], (uint32_t)s_ntris[code]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    base = end;
+  }
+  if (in) counts[g] = pack_counts(nv, (int)s_nt[wv][lane]);
 }
 
 // ------------------------------------------------------------------ scan
