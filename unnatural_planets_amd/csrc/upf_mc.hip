@@ -31,26 +31,36 @@ namespace upf {
 
 #include "upf_mc_device.hpp"
 
-// ------------------------------------------------------------------ classify / emit kernels
-// Work item w = (z block, 4-row block, group column), column fastest, spread over the
-// XCDs in contiguous ranges: the groups of one row (one cache line of S and of counts)
-// and the row blocks that share a boundary row then meet in the same L2.
-__global__ __launch_bounds__(256) void k_mc_classify(const float* __restrict__ f, const McDims d, float iso,
-                                                     int64_t items, uint64_t* __restrict__ S,
-                                                     uint32_t* __restrict__ counts) {
-  __shared__ uint8_t s_ntris[256];
-  s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
-  __syncthreads();
-  const int64_t w = xcd_item(blockIdx.x, items);
-  if (w >= items) return;
-  const int rbs = (d.ny + 3) / 4;
-  const int xb = (int)(w % d.gpr);
-  const int64_t r = w / d.gpr;
-  const int rb = (int)(r % rbs), zb = (int)(r / rbs);
-  const int j = uniform(rb * 4 + (threadIdx.x >> 6));
-  if (j >= d.ny) return;
-  const int k0 = zb * kMcClassZ;
-  mc_classify_column(f, d, iso, xb, j, k0, min(k0 + kMcClassZ, d.nz), s_ntris, S, counts);
+// ------------------------------------------------------------------ sign words / emit kernels
+// Sign words of a stored field: bit x of S[g] = (node x of the group's row segment
+// exists and f < iso), the words the field kernel stores when it evaluates a grid, so a
+// stored field is classified by the same sign-word kernel as a fused one.  One wave per
+// grid row and run of kSignGroups consecutive groups (blockIdx.y): kSignGroups loads in
+// flight per lane, one ballot per group, and lane u stores the word of group u.  The
+// field is read once, 4 B per node.
+constexpr int kSignGroups = 8;
+
+__global__ __launch_bounds__(256) void k_mc_signs(const float* __restrict__ f, const McDims d, float iso,
+                                                  int64_t rows, uint64_t* __restrict__ S) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;  // whole waves
+  const int lane = threadIdx.x & 63;
+  const int g0 = blockIdx.y * kSignGroups;
+  const float* fr = f + row * d.nx;
+  float v[kSignGroups];
+#pragma unroll
+  for (int u = 0; u < kSignGroups; u++) {
+    const int x = (g0 + u) * 64 + lane;
+    v[u] = x < d.nx ? fr[x] : 0.0f;
+  }
+  uint64_t mine = 0ull;
+#pragma unroll
+  for (int u = 0; u < kSignGroups; u++) {
+    const int x = (g0 + u) * 64 + lane;
+    const uint64_t m = ballot64(x < d.nx && v[u] < iso);
+    if (lane == u) mine = m;
+  }
+  if (lane < kSignGroups && g0 + lane < d.gpr) S[row * d.gpr + g0 + lane] = mine;
 }
 
 __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, const McDims d, const DGrid g,
@@ -328,11 +338,11 @@ int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws
   const McLayout L = mc_layout(g);
   if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
   const McPtrs P = mc_ptrs(L, d_ws);
-  const int64_t items = (int64_t)L.d.gpr * ((L.d.ny + 3) / 4) * ((L.d.nz + kMcClassZ - 1) / kMcClassZ);
-  hipLaunchKernelGGL(k_mc_classify, dim3((unsigned)xcd_grid(items)), dim3(256), 0, s, d_field, L.d, iso, items, P.S,
-                     P.counts);
+  const int64_t rows = (int64_t)L.d.ny * L.d.nz;
+  hipLaunchKernelGGL(k_mc_signs, dim3((unsigned)((rows + 3) / 4), (unsigned)((L.d.gpr + kSignGroups - 1) / kSignGroups)),
+                     dim3(256), 0, s, d_field, L.d, iso, rows, P.S);
   UPF_LAUNCH_CHECK();
-  return mc_scan_launch(P.counts, L.d.groups, P.tiles, P.total, P.voff, P.act, nullptr, s);
+  return mc_classify_signs_scan(g, d_ws, ws_bytes, s);
 }
 
 int mc_scan_launch(const uint32_t* counts, int64_t groups, uint4* tiles, uint4* total, uint32_t* voff, uint2* act,
