@@ -218,8 +218,9 @@ def bench_material(upf, S, torch, reps: int):
 
 
 def bench_batch(upf, S, torch):
-    """The uneven 64-chunk batch (committed manifest, seed 0): per-chunk launches
-    (field, count with its host read-back, emit) against one autotuned, graph-captured
+    """The uneven 64-chunk batch (committed manifest, seed 0): per-chunk launches (one
+    stream-ordered upf_field_mesh_async per chunk, the fastest single-grid path: fused
+    field + classification, no host read-back) against one autotuned, graph-captured
     batched run.  Buffers are allocated before either timed region."""
     import json
 
@@ -239,23 +240,19 @@ def bench_batch(upf, S, torch):
         pos = torch.empty((nv_all, 3), dtype=torch.float32, device="cuda")
         nrm = torch.empty_like(pos)
         idx = torch.empty((nt_all, 3), dtype=torch.int32, device="cuda")
-        nv, nt = ctypes.c_int64(), ctypes.c_int64()
+        counts = torch.zeros((len(man), 2), dtype=torch.int64, device="cuda")
+        mb = S.MeshBuffers()
+        mb.positions, mb.normals, mb.indices = pos.data_ptr(), nrm.data_ptr(), idx.data_ptr()
+        mb.cap_vertices, mb.cap_triangles = nv_all, nt_all
 
         def per_chunk():
             for i, m in enumerate(man):
                 g = chunks[i].grid
-                upf.check(L.upf_field_eval_grid(ctypes.byref(chunks[i].spec), ctypes.byref(g),
-                                                ctypes.c_void_p(fields[i].data_ptr()), sp), "eval")
                 w = ws[m["n"]]
-                upf.check(L.upf_mc_count(ctypes.c_void_p(fields[i].data_ptr()), ctypes.byref(g), 0.0,
-                                         ctypes.c_void_p(w.buf.data_ptr()), w.bytes, ctypes.byref(nv),
-                                         ctypes.byref(nt), sp), "count")
-                mb = S.MeshBuffers()
-                mb.positions, mb.normals, mb.indices = pos.data_ptr(), nrm.data_ptr(), idx.data_ptr()
-                mb.cap_vertices, mb.cap_triangles = nv_all, nt_all
-                mb.n_vertices, mb.n_triangles = nv.value, nt.value
-                upf.check(L.upf_mc_emit(ctypes.c_void_p(fields[i].data_ptr()), ctypes.byref(g), 0.0, ctypes.byref(mb),
-                                        ctypes.c_void_p(w.buf.data_ptr()), w.bytes, sp), "emit")
+                upf.check(L.upf_field_mesh_async(ctypes.byref(chunks[i].spec), ctypes.byref(g), 0.0,
+                                                 ctypes.c_void_p(fields[i].data_ptr()), None, ctypes.byref(mb),
+                                                 ctypes.c_void_p(counts[i].data_ptr()),
+                                                 ctypes.c_void_p(w.buf.data_ptr()), w.bytes, sp), "field_mesh_async")
 
         def timed(fn, reps):
             fn()

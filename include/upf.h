@@ -629,7 +629,9 @@ typedef struct upf_chunk {
 typedef struct upf_batch_opts {
   int32_t field_variant;  /* upf_eval_variant                                  */
   int32_t blocks_per_cu;  /* persistent workgroups per compute unit (0 = 4)    */
-  int32_t classify_z;     /* z planes per classify item: 16, 32 or 64 (0 = 32) */
+  int32_t classify_z;     /* accepted values 0, 16, 32, 64; no effect (the
+                             classification reads the sign words the field
+                             kernel stores, one lane per group)           */
   int32_t reserved;
 } upf_batch_opts;
 

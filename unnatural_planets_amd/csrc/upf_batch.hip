@@ -3,9 +3,10 @@
 // task, sources/generator.cpp:159).
 //
 // Work items (a wavefront each), enumerated chunk after chunk:
-//   field     64 x nodes of a pair of grid rows            (chunk, xb, j / 2, k)
-//   classify  one group column over classify_z planes      (chunk, xb, j, kz)
-//   emit      one active group of the concatenated scan
+//   field     64 x nodes of a pair of grid rows            (chunk, xb, j / 2, k), which
+//             also stores the rows' sign words
+//   classify  one marching-cubes group per lane, from the sign words alone
+//   emit      packed passes over the active groups of the concatenated scan
 // A fixed grid of workgroups (blocks_per_cu per compute unit) strides over each list,
 // so small chunks share the machine with large ones and nothing waits on the host.
 // The marching-cubes groups of all chunks are concatenated, so a single scan numbers
@@ -35,7 +36,6 @@ struct BChunk {
   int64_t node_off;    // first field value of the chunk
   int64_t group_off;   // first marching-cubes group of the chunk
   int64_t row_off;     // first field item (row pairs)
-  int64_t col_off;     // first classify item
 };
 
 // chunk holding item r of a list whose per-chunk starts are at member `off`
@@ -53,8 +53,8 @@ __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ ch, int n, 
 // Six waves per SIMD at most 80 VGPRs, as the single-grid row kernel (the row variant
 // needs 86 unconstrained; 2.665 -> 2.63 ms for the 64-chunk batch).
 template <int VARIANT, bool EXT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_field(const BChunk* __restrict__ ch, int n_chunks, int64_t n_items,
-                                                     float* __restrict__ field) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_field(
+    const BChunk* __restrict__ ch, int n_chunks, int64_t n_items, float* __restrict__ field, uint64_t* __restrict__ S) {
   __shared__ float4 s_slice[4][kRowSlice];
   __shared__ float4 s_gtab[16];
   __shared__ float4 s_oct[4][kRowOctTab];
@@ -87,33 +87,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     } else {
       row_field<2>(ctx, C.spec, x, y, z, v);
     }
-    if (i < C.grid.n[0]) {
+    const bool valid = i < C.grid.n[0];
+    if (valid) {
       const int64_t o = C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + i;
       field[o] = v[0];
       if (two) field[o + C.grid.n[0]] = v[1];
     }
+    // the rows' sign words (bit = node exists and f < iso), as the single-grid kernels
+    const uint64_t m0 = ballot64(valid && v[0] < C.iso), m1 = ballot64(valid && v[1] < C.iso);
+    if (lane == 0) {
+      const int64_t w = C.group_off + ((int64_t)k * C.grid.n[1] + j) * C.gx + xb;
+      S[w] = m0;
+      if (two) S[w + C.gx] = m1;
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void k_batch_classify(const BChunk* __restrict__ ch, int n_chunks, int64_t n_items,
-                                                        int kz, const float* __restrict__ field,
-                                                        uint64_t* __restrict__ S, uint32_t* __restrict__ counts) {
+// Counts of every group of every chunk (global id g: chunk, then its local group) from
+// the sign words; lanes of one wave may belong to two chunks.
+__global__ __launch_bounds__(256) void k_batch_classify(const BChunk* __restrict__ ch, int n_chunks, int64_t groups,
+                                                        const uint64_t* __restrict__ S,
+                                                        uint32_t* __restrict__ counts) {
   __shared__ uint8_t s_ntris[256];
+  __shared__ SignPass s_pass[4];
   s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
   __syncthreads();
-  const int wave = uniform(threadIdx.x >> 6);
-  const int64_t stride = (int64_t)gridDim.x * 4;
-  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < n_items; r += stride) {
-    const int c = uniform(find_chunk<&BChunk::col_off>(ch, n_chunks, r));
-    const BChunk& C = ch[c];
-    const int64_t local = r - C.col_off;
-    const int xb = (int)(local % C.md.gpr);
-    const int64_t rest = local / C.md.gpr;
-    const int j = (int)(rest % C.md.ny);
-    const int k0 = (int)(rest / C.md.ny) * kz;
-    mc_classify_column(field + C.node_off, C.md, C.iso, xb, j, k0, min(k0 + kz, C.md.nz), s_ntris,
-                       S + C.group_off, counts + C.group_off);
-  }
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool in = g < groups;
+  const int c = find_chunk<&BChunk::group_off>(ch, n_chunks, in ? g : groups - 1);
+  const BChunk& C = ch[c];
+  const uint32_t cnt = classify_signs_group(S + C.group_off, C.md, g - C.group_off, in, s_ntris, s_pass[threadIdx.x >> 6]);
+  if (in) counts[g] = cnt;
 }
 
 // Per-chunk vertex / triangle ranges from the dense exclusive offsets.
@@ -182,20 +186,6 @@ struct upf_batch {
 namespace upf {
 namespace {
 
-int64_t class_items(const upf_batch* b, int kz) {
-  int64_t t = 0;
-  for (const BChunk& c : b->host) t += (int64_t)c.md.gpr * c.md.ny * ((c.md.nz + kz - 1) / kz);
-  return t;
-}
-
-void set_col_offsets(upf_batch* b, int kz) {
-  int64_t t = 0;
-  for (BChunk& c : b->host) {
-    c.col_off = t;
-    t += (int64_t)c.md.gpr * c.md.ny * ((c.md.nz + kz - 1) / kz);
-  }
-}
-
 int normalize_opts(const upf_batch_opts* in, upf_batch_opts* out) {
   upf_batch_opts o{};
   if (in) o = *in;
@@ -210,7 +200,6 @@ int normalize_opts(const upf_batch_opts* in, upf_batch_opts* out) {
 }
 
 int upload(upf_batch* b) {
-  set_col_offsets(b, b->opts.classify_z);
   UPF_HIP_CHECK(hipMemcpy(b->dev, b->host.data(), sizeof(BChunk) * b->n, hipMemcpyHostToDevice));
   return UPF_OK;
 }
@@ -230,15 +219,15 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
   const unsigned grid = (unsigned)(b->cus * b->opts.blocks_per_cu);
   if (b->any_ext)
     hipLaunchKernelGGL((k_batch_field<UPF_EVAL_PER_LANE, true>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
-                       field);
+                       field, S);
   else if (b->opts.field_variant == UPF_EVAL_PER_LANE)
     hipLaunchKernelGGL((k_batch_field<UPF_EVAL_PER_LANE, false>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
-                       field);
+                       field, S);
   else
     hipLaunchKernelGGL((k_batch_field<UPF_EVAL_ROWS, false>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
-                       field);
-  hipLaunchKernelGGL(k_batch_classify, dim3(grid), dim3(256), 0, s, b->dev, b->n, class_items(b, b->opts.classify_z),
-                     b->opts.classify_z, field, S, counts);
+                       field, S);
+  hipLaunchKernelGGL(k_batch_classify, dim3((unsigned)((b->groups + 255) / 256)), dim3(256), 0, s, b->dev, b->n,
+                     b->groups, S, counts);
   UPF_LAUNCH_CHECK();
   int st = mc_scan_launch(counts, b->groups, tiles, total, voff, act, toff, s);
   if (st) return st;
@@ -391,9 +380,8 @@ int upf_batch_autotune(upf_batch* b, float* d_field, void* d_ws, size_t ws_bytes
   int st = UPF_OK;
   const int variants[2] = {UPF_EVAL_ROWS, UPF_EVAL_PER_LANE};
   const int bpcs[3] = {2, 4, 8};
-  const int kzs[3] = {16, 32, 64};
-  for (int cand = 0; cand < 18 && !st; cand++) {
-    const upf_batch_opts o{variants[cand / 9], bpcs[(cand / 3) % 3], kzs[cand % 3], 0};
+  for (int cand = 0; cand < 6 && !st; cand++) {
+    const upf_batch_opts o{variants[cand / 3], bpcs[cand % 3], start.classify_z, 0};
     if ((st = upf_batch_set_opts(b, &o))) break;
     if ((st = run(b, d_field, d_ws, ws_bytes, d_pos, d_nrm, d_idx, cap_v, cap_t, d_meshes, s))) break;  // warm-up
     if (hipEventRecord(e0, s) != hipSuccess) { st = UPF_ERR_HIP; break; }

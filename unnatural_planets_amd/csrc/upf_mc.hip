@@ -85,101 +85,16 @@ __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, co
                       s_pass[wave], pos, nrm, idx, cap_v, cap_t);
 }
 
-// Classification from the sign masks alone (the field kernel wrote them).  One lane per
-// group: its crossing masks come from S of the group, of its +x neighbour group and of
-// the +y / +z rows (8 words), so vertex counts are a few 64-bit operations per lane.
-// Triangle counts need the codes of the mixed cells (corners not all on one side; the
-// other codes have no triangles).  Each lane lists its group's mixed cells in LDS; in
-// passes of at most 64 cells (a group's cells never split), lane t takes listed cell t,
-// builds its code from the group's 8 words (in LDS) and adds the code's triangle count
-// to the group's LDS counter.  A wave with ~6 active groups of ~7 mixed cells each needs
-// one pass, instead of one wave-wide step per active group.
-
 __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __restrict__ S, const McDims d,
                                                            uint32_t* __restrict__ counts) {
   __shared__ uint8_t s_ntris[256];
-  __shared__ uint64_t s_w[4][64][8];
-  __shared__ uint32_t s_nt[4][64];
-  __shared__ uint16_t s_item[4][64];
+  __shared__ SignPass s_pass[4];
   s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
   __syncthreads();
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int lane = threadIdx.x & 63;
   const bool in = g < d.groups;
-  uint64_t a = 0, a1 = 0, ay = 0, ay1 = 0, b = 0, b1 = 0, by = 0, by1 = 0, cells = 0;
-  int nv = 0;
-  if (in) {
-    const uint32_t row = (uint32_t)(g / d.gpr);
-    const int xb = (int)(g - (int64_t)row * d.gpr);
-    const int k = (int)(row / (uint32_t)d.ny), j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
-    const bool nx1 = xb + 1 < d.gpr, hy = j + 1 < d.ny, hz = k + 1 < d.nz;
-    const int64_t plane = (int64_t)d.ny * d.gpr;
-    a = S[g];
-    const uint64_t ax = nx1 ? S[g + 1] : 0ull;
-    ay = hy ? S[g + d.gpr] : 0ull;
-    const uint64_t axy = (hy && nx1) ? S[g + d.gpr + 1] : 0ull;
-    b = hz ? S[g + plane] : 0ull;
-    const uint64_t bx = (hz && nx1) ? S[g + plane + 1] : 0ull;
-    by = (hz && hy) ? S[g + plane + d.gpr] : 0ull;
-    const uint64_t bxy = (hz && hy && nx1) ? S[g + plane + d.gpr + 1] : 0ull;
-    const int first = xb * 64;
-    const int nval = min(64, d.nx - first), nvx = min(64, d.nx - 1 - first);
-    const uint64_t vmask = nval >= 64 ? ~0ull : ((1ull << nval) - 1);
-    const uint64_t xmask = nvx >= 64 ? ~0ull : (nvx > 0 ? ((1ull << nvx) - 1) : 0ull);
-    // bit i of *1 = node i + 1 of the same row
-    a1 = (a >> 1) | (ax << 63); ay1 = (ay >> 1) | (axy << 63);
-    b1 = (b >> 1) | (bx << 63); by1 = (by >> 1) | (bxy << 63);
-    nv = __popcll((a ^ a1) & xmask) + (hy ? __popcll((a ^ ay) & vmask) : 0) + (hz ? __popcll((a ^ b) & vmask) : 0);
-    cells = (hy && hz) ? xmask : 0ull;
-  }
-  const uint64_t any = (a | a1 | ay | ay1 | b | b1 | by | by1) & cells;
-  const uint64_t all = (a & a1 & ay & ay1 & b & b1 & by & by1) & cells;
-  const uint64_t mixed = any & ~all;
-  const int nmix = __popcll(mixed);
-  const int wv = threadIdx.x >> 6;
-  // the words in code-bit order: bit q of cell c's code is bit c of w[q]
-  uint64_t* w = s_w[wv][lane];
-  w[0] = a; w[1] = a1; w[2] = ay1; w[3] = ay; w[4] = b; w[5] = b1; w[6] = by1; w[7] = by;
-  s_nt[wv][lane] = 0u;
-  int incl = nmix;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += v;
-  }
-  const int total = __shfl(incl, 63, 64);
-  for (int base = 0; base < total;) {
-    // the groups whose cells lie within [base, base + 64); the first group left starts at
-    // base and has at most 64 cells, so every pass takes one
-    const bool take = nmix > 0 && incl - nmix >= base && incl <= base + 64;
-    const uint64_t tk = ballot64(take);
-    const int end = __shfl(incl, 63 - __clzll((long long)tk), 64);
-    if (take) {
-      uint64_t m = mixed;
-      int p = incl - nmix - base;
-      while (m) {
-        s_item[wv][p++] = (uint16_t)((lane << 6) | (__ffsll((unsigned long long)m) - 1));
-        m &= m - 1;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < end - base) {
-      const int e = s_item[wv][lane];
-      const int grp = e >> 6, c = e & 63;
-      const uint64_t* gw = s_w[wv][grp];
-      int code = 0;
-#pragma unroll
-      for (int q = 0; q < 8; q++) code |= (int)((gw[q] >> c) & 1ull) << q;
-      atomicAdd(&s_nt[wv][grp], (uint32_t)s_ntris[code]);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    base = end;
-  }
-  if (in) counts[g] = pack_counts(nv, (int)s_nt[wv][lane]);
+  const uint32_t c = classify_signs_group(S, d, g, in, s_ntris, s_pass[threadIdx.x >> 6]);
+  if (in) counts[g] = c;
 }
 
 // ------------------------------------------------------------------ scan

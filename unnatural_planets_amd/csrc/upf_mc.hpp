@@ -9,7 +9,6 @@
 
 namespace upf {
 
-constexpr int kMcClassZ = 32;  // z planes swept by one classify workgroup
 
 struct McDims {
   int nx, ny, nz, gpr;  // gpr = 64-node groups per row

@@ -56,52 +56,100 @@ __device__ __forceinline__ uint32_t pack_counts(int nv, int nt) {
 }
 
 // ------------------------------------------------------------------ classify
-// One wave: group column xb of row j, planes k0 <= k < k1.  The current z plane stays in
-// registers; each step uses plane k+1 (loaded the step before) and loads plane k+2's rows
-// j and j+1 at x and x+1 (clamped, so every load is unconditional and the loads of a step
-// issue back to back).
-__device__ __forceinline__ void mc_classify_column(const float* __restrict__ f, const McDims& d, float iso, int xb,
-                                                   int j, int k0, int k1, const uint8_t* s_ntris,
-                                                   uint64_t* __restrict__ S, uint32_t* __restrict__ counts) {
+// Packed counts of group g (local id in a grid with dims d; `in` false past the end)
+// from the sign words alone (written by the field kernel or k_mc_signs), one lane per
+// group.  Crossing masks come from S of the group, of its +x neighbour group and of the
+// +y / +z rows (8 words), so vertex counts are a few 64-bit operations per lane.
+// Triangle counts need the codes of the mixed cells (corners not all on one side; the
+// other codes have no triangles).  Each lane lists its group's mixed cells in LDS; in
+// passes of at most 64 cells (a group's cells never split), lane t takes listed cell t,
+// builds its code from the group's 8 words (in LDS) and adds the code's triangle count
+// to the group's LDS counter.  A wave with ~6 active groups of ~7 mixed cells each needs
+// one pass, instead of one wave-wide step per active group.  Lanes may belong to
+// different grids (batched chunks): everything but the pass bookkeeping is per lane.
+
+struct SignPass {  // one wave's LDS
+  uint64_t w[64][8];
+  uint32_t nt[64];
+  uint16_t item[64];
+};
+
+__device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restrict__ S, const McDims& d, int64_t g,
+                                                         bool in, const uint8_t* s_ntris, SignPass& P) {
   const int lane = threadIdx.x & 63;
-  const int x = xb * 64 + lane;
-  const bool valid = x < d.nx;
-  const int xc = min(x, d.nx - 1), x1 = min(x + 1, d.nx - 1);
-  const int j1 = min(j + 1, d.ny - 1);
-  const bool hx = valid && x + 1 < d.nx, hy = valid && j + 1 < d.ny;
-  const int64_t r0 = (int64_t)j * d.nx, r1 = (int64_t)j1 * d.nx;
-  // current plane: a0 = (x, j), a1 = (x+1, j), a2 = (x+1, j+1), a3 = (x, j+1)
-  const float* p = f + (int64_t)k0 * d.nxy;
-  float a0 = p[r0 + xc], a1 = p[r0 + x1], a2 = p[r1 + x1], a3 = p[r1 + xc];
-  // plane k+1 is loaded one step ahead, so two planes of loads are in flight per step
-  const float* q1 = f + (int64_t)min(k0 + 1, d.nz - 1) * d.nxy;
-  float c0 = q1[r0 + xc], c1 = q1[r0 + x1], c2 = q1[r1 + x1], c3 = q1[r1 + xc];
-  for (int k = k0; k < k1; k++) {
-    const bool hz = valid && k + 1 < d.nz;
-    const float b0 = c0, b1 = c1, b2 = c2, b3 = c3;
-    const float* q = f + (int64_t)min(k + 2, d.nz - 1) * d.nxy;
-    c0 = q[r0 + xc]; c1 = q[r0 + x1]; c2 = q[r1 + x1]; c3 = q[r1 + xc];
-    const bool s0 = valid && a0 < iso;
-    const bool cx = hx && ((a1 < iso) != s0);
-    const bool cy = hy && ((a3 < iso) != s0);
-    const bool cz = hz && ((b0 < iso) != s0);
-    int nt = 0;
-    if (hx && hy && hz) {
-      const int code = (a0 < iso) | ((a1 < iso) << 1) | ((a2 < iso) << 2) | ((a3 < iso) << 3) | ((b0 < iso) << 4) |
-                       ((b1 < iso) << 5) | ((b2 < iso) << 6) | ((b3 < iso) << 7);
-      nt = s_ntris[code];
-    }
-    const uint64_t sm = ballot64(s0);
-    const int nv = __popcll(ballot64(cx)) + __popcll(ballot64(cy)) + __popcll(ballot64(cz));
-    // triangle total from bit-sliced ballots (nt <= 5)
-    const int ntot = __popcll(ballot64(nt & 1)) + 2 * __popcll(ballot64(nt & 2)) + 4 * __popcll(ballot64(nt & 4));
-    if (lane == 0) {
-      const int64_t g = ((int64_t)k * d.ny + j) * d.gpr + xb;
-      S[g] = sm;
-      counts[g] = pack_counts(nv, ntot);
-    }
-    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+  uint64_t a = 0, a1 = 0, ay = 0, ay1 = 0, b = 0, b1 = 0, by = 0, by1 = 0, cells = 0;
+  int nv = 0;
+  if (in) {
+    const uint32_t row = (uint32_t)(g / d.gpr);
+    const int xb = (int)(g - (int64_t)row * d.gpr);
+    const int k = (int)(row / (uint32_t)d.ny), j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
+    const bool nx1 = xb + 1 < d.gpr, hy = j + 1 < d.ny, hz = k + 1 < d.nz;
+    const int64_t plane = (int64_t)d.ny * d.gpr;
+    a = S[g];
+    const uint64_t ax = nx1 ? S[g + 1] : 0ull;
+    ay = hy ? S[g + d.gpr] : 0ull;
+    const uint64_t axy = (hy && nx1) ? S[g + d.gpr + 1] : 0ull;
+    b = hz ? S[g + plane] : 0ull;
+    const uint64_t bx = (hz && nx1) ? S[g + plane + 1] : 0ull;
+    by = (hz && hy) ? S[g + plane + d.gpr] : 0ull;
+    const uint64_t bxy = (hz && hy && nx1) ? S[g + plane + d.gpr + 1] : 0ull;
+    const int first = xb * 64;
+    const int nval = min(64, d.nx - first), nvx = min(64, d.nx - 1 - first);
+    const uint64_t vmask = nval >= 64 ? ~0ull : ((1ull << nval) - 1);
+    const uint64_t xmask = nvx >= 64 ? ~0ull : (nvx > 0 ? ((1ull << nvx) - 1) : 0ull);
+    // bit i of *1 = node i + 1 of the same row
+    a1 = (a >> 1) | (ax << 63); ay1 = (ay >> 1) | (axy << 63);
+    b1 = (b >> 1) | (bx << 63); by1 = (by >> 1) | (bxy << 63);
+    nv = __popcll((a ^ a1) & xmask) + (hy ? __popcll((a ^ ay) & vmask) : 0) + (hz ? __popcll((a ^ b) & vmask) : 0);
+    cells = (hy && hz) ? xmask : 0ull;
   }
+  const uint64_t any = (a | a1 | ay | ay1 | b | b1 | by | by1) & cells;
+  const uint64_t all = (a & a1 & ay & ay1 & b & b1 & by & by1) & cells;
+  const uint64_t mixed = any & ~all;
+  const int nmix = __popcll(mixed);
+  // the words in code-bit order: bit q of cell c's code is bit c of w[q]
+  uint64_t* w = P.w[lane];
+  w[0] = a; w[1] = a1; w[2] = ay1; w[3] = ay; w[4] = b; w[5] = b1; w[6] = by1; w[7] = by;
+  P.nt[lane] = 0u;
+  int incl = nmix;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  const int total = __shfl(incl, 63, 64);
+  for (int base = 0; base < total;) {
+    // the groups whose cells lie within [base, base + 64); the first group left starts at
+    // base and has at most 64 cells, so every pass takes one
+    const bool take = nmix > 0 && incl - nmix >= base && incl <= base + 64;
+    const uint64_t tk = ballot64(take);
+    const int end = __shfl(incl, 63 - __clzll((long long)tk), 64);
+    if (take) {
+      uint64_t m = mixed;
+      int p = incl - nmix - base;
+      while (m) {
+        P.item[p++] = (uint16_t)((lane << 6) | (__ffsll((unsigned long long)m) - 1));
+        m &= m - 1;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < end - base) {
+      const int e = P.item[lane];
+      const int grp = e >> 6, c = e & 63;
+      const uint64_t* gw = P.w[grp];
+      int code = 0;
+#pragma unroll
+      for (int q = 0; q < 8; q++) code |= (int)((gw[q] >> c) & 1ull) << q;
+      atomicAdd(&P.nt[grp], (uint32_t)s_ntris[code]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    base = end;
+  }
+  return pack_counts(nv, (int)P.nt[lane]);
 }
 
 // ------------------------------------------------------------------ emit
