@@ -379,9 +379,12 @@ int upf_batch_autotune(upf_batch* b, float* d_field, void* d_ws, size_t ws_bytes
   float best_ms = 1e30f;
   int st = UPF_OK;
   const int variants[2] = {UPF_EVAL_ROWS, UPF_EVAL_PER_LANE};
-  const int bpcs[3] = {2, 4, 8};
-  for (int cand = 0; cand < 6 && !st; cand++) {
-    const upf_batch_opts o{variants[cand / 3], bpcs[cand % 3], start.classify_z, 0};
+  // 6 = the row variant's resident workgroups per CU (six waves per SIMD), 8 the
+  // per-lane variant's: persistent waves take equal shares, so a grid of exactly the
+  // resident size has no tail of late waves
+  const int bpcs[4] = {2, 4, 6, 8};
+  for (int cand = 0; cand < 8 && !st; cand++) {
+    const upf_batch_opts o{variants[cand / 4], bpcs[cand % 4], start.classify_z, 0};
     if ((st = upf_batch_set_opts(b, &o))) break;
     if ((st = run(b, d_field, d_ws, ws_bytes, d_pos, d_nrm, d_idx, cap_v, cap_t, d_meshes, s))) break;  // warm-up
     if (hipEventRecord(e0, s) != hipSuccess) { st = UPF_ERR_HIP; break; }

@@ -92,6 +92,13 @@ int to_device(const upf_field_spec* in, DSpec* out) {
   return UPF_OK;
 }
 
+int resident_blocks(const void* kernel, int threads) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 1;
+  return per_cu * device_cus();
+}
+
 int device_cus() {
   static std::atomic<int> cache[64];
   int dev = 0;

@@ -311,7 +311,9 @@ int upf_field_adjoint_grid(const upf_field_spec* spec, const upf_grid* grid, con
     return UPF_OK;
   }
   const int64_t items = (int64_t)((g.n[0] + 63) / 64) * g.n[1] * g.n[2];
-  const int nb = (int)std::min<int64_t>((items + 3) / 4, (int64_t)device_cus() * 8);
+  const void* kern = d_dLdtheta ? reinterpret_cast<const void*>(&k_adjoint_rows<true>)
+                                : reinterpret_cast<const void*>(&k_adjoint_rows<false>);
+  const int nb = (int)std::min<int64_t>((items + 3) / 4, (int64_t)resident_blocks(kern, 256));
   if (d_dLdtheta)
     hipLaunchKernelGGL((k_adjoint_rows<true>), dim3(nb), dim3(256), 0, as_stream(stream), s, g, items, d_dLdf, d_f,
                        d_dfdx, d_dLdtheta);

@@ -24,6 +24,12 @@ int field_grid_launch(const DSpec& s, const DGrid& g, float* d_field, int varian
 // Compute units of the current device (cached per device; 256 if the query fails).
 int device_cus();
 
+// Workgroups of `threads` threads of `kernel` that are resident on the current device at
+// once (the occupancy its registers and LDS allow, times the compute units): the size of
+// a persistent grid whose waves each take an equal share, so that no wave starts its
+// share after others have finished theirs.
+int resident_blocks(const void* kernel, int threads);
+
 // Workgroups are dispatched round-robin over the XCDs (workgroup L runs on XCD L % 8),
 // and each XCD has its own L2.  Work item of workgroup L when n items are split into 8
 // contiguous ranges, one per XCD: neighbouring items (which share cache lines of their
