@@ -119,8 +119,29 @@ enum upf_shape_id {
   UPF_SHAPE_TWISTED_HEX_PRISM = 27,/* no params; apothem 650, length 900 (sdf.cpp:234-242) */
   UPF_SHAPE_TETRAHEDRON = 28,  /* no params; regular, circumradius 600, exact distance,
                                   minus 100 (sdf.cpp:50-53)                                 */
-  UPF_SHAPE_COUNT = 29
+  UPF_SHAPE_SINE_NET = 29,     /* params: scale, d1, d2; weights: shape_weights (below).
+                                  The architecture of sdfBunny / sdfMonkeyHead
+                                  (sdf.cpp:252-303), whose weight tables the caller passes */
+  UPF_SHAPE_COUNT = 30
 };
+
+/* UPF_SHAPE_SINE_NET: a sine network of width 16 evaluated at q = (x, z, y) * scale:
+ *   q outside the unit ball: f = (|q| - 0.8) / scale;
+ *   h0 = sin(A q.y + B q.z + C q.x + D)                   (16 units, 4 blocks of 4)
+ *   h1[o] = sin(sum_b M1[o][b] h0[b] + V1[o]) / d1 + h0[o]   (block o = 0..3)
+ *   h2[o] = sin(sum_b M2[o][b] h1[b] + V2[o]) / d2 + h1[o]
+ *   f = (sum_b dot(h2[b], U[b]) + c) / scale.
+ * shape_weights points to UPF_SINE_NET_WEIGHTS floats in the memory the evaluating call
+ * reads (device memory for the GPU entry points), in the order the reference's source
+ * writes them:
+ *   [0, 64)     layer 0, block b at 16 b: A[4], B[4], C[4], D[4] (sdfBunnyImpl's
+ *               "- p[0] * Vec4(...)" is C negated);
+ *   [64, 336)   layer 1, block o at 64 + 68 o: M1[o][0..3] (16 floats each, the Mat4
+ *               constructor's order, column-major: element 4 col + row), then V1[o][4];
+ *   [336, 608)  layer 2, the same layout;
+ *   [608, 624)  U[0..3] (4 floats each); [624] c.
+ * The reference uses scale 0.0005, d1 1.0, d2 1.4 for both shapes. */
+#define UPF_SINE_NET_WEIGHTS 625
 
 enum upf_noise_id {
   UPF_NOISE_GRADIENT = 1, /* 3D lattice gradient noise, quintic fade, 12-edge gradients   */
@@ -223,7 +244,8 @@ typedef struct upf_field_spec {
   int32_t elevation_mode; /* upf_elevation_mode_id; 0 = the generic layer stack */
   float shape_params[UPF_MAX_SHAPE_PARAMS];
   float elevation_scale;
-  float reserved[3];
+  int32_t reserved;
+  const float* shape_weights; /* UPF_SHAPE_SINE_NET's weights, else NULL */
   upf_layer layers[UPF_MAX_LAYERS];
 } upf_field_spec;
 

@@ -317,6 +317,7 @@ struct SpecT {
   int shape = 0, compose = 0, n_layers = 0, elevation_mode = 0;
   T shape_params[UPF_MAX_SHAPE_PARAMS];
   T elevation_scale;
+  const float* shape_weights = nullptr;  // UPF_SHAPE_SINE_NET (host memory here)
   LayerT<T> layers[UPF_MAX_LAYERS];
 };
 
@@ -326,6 +327,7 @@ SpecT<T> spec_to(const upf_field_spec& s) {
   r.shape = s.shape; r.compose = s.compose; r.n_layers = s.n_layers; r.elevation_mode = s.elevation_mode;
   for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++) r.shape_params[i] = T(s.shape_params[i]);
   r.elevation_scale = T(s.elevation_scale);
+  r.shape_weights = s.shape_weights;
   for (int l = 0; l < UPF_MAX_LAYERS; l++) {
     const upf_layer& a = s.layers[l];
     LayerT<T>& b = r.layers[l];
@@ -575,6 +577,45 @@ template <class T> T shape_extended(int shape, const T* a, const T p[3]) {
   }
 }
 
+// sdfBunny / sdfMonkeyHead (sdf.cpp:252-303; the networks at :252-291, the swizzle and
+// scale at :293-303): the shared architecture with the caller's weights w (the order of
+// the reference's source, the layout of UPF_SINE_NET_WEIGHTS), params (scale, d1, d2).
+// Plain operations in the reference's order: each layer-0 unit y A + z B + x C + D, each
+// hidden unit the four Mat4 * Vec4 rows (column-major) summed block by block, then V,
+// the output the four dot products, then the constant.
+template <class T> T sine_net(const float* w, const T* a, const T p[3]) {
+  const T scale = a[0], d1 = a[1], d2 = a[2];
+  const T q[3] = {p[0] * scale, p[2] * scale, p[1] * scale};  // Vec3(p[0], p[2], p[1]) * scale
+  const T len = sqrt_(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+  if (val(len) > 1.0) return (len - T(0.8f)) / scale;  // :255-256
+  T h0[16], h1[16], h2[16];
+  for (int b = 0; b < 4; b++)
+    for (int i = 0; i < 4; i++) {
+      const float* L = w + 16 * b;
+      h0[4 * b + i] = sin_(q[1] * T(L[i]) + q[2] * T(L[4 + i]) + q[0] * T(L[8 + i]) + T(L[12 + i]));
+    }
+  auto mat_row = [&](const float* M, const T* v, int r) {  // row r of Mat4(M) * v
+    return T(M[r]) * v[0] + T(M[4 + r]) * v[1] + T(M[8 + r]) * v[2] + T(M[12 + r]) * v[3];
+  };
+  auto hidden = [&](const float* W, const T* hin, T* hout, T d) {
+    for (int o = 0; o < 4; o++) {
+      const float* B = W + 68 * o;
+      for (int r = 0; r < 4; r++) {
+        const T sum = mat_row(B, hin, r) + mat_row(B + 16, hin + 4, r) + mat_row(B + 32, hin + 8, r) +
+                      mat_row(B + 48, hin + 12, r) + T(B[64 + r]);
+        hout[4 * o + r] = sin_(sum) / d + hin[4 * o + r];
+      }
+    }
+  };
+  hidden(w + 64, h0, h1, d1);
+  hidden(w + 336, h1, h2, d2);
+  auto dot4 = [&](const T* h, const float* u) {
+    return h[0] * T(u[0]) + h[1] * T(u[1]) + h[2] * T(u[2]) + h[3] * T(u[3]);
+  };
+  const T f = dot4(h2, w + 608) + dot4(h2 + 4, w + 612) + dot4(h2 + 8, w + 616) + dot4(h2 + 12, w + 620) + T(w[624]);
+  return f / scale;
+}
+
 template <class T> T shape_eval(const SpecT<T>& s, const T p[3]) {
   const T* a = s.shape_params;
   switch (s.shape) {
@@ -597,6 +638,7 @@ template <class T> T shape_eval(const SpecT<T>& s, const T p[3]) {
       const T zc = p[2] - clamp_(p[2], -h2, h2);
       return sqrt_(p[0] * p[0] + p[1] * p[1] + zc * zc) - a[1];
     }
+    case UPF_SHAPE_SINE_NET: return sine_net(s.shape_weights, a, p);
     default: return shape_extended(s.shape, a, p);
   }
 }

@@ -82,7 +82,7 @@ def test_flop_table_counts_extended_shapes(oracle, upf, name):
 
 def test_shape_ids_validated(upf):
     s = S.field_spec("sphere", (1.0,), [])
-    s.shape = 29
+    s.shape = 30
     assert upf.lib().upf_spec_validate(ctypes.byref(s)) == S.ERR_BAD_SPEC
     for name in FURTHER:
         assert upf.lib().upf_spec_validate(ctypes.byref(shape_spec(name))) == S.OK

@@ -30,6 +30,7 @@ int shape_params_used(int shape) {
     case UPF_SHAPE_HEXAGONAL_PRISM: case UPF_SHAPE_TWISTED_HEX_PRISM: case UPF_SHAPE_TETRAHEDRON: return 0;
     case UPF_SHAPE_FIBERS: return 3;
     case UPF_SHAPE_ASTEROID: return 2;
+    case UPF_SHAPE_SINE_NET: return 3;
     default: return -1;
   }
 }
@@ -47,6 +48,11 @@ int to_device(const upf_field_spec* in, DSpec* out) {
     if (!finite(in->shape_params[i])) return UPF_ERR_INVALID_VALUE;
   if (!finite(in->elevation_scale)) return UPF_ERR_INVALID_VALUE;
   if (in->compose == UPF_COMPOSE_ELEVATION && in->elevation_scale == 0.0f) return UPF_ERR_INVALID_VALUE;
+  if (in->shape == UPF_SHAPE_SINE_NET) {  // weights, and nonzero scale and divisors
+    if (!in->shape_weights) return UPF_ERR_NULL;
+    if (in->shape_params[0] == 0.0f || in->shape_params[1] == 0.0f || in->shape_params[2] == 0.0f)
+      return UPF_ERR_INVALID_VALUE;
+  }
   std::memset(out, 0, sizeof(*out));
   out->shape = in->shape;
   out->compose = in->compose;
@@ -54,6 +60,7 @@ int to_device(const upf_field_spec* in, DSpec* out) {
   for (int i = 0; i < UPF_MAX_SHAPE_PARAMS; i++) out->sp[i] = in->shape_params[i];
   out->escale = in->elevation_scale;
   out->mode = mode;
+  out->w = in->shape == UPF_SHAPE_SINE_NET ? in->shape_weights : nullptr;
   out->all_add = 1;
   for (int l = 0; l < in->n_layers; l++) {
     const upf_layer& a = in->layers[l];
@@ -187,6 +194,7 @@ OpCount sq_root(OpCount a) { return op(std::sqrt(a.v), 1, 5); }
 OpCount pow_v(OpCount a, float e) { return op(std::pow(a.v, e), 1, 5); }
 OpCount floor_v(OpCount a) { return {std::floor(a.v)}; }
 OpCount sin_v(OpCount a) { return op(std::sin(a.v), 1, 5); }
+OpCount fma_s(float w, OpCount h, OpCount acc) { return op(std::fma(w, h.v, acc.v), 2, 8); }
 OpCount cos_v(OpCount a) { return op(std::cos(a.v), 1, 5); }
 OpCount atan2_v(OpCount y, OpCount x) { return op(std::atan2(y.v, x.v), 1, 12); }
 OpCount acos_v(OpCount a) { return op(std::acos(a.v), 1, 7); }
@@ -207,6 +215,12 @@ OpTally shape_ext_tally(int shape, const float* sp) {
     // octave sum 3, the radial composition 3
     g_tally.fwd = 5 + 6 + 2 * (3 + 75) + 3 + 3;
     g_tally.grad = 5 * 4 + 6 * 10 + 2 * (3 + 138) + 3 + 3 * 4 + 18;
+    return g_tally;
+  }
+  if (shape == UPF_SHAPE_SINE_NET) {  // a point inside the unit ball of q (the full network)
+    static const float zero_w[UPF_SINE_NET_WEIGHTS] = {};
+    const float s = sp[0];
+    (void)upf::sine_net_t<OpCount>(zero_w, s, sp[1], sp[2], OpCount{0.3f / s}, OpCount{0.2f / s}, OpCount{-0.4f / s});
     return g_tally;
   }
   if (shape >= UPF_SHAPE_H3O) {

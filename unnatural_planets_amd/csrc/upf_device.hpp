@@ -41,7 +41,7 @@ struct DSpec {
   float sp[UPF_MAX_SHAPE_PARAMS];
   float escale;
   int32_t mode;  // upf_elevation_mode_id
-  float pad[2];
+  const float* w;  // UPF_SHAPE_SINE_NET's weights (device memory)
   DLayer L[UPF_MAX_LAYERS];
 };
 
@@ -247,8 +247,9 @@ __device__ __forceinline__ float noise_simplex(uint32_t seed, float x, float y, 
 // ------------------------------------------------------------------ shapes
 // The further shapes (ids UPF_SHAPE_H3O and above): out of line, so that the field
 // kernels carry one call instead of their code; parameters pass by value.
-__device__ __noinline__ float shape_more(int shape, float p0, float p1, float p2, float p3, float x, float y,
-                                         float z) {
+__device__ __noinline__ float shape_more(int shape, float p0, float p1, float p2, float p3, const float* w, float x,
+                                         float y, float z) {
+  if (shape == UPF_SHAPE_SINE_NET) return sine_net_t<float>(w, p0, p1, p2, x, y, z);
   const float sp[4] = {p0, p1, p2, p3};
   if (shape == UPF_SHAPE_ASTEROID) {  // sources/sdf.cpp:333-353
     const float l = sqrt_f(x * x + y * y + z * z);
@@ -290,7 +291,7 @@ __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, fl
       return 0.0f;
     default:
       if (EXT && s.shape >= UPF_SHAPE_H3O)
-        return shape_more(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], x, y, z);
+        return shape_more(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], s.w, x, y, z);
       return shape_ext<float>(s.shape, s.sp, x, y, z, 0.0f);
   }
 }

@@ -124,8 +124,9 @@ __device__ __forceinline__ float octave_noise_d(const DLayer& L, int k, float x,
 
 // ------------------------------------------------------------------ shapes + gradient
 // The further shapes with their position gradient (forward mode), out of line.
-__device__ __noinline__ Grad shape_more_d(int shape, float p0, float p1, float p2, float p3, float x, float y,
-                                          float z) {
+__device__ __noinline__ Grad shape_more_d(int shape, float p0, float p1, float p2, float p3, const float* w, float x,
+                                          float y, float z) {
+  if (shape == UPF_SHAPE_SINE_NET) return sine_net_t<Grad>(w, p0, p1, p2, gvar(x, 0), gvar(y, 1), gvar(z, 2));
   const float sp[4] = {p0, p1, p2, p3};
   if (shape == UPF_SHAPE_ASTEROID) {
     const float l = sqrt_f(x * x + y * y + z * z);
@@ -208,7 +209,7 @@ __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, 
     case UPF_SHAPE_NONE:
       return 0.0f;
     default: {  // extended shapes: forward-mode position gradient, no parameter gradient
-      const Grad r = EXT && s.shape >= UPF_SHAPE_H3O ? shape_more_d(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], x, y, z)
+      const Grad r = EXT && s.shape >= UPF_SHAPE_H3O ? shape_more_d(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], s.w, x, y, z)
                                               : shape_ext<Grad>(s.shape, s.sp, gvar(x, 0), gvar(y, 1), gvar(z, 2),
                                                                 gc(0.0f));
       g[0] = r.dx; g[1] = r.dy; g[2] = r.dz;

@@ -338,7 +338,7 @@ __device__ __forceinline__ float m_shape(const DSpec& s, float x, float y, float
     }
     case UPF_SHAPE_NONE: return 0.0f;
     default:
-      if (s.shape >= UPF_SHAPE_H3O) return shape_more(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], x, y, z);
+      if (s.shape >= UPF_SHAPE_H3O) return shape_more(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], s.w, x, y, z);
       return shape_ext<float>(s.shape, s.sp, x, y, z, 0.0f);
   }
 }

@@ -75,9 +75,14 @@ UPF_HD Grad pow_v(const Grad& a, float e) {
 UPF_HD float floor_v(float a) { return floorf(a); }
 UPF_HD Grad floor_v(const Grad& a) { return gc(floorf(a.v)); }
 UPF_HD float sin_v(float a) { return sinf(a); }
+// acc + w h with a constant weight w: one fused multiply-add per component
+UPF_HD float fma_s(float w, float h, float acc) { return fmaf(w, h, acc); }
 UPF_HD Grad sin_v(const Grad& a) {
   const float c = cosf(a.v);
   return {sinf(a.v), c * a.dx, c * a.dy, c * a.dz};
+}
+UPF_HD Grad fma_s(float w, const Grad& h, const Grad& acc) {
+  return {fmaf(w, h.v, acc.v), fmaf(w, h.dx, acc.dx), fmaf(w, h.dy, acc.dy), fmaf(w, h.dz, acc.dz)};
 }
 UPF_HD float cos_v(float a) { return cosf(a); }
 UPF_HD Grad cos_v(const Grad& a) {
@@ -259,6 +264,46 @@ template <class T> UPF_HD T shape_ext(int shape, const float* spf, T x, T y, T z
 }
 
 template <class T> UPF_HD T smooth_max_s(T a, T b, T k, T zero) { return -smooth_min_v(-a, -b, k, zero); }
+
+// UPF_SHAPE_SINE_NET, the architecture of sdfBunny and sdfMonkeyHead (sources/sdf.cpp:
+// 252-303): a sine network of width 16 at q = (x, z, y) * scale, weights w in the layout
+// documented at UPF_SINE_NET_WEIGHTS (the order of the reference's source; the Mat4
+// blocks column-major), residual hidden layers divided by d1 and d2.  Outside the unit
+// ball (|q| > 1) the distance is (|q| - 0.8) / scale, as in the reference.
+template <class T> UPF_HD T sine_net_t(const float* w, float scale, float d1, float d2, T x, T y, T z) {
+  const T qx = x * scale, qy = z * scale, qz = y * scale;
+  const T len = len3(qx, qy, qz);
+  if (value_of(len) > 1.0f) return (len - 0.8f) / scale;
+  T h0[16], h1[16];
+  for (int b = 0; b < 4; b++)
+    for (int i = 0; i < 4; i++) {
+      const float* L = w + 16 * b;  // A, B, C, D of block b
+      h0[4 * b + i] = sin_v(fma_s(L[8 + i], qx, fma_s(L[4 + i], qz, fma_s(L[i], qy, lift(x, L[12 + i])))));
+    }
+  // h_out[o] = sin(sum_b M[o][b] h_in[b] + V[o]) / d + h_in[o]; M[o][b] element (col c,
+  // row r) at 16 b + 4 c + r of the block's 68 floats, V[o] at 64
+  for (int o = 0; o < 4; o++) {
+    const float* B = w + 64 + 68 * o;
+    for (int r = 0; r < 4; r++) {
+      T acc = lift(x, B[64 + r]);
+      for (int b = 0; b < 4; b++)
+        for (int c = 0; c < 4; c++) acc = fma_s(B[16 * b + 4 * c + r], h0[4 * b + c], acc);
+      h1[4 * o + r] = sin_v(acc) / d1 + h0[4 * o + r];
+    }
+  }
+  for (int o = 0; o < 4; o++) {  // layer 2 into h0 (layer 0 is no longer needed)
+    const float* B = w + 336 + 68 * o;
+    for (int r = 0; r < 4; r++) {
+      T acc = lift(x, B[64 + r]);
+      for (int b = 0; b < 4; b++)
+        for (int c = 0; c < 4; c++) acc = fma_s(B[16 * b + 4 * c + r], h1[4 * b + c], acc);
+      h0[4 * o + r] = sin_v(acc) / d2 + h1[4 * o + r];
+    }
+  }
+  T f = lift(x, w[624]);
+  for (int u = 0; u < 16; u++) f = fma_s(w[608 + u], h0[u], f);
+  return f / scale;
+}
 
 // The further shapes of sources/sdf.cpp (ids UPF_SHAPE_H3O .. UPF_SHAPE_TETRAHEDRON; the
 // asteroid, which samples noise, is evaluated by the caller).  cage's atan2(a, b) is

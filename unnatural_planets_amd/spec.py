@@ -26,7 +26,8 @@ SHAPE = {"none": 0, "sphere": 1, "box": 2, "torus": 3, "capsule": 4, "cylinder":
          "double_torus": 8, "torus_cross": 9, "h2o": 10, "bowl": 11, "inside_cube": 12, "wormhole": 13,
          "pipe": 14, "hemispheres": 15, "h3o": 16, "h4o": 17, "gear": 18, "mandelbulb": 19, "mobius": 20,
          "fibers": 21, "asteroid": 22, "twisted_plane": 23, "belt": 24, "triangular_prism": 25,
-         "hexagonal_prism": 26, "twisted_hex_prism": 27, "tetrahedron": 28}
+         "hexagonal_prism": 26, "twisted_hex_prism": 27, "tetrahedron": 28, "sine_net": 29}
+SINE_NET_WEIGHTS = 625
 # the reference's parameters (sources/sdf.cpp, metres) for the parametric extended shapes
 SHAPE_REFERENCE_PARAMS = {"cylinder": (10000.0, 800.0), "octahedron": (800.0, 100.0),
                           "double_torus": (900.0, 400.0, 200.0), "torus_cross": (750.0, 400.0, 200.0),
@@ -55,7 +56,8 @@ class FieldSpec(ctypes.Structure):
     _fields_ = [
         ("shape", ctypes.c_int32), ("compose", ctypes.c_int32), ("n_layers", ctypes.c_int32),
         ("elevation_mode", ctypes.c_int32), ("shape_params", ctypes.c_float * MAX_SHAPE_PARAMS),
-        ("elevation_scale", ctypes.c_float), ("reserved", ctypes.c_float * 3),
+        ("elevation_scale", ctypes.c_float), ("reserved", ctypes.c_int32),
+        ("shape_weights", ctypes.c_void_p),
         ("layers", Layer * MAX_LAYERS),
     ]
 
