@@ -21,6 +21,9 @@ CASES = {
     "sphere_fbm_all_add": (S.spec_sphere_fbm(1234), S.uniform_grid(-1.3, 1.3, 66)),
     "value_ridged": (value_ridged(), S.uniform_grid(-1.3, 1.3, 64)),
     "navigation_torus": (upfm._with_compose(S.spec_torus_fbm_voronoi(7), "navigation"), S.uniform_grid(-1.6, 1.6, 65)),
+    "simplex_in_stack": (S.field_spec("torus", (1.0, 0.4), [S.layer("simplex", "fbm", 3, 2.0, 2.0, 0.5, 0.05, 9),
+                                                            S.layer("gradient", "ridged", 3, 1.5, 2.0, 0.5, 0.04, 10)]),
+                         S.uniform_grid(-1.6, 1.6, 64)),
     "octahedron_metres": (S.field_spec("octahedron", (800.0, 100.0),
                                        [S.layer("gradient", "fbm", 3, 0.002, 2.0, 0.5, 1.0, 17)], elevation_scale=30.0),
                           S.uniform_grid(-1200.0, 1200.0, 67)),

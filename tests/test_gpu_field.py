@@ -148,7 +148,7 @@ def _grid(o, sp, dims):
 
 
 @pytest.mark.parametrize("case", ["sphere_fbm", "torus_ridged_voronoi", "value_ridged", "cell_highfreq",
-                                  "grad_highfreq", "ragged"])
+                                  "grad_highfreq", "ragged", "simplex_in_stack"])
 def test_row_kernel_matches_per_lane(upf, oracle, case):
     """The wave-cooperative row kernel and the per-lane kernel agree to a few ulps (the
     row kernel regroups the interpolation sums; both are within 1e-5 of the oracle), on
@@ -166,6 +166,9 @@ def test_row_kernel_matches_per_lane(upf, oracle, case):
         s = S.field_spec("sphere", (1.0,), [S.layer("cellular", "fbm", 3, 40.0, 2.0, 0.5, 0.05, 3, jitter=0.7)])
     elif case == "grad_highfreq":  # > 62 lattice x per row at the top octaves
         s = S.field_spec("sphere", (1.0,), [S.layer("gradient", "fbm", 8, 9.0, 2.0, 0.5, 0.05, 4)])
+    elif case == "simplex_in_stack":  # a simplex layer of the generic stack: per lane inside the row kernel
+        s = S.field_spec("torus", (1.0, 0.4), [S.layer("simplex", "fbm", 3, 2.0, 2.0, 0.5, 0.05, 9),
+                                               S.layer("gradient", "ridged", 3, 1.5, 2.0, 0.5, 0.04, 10)])
     else:
         s = S.spec_torus_ridged_voronoi(7)
         g = _grid((-1.5, -1.2, -0.7), (0.021, 0.033, 0.05), (131, 45, 29))

@@ -345,7 +345,11 @@ __device__ __forceinline__ void row_layer_value(const RowCtx& c, const DLayer& L
                                                 float* out) {
   if (L.noise == UPF_NOISE_GRADIENT) row_layer_t<UPF_NOISE_GRADIENT, R>(c, L, x, y, z, out);
   else if (L.noise == UPF_NOISE_VALUE) row_layer_t<UPF_NOISE_VALUE, R>(c, L, x, y, z, out);
-  else row_layer_t<UPF_NOISE_CELLULAR, R>(c, L, x, y, z, out);
+  else if (L.noise == UPF_NOISE_CELLULAR) row_layer_t<UPF_NOISE_CELLULAR, R>(c, L, x, y, z, out);
+  else {  // simplex: its lattice is skewed, not axis-aligned columns; per lane
+#pragma unroll
+    for (int r = 0; r < R; r++) out[r] = layer_value(L, x, y[r], z);
+  }
 }
 
 template <int R>

@@ -221,7 +221,9 @@ __device__ __forceinline__ float row_layer_grad(const RowCtx& c, const DLayer& L
     return row_layer_grad_t<THETA, UPF_NOISE_GRADIENT>(c, L, l, x, y, z, coeff, coeff_theta, g, acc);
   if (L.noise == UPF_NOISE_VALUE)
     return row_layer_grad_t<THETA, UPF_NOISE_VALUE>(c, L, l, x, y, z, coeff, coeff_theta, g, acc);
-  return row_layer_grad_t<THETA, UPF_NOISE_CELLULAR>(c, L, l, x, y, z, coeff, coeff_theta, g, acc);
+  if (L.noise == UPF_NOISE_CELLULAR)
+    return row_layer_grad_t<THETA, UPF_NOISE_CELLULAR>(c, L, l, x, y, z, coeff, coeff_theta, g, acc);
+  return layer_grad<THETA>(L, l, x, y, z, coeff, coeff_theta, g, acc);  // simplex: per lane
 }
 
 // Row version of field_grad: the same passes, with the row layer functions.
