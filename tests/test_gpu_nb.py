@@ -3,10 +3,11 @@
 With a band that holds the surface (band >= L x half the block diagonal: every point of
 a block lies within half its diagonal of one of its corners; the oracle's FP64 gradient
 gives max |grad f| = 1.31 for the sphere workload and 1.70 for the torus one, and the
-tests use L = 2), the narrow-band mesh is the dense mesh of the per-lane field
-up to order.  Sorting its vertices by key (edge id = node * 3 + axis) gives the dense
+tests use L = 2), the narrow-band mesh is the dense mesh of the row-kernel field up to
+order: the bricks are evaluated in rows with the row kernels' lattice column folds and
+reads (`brick_field`), which give the row kernel's values bit for bit.  Sorting its vertices by key (edge id = node * 3 + axis) gives the dense
 order, so:
-  * vs the oracle's marching cubes of the GPU per-lane field (129^3, 257^3 sphere fBm):
+  * vs the oracle's marching cubes of the GPU row-kernel field (129^3, 257^3 sphere fBm):
     same vertex count, positions bit-identical after the key sort, same triangles;
   * vs the GPU dense path at 513^3 (torus, ridged + Voronoi + mask): the same;
   * normals are the normalized analytic gradient: vs the oracle's FP64 dual gradient at
@@ -63,7 +64,7 @@ def test_nb_equals_oracle_dense_mc(upf, oracle, n):
     s = S.spec_sphere_fbm(1234)
     g = S.uniform_grid(-1.3, 1.3, n)
     keys, pos, nrm, idx, st = nb_numpy(s, g, band_for(g))
-    f = upfm.field_eval_grid(s, g, variant=upfm.EVAL_PER_LANE).cpu().numpy()
+    f = upfm.field_eval_grid(s, g, variant=upfm.EVAL_ROWS).cpu().numpy()
     rpos, _, ridx = oracle.mc(f, g, normals=False)
     sk, spos, _, stri = canonical(keys, pos, None, idx)
     print(f"{n}^3: {st.active_blocks}/{st.blocks} blocks active, V={len(pos)} T={len(idx)}")
@@ -94,7 +95,7 @@ def test_nb_equals_dense_gpu_torus_513(upf):
     s = S.spec_torus_ridged_voronoi(42)
     g = S.uniform_grid(-1.6, 1.6, 513)
     keys, pos, nrm, idx, st = nb_numpy(s, g, band_for(g))
-    f = upfm.field_eval_grid(s, g, variant=upfm.EVAL_PER_LANE)
+    f = upfm.field_eval_grid(s, g, variant=upfm.EVAL_ROWS)
     dense = upfm.mc_extract(f, g, normals=False)
     del f
     torch.cuda.empty_cache()

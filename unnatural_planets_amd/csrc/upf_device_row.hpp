@@ -142,6 +142,31 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
   }
 }
 
+// Column task t of a packed task list: the column's 4 lattice points from table entry e
+// (written by the task list's first lanes, layout below), hashed and folded in y then z
+// into the task's entry of the slice.
+template <int NOISE>
+__device__ __forceinline__ void col_fold(const RowCtx& c, int e, int t, bool store, float2* slice) {
+  const float4 p0 = c.oct[3 * e], p1 = c.oct[3 * e + 1];
+  const int off = __float_as_int(c.oct[3 * e + 2].x);
+  const uint32_t hx = (uint32_t)(t + off) * kPX;
+  const uint32_t h0 = (__float_as_uint(p1.x) ^ hx) * kHM, h1 = (__float_as_uint(p1.y) ^ hx) * kHM;
+  const uint32_t h2 = (__float_as_uint(p1.z) ^ hx) * kHM, h3 = (__float_as_uint(p1.w) ^ hx) * kHM;
+  const float ty = p0.x, tz = p0.y, uy = p0.z, uz = p0.w;
+  float A, B = 0.0f;
+  if (NOISE == UPF_NOISE_GRADIENT) {
+    const float4 g0 = c.gtab[h0 >> 28], g1 = c.gtab[h1 >> 28], g2 = c.gtab[h2 >> 28], g3 = c.gtab[h3 >> 28];
+    const float ty1 = ty - 1.0f, tz1 = tz - 1.0f;
+    A = lerpf(lerpf(g0.x, g1.x, uy), lerpf(g2.x, g3.x, uy), uz);
+    const float b0 = fmaf(g0.z, tz, g0.y * ty), b1 = fmaf(g1.z, tz, g1.y * ty1);
+    const float b2 = fmaf(g2.z, tz1, g2.y * ty), b3 = fmaf(g3.z, tz1, g3.y * ty1);
+    B = lerpf(lerpf(b0, b1, uy), lerpf(b2, b3, uy), uz);
+  } else {
+    A = lerpf(lerpf(lattice_value(h0), lattice_value(h1), uy), lerpf(lattice_value(h2), lattice_value(h3), uy), uz);
+  }
+  if (store) slice[t] = make_float2(A, B);
+}
+
 // Columns of every octave of a gradient / value layer at once, for R rows that share x
 // (same column spans) and z and differ in y: column task t (region entry t of the packed
 // regions, row after row) goes to lane t % 64 of round t / 64, whatever its row and
@@ -186,25 +211,7 @@ __device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, 
 #pragma unroll
     for (int j = 1; j < UPF_MAX_OCTAVES; j++)
       if (j < oct) k += tt >= base[j];
-    const int e = r * oct + k;
-    const float4 p0 = c.oct[3 * e], p1 = c.oct[3 * e + 1];
-    const int off = __float_as_int(c.oct[3 * e + 2].x);
-    const uint32_t hx = (uint32_t)(t + off) * kPX;
-    const uint32_t h0 = (__float_as_uint(p1.x) ^ hx) * kHM, h1 = (__float_as_uint(p1.y) ^ hx) * kHM;
-    const uint32_t h2 = (__float_as_uint(p1.z) ^ hx) * kHM, h3 = (__float_as_uint(p1.w) ^ hx) * kHM;
-    const float ty = p0.x, tz = p0.y, uy = p0.z, uz = p0.w;
-    float A, B = 0.0f;
-    if (NOISE == UPF_NOISE_GRADIENT) {
-      const float4 g0 = c.gtab[h0 >> 28], g1 = c.gtab[h1 >> 28], g2 = c.gtab[h2 >> 28], g3 = c.gtab[h3 >> 28];
-      const float ty1 = ty - 1.0f, tz1 = tz - 1.0f;
-      A = lerpf(lerpf(g0.x, g1.x, uy), lerpf(g2.x, g3.x, uy), uz);
-      const float b0 = fmaf(g0.z, tz, g0.y * ty), b1 = fmaf(g1.z, tz, g1.y * ty1);
-      const float b2 = fmaf(g2.z, tz1, g2.y * ty), b3 = fmaf(g3.z, tz1, g3.y * ty1);
-      B = lerpf(lerpf(b0, b1, uy), lerpf(b2, b3, uy), uz);
-    } else {
-      A = lerpf(lerpf(lattice_value(h0), lattice_value(h1), uy), lerpf(lattice_value(h2), lattice_value(h3), uy), uz);
-    }
-    if (t < R * total) slice[t] = make_float2(A, B);
+    col_fold<NOISE>(c, r * oct + k, t, t < R * total, slice);
   }
 }
 
@@ -393,6 +400,156 @@ __device__ __forceinline__ void row_field(const RowCtx& c, const DSpec& s, float
       case UPF_COMPOSE_ELEVATION: out[r] = shape / s.escale; break;
       default: out[r] = e; break;
     }
+  }
+}
+
+// ------------------------------------------------------------------ brick rows
+// A wave over R <= kBrickRows rows that share their x nodes but not y and z (the rows
+// of 9 nodes of a narrow-band brick): lane r * nxr + i holds node i of row r, and the
+// lanes past R * nxr stand in for the last node of the last row, so that lanes 0 and 63
+// hold the row's extreme x.  The collapsed columns of every (row, octave) go into the
+// wave's slice, row r's regions after row r-1's, and each lane reads its own row's.  The
+// lattice and the column folds are those of the row kernels, so a node agrees with them
+// and with the per-lane evaluation to a few ulps.  Simplex layers, and layers whose
+// regions do not fit the slice, run per lane.
+constexpr int kBrickRows = 7;
+constexpr int kBrickOctTab = 3 * kBrickRows * UPF_MAX_OCTAVES;  // float4 entries of the (row, octave) table
+
+// Quotient a / b of small non-negative integers (a < 4096, 1 <= b <= 512; here a task or
+// lane index and a region size) with the divisor given as rb = 1 / b in float32:
+// (a + 1/2) / b lies at least 1 / (2 b) >= 2^-10 away from an integer, and the roundings
+// of rb and of the product move it by less than 4096 * 2^-23 = 2^-11, so truncation
+// gives the exact quotient (an integer division by a variable is a long correction
+// sequence).
+__device__ __forceinline__ int small_div(int a, float rb) { return (int)(((float)a + 0.5f) * rb); }
+
+// The table entry of (row r, octave k) is lane r * oct + k; yz(r, y, z) gives row r's y, z.
+template <int NOISE, class YZ>
+__device__ __forceinline__ void col_write_rows(const RowCtx& c, const DLayer& L, int oct, int R, YZ yz,
+                                               const int* lo, const int* base, int total, float2* slice) {
+  if (c.lane < R * oct) {
+    const int r = small_div(c.lane, 1.0f / (float)oct);
+    const int k = c.lane - r * oct;
+    float f = L.freq[0];
+    int off = lo[0] - base[0];
+#pragma unroll
+    for (int j = 1; j < UPF_MAX_OCTAVES; j++)
+      if (j < oct && k == j) { f = L.freq[j]; off = lo[j] - base[j]; }
+    off -= r * total;
+    float yr, zr;
+    yz(r, yr, zr);
+    const float qy = yr * f, qz = zr * f;
+    const float fy = floorf(qy), fz = floorf(qz);
+    const float ty = qy - fy, tz = qz - fz;
+    const uint32_t y0 = (uint32_t)(int)fy * kPY, z0 = (uint32_t)(int)fz * kPZ;
+    const uint32_t sk = L.seed + (uint32_t)k;
+    c.oct[3 * c.lane] = make_float4(ty, tz, fade(ty), fade(tz));
+    c.oct[3 * c.lane + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
+                                        __uint_as_float(sk ^ y0 ^ (z0 + kPZ)),
+                                        __uint_as_float(sk ^ (y0 + kPY) ^ (z0 + kPZ)));
+    c.oct[3 * c.lane + 2] = make_float4(__int_as_float(off), 0.0f, 0.0f, 0.0f);
+  }
+  wave_sync();
+  const float rtotal = 1.0f / (float)total;
+  for (int rd = 0; rd * 64 < R * total; rd++) {
+    const int t = 64 * rd + c.lane;
+    const int r = min(small_div(t, rtotal), R - 1);
+    const int tt = t - r * total;
+    int k = 0;
+#pragma unroll
+    for (int j = 1; j < UPF_MAX_OCTAVES; j++)
+      if (j < oct) k += tt >= base[j];
+    col_fold<NOISE>(c, r * oct + k, t, t < R * total, slice);
+  }
+}
+
+// One gradient / value / cellular layer at this lane's node (x, y, z) of row `row`:
+// the writes and reads of row_layer_t, row by row, so the values are bit for bit those
+// of the row kernels at the same nodes (as long as neither falls back to the per-lane
+// functions).
+template <int NOISE, class YZ>
+__device__ __forceinline__ float brick_layer_t(const RowCtx& c, const DLayer& L, int R, int row, YZ yz, float x,
+                                               float y, float z) {
+  constexpr bool CELL = NOISE == UPF_NOISE_CELLULAR;
+  const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
+  float2* slice = reinterpret_cast<float2*>(c.slice);
+  int lo[UPF_MAX_OCTAVES], base[UPF_MAX_OCTAVES];
+  int total = 0;
+  bool fits = true;
+#pragma unroll
+  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+    if (k < oct) {
+      const float q = x * L.freq[k];
+      int hi;
+      lane_range(CELL ? (int)rintf(q) : (int)floorf(q), lo[k], hi);
+      const int n = hi - lo[k] + (CELL ? 3 : 2);
+      base[k] = total;
+      total += CELL ? 72 : n;
+      fits = fits && (!CELL || n <= 7);
+    }
+  }
+  if (!fits || R * total > kRowSlice2) return layer_value(L, x, y, z);
+  if (CELL) {
+    for (int r = 0; r < R; r++) {
+      float yr, zr;
+      yz(r, yr, zr);
+#pragma unroll
+      for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+        if (k < oct) {
+          const float f = L.freq[k];
+          cell_write(c, L, L.seed + (uint32_t)k, yr * f, zr * f, lo[k], slice + r * total + base[k]);
+        }
+      }
+    }
+  } else {
+    col_write_rows<NOISE>(c, L, oct, R, yz, lo, base, total, slice);
+  }
+  wave_sync();
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+    if (k < oct) {
+      const float q = x * L.freq[k];
+      float n;
+      if (CELL) n = cell_read(q, slice + row * total + base[k] + ((int)rintf(q) - lo[k]));
+      else col_read<NOISE, 1>(q, lo[k], slice + base[k] + row * total, total, &n);
+      acc = fractal_acc(L, k, n, acc);
+    }
+  }
+  wave_sync();
+  return acc;
+}
+
+// f at this lane's node (field_eval<false> with the layers from the brick rows).
+template <class YZ>
+__device__ __forceinline__ float brick_field(const RowCtx& c, const DSpec& s, int R, int row, YZ yz, float x, float y,
+                                             float z) {
+  const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
+                         s.compose == UPF_COMPOSE_ELEVATION_RAW;
+  float elev = 0.0f;
+  if (need_elev) {
+    float acc = 0.0f, m = 0.0f;
+    for (int l = 0; l < s.n_layers; l++) {
+      const DLayer& L = s.L[l];
+      float raw;
+      if (L.noise == UPF_NOISE_GRADIENT) raw = brick_layer_t<UPF_NOISE_GRADIENT>(c, L, R, row, yz, x, y, z);
+      else if (L.noise == UPF_NOISE_VALUE) raw = brick_layer_t<UPF_NOISE_VALUE>(c, L, R, row, yz, x, y, z);
+      else if (L.noise == UPF_NOISE_CELLULAR) raw = brick_layer_t<UPF_NOISE_CELLULAR>(c, L, R, row, yz, x, y, z);
+      else raw = layer_value(L, x, y, z);
+      const float v = L.weight * raw;
+      if (L.blend == UPF_BLEND_ADD) acc = acc + v;
+      else if (L.blend == UPF_BLEND_MASK) m = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
+      else acc = fmaf(m, v - acc, acc);
+    }
+    elev = acc;
+  }
+  const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<false>(s, x, y, z);
+  switch (s.compose) {
+    case UPF_COMPOSE_LAND: return shape - s.escale * elev;
+    case UPF_COMPOSE_WATER: return shape;
+    case UPF_COMPOSE_NAVIGATION: return shape - s.escale * fmaxf(elev, 0.0f);
+    case UPF_COMPOSE_ELEVATION: return shape / s.escale;
+    default: return elev;
   }
 }
 

@@ -30,6 +30,7 @@
 #include <cmath>
 
 #include "upf_device_grad.hpp"
+#include "upf_device_row.hpp"
 #include "upf_internal.hpp"
 #include "upf_mc.hpp"
 
@@ -134,6 +135,9 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
                                                         uint16_t* __restrict__ tabs, uint32_t* __restrict__ bcounts,
                                                         uint32_t* __restrict__ unresolved) {
   __shared__ float s_f[kBrick];
+  __shared__ float4 s_slice[4][EXT ? 1 : kRowSlice];
+  __shared__ float4 s_gtab[16];
+  __shared__ float4 s_oct[4][EXT ? 1 : kBrickOctTab];
   const int64_t a = blockIdx.x;
   const uint32_t b = act[a].x;
   int bx, by, bz;
@@ -141,13 +145,45 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
   const int gi0 = bx * kB, gj0 = by * kB, gk0 = bz * kB;
   const bool lx = bx == d.bx - 1, ly = by == d.by - 1, lz = bz == d.bz - 1;
   float* __restrict__ brick = bricks + a * kBrick;
-  for (int n = threadIdx.x; n < kBrick; n += kNbThreads) {
-    const int k = n / (kBN * kBN), r = n - k * kBN * kBN;
-    const int j = r / kBN, i = r - j * kBN;
-    const float v = field_eval<EXT>(s, node_coord(g.o[0], g.s[0], gi0 + i), node_coord(g.o[1], g.s[1], gj0 + j),
-                               node_coord(g.o[2], g.s[2], gk0 + k));
-    s_f[n] = v;
-    brick[n] = v;
+  if (EXT) {  // elevation modes and further shapes: per node
+    for (int n = threadIdx.x; n < kBrick; n += kNbThreads) {
+      const int k = n / (kBN * kBN), r = n - k * kBN * kBN;
+      const int j = r / kBN, i = r - j * kBN;
+      const float v = field_eval<EXT>(s, node_coord(g.o[0], g.s[0], gi0 + i), node_coord(g.o[1], g.s[1], gj0 + j),
+                                      node_coord(g.o[2], g.s[2], gk0 + k));
+      s_f[n] = v;
+      brick[n] = v;
+    }
+  } else {
+    // the brick's 81 rows of 9 nodes (row = j + 9 k) in groups of kBrickRows per wave,
+    // the waves taking groups wave, wave + 4, ...; lane 7 * 9 = 63 stands in for node 8
+    // of the group's last row
+    if (threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], lane);
+    const int rl = lane / kBN;
+    const bool node_lane = lane < kBrickRows * kBN;
+    const int i = node_lane ? lane - rl * kBN : kBN - 1;
+    const float x = node_coord(g.o[0], g.s[0], gi0 + i);
+    for (int grp = wave; grp * kBrickRows < kBN * kBN; grp += 4) {
+      const int row0 = grp * kBrickRows;
+      const int R = min(kBrickRows, kBN * kBN - row0);
+      const int r = node_lane ? min(rl, R - 1) : R - 1;  // lanes past the group's rows repeat its last row
+      auto yz = [&](int rr, float& yy, float& zz) {
+        const int br = row0 + rr, kk = br / kBN, jj = br - kk * kBN;
+        yy = node_coord(g.o[1], g.s[1], gj0 + jj);
+        zz = node_coord(g.o[2], g.s[2], gk0 + kk);
+      };
+      float y, z;
+      yz(r, y, z);
+      const float v = brick_field(ctx, s, R, r, yz, x, y, z);
+      if (node_lane && rl < R) {
+        const int n = (row0 + r) * kBN + i;
+        s_f[n] = v;
+        brick[n] = v;
+      }
+    }
   }
   __syncthreads();
 
