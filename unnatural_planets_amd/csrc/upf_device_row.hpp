@@ -142,14 +142,13 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
   }
 }
 
-// Column task t of a packed task list: the column's 4 lattice points from table entry e
-// (written by the task list's first lanes, layout below), hashed and folded in y then z
-// into the task's entry of the slice.
+// One lattice column X of a (row, octave): p0 = (ty, tz, uy, uz), p1 = the 4 combo seeds
+// (layout below); its 4 lattice points hashed and folded in y, then z.  col_fold: column
+// task t of a packed task list with the table entry e (written by the task list's first
+// lanes), into the task's entry of the slice.
 template <int NOISE>
-__device__ __forceinline__ void col_fold(const RowCtx& c, int e, int t, bool store, float2* slice) {
-  const float4 p0 = c.oct[3 * e], p1 = c.oct[3 * e + 1];
-  const int off = __float_as_int(c.oct[3 * e + 2].x);
-  const uint32_t hx = (uint32_t)(t + off) * kPX;
+__device__ __forceinline__ float2 fold_column(const RowCtx& c, const float4 p0, const float4 p1, uint32_t X) {
+  const uint32_t hx = X * kPX;
   const uint32_t h0 = (__float_as_uint(p1.x) ^ hx) * kHM, h1 = (__float_as_uint(p1.y) ^ hx) * kHM;
   const uint32_t h2 = (__float_as_uint(p1.z) ^ hx) * kHM, h3 = (__float_as_uint(p1.w) ^ hx) * kHM;
   const float ty = p0.x, tz = p0.y, uy = p0.z, uz = p0.w;
@@ -164,7 +163,15 @@ __device__ __forceinline__ void col_fold(const RowCtx& c, int e, int t, bool sto
   } else {
     A = lerpf(lerpf(lattice_value(h0), lattice_value(h1), uy), lerpf(lattice_value(h2), lattice_value(h3), uy), uz);
   }
-  if (store) slice[t] = make_float2(A, B);
+  return make_float2(A, B);
+}
+
+template <int NOISE>
+__device__ __forceinline__ void col_fold(const RowCtx& c, int e, int t, bool store, float2* slice) {
+  const float4 p0 = c.oct[3 * e], p1 = c.oct[3 * e + 1];
+  const int off = __float_as_int(c.oct[3 * e + 2].x);
+  const float2 v = fold_column<NOISE>(c, p0, p1, (uint32_t)(t + off));
+  if (store) slice[t] = v;
 }
 
 // Columns of every octave of a gradient / value layer at once, for R rows that share x
@@ -404,16 +411,18 @@ __device__ __forceinline__ void row_field(const RowCtx& c, const DSpec& s, float
 }
 
 // ------------------------------------------------------------------ brick rows
-// A wave over R <= kBrickRows rows that share their x nodes but not y and z (the rows
-// of 9 nodes of a narrow-band brick): lane r * nxr + i holds node i of row r, and the
-// lanes past R * nxr stand in for the last node of the last row, so that lanes 0 and 63
-// hold the row's extreme x.  The collapsed columns of every (row, octave) go into the
-// wave's slice, row r's regions after row r-1's, and each lane reads its own row's.  The
-// lattice and the column folds are those of the row kernels, so a node agrees with them
-// and with the per-lane evaluation to a few ulps.  Simplex layers, and layers whose
-// regions do not fit the slice, run per lane.
+// A wave over the rows of a brick that share their x nodes but not y and z (the rows of
+// 9 nodes of a narrow-band brick): M groups of kBrickRows rows, row rho = kBrickRows m + r,
+// lane r * nxr + i holding node i of row r of every group (the lanes past kBrickRows *
+// nxr stand in for the last node of row kBrickRows - 1, so that lanes 0 and 63 hold the
+// extreme x).  The collapsed columns of every (row, octave) go into the wave's slice, row
+// rho's regions at rho * total, and each lane reads its M rows with one x part.  The
+// lattice and the column folds and reads are those of the row kernels, so brick values
+// equal the row kernels' at the same nodes bit for bit (while neither falls back to the
+// per-lane functions).  Simplex layers, and layers whose regions do not fit the slice,
+// run per lane.
 constexpr int kBrickRows = 7;
-constexpr int kBrickOctTab = 3 * kBrickRows * UPF_MAX_OCTAVES;  // float4 entries of the (row, octave) table
+constexpr int kBrickGroups = 3;  // M: groups of rows per wave
 
 // Quotient a / b of small non-negative integers (a < 4096, 1 <= b <= 512; here a task or
 // lane index and a region size) with the divisor given as rb = 1 / b in float32:
@@ -423,54 +432,89 @@ constexpr int kBrickOctTab = 3 * kBrickRows * UPF_MAX_OCTAVES;  // float4 entrie
 // sequence).
 __device__ __forceinline__ int small_div(int a, float rb) { return (int)(((float)a + 0.5f) * rb); }
 
-// The table entry of (row r, octave k) is lane r * oct + k; yz(r, y, z) gives row r's y, z.
-template <int NOISE, class YZ>
-__device__ __forceinline__ void col_write_rows(const RowCtx& c, const DLayer& L, int oct, int R, YZ yz,
-                                               const int* lo, const int* base, int total, float2* slice) {
-  if (c.lane < R * oct) {
-    const int r = small_div(c.lane, 1.0f / (float)oct);
-    const int k = c.lane - r * oct;
+// One gradient / value layer at this lane's M nodes (x, y[m], z[m]) of rows
+// rho = kBrickRows m + rl; yz(rho, y, z) gives row rho's coordinates.  The (row, octave)
+// table has two float4 per entry (ty, tz, uy, uz; the combo seeds), entry rho * oct + k.
+template <int NOISE, int M, class YZ>
+__device__ __forceinline__ void brick_layer_m(const RowCtx& c, const DLayer& L, int rl, YZ yz, float x,
+                                              const float* y, const float* z, float* out) {
+  constexpr int NR = kBrickRows * M;
+  const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
+  float2* slice = reinterpret_cast<float2*>(c.slice);
+  int lo[UPF_MAX_OCTAVES], base[UPF_MAX_OCTAVES];
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+    if (k < oct) {
+      int hi;
+      lane_range((int)floorf(x * L.freq[k]), lo[k], hi);
+      base[k] = total;
+      total += hi - lo[k] + 2;
+    }
+  }
+  if (NR * total > kRowSlice2) {
+#pragma unroll
+    for (int m = 0; m < M; m++) out[m] = layer_value(L, x, y[m], z[m]);
+    return;
+  }
+  const float roct = 1.0f / (float)oct;
+  for (int e = c.lane; e < NR * oct; e += 64) {
+    const int rho = small_div(e, roct);
+    const int k = e - rho * oct;
     float f = L.freq[0];
-    int off = lo[0] - base[0];
 #pragma unroll
     for (int j = 1; j < UPF_MAX_OCTAVES; j++)
-      if (j < oct && k == j) { f = L.freq[j]; off = lo[j] - base[j]; }
-    off -= r * total;
+      if (j < oct && k == j) f = L.freq[j];
     float yr, zr;
-    yz(r, yr, zr);
+    yz(rho, yr, zr);
     const float qy = yr * f, qz = zr * f;
     const float fy = floorf(qy), fz = floorf(qz);
     const float ty = qy - fy, tz = qz - fz;
     const uint32_t y0 = (uint32_t)(int)fy * kPY, z0 = (uint32_t)(int)fz * kPZ;
     const uint32_t sk = L.seed + (uint32_t)k;
-    c.oct[3 * c.lane] = make_float4(ty, tz, fade(ty), fade(tz));
-    c.oct[3 * c.lane + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
-                                        __uint_as_float(sk ^ y0 ^ (z0 + kPZ)),
-                                        __uint_as_float(sk ^ (y0 + kPY) ^ (z0 + kPZ)));
-    c.oct[3 * c.lane + 2] = make_float4(__int_as_float(off), 0.0f, 0.0f, 0.0f);
+    c.oct[2 * e] = make_float4(ty, tz, fade(ty), fade(tz));
+    c.oct[2 * e + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
+                                   __uint_as_float(sk ^ y0 ^ (z0 + kPZ)), __uint_as_float(sk ^ (y0 + kPY) ^ (z0 + kPZ)));
   }
   wave_sync();
   const float rtotal = 1.0f / (float)total;
-  for (int rd = 0; rd * 64 < R * total; rd++) {
+  for (int rd = 0; rd * 64 < NR * total; rd++) {
     const int t = 64 * rd + c.lane;
-    const int r = min(small_div(t, rtotal), R - 1);
-    const int tt = t - r * total;
-    int k = 0;
+    const int rho = min(small_div(t, rtotal), NR - 1);
+    const int tt = t - rho * total;
+    int k = 0, off = lo[0] - base[0];  // column X = tt + off of octave k
 #pragma unroll
     for (int j = 1; j < UPF_MAX_OCTAVES; j++)
-      if (j < oct) k += tt >= base[j];
-    col_fold<NOISE>(c, r * oct + k, t, t < R * total, slice);
+      if (j < oct && tt >= base[j]) { k = j; off = lo[j] - base[j]; }
+    const int e = rho * oct + k;
+    const float2 v = fold_column<NOISE>(c, c.oct[2 * e], c.oct[2 * e + 1], (uint32_t)(tt + off));
+    if (t < NR * total) slice[t] = v;
   }
+  wave_sync();
+  float acc[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) acc[m] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+    if (k < oct) {
+      float n[M];
+      col_read<NOISE, M>(x * L.freq[k], lo[k], slice + base[k] + rl * total, kBrickRows * total, n);
+#pragma unroll
+      for (int m = 0; m < M; m++) acc[m] = fractal_acc(L, k, n[m], acc[m]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int m = 0; m < M; m++) out[m] = acc[m];
 }
 
-// One gradient / value / cellular layer at this lane's node (x, y, z) of row `row`:
-// the writes and reads of row_layer_t, row by row, so the values are bit for bit those
-// of the row kernels at the same nodes (as long as neither falls back to the per-lane
-// functions).
-template <int NOISE, class YZ>
-__device__ __forceinline__ float brick_layer_t(const RowCtx& c, const DLayer& L, int R, int row, YZ yz, float x,
-                                               float y, float z) {
-  constexpr bool CELL = NOISE == UPF_NOISE_CELLULAR;
+// One cellular layer at this lane's node of row rl of a group of kBrickRows rows
+// (yz(r, y, z) for r < kBrickRows): the feature points of each row written once (63
+// lanes), each lane reading its own row's; as row_layer_t.
+template <class YZ>
+__device__ __forceinline__ float brick_cell_layer(const RowCtx& c, const DLayer& L, int rl, YZ yz, float x, float y,
+                                                  float z) {
+  constexpr int R = kBrickRows;
   const int oct = L.fractal == UPF_FRACTAL_NONE ? 1 : L.octaves;
   float2* slice = reinterpret_cast<float2*>(c.slice);
   int lo[UPF_MAX_OCTAVES], base[UPF_MAX_OCTAVES];
@@ -479,30 +523,24 @@ __device__ __forceinline__ float brick_layer_t(const RowCtx& c, const DLayer& L,
 #pragma unroll
   for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
     if (k < oct) {
-      const float q = x * L.freq[k];
       int hi;
-      lane_range(CELL ? (int)rintf(q) : (int)floorf(q), lo[k], hi);
-      const int n = hi - lo[k] + (CELL ? 3 : 2);
+      lane_range((int)rintf(x * L.freq[k]), lo[k], hi);
       base[k] = total;
-      total += CELL ? 72 : n;
-      fits = fits && (!CELL || n <= 7);
+      total += 72;
+      fits = fits && hi - lo[k] + 3 <= 7;
     }
   }
   if (!fits || R * total > kRowSlice2) return layer_value(L, x, y, z);
-  if (CELL) {
-    for (int r = 0; r < R; r++) {
-      float yr, zr;
-      yz(r, yr, zr);
+  for (int r = 0; r < R; r++) {
+    float yr, zr;
+    yz(r, yr, zr);
 #pragma unroll
-      for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
-        if (k < oct) {
-          const float f = L.freq[k];
-          cell_write(c, L, L.seed + (uint32_t)k, yr * f, zr * f, lo[k], slice + r * total + base[k]);
-        }
+    for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
+      if (k < oct) {
+        const float f = L.freq[k];
+        cell_write(c, L, L.seed + (uint32_t)k, yr * f, zr * f, lo[k], slice + r * total + base[k]);
       }
     }
-  } else {
-    col_write_rows<NOISE>(c, L, oct, R, yz, lo, base, total, slice);
   }
   wave_sync();
   float acc = 0.0f;
@@ -510,46 +548,64 @@ __device__ __forceinline__ float brick_layer_t(const RowCtx& c, const DLayer& L,
   for (int k = 0; k < UPF_MAX_OCTAVES; k++) {
     if (k < oct) {
       const float q = x * L.freq[k];
-      float n;
-      if (CELL) n = cell_read(q, slice + row * total + base[k] + ((int)rintf(q) - lo[k]));
-      else col_read<NOISE, 1>(q, lo[k], slice + base[k] + row * total, total, &n);
-      acc = fractal_acc(L, k, n, acc);
+      acc = fractal_acc(L, k, cell_read(q, slice + rl * total + base[k] + ((int)rintf(q) - lo[k])), acc);
     }
   }
   wave_sync();
   return acc;
 }
 
-// f at this lane's node (field_eval<false> with the layers from the brick rows).
-template <class YZ>
-__device__ __forceinline__ float brick_field(const RowCtx& c, const DSpec& s, int R, int row, YZ yz, float x, float y,
-                                             float z) {
+// f at this lane's M nodes (field_eval<false> with the layers from the brick rows).
+template <int M, class YZ>
+__device__ __forceinline__ void brick_field(const RowCtx& c, const DSpec& s, int rl, YZ yz, float x, const float* y,
+                                            const float* z, float* out) {
   const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
                          s.compose == UPF_COMPOSE_ELEVATION_RAW;
-  float elev = 0.0f;
+  float elev[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) elev[m] = 0.0f;
   if (need_elev) {
-    float acc = 0.0f, m = 0.0f;
+    float acc[M], msk[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m] = msk[m] = 0.0f;
     for (int l = 0; l < s.n_layers; l++) {
       const DLayer& L = s.L[l];
-      float raw;
-      if (L.noise == UPF_NOISE_GRADIENT) raw = brick_layer_t<UPF_NOISE_GRADIENT>(c, L, R, row, yz, x, y, z);
-      else if (L.noise == UPF_NOISE_VALUE) raw = brick_layer_t<UPF_NOISE_VALUE>(c, L, R, row, yz, x, y, z);
-      else if (L.noise == UPF_NOISE_CELLULAR) raw = brick_layer_t<UPF_NOISE_CELLULAR>(c, L, R, row, yz, x, y, z);
-      else raw = layer_value(L, x, y, z);
-      const float v = L.weight * raw;
-      if (L.blend == UPF_BLEND_ADD) acc = acc + v;
-      else if (L.blend == UPF_BLEND_MASK) m = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
-      else acc = fmaf(m, v - acc, acc);
+      float raw[M];
+      if (L.noise == UPF_NOISE_GRADIENT) {
+        brick_layer_m<UPF_NOISE_GRADIENT, M>(c, L, rl, yz, x, y, z, raw);
+      } else if (L.noise == UPF_NOISE_VALUE) {
+        brick_layer_m<UPF_NOISE_VALUE, M>(c, L, rl, yz, x, y, z, raw);
+      } else if (L.noise == UPF_NOISE_CELLULAR) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+          auto yz_m = [&](int r, float& yy, float& zz) { yz(kBrickRows * m + r, yy, zz); };
+          raw[m] = brick_cell_layer(c, L, rl, yz_m, x, y[m], z[m]);
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < M; m++) raw[m] = layer_value(L, x, y[m], z[m]);
+      }
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const float v = L.weight * raw[m];
+        if (L.blend == UPF_BLEND_ADD) acc[m] = acc[m] + v;
+        else if (L.blend == UPF_BLEND_MASK) msk[m] = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
+        else acc[m] = fmaf(msk[m], v - acc[m], acc[m]);
+      }
     }
-    elev = acc;
+#pragma unroll
+    for (int m = 0; m < M; m++) elev[m] = acc[m];
   }
-  const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<false>(s, x, y, z);
-  switch (s.compose) {
-    case UPF_COMPOSE_LAND: return shape - s.escale * elev;
-    case UPF_COMPOSE_WATER: return shape;
-    case UPF_COMPOSE_NAVIGATION: return shape - s.escale * fmaxf(elev, 0.0f);
-    case UPF_COMPOSE_ELEVATION: return shape / s.escale;
-    default: return elev;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<false>(s, x, y[m], z[m]);
+    switch (s.compose) {
+      case UPF_COMPOSE_LAND: out[m] = shape - s.escale * elev[m]; break;
+      case UPF_COMPOSE_WATER: out[m] = shape; break;
+      case UPF_COMPOSE_NAVIGATION: out[m] = shape - s.escale * fmaxf(elev[m], 0.0f); break;
+      case UPF_COMPOSE_ELEVATION: out[m] = shape / s.escale; break;
+      default: out[m] = elev[m]; break;
+    }
   }
 }
 

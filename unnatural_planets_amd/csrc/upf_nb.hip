@@ -133,11 +133,13 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
                                                         const uint2* __restrict__ act,
                                                         const int32_t* __restrict__ map, float* __restrict__ bricks,
                                                         uint16_t* __restrict__ tabs, uint32_t* __restrict__ bcounts,
-                                                        uint32_t* __restrict__ unresolved) {
-  __shared__ float s_f[kBrick];
+                                                        uint32_t* __restrict__ unresolved, int oct_max) {
+  __shared__ float s_fe[EXT ? kBrick : 1];
   __shared__ float4 s_slice[4][EXT ? 1 : kRowSlice];
   __shared__ float4 s_gtab[16];
-  __shared__ float4 s_oct[4][EXT ? 1 : kBrickOctTab];
+  // the (row, octave) tables of the 4 waves, 2 * kBrickRows * kBrickGroups * oct_max float4
+  // each (launch-sized: the spec's largest octave count sets the LDS, hence the occupancy)
+  extern __shared__ float4 s_oct[];
   const int64_t a = blockIdx.x;
   const uint32_t b = act[a].x;
   int bx, by, bz;
@@ -145,6 +147,10 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
   const int gi0 = bx * kB, gj0 = by * kB, gk0 = bz * kB;
   const bool lx = bx == d.bx - 1, ly = by == d.by - 1, lz = bz == d.bz - 1;
   float* __restrict__ brick = bricks + a * kBrick;
+  // the brick's values in LDS; the row path stores them over its slices once every wave
+  // has finished with its slice (LDS sets the occupancy here)
+  float* s_f = EXT ? s_fe : reinterpret_cast<float*>(&s_slice[0][0]);
+  static_assert(sizeof(float) * kBrick <= sizeof(float4) * 4 * kRowSlice, "the brick fits the slices");
   if (EXT) {  // elevation modes and further shapes: per node
     for (int n = threadIdx.x; n < kBrick; n += kNbThreads) {
       const int k = n / (kBN * kBN), r = n - k * kBN * kBN;
@@ -155,33 +161,41 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
       brick[n] = v;
     }
   } else {
-    // the brick's 81 rows of 9 nodes (row = j + 9 k) in groups of kBrickRows per wave,
-    // the waves taking groups wave, wave + 4, ...; lane 7 * 9 = 63 stands in for node 8
-    // of the group's last row
+    // the brick's 81 rows of 9 nodes (row = j + 9 k) in 12 groups of kBrickRows; wave w
+    // takes groups w, w + 4, w + 8 (kBrickGroups of them), its row rho = 7 m + r being
+    // brick row 7 (w + 4 m) + r (the 3 rows past 80 repeat row 80 and are not stored);
+    // lane 7 * 9 = 63 stands in for node 8 of row 6
+    static_assert(kBrickRows * kBrickGroups * 4 >= kBN * kBN, "four waves cover the brick's rows");
     if (threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], lane);
-    const int rl = lane / kBN;
+    const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct + wave * (2 * kBrickRows * kBrickGroups * oct_max),
+                                    lane);
     const bool node_lane = lane < kBrickRows * kBN;
+    const int rl = node_lane ? lane / kBN : kBrickRows - 1;
     const int i = node_lane ? lane - rl * kBN : kBN - 1;
     const float x = node_coord(g.o[0], g.s[0], gi0 + i);
-    for (int grp = wave; grp * kBrickRows < kBN * kBN; grp += 4) {
-      const int row0 = grp * kBrickRows;
-      const int R = min(kBrickRows, kBN * kBN - row0);
-      const int r = node_lane ? min(rl, R - 1) : R - 1;  // lanes past the group's rows repeat its last row
-      auto yz = [&](int rr, float& yy, float& zz) {
-        const int br = row0 + rr, kk = br / kBN, jj = br - kk * kBN;
-        yy = node_coord(g.o[1], g.s[1], gj0 + jj);
-        zz = node_coord(g.o[2], g.s[2], gk0 + kk);
-      };
-      float y, z;
-      yz(r, y, z);
-      const float v = brick_field(ctx, s, R, r, yz, x, y, z);
-      if (node_lane && rl < R) {
-        const int n = (row0 + r) * kBN + i;
-        s_f[n] = v;
-        brick[n] = v;
+    auto brick_row = [&](int rho) {
+      const int m = rho / kBrickRows;
+      return min(kBrickRows * (wave + 4 * m) + (rho - m * kBrickRows), kBN * kBN - 1);
+    };
+    auto yz = [&](int rho, float& yy, float& zz) {
+      const int br = brick_row(rho), kk = br / kBN, jj = br - kk * kBN;
+      yy = node_coord(g.o[1], g.s[1], gj0 + jj);
+      zz = node_coord(g.o[2], g.s[2], gk0 + kk);
+    };
+    float y[kBrickGroups], z[kBrickGroups], v[kBrickGroups];
+#pragma unroll
+    for (int m = 0; m < kBrickGroups; m++) yz(kBrickRows * m + rl, y[m], z[m]);
+    brick_field<kBrickGroups>(ctx, s, rl, yz, x, y, z, v);
+    __syncthreads();  // every wave done with its slice
+#pragma unroll
+    for (int m = 0; m < kBrickGroups; m++) {
+      const int br = kBrickRows * (wave + 4 * m) + rl;
+      if (node_lane && br < kBN * kBN) {
+        const int n = br * kBN + i;
+        s_f[n] = v[m];
+        brick[n] = v[m];
       }
     }
   }
@@ -473,10 +487,16 @@ int upf_nb_extract(const upf_field_spec* spec, const upf_grid* grid, float iso, 
   hipLaunchKernelGGL(k_nb_map, nblk(nact), dim3(256), 0, hs, act, nact, map);
   if (modes)
     hipLaunchKernelGGL(k_nb_fine<true>, dim3((unsigned)nact), dim3(kNbThreads), 0, hs, s, g, d, iso, act, map, bricks,
-                       tabs, bcounts, reinterpret_cast<uint32_t*>(total_b + 1));
-  else
-    hipLaunchKernelGGL(k_nb_fine<false>, dim3((unsigned)nact), dim3(kNbThreads), 0, hs, s, g, d, iso, act, map, bricks,
-                       tabs, bcounts, reinterpret_cast<uint32_t*>(total_b + 1));
+                       tabs, bcounts, reinterpret_cast<uint32_t*>(total_b + 1), 0);
+  else {
+    int oct_max = 1;  // octaves of the largest gradient / value layer (the brick rows' table)
+    for (int l = 0; l < s.n_layers; l++)
+      if (s.L[l].noise == UPF_NOISE_GRADIENT || s.L[l].noise == UPF_NOISE_VALUE)
+        oct_max = std::max(oct_max, s.L[l].fractal == UPF_FRACTAL_NONE ? 1 : s.L[l].octaves);
+    const size_t lds = sizeof(float4) * 4 * 2 * kBrickRows * kBrickGroups * oct_max;
+    hipLaunchKernelGGL(k_nb_fine<false>, dim3((unsigned)nact), dim3(kNbThreads), lds, hs, s, g, d, iso, act, map,
+                       bricks, tabs, bcounts, reinterpret_cast<uint32_t*>(total_b + 1), oct_max);
+  }
   UPF_LAUNCH_CHECK();
   if ((st = mc_scan_launch(bcounts, nact, tiles, total_b, voff, nullptr, toff, hs, kScanBlocks))) return st;
   uint4 hb[2];
