@@ -142,6 +142,43 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
   }
 }
 
+// The single-octave columns of two rows (y0, y1; shared z and x) in one round when each
+// row needs at most 8: lanes 32 h + 4 col + combo, the same lattice points, hashes and
+// folds as col_write (so the same entries), row h's at region + h * stride.
+template <int NOISE>
+__device__ __forceinline__ void col_write_pair(const RowCtx& c, uint32_t seed, float qy0, float qy1, float qz, int lo,
+                                               int n, float2* region, int stride) {
+  const int h = c.lane >> 5;
+  const float qy = h ? qy1 : qy0;
+  const float fy = floorf(qy), fz = floorf(qz);
+  const float ty = qy - fy, tz = qz - fz;
+  const float uy = fade(ty), uz = fade(tz);
+  const int iz = __builtin_amdgcn_readfirstlane((int)fz);
+  const uint32_t y0 = (uint32_t)(int)fy * kPY, z0 = (uint32_t)iz * kPZ;
+  const int combo = c.lane & 3;
+  const uint32_t s = seed ^ ((combo & 1) ? y0 + kPY : y0) ^ ((combo & 2) ? z0 + kPZ : z0);
+  const float ay = (combo & 1) ? ty - 1.0f : ty, az = (combo & 2) ? tz - 1.0f : tz;
+  const int xo = (c.lane >> 2) & 7;
+  const uint32_t hh = (s ^ ((uint32_t)lo * kPX + (uint32_t)xo * kPX)) * kHM;
+  float a, b;
+  if (NOISE == UPF_NOISE_GRADIENT) {
+    const float4 g = c.gtab[hh >> 28];
+    a = g.x;
+    b = fmaf(g.z, az, g.y * ay);
+  } else {
+    a = lattice_value(hh);
+    b = 0.0f;
+  }
+  const float ya = lerpf(a, quad_xor1(a), uy);
+  const float za = lerpf(ya, quad_xor2(ya), uz);
+  float zb = 0.0f;
+  if (NOISE == UPF_NOISE_GRADIENT) {
+    const float yb = lerpf(b, quad_xor1(b), uy);
+    zb = lerpf(yb, quad_xor2(yb), uz);
+  }
+  if (combo == 0 && xo < n) region[h * stride + xo] = make_float2(za, zb);
+}
+
 // One lattice column X of a (row, octave): p0 = (ty, tz, uy, uz), p1 = the 4 combo seeds
 // (layout below); its 4 lattice points hashed and folded in y, then z.  col_fold: column
 // task t of a packed task list with the table entry e (written by the task list's first
@@ -323,6 +360,8 @@ __device__ __forceinline__ void row_layer_t(const RowCtx& c, const DLayer& L, fl
           cell_write(c, L, L.seed + (uint32_t)k, y[r] * f, z * f, lo[k], slice + r * total + base[k]);
         }
       }
+  } else if (oct == 1 && R == 2 && total <= 8) {
+    col_write_pair<NOISE>(c, L.seed, y[0] * L.freq[0], y[R - 1] * L.freq[0], z * L.freq[0], lo[0], total, slice, total);
   } else if (oct == 1) {
 #pragma unroll
     for (int r = 0; r < R; r++) col_write<NOISE>(c, L.seed, y[r] * L.freq[0], z * L.freq[0], lo[0], total, slice + r * total);
