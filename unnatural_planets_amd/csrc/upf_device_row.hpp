@@ -142,14 +142,17 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
   }
 }
 
-// The single-octave columns of two rows (y0, y1; shared z and x) in one round when each
-// row needs at most 8: lanes 32 h + 4 col + combo, the same lattice points, hashes and
-// folds as col_write (so the same entries), row h's at region + h * stride.
-template <int NOISE>
-__device__ __forceinline__ void col_write_pair(const RowCtx& c, uint32_t seed, float qy0, float qy1, float qz, int lo,
-                                               int n, float2* region, int stride) {
-  const int h = c.lane >> 5;
-  const float qy = h ? qy1 : qy0;
+// The single-octave columns of R rows (qy[R]; shared z and x) in one round when each row
+// needs at most 16 / R: lanes (64 / R) h + 4 col + combo, the same lattice points, hashes
+// and folds as col_write (so the same entries), row h's at region + h * stride.
+template <int NOISE, int R>
+__device__ __forceinline__ void col_write_group(const RowCtx& c, uint32_t seed, const float* qyr, float qz, int lo,
+                                                int n, float2* region, int stride) {
+  const int h = c.lane / (64 / R);
+  float qy = qyr[0];
+#pragma unroll
+  for (int q = 1; q < R; q++)
+    if (h == q) qy = qyr[q];
   const float fy = floorf(qy), fz = floorf(qz);
   const float ty = qy - fy, tz = qz - fz;
   const float uy = fade(ty), uz = fade(tz);
@@ -158,7 +161,7 @@ __device__ __forceinline__ void col_write_pair(const RowCtx& c, uint32_t seed, f
   const int combo = c.lane & 3;
   const uint32_t s = seed ^ ((combo & 1) ? y0 + kPY : y0) ^ ((combo & 2) ? z0 + kPZ : z0);
   const float ay = (combo & 1) ? ty - 1.0f : ty, az = (combo & 2) ? tz - 1.0f : tz;
-  const int xo = (c.lane >> 2) & 7;
+  const int xo = (c.lane >> 2) & (16 / R - 1);
   const uint32_t hh = (s ^ ((uint32_t)lo * kPX + (uint32_t)xo * kPX)) * kHM;
   float a, b;
   if (NOISE == UPF_NOISE_GRADIENT) {
@@ -226,7 +229,7 @@ template <int NOISE, int R>
 __device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, int oct, const float* y, float z,
                                               const int* lo, const int* base, int total, float2* slice) {
   if (c.lane < R * oct) {
-    const int r = (R == 2 && c.lane >= oct) ? 1 : 0;
+    const int r = R == 1 ? 0 : (R == 2 ? (c.lane >= oct ? 1 : 0) : c.lane / oct);
     const int k = c.lane - r * oct;
     float f = L.freq[0];
     int off = lo[0] - base[0];
@@ -234,7 +237,10 @@ __device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, 
     for (int j = 1; j < UPF_MAX_OCTAVES; j++)
       if (j < oct && k == j) { f = L.freq[j]; off = lo[j] - base[j]; }
     off -= r * total;
-    const float yr = (R == 2 && r) ? y[R - 1] : y[0];
+    float yr = y[0];
+#pragma unroll
+    for (int q = 1; q < R; q++)
+      if (r == q) yr = y[q];
     const float qy = yr * f, qz = z * f;
     const float fy = floorf(qy), fz = floorf(qz);
     const float ty = qy - fy, tz = qz - fz;
@@ -249,7 +255,9 @@ __device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, 
   wave_sync();
   for (int rd = 0; rd * 64 < R * total; rd++) {
     const int t = 64 * rd + c.lane;
-    const int r = (R == 2 && t >= total) ? 1 : 0;
+    int r = 0;
+#pragma unroll
+    for (int q = 1; q < R; q++) r += t >= q * total;
     const int tt = t - r * total;
     int k = 0;
 #pragma unroll
@@ -360,8 +368,11 @@ __device__ __forceinline__ void row_layer_t(const RowCtx& c, const DLayer& L, fl
           cell_write(c, L, L.seed + (uint32_t)k, y[r] * f, z * f, lo[k], slice + r * total + base[k]);
         }
       }
-  } else if (oct == 1 && R == 2 && total <= 8) {
-    col_write_pair<NOISE>(c, L.seed, y[0] * L.freq[0], y[R - 1] * L.freq[0], z * L.freq[0], lo[0], total, slice, total);
+  } else if (oct == 1 && R > 1 && total <= 16 / R) {
+    float qy[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) qy[r] = y[r] * L.freq[0];
+    col_write_group<NOISE, R>(c, L.seed, qy, z * L.freq[0], lo[0], total, slice, total);
   } else if (oct == 1) {
 #pragma unroll
     for (int r = 0; r < R; r++) col_write<NOISE>(c, L.seed, y[r] * L.freq[0], z * L.freq[0], lo[0], total, slice + r * total);

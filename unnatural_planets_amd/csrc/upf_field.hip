@@ -46,44 +46,55 @@ __global__ __launch_bounds__(256) void k_eval_grid(const DSpec s, const DGrid g,
   if (S) write_signs(S, g, blockIdx.x, j, k, valid, v, iso);
 }
 
-// Wave-cooperative rows (upf_device_row.hpp): each wave evaluates 64 x nodes of two
-// rows j, j+1 of one plane (they share the lattice column spans and the x part of the
-// interpolation); lanes past the end of the row stand in for the last node so that all
-// 64 lanes take part in the shared lattice work, and only real nodes are stored.  A
-// workgroup covers 64 x nodes of kRowPairs row pairs.
-constexpr int kRowPairs = 4;
+// Wave-cooperative rows (upf_device_row.hpp): each wave evaluates 64 x nodes of
+// kRowsPerWave rows j .. j + kRowsPerWave - 1 of one plane (they share the lattice column
+// spans and the x part of the interpolation); lanes past the end of the row stand in for
+// the last node so that all 64 lanes take part in the shared lattice work, rows past the
+// end of the plane repeat its last row, and only real nodes are stored.  A workgroup
+// covers 64 x nodes of kRowWaves row groups.
+// Two rows: four rows per wave would share more (the x part over 4 rows) but measured
+// 3.5 ms at the same workload: the fully unrolled 4-row layers spill (400-600 B of
+// scratch at 80 to 128 VGPRs).
+constexpr int kRowsPerWave = 2;
+constexpr int kRowWaves = 4;
+static_assert(kRowsPerWave <= kRowMaxRows, "the per-octave table holds kRowMaxRows rows");
 
 // Six waves per SIMD (at most 80 VGPRs, from 87 unconstrained: a 16-byte spill on a cold
 // path): the kernel keeps the VALU busy ~93% of the time at five waves and the extra
 // wave fills the issue slots the others leave; 1.80 -> 1.63 ms on the 512^3 torus
 // workload (seven waves 1.65 ms, eight 1.85 ms with spills in the hot loop).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_eval_grid_row(const DSpec s, const DGrid g, float* __restrict__ out,
-                                                       uint64_t* __restrict__ S, float iso) {
-  __shared__ float4 s_slice[kRowPairs][kRowSlice];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_eval_grid_row(
+    const DSpec s, const DGrid g, float* __restrict__ out, uint64_t* __restrict__ S, float iso) {
+  constexpr int R = kRowsPerWave;
+  __shared__ float4 s_slice[kRowWaves][kRowSlice];
   __shared__ float4 s_gtab[16];
-  __shared__ float4 s_oct[kRowPairs][kRowOctTab];
+  __shared__ float4 s_oct[kRowWaves][kRowOctTab];
   if (threadIdx.y == 0 && threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
   __syncthreads();
   const int lane = threadIdx.x;
   const int i = blockIdx.x * kGridBX + lane;
-  const int j = __builtin_amdgcn_readfirstlane((blockIdx.y * kRowPairs + threadIdx.y) * 2);
+  const int j = __builtin_amdgcn_readfirstlane((blockIdx.y * kRowWaves + threadIdx.y) * R);
   const int k = blockIdx.z;
   if (j >= g.n[1]) return;
-  const bool two = j + 1 < g.n[1];
+  const int rows = min(R, g.n[1] - j);  // real rows of this wave
   const int ic = min(i, g.n[0] - 1);
   const float x = node_coord(g.o[0], g.s[0], ic);
-  const float y[2] = {node_coord(g.o[1], g.s[1], j), node_coord(g.o[1], g.s[1], two ? j + 1 : j)};
+  float y[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) y[r] = node_coord(g.o[1], g.s[1], j + min(r, rows - 1));
   const float z = node_coord(g.o[2], g.s[2], k);
   const RowCtx c = make_row_ctx(s_slice[threadIdx.y], s_gtab, s_oct[threadIdx.y], lane);
-  float v[2];
-  row_field<2>(c, s, x, y, z, v);
+  float v[R];
+  row_field<R>(c, s, x, y, z, v);
   const bool valid = i < g.n[0];
   const size_t row0 = ((size_t)k * g.n[1] + j) * g.n[0];
-  if (valid) out[row0 + i] = v[0];
-  if (two && valid) out[row0 + g.n[0] + i] = v[1];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (r < rows && valid) out[row0 + (size_t)r * g.n[0] + i] = v[r];
   if (S) {
-    write_signs(S, g, blockIdx.x, j, k, valid, v[0], iso);
-    if (two) write_signs(S, g, blockIdx.x, j + 1, k, valid, v[1], iso);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (r < rows) write_signs(S, g, blockIdx.x, j + r, k, valid, v[r], iso);
   }
 }
 
@@ -215,8 +226,9 @@ int field_grid_launch(const DSpec& s, const DGrid& g, float* d_field, int varian
     else
       hipLaunchKernelGGL(k_eval_grid<false>, blocks, dim3(kGridBX, kGridBY, 1), 0, stream, s, g, d_field, S, iso);
   } else {
-    const dim3 rblocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + 2 * kRowPairs - 1) / (2 * kRowPairs), g.n[2]);
-    hipLaunchKernelGGL(k_eval_grid_row, rblocks, dim3(kGridBX, kRowPairs, 1), 0, stream, s, g, d_field, S, iso);
+    constexpr int rows_per_block = kRowsPerWave * kRowWaves;
+    const dim3 rblocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + rows_per_block - 1) / rows_per_block, g.n[2]);
+    hipLaunchKernelGGL(k_eval_grid_row, rblocks, dim3(kGridBX, kRowWaves, 1), 0, stream, s, g, d_field, S, iso);
   }
   UPF_LAUNCH_CHECK();
   return UPF_OK;
