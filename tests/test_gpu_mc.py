@@ -169,13 +169,17 @@ def _grid(dims, lo, hi):
     return g
 
 
-@pytest.mark.parametrize("dims", [(128, 128, 128), (97, 70, 33), (2, 2, 2)])
+@pytest.mark.parametrize("dims", [(128, 128, 128), (97, 70, 33), (2, 2, 2), "lakes"])
 def test_field_mesh_async_equals_two_calls(upf, dims):
     """upf_field_mesh_async (no host synchronisation) = upf_field_mesh_count + upf_mc_emit
-    bit for bit, twice in a row on the same buffers; rows past the counts stay untouched."""
+    bit for bit, twice in a row on the same buffers; rows past the counts stay untouched.
+    The elevation-mode case runs the per-lane kernel, which stores the sign words too."""
     torch = _torch()
-    s = S.spec_sphere_fbm(1234) if dims[0] == 128 else S.spec_torus_ridged_voronoi(42)
-    g = _grid(dims, -1.3, 1.3)
+    if dims == "lakes":
+        s, g = S.spec_elevation_mode("lakes", 5), S.grid_reference(72)
+    else:
+        s = S.spec_sphere_fbm(1234) if dims[0] == 128 else S.spec_torus_ridged_voronoi(42)
+        g = _grid(dims, -1.3, 1.3)
     f_ref, ws, nv, nt = upf.field_mesh_count(s, g, 0.0)
     ref = upf.mc_emit_counted(f_ref, g, 0.0, ws, nv, nt)
     cap_v, cap_t = nv + 100, nt + 100
