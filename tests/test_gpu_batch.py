@@ -52,6 +52,30 @@ def test_batch_matches_per_chunk(upf):
     _check(upf, b, ref)
 
 
+def test_batch_with_compositions_matches_per_chunk(upf):
+    """Chunks whose shapes are compositions next to primitives: the batch then runs its
+    composition-capable field instantiation, the per-chunk path one per chunk."""
+    from unnatural_planets_amd.batch import Batch
+    man = _manifest(3)
+    extra = [("cylinder", (1.6, 0.8), 41), ("octahedron", (1.1, 0.1), 57), ("double_torus", (0.9, 0.35, 0.2), 48)]
+    chunks = (S.Chunk * (len(man) + len(extra)))()
+    base = S.batch_chunks(man)
+    for i in range(len(man)):
+        chunks[i] = base[i]
+    for i, (name, params, n) in enumerate(extra):
+        c = chunks[len(man) + i]
+        c.spec = S.field_spec(name, params, [S.layer("gradient", "fbm", 3, 2.0, 2.0, 0.5, 0.05, 90 + i)])
+        c.grid = S.uniform_grid(-1.6, 1.6, n)
+        c.iso = 0.0
+    print("case: batch of primitives and compositions", flush=True)
+    b = Batch(chunks)
+    b.size_outputs()
+    b.run()
+    ref = _reference(upf, chunks)
+    assert all(m.n_triangles > 100 for _, m in ref)
+    _check(upf, b, ref)
+
+
 def test_batch_graph_replays_and_autotune(upf):
     import torch
 

@@ -107,6 +107,40 @@ def test_nb_equals_dense_gpu_torus_513(upf):
     np.testing.assert_array_equal(stri, canon_tris(dense.indices.cpu().numpy().astype(np.int64)))
 
 
+def test_nb_fallback_layer_equals_dense_rows(upf):
+    """A layer too fine for both row forms (over 512 lattice columns per 64-node dense row,
+    over 512 / 21 per 9-node brick row) runs per lane in both, beside the other layers'
+    row forms, so the meshes stay equal bit for bit."""
+    s = S.spec_sphere_fbm(1234)
+    s.layers[s.n_layers] = S.layer("gradient", "none", 1, 900.0, 2.0, 0.5, 1e-4, 77)
+    s.n_layers += 1
+    g = S.uniform_grid(-1.3, 1.3, 257)
+    keys, pos, nrm, idx, st = nb_numpy(s, g, band_for(g))
+    f = upfm.field_eval_grid(s, g, variant=upfm.EVAL_ROWS)
+    dense = upfm.mc_extract(f, g, normals=False)
+    sk, spos, _, stri = canonical(keys, pos, None, idx)
+    print(f"fallback layer 257^3: V={len(pos)} T={len(idx)}")
+    assert len(sk) == dense.n_vertices and len(stri) == dense.n_triangles
+    np.testing.assert_array_equal(spos, dense.positions.cpu().numpy())
+    np.testing.assert_array_equal(stri, canon_tris(dense.indices.cpu().numpy().astype(np.int64)))
+
+
+def test_nb_composition_shape_equals_dense_rows(upf):
+    """A composition shape (cylinder) under the fBm layer: the bricks run the
+    composition-capable instantiation and still give the dense row field's mesh."""
+    s = S.field_spec("cylinder", (1.6, 0.8), [S.layer("gradient", "fbm", 4, 2.0, 2.0, 0.5, 0.05, 31)])
+    g = S.uniform_grid(-1.3, 1.3, 129)
+    keys, pos, nrm, idx, st = nb_numpy(s, g, band_for(g))
+    f = upfm.field_eval_grid(s, g, variant=upfm.EVAL_ROWS)
+    dense = upfm.mc_extract(f, g, normals=False)
+    sk, spos, _, stri = canonical(keys, pos, None, idx)
+    print(f"cylinder 129^3: V={len(pos)} T={len(idx)}")
+    assert len(stri) > 1000
+    assert len(sk) == dense.n_vertices and len(stri) == dense.n_triangles
+    np.testing.assert_array_equal(spos, dense.positions.cpu().numpy())
+    np.testing.assert_array_equal(stri, canon_tris(dense.indices.cpu().numpy().astype(np.int64)))
+
+
 def test_nb_errors(upf):
     s = S.spec_torus_ridged_voronoi(42)
     g = S.uniform_grid(-1.6, 1.6, 257)

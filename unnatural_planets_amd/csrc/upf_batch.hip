@@ -52,7 +52,7 @@ __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ ch, int n, 
 
 // Six waves per SIMD at most 80 VGPRs, as the single-grid row kernel (the row variant
 // needs 86 unconstrained; 2.665 -> 2.63 ms for the 64-chunk batch).
-template <int VARIANT, bool EXT>
+template <int VARIANT, bool EXT, bool COMPOSED = true>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_field(
     const BChunk* __restrict__ ch, int n_chunks, int64_t n_items, float* __restrict__ field, uint64_t* __restrict__ S) {
   __shared__ float4 s_slice[4][kRowSlice];
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       v[0] = field_eval<EXT>(C.spec, x, y[0], z);
       v[1] = two ? field_eval<EXT>(C.spec, x, y[1], z) : v[0];
     } else {
-      row_field<2>(ctx, C.spec, x, y, z, v);
+      row_field<2, COMPOSED>(ctx, C.spec, x, y, z, v);
     }
     const bool valid = i < C.grid.n[0];
     if (valid) {
@@ -176,6 +176,7 @@ struct upf_batch {
   int cus = 256;
   int64_t nodes = 0, groups = 0, rows = 0;
   bool any_ext = false;  // a chunk with an elevation mode or a further shape: the field runs per lane
+  bool any_composed = false;  // a chunk whose shape is a composition (spec_composed)
   upf_batch_opts opts{};
   size_t off_S = 0, off_counts = 0, off_voff = 0, off_toff = 0, off_act = 0, off_tiles = 0, off_total = 0,
          ws_bytes = 0;
@@ -223,9 +224,12 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
   else if (b->opts.field_variant == UPF_EVAL_PER_LANE)
     hipLaunchKernelGGL((k_batch_field<UPF_EVAL_PER_LANE, false>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
                        field, S);
+  else if (b->any_composed)
+    hipLaunchKernelGGL((k_batch_field<UPF_EVAL_ROWS, false, true>), dim3(grid), dim3(256), 0, s, b->dev, b->n,
+                       b->rows, field, S);
   else
-    hipLaunchKernelGGL((k_batch_field<UPF_EVAL_ROWS, false>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
-                       field, S);
+    hipLaunchKernelGGL((k_batch_field<UPF_EVAL_ROWS, false, false>), dim3(grid), dim3(256), 0, s, b->dev, b->n,
+                       b->rows, field, S);
   hipLaunchKernelGGL(k_batch_classify, dim3((unsigned)((b->groups + 255) / 256)), dim3(256), 0, s, b->dev, b->n,
                      b->groups, S, counts);
   UPF_LAUNCH_CHECK();
@@ -267,6 +271,7 @@ int upf_batch_create(const upf_chunk* chunks, int32_t n_chunks, const upf_batch_
     C.md = mc_layout(C.grid).d;
     C.iso = chunks[c].iso;
     if (spec_ext(C.spec)) b->any_ext = true;
+    if (spec_composed(C.spec)) b->any_composed = true;
     C.gx = (C.grid.n[0] + 63) / 64;
     C.node_off = b->nodes;
     C.group_off = b->groups;

@@ -56,6 +56,14 @@ __host__ __device__ inline bool spec_ext(const DSpec& s) {
   return s.mode != UPF_ELEV_LAYERS || s.shape >= UPF_SHAPE_H3O;
 }
 
+// Whether a spec's shape is one of the compositions (cylinder .. hemispheres).  Kernels
+// instantiated with COMPOSED = false hold only the primitives (none, sphere, box, torus,
+// capsule): the compositions' code in the shape switch costs a hot kernel ~3% even when
+// never taken (registers and code size), so the host picks the instantiation.
+__host__ __device__ inline bool spec_composed(const DSpec& s) {
+  return s.shape >= UPF_SHAPE_CYLINDER && s.shape < UPF_SHAPE_H3O;
+}
+
 // ------------------------------------------------------------------ hash and lattice
 constexpr uint32_t kPX = 501125321u;
 constexpr uint32_t kPY = 1136930381u;
@@ -266,7 +274,7 @@ __device__ __noinline__ float shape_more(int shape, float p0, float p1, float p2
   return shape_more_t<float>(shape, sp, x, y, z, 0.0f);
 }
 
-template <bool EXT = true>
+template <bool EXT = true, bool COMPOSED = true>
 __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, float z) {
   switch (s.shape) {
     case UPF_SHAPE_SPHERE:
@@ -292,7 +300,8 @@ __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, fl
     default:
       if (EXT && s.shape >= UPF_SHAPE_H3O)
         return shape_more(s.shape, s.sp[0], s.sp[1], s.sp[2], s.sp[3], s.w, x, y, z);
-      return shape_ext<float>(s.shape, s.sp, x, y, z, 0.0f);
+      if (COMPOSED) return shape_ext<float>(s.shape, s.sp, x, y, z, 0.0f);
+      return 0.0f;  // not reached: the host launches COMPOSED = false for the primitives only
   }
 }
 

@@ -439,7 +439,7 @@ __device__ __forceinline__ void row_elevation(const RowCtx& c, const DSpec& s, f
 }
 
 // f at (x, y[r], z) for the R rows of the wave.
-template <int R>
+template <int R, bool COMPOSED = true>
 __device__ __forceinline__ void row_field(const RowCtx& c, const DSpec& s, float x, const float* y, float z,
                                           float* out) {
   const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
@@ -448,7 +448,7 @@ __device__ __forceinline__ void row_field(const RowCtx& c, const DSpec& s, float
   if (need_elev) row_elevation<R>(c, s, x, y, z, elev);
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<false>(s, x, y[r], z);
+    const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<false, COMPOSED>(s, x, y[r], z);
     const float e = need_elev ? elev[r] : 0.0f;
     switch (s.compose) {
       case UPF_COMPOSE_LAND: out[r] = shape - s.escale * e; break;
@@ -606,7 +606,7 @@ __device__ __forceinline__ float brick_cell_layer(const RowCtx& c, const DLayer&
 }
 
 // f at this lane's M nodes (field_eval<false> with the layers from the brick rows).
-template <int M, class YZ>
+template <int M, bool COMPOSED = true, class YZ>
 __device__ __forceinline__ void brick_field(const RowCtx& c, const DSpec& s, int rl, YZ yz, float x, const float* y,
                                             const float* z, float* out) {
   const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
@@ -648,7 +648,7 @@ __device__ __forceinline__ void brick_field(const RowCtx& c, const DSpec& s, int
   }
 #pragma unroll
   for (int m = 0; m < M; m++) {
-    const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<false>(s, x, y[m], z[m]);
+    const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<false, COMPOSED>(s, x, y[m], z[m]);
     switch (s.compose) {
       case UPF_COMPOSE_LAND: out[m] = shape - s.escale * elev[m]; break;
       case UPF_COMPOSE_WATER: out[m] = shape; break;

@@ -63,6 +63,7 @@ static_assert(kRowsPerWave <= kRowMaxRows, "the per-octave table holds kRowMaxRo
 // path): the kernel keeps the VALU busy ~93% of the time at five waves and the extra
 // wave fills the issue slots the others leave; 1.80 -> 1.63 ms on the 512^3 torus
 // workload (seven waves 1.65 ms, eight 1.85 ms with spills in the hot loop).
+template <bool COMPOSED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_eval_grid_row(
     const DSpec s, const DGrid g, float* __restrict__ out, uint64_t* __restrict__ S, float iso) {
   constexpr int R = kRowsPerWave;
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   const float z = node_coord(g.o[2], g.s[2], k);
   const RowCtx c = make_row_ctx(s_slice[threadIdx.y], s_gtab, s_oct[threadIdx.y], lane);
   float v[R];
-  row_field<R>(c, s, x, y, z, v);
+  row_field<R, COMPOSED>(c, s, x, y, z, v);
   const bool valid = i < g.n[0];
   const size_t row0 = ((size_t)k * g.n[1] + j) * g.n[0];
 #pragma unroll
@@ -228,7 +229,10 @@ int field_grid_launch(const DSpec& s, const DGrid& g, float* d_field, int varian
   } else {
     constexpr int rows_per_block = kRowsPerWave * kRowWaves;
     const dim3 rblocks((g.n[0] + kGridBX - 1) / kGridBX, (g.n[1] + rows_per_block - 1) / rows_per_block, g.n[2]);
-    hipLaunchKernelGGL(k_eval_grid_row, rblocks, dim3(kGridBX, kRowWaves, 1), 0, stream, s, g, d_field, S, iso);
+    if (spec_composed(s))
+      hipLaunchKernelGGL(k_eval_grid_row<true>, rblocks, dim3(kGridBX, kRowWaves, 1), 0, stream, s, g, d_field, S, iso);
+    else
+      hipLaunchKernelGGL(k_eval_grid_row<false>, rblocks, dim3(kGridBX, kRowWaves, 1), 0, stream, s, g, d_field, S, iso);
   }
   UPF_LAUNCH_CHECK();
   return UPF_OK;

@@ -128,7 +128,7 @@ __global__ void k_nb_map(const uint2* __restrict__ act, int64_t nact, int32_t* _
   if (r < nact) map[act[r].x] = (int32_t)r;
 }
 
-template <bool EXT>
+template <bool EXT, bool COMPOSED = true>
 __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGrid g, const NbDims d, float iso,
                                                         const uint2* __restrict__ act,
                                                         const int32_t* __restrict__ map, float* __restrict__ bricks,
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
     float y[kBrickGroups], z[kBrickGroups], v[kBrickGroups];
 #pragma unroll
     for (int m = 0; m < kBrickGroups; m++) yz(kBrickRows * m + rl, y[m], z[m]);
-    brick_field<kBrickGroups>(ctx, s, rl, yz, x, y, z, v);
+    brick_field<kBrickGroups, COMPOSED>(ctx, s, rl, yz, x, y, z, v);
     __syncthreads();  // every wave done with its slice
 #pragma unroll
     for (int m = 0; m < kBrickGroups; m++) {
@@ -494,8 +494,12 @@ int upf_nb_extract(const upf_field_spec* spec, const upf_grid* grid, float iso, 
       if (s.L[l].noise == UPF_NOISE_GRADIENT || s.L[l].noise == UPF_NOISE_VALUE)
         oct_max = std::max(oct_max, s.L[l].fractal == UPF_FRACTAL_NONE ? 1 : s.L[l].octaves);
     const size_t lds = sizeof(float4) * 4 * 2 * kBrickRows * kBrickGroups * oct_max;
-    hipLaunchKernelGGL(k_nb_fine<false>, dim3((unsigned)nact), dim3(kNbThreads), lds, hs, s, g, d, iso, act, map,
-                       bricks, tabs, bcounts, reinterpret_cast<uint32_t*>(total_b + 1), oct_max);
+    if (spec_composed(s))
+      hipLaunchKernelGGL((k_nb_fine<false, true>), dim3((unsigned)nact), dim3(kNbThreads), lds, hs, s, g, d, iso, act,
+                         map, bricks, tabs, bcounts, reinterpret_cast<uint32_t*>(total_b + 1), oct_max);
+    else
+      hipLaunchKernelGGL((k_nb_fine<false, false>), dim3((unsigned)nact), dim3(kNbThreads), lds, hs, s, g, d, iso, act,
+                         map, bricks, tabs, bcounts, reinterpret_cast<uint32_t*>(total_b + 1), oct_max);
   }
   UPF_LAUNCH_CHECK();
   if ((st = mc_scan_launch(bcounts, nact, tiles, total_b, voff, nullptr, toff, hs, kScanBlocks))) return st;
