@@ -46,8 +46,21 @@ struct RowCtx {
   float4* oct;          // this wave's per-octave table (kRowOctTab entries)
   int lane;
   uint32_t col_px;      // (lane / 4) * PX: the lane's column offset in the lattice hash
-  int cell_lx, cell_c9, cell_dy, cell_dz;  // the lane's feature point in the cellular writer
 };
+
+// The lane's feature point in the cellular writers: (lx, c9) = (lane % 7, lane / 7),
+// c9 = dy + 3 dz.  The lane index passes through an empty asm so that the compiler cannot
+// hoist these values to the kernel's start: held in registers across the whole layer
+// loop they were spilled to scratch (1.8 KB per wave, ~450 MB of HBM writes per 512^3
+// launch); recomputed at the writer they cost a few VALU instructions per call.
+struct CellLane {
+  int lx, c9, dy, dz;
+};
+__device__ __forceinline__ CellLane cell_lane(int lane) {
+  asm volatile("" : "+v"(lane));
+  const int c9 = lane / 7;
+  return {lane % 7, c9, c9 % 3, c9 / 3};
+}
 
 __device__ __forceinline__ RowCtx make_row_ctx(float4* slice, const float4* gtab, float4* oct, int lane) {
   RowCtx c;
@@ -56,10 +69,6 @@ __device__ __forceinline__ RowCtx make_row_ctx(float4* slice, const float4* gtab
   c.oct = oct;
   c.lane = lane;
   c.col_px = (uint32_t)(lane >> 2) * 501125321u;
-  c.cell_lx = lane % 7;
-  c.cell_c9 = lane / 7;
-  c.cell_dy = c.cell_c9 % 3;
-  c.cell_dz = c.cell_c9 / 3;
   return c;
 }
 
@@ -295,8 +304,9 @@ __device__ __forceinline__ void cell_write(const RowCtx& c, const DLayer& L, uin
                                            int lo, float2* region) {
   const int cy = __builtin_amdgcn_readfirstlane((int)rintf(qy));
   const int cz = __builtin_amdgcn_readfirstlane((int)rintf(qz));
-  const int lx = c.cell_lx, c9 = c.cell_c9;
-  const int kx = lo - 1 + lx, ky = cy - 1 + c.cell_dy, kz = cz - 1 + c.cell_dz;
+  const CellLane cl = cell_lane(c.lane);
+  const int lx = cl.lx, c9 = cl.c9;
+  const int kx = lo - 1 + lx, ky = cy - 1 + cl.dy, kz = cz - 1 + cl.dz;
   const uint32_t h = (seed ^ ((uint32_t)kx * kPX) ^ ((uint32_t)ky * kPY) ^ ((uint32_t)kz * kPZ)) * kHM;
   const uint32_t m = h ^ (h >> 15);
   const float px = fmaf((float)(m & 1023u), L.js, (float)kx - L.j0);

@@ -115,8 +115,9 @@ __device__ __forceinline__ void cell_write_d(const RowCtx& c, const DLayer& L, u
                                              int lo, float4* region) {
   const int cy = __builtin_amdgcn_readfirstlane((int)rintf(qy));
   const int cz = __builtin_amdgcn_readfirstlane((int)rintf(qz));
-  const int lx = c.cell_lx, c9 = c.cell_c9;
-  const int kx = lo - 1 + lx, ky = cy - 1 + c.cell_dy, kz = cz - 1 + c.cell_dz;
+  const CellLane cl = cell_lane(c.lane);
+  const int lx = cl.lx, c9 = cl.c9;
+  const int kx = lo - 1 + lx, ky = cy - 1 + cl.dy, kz = cz - 1 + cl.dz;
   const uint32_t h = (seed ^ ((uint32_t)kx * kPX) ^ ((uint32_t)ky * kPY) ^ ((uint32_t)kz * kPZ)) * kHM;
   const uint32_t m = h ^ (h >> 15);
   const float px = fmaf((float)(m & 1023u), L.js, (float)kx - L.j0);
