@@ -221,12 +221,31 @@ __device__ __forceinline__ float shape_eval_d(const DSpec& s, float x, float y, 
 // ------------------------------------------------------------------ θ accumulator
 // Per-wave accumulation of dLdf * df/dθ: every call site is wave-uniform; each call
 // reduces across the 64 lanes and lane 0 adds into this wave's slice of LDS.
+// v plus the value of the lane that DPP control CTRL names, in the rows ROWS enables
+// (the other rows add 0).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+// Sum of v over the 64 lanes, complete in lane 63: pair and quad sums (quad_perm 1032,
+// 2301), 8- and 16-lane sums (half-row and row mirrors), then rows 1 and 3 add lane 15
+// of the row before (row_bcast:15) and rows 2 and 3 add lane 31 (row_bcast:31).  Six DPP
+// adds; the butterfly over __shfl_xor was six dependent LDS permutes.
+__device__ __forceinline__ float wave_sum_to_lane63(float v) {
+  v = dpp_add<0xB1, 0xF>(v);
+  v = dpp_add<0x4E, 0xF>(v);
+  v = dpp_add<0x141, 0xF>(v);
+  v = dpp_add<0x140, 0xF>(v);
+  v = dpp_add<0x142, 0xA>(v);
+  return dpp_add<0x143, 0xC>(v);
+}
+
 struct ThetaAcc {
   float* slice;  // LDS, UPF_THETA_COUNT floats for this wave
   __device__ __forceinline__ void add(int idx, float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) slice[idx] += v;
+    v = wave_sum_to_lane63(v);
+    if ((threadIdx.x & 63) == 63) slice[idx] += v;
   }
 };
 
