@@ -162,6 +162,80 @@ def bench_bake(upf, S, torch, reps: int):
     }
 
 
+def bench_post(upf, S, torch, reps: int):
+    """Mesh post-processing on the 512^3 torus mesh (meshRemoveDisconnected, the water
+    mask of sources/meshGeneration.cpp:41,62-95) and texture post-processing of 4096^2
+    images (dilation by 7 and U8, sources/textures.cpp:73-81; the glTF conversions of
+    sources/generator.cpp:136-139).  The mesh calls return their counts to the host, so
+    each is timed on the host clock around a synchronized call; the image calls are timed
+    with HIP events on the launch stream."""
+    import time
+    s = S.spec_torus_ridged_voronoi(42)
+    g = S.grid_torus(512)
+    mesh = upf.mesh_generate_generic(s, g, remove_disconnected=False)
+    torch.cuda.synchronize()
+
+    def host_timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, r
+
+    t_rd, kept = host_timed(lambda: upf.mesh_remove_disconnected(mesh))
+    t_wm, water = host_timed(lambda: upf.mesh_water_mask(s, mesh, 0.0))
+    out = {"mesh": {"workload": "512^3 torus + ridged + Voronoi mesh", "vertices": mesh.n_vertices,
+                    "triangles": mesh.n_triangles, "remove_disconnected_ms": round(t_rd * 1e3, 3),
+                    "kept_triangles": kept.n_triangles, "water_mask_ms": round(t_wm * 1e3, 3),
+                    "water_triangles": water.n_triangles}}
+    del mesh, kept, water
+
+    n = 4096
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    holes = torch.rand((n // 8, n // 8), device="cuda", generator=gen) < 0.3
+    holes = holes.repeat_interleave(8, 0).repeat_interleave(8, 1)
+    imgs = []
+    for c in (3, 2, 1):
+        im = torch.rand((n, n, c), device="cuda", generator=gen)
+        im[holes] = float("nan")
+        imgs.append(im.squeeze(2) if c == 1 else im)
+    stream = torch.cuda.current_stream()
+
+    def ev_timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    work = [im.clone() for im in imgs]
+
+    def dilate_all():
+        for w, im in zip(work, imgs):
+            w.copy_(im)
+            upf.image_dilate(w, 7)
+
+    t_copy = ev_timed(lambda: [w.copy_(im) for w, im in zip(work, imgs)])
+    t_dil = ev_timed(dilate_all) - t_copy
+    u8 = [upf.image_to_u8(w) for w in work]
+    t_u8 = ev_timed(lambda: [upf.image_to_u8(w) for w in work])
+    t_nrm = ev_timed(lambda: upf.image_height_to_normal(u8[2], 1.0))
+    t_pbr = ev_timed(lambda: upf.image_special_to_gltf_pbr(u8[1]))
+    texels = n * n
+    # dilation: each of the 7 rounds reads and writes the 6 channels (24 B) of a texel
+    dil_bytes = 7 * texels * 6 * 4 * 2
+    out["texture"] = {"workload": "4096^2 albedo (3 ch), special (2 ch), height (1 ch), 30% uncovered in 8x8 blocks",
+                      "dilate7_ms": round(t_dil * 1e3, 3), "dilate_gbs": round(dil_bytes / t_dil / 1e9, 1),
+                      "to_u8_ms": round(t_u8 * 1e3, 3), "height_to_normal_ms": round(t_nrm * 1e3, 3),
+                      "special_to_gltf_ms": round(t_pbr * 1e3, 3)}
+    return out
+
+
 def bench_material(upf, S, torch, reps: int):
     """The full material stack (coloringDefault, poles and flowers on) on a reference
     planet: sphere r=1100 m + the lakes elevation mode (seed 3) meshed on the 128^3
@@ -408,6 +482,7 @@ def main() -> int:
     ap.add_argument("--no-bake", action="store_true")
     ap.add_argument("--no-nb", action="store_true", help="skip the narrow-band 8-planet workload")
     ap.add_argument("--no-material", action="store_true", help="skip the full material stack")
+    ap.add_argument("--no-post", action="store_true", help="skip the mesh and texture post-processing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -604,6 +679,9 @@ def main() -> int:
 
     if not args.no_material:
         out["material"] = bench_material(upf, S, torch, max(3, args.steps // 2))
+
+    if not args.no_post:
+        out["post"] = bench_post(upf, S, torch, max(3, args.steps // 2))
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(spec, grid, args.cpu_seconds)

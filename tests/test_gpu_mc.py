@@ -73,6 +73,30 @@ def test_mc_matches_oracle_torus_96(upf, oracle):
     assert boundary == 0 and bad == 0 and oriented
 
 
+@pytest.mark.parametrize("kind", ["sphere_192", "plane_x"])
+def test_mc_matches_oracle_few_cells_per_group(upf, oracle, kind):
+    """Surfaces that cross only a few cells of each 64-cell group (a sphere seen along x,
+    a plane x = const), so that the emission packs up to 8 groups into one pass: triangle
+    ids then come from each group's own offset within the pass."""
+    if kind == "sphere_192":
+        n = 192
+        c = np.linspace(-1.0, 1.0, n, dtype=np.float64)
+        z, y, x = np.meshgrid(c, c, c, indexing="ij")
+        f_np = (np.sqrt(x * x + y * y + z * z) - 0.77).astype(np.float32)
+        g = S.uniform_grid(-1.0, 1.0, n)
+    else:
+        g = S.Grid()
+        g.origin[:] = (0.0, 0.0, 0.0)
+        g.spacing[:] = (1.0, 1.0, 1.0)
+        g.dims[:] = (300, 40, 24)
+        xs = np.arange(300, dtype=np.float32)
+        f_np = np.broadcast_to(xs - 150.37 + 0.2 * np.sin(np.arange(40))[:, None], (24, 40, 300)).astype(np.float32)
+        f_np = np.ascontiguousarray(f_np)
+    print(f"case: mc few cells per group {kind}", flush=True)
+    m, gp, gi = _mc_same_field(upf, oracle, f_np, g)
+    assert m.n_triangles > 1000
+
+
 @pytest.mark.parametrize("dims", [(2, 2, 2), (70, 33, 5), (64, 64, 3), (65, 3, 7), (130, 9, 4)])
 def test_mc_ragged_and_tiny(upf, oracle, dims):
     rng = np.random.default_rng(sum(dims))

@@ -117,12 +117,8 @@ __device__ __forceinline__ uint4 unpack_counts(uint32_t c) {
 __device__ __forceinline__ uint4 block_exclusive_scan(uint4 v, uint4* total) {
   __shared__ uint4 wsum[kScanThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint4 inc = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned ax = __shfl_up(inc.x, off, 64), ay = __shfl_up(inc.y, off, 64), az = __shfl_up(inc.z, off, 64);
-    if (lane >= off) { inc.x += ax; inc.y += ay; inc.z += az; }
-  }
+  const uint4 inc = make_uint4((unsigned)wave_inclusive_sum((int)v.x), (unsigned)wave_inclusive_sum((int)v.y),
+                               (unsigned)wave_inclusive_sum((int)v.z), 0u);
   if (lane == 63) wsum[wave] = inc;
   __syncthreads();
   uint4 before = make_uint4(0, 0, 0, 0), all = make_uint4(0, 0, 0, 0);

@@ -50,6 +50,32 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// v plus the value of the lane that DPP control CTRL names, in the rows ROWS enables
+// (lanes of other rows, and lanes whose source lies outside their row, add 0).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_add_i(int v) {
+  return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+
+// Inclusive sum of v over lanes 0..lane for the first LANES lanes (8 or 64; the other
+// lanes get partial sums): within each 16-lane row by row shifts of 1, 2, 4 and 8, then
+// rows 1 and 3 add lane 15 of the row before (row_bcast:15) and rows 2 and 3 add lane 31
+// (row_bcast:31).  DPP adds in place of the LDS permutes of a __shfl_up scan.  Every
+// lane of the wave must be active.
+template <int LANES = 64>
+__device__ __forceinline__ int wave_inclusive_sum(int v) {
+  static_assert(LANES == 8 || LANES == 64, "8 or 64 lanes");
+  v = dpp_add_i<0x111, 0xF>(v);
+  v = dpp_add_i<0x112, 0xF>(v);
+  v = dpp_add_i<0x114, 0xF>(v);
+  if (LANES == 64) {
+    v = dpp_add_i<0x118, 0xF>(v);
+    v = dpp_add_i<0x142, 0xA>(v);
+    v = dpp_add_i<0x143, 0xC>(v);
+  }
+  return v;
+}
+
 // packed per-group counts: vertices (bits 0-7, <= 192), triangles (8-16, <= 320), active (17)
 __device__ __forceinline__ uint32_t pack_counts(int nv, int nt) {
   return (uint32_t)nv | ((uint32_t)nt << 8) | ((nv | nt) ? (1u << 17) : 0u);
@@ -111,19 +137,14 @@ __device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restr
   uint64_t* w = P.w[lane];
   w[0] = a; w[1] = a1; w[2] = ay1; w[3] = ay; w[4] = b; w[5] = b1; w[6] = by1; w[7] = by;
   P.nt[lane] = 0u;
-  int incl = nmix;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += v;
-  }
-  const int total = __shfl(incl, 63, 64);
+  const int incl = wave_inclusive_sum(nmix);
+  const int total = __builtin_amdgcn_readlane(incl, 63);
   for (int base = 0; base < total;) {
     // the groups whose cells lie within [base, base + 64); the first group left starts at
     // base and has at most 64 cells, so every pass takes one
     const bool take = nmix > 0 && incl - nmix >= base && incl <= base + 64;
     const uint64_t tk = ballot64(take);
-    const int end = __shfl(incl, 63 - __clzll((long long)tk), 64);
+    const int end = __builtin_amdgcn_readlane(incl, 63 - __clzll((long long)tk));
     if (take) {
       uint64_t m = mixed;
       int p = incl - nmix - base;
@@ -281,15 +302,11 @@ __device__ __forceinline__ int mc_emit_pass(const float* __restrict__ f, const M
     items = (any & ~all) | n[0].bx | n[0].by | n[0].bz;
     nit = __popcll(items);
   }
-  int incl = nit;
-#pragma unroll
-  for (int o = 1; o < kEmitPassGroups; o <<= 1) {
-    const int v = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += v;
-  }
+  static_assert(kEmitPassGroups == 8, "the group scan covers 8 lanes");
+  const int incl = wave_inclusive_sum<kEmitPassGroups>(nit);
   const int pre = incl - nit;
   const int npass = __popcll(ballot64(lane < ng && incl <= 64));  // >= 1
-  const int nitems = __shfl(incl, npass - 1, 64);
+  const int nitems = __builtin_amdgcn_readlane(incl, npass - 1);
   if (lane < npass) P.pre[lane] = pre;
   // ---- 3. the list of cells
   for (int b = 0; b < npass; b++) {
@@ -397,14 +414,11 @@ __device__ __forceinline__ int mc_emit_pass(const float* __restrict__ f, const M
            (bit1(6) << 6) | (bit(6) << 7);
   }
   const int nt = upf_mc_ntris[code];
-  int inc = nt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(inc, o, 64);
-    if (lane >= o) inc += v;
-  }
+  const int inc = wave_inclusive_sum(nt);
   const int exc = inc - nt;
-  const int seg = __shfl(exc, P.pre[b], 64);  // the triangles of the pass's earlier groups
+  // the triangles of the pass's earlier groups: exc at the first cell of this lane's group
+  // (b is the lane's own group, so the source lane differs between lanes)
+  const int seg = __shfl(exc, P.pre[b], 64);
   if (nt) {
     const uint32_t tid0 = P.toff[b] + (uint32_t)(exc - seg);
     // the cell's triangulation row in one 16-byte load; edge owners from the immediate

@@ -63,12 +63,7 @@ __device__ __forceinline__ void block_coords(const NbDims& d, uint32_t b, int& b
 __device__ __forceinline__ int wg_exclusive_sum(int v, int* total) {
   __shared__ int s_w[kNbThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int inc = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int u = __shfl_up(inc, o, 64);
-    if (lane >= o) inc += u;
-  }
+  const int inc = wave_inclusive_sum(v);
   if (lane == 63) s_w[wave] = inc;
   __syncthreads();
   int before = 0, all = 0;
