@@ -45,8 +45,17 @@ struct RowCtx {
   const float4* gtab;   // 16 gradient vectors in LDS
   float4* oct;          // this wave's per-octave table (kRowOctTab entries)
   int lane;
-  uint32_t col_px;      // (lane / 4) * PX: the lane's column offset in the lattice hash
 };
+
+// The lane index, read where it is used.  The empty asm keeps the compiler from hoisting
+// lane-derived values (LDS addresses, lane masks, hash offsets) to the kernel's start,
+// where, live across the whole layer loop, they were spilled to scratch; recomputed at
+// each layer call they cost one or two VALU instructions each.
+__device__ __forceinline__ int lane_of(const RowCtx& c) {
+  int l = c.lane;
+  asm volatile("" : "+v"(l));
+  return l;
+}
 
 // The lane's feature point in the cellular writers: (lx, c9) = (lane % 7, lane / 7),
 // c9 = dy + 3 dz.  The lane index passes through an empty asm so that the compiler cannot
@@ -68,7 +77,6 @@ __device__ __forceinline__ RowCtx make_row_ctx(float4* slice, const float4* gtab
   c.gtab = gtab;
   c.oct = oct;
   c.lane = lane;
-  c.col_px = (uint32_t)(lane >> 2) * 501125321u;
   return c;
 }
 
@@ -124,10 +132,11 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
   const float uy = fade(ty), uz = fade(tz);
   const int iy = __builtin_amdgcn_readfirstlane((int)fy), iz = __builtin_amdgcn_readfirstlane((int)fz);
   const uint32_t y0 = (uint32_t)iy * kPY, z0 = (uint32_t)iz * kPZ;
-  const int combo = c.lane & 3;
+  const int lane = lane_of(c);
+  const int combo = lane & 3;
   const uint32_t s = seed ^ ((combo & 1) ? y0 + kPY : y0) ^ ((combo & 2) ? z0 + kPZ : z0);
   const float ay = (combo & 1) ? ty - 1.0f : ty, az = (combo & 2) ? tz - 1.0f : tz;
-  const uint32_t xbase = (uint32_t)lo * kPX + c.col_px;
+  const uint32_t xbase = (uint32_t)lo * kPX + (uint32_t)(lane >> 2) * kPX;
   for (int r = 0; r * 16 < n; r++) {
     const uint32_t h = (s ^ (xbase + (uint32_t)(16 * r) * kPX)) * kHM;
     float a, b;
@@ -146,7 +155,7 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
       const float yb = lerpf(b, quad_xor1(b), uy);
       zb = lerpf(yb, quad_xor2(yb), uz);
     }
-    const int xo = (c.lane >> 2) + 16 * r;
+    const int xo = (lane >> 2) + 16 * r;
     if (combo == 0 && xo < n) region[xo] = make_float2(za, zb);
   }
 }
@@ -157,7 +166,8 @@ __device__ __forceinline__ void col_write(const RowCtx& c, uint32_t seed, float 
 template <int NOISE, int R>
 __device__ __forceinline__ void col_write_group(const RowCtx& c, uint32_t seed, const float* qyr, float qz, int lo,
                                                 int n, float2* region, int stride) {
-  const int h = c.lane / (64 / R);
+  const int lane = lane_of(c);
+  const int h = lane / (64 / R);
   float qy = qyr[0];
 #pragma unroll
   for (int q = 1; q < R; q++)
@@ -167,10 +177,10 @@ __device__ __forceinline__ void col_write_group(const RowCtx& c, uint32_t seed, 
   const float uy = fade(ty), uz = fade(tz);
   const int iz = __builtin_amdgcn_readfirstlane((int)fz);
   const uint32_t y0 = (uint32_t)(int)fy * kPY, z0 = (uint32_t)iz * kPZ;
-  const int combo = c.lane & 3;
+  const int combo = lane & 3;
   const uint32_t s = seed ^ ((combo & 1) ? y0 + kPY : y0) ^ ((combo & 2) ? z0 + kPZ : z0);
   const float ay = (combo & 1) ? ty - 1.0f : ty, az = (combo & 2) ? tz - 1.0f : tz;
-  const int xo = (c.lane >> 2) & (16 / R - 1);
+  const int xo = (lane >> 2) & (16 / R - 1);
   const uint32_t hh = (s ^ ((uint32_t)lo * kPX + (uint32_t)xo * kPX)) * kHM;
   float a, b;
   if (NOISE == UPF_NOISE_GRADIENT) {
@@ -237,9 +247,10 @@ __device__ __forceinline__ void col_fold(const RowCtx& c, int e, int t, bool sto
 template <int NOISE, int R>
 __device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, int oct, const float* y, float z,
                                               const int* lo, const int* base, int total, float2* slice) {
-  if (c.lane < R * oct) {
-    const int r = R == 1 ? 0 : (R == 2 ? (c.lane >= oct ? 1 : 0) : c.lane / oct);
-    const int k = c.lane - r * oct;
+  const int lane = lane_of(c);
+  if (lane < R * oct) {
+    const int r = R == 1 ? 0 : (R == 2 ? (lane >= oct ? 1 : 0) : lane / oct);
+    const int k = lane - r * oct;
     float f = L.freq[0];
     int off = lo[0] - base[0];
 #pragma unroll
@@ -255,15 +266,15 @@ __device__ __forceinline__ void col_write_all(const RowCtx& c, const DLayer& L, 
     const float ty = qy - fy, tz = qz - fz;
     const uint32_t y0 = (uint32_t)(int)fy * kPY, z0 = (uint32_t)(int)fz * kPZ;
     const uint32_t sk = L.seed + (uint32_t)k;
-    c.oct[3 * c.lane] = make_float4(ty, tz, fade(ty), fade(tz));
-    c.oct[3 * c.lane + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
+    c.oct[3 * lane] = make_float4(ty, tz, fade(ty), fade(tz));
+    c.oct[3 * lane + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
                                         __uint_as_float(sk ^ y0 ^ (z0 + kPZ)),
                                         __uint_as_float(sk ^ (y0 + kPY) ^ (z0 + kPZ)));
-    c.oct[3 * c.lane + 2] = make_float4(__int_as_float(off), 0.0f, 0.0f, 0.0f);
+    c.oct[3 * lane + 2] = make_float4(__int_as_float(off), 0.0f, 0.0f, 0.0f);
   }
   wave_sync();
   for (int rd = 0; rd * 64 < R * total; rd++) {
-    const int t = 64 * rd + c.lane;
+    const int t = 64 * rd + lane;
     int r = 0;
 #pragma unroll
     for (int q = 1; q < R; q++) r += t >= q * total;
@@ -518,7 +529,8 @@ __device__ __forceinline__ void brick_layer_m(const RowCtx& c, const DLayer& L, 
     return;
   }
   const float roct = 1.0f / (float)oct;
-  for (int e = c.lane; e < NR * oct; e += 64) {
+  const int lane = lane_of(c);
+  for (int e = lane; e < NR * oct; e += 64) {
     const int rho = small_div(e, roct);
     const int k = e - rho * oct;
     float f = L.freq[0];
@@ -539,7 +551,7 @@ __device__ __forceinline__ void brick_layer_m(const RowCtx& c, const DLayer& L, 
   wave_sync();
   const float rtotal = 1.0f / (float)total;
   for (int rd = 0; rd * 64 < NR * total; rd++) {
-    const int t = 64 * rd + c.lane;
+    const int t = 64 * rd + lane;
     const int rho = min(small_div(t, rtotal), NR - 1);
     const int tt = t - rho * total;
     int k = 0, off = lo[0] - base[0];  // column X = tt + off of octave k

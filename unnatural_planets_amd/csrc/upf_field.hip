@@ -59,12 +59,13 @@ constexpr int kRowsPerWave = 2;
 constexpr int kRowWaves = 4;
 static_assert(kRowsPerWave <= kRowMaxRows, "the per-octave table holds kRowMaxRows rows");
 
-// Six waves per SIMD (at most 80 VGPRs, from 87 unconstrained: a 16-byte spill on a cold
-// path): the kernel keeps the VALU busy ~93% of the time at five waves and the extra
-// wave fills the issue slots the others leave; 1.80 -> 1.63 ms on the 512^3 torus
-// workload (seven waves 1.65 ms, eight 1.85 ms with spills in the hot loop).
+// Seven waves per SIMD (at most 72 VGPRs): the kernel keeps the VALU busy ~93% of the
+// time at five waves and the extra waves fill the issue slots the others leave.  On the
+// 512^3 torus workload: five waves 1.80 ms, six 1.56 ms, seven 1.45 ms once the
+// lane-derived values are recomputed at their use (lane_of; before, seven waves spilled
+// 11 VGPRs to scratch, 2.5 GB of writes per launch), eight 1.69 ms (29 VGPRs spilled).
 template <bool COMPOSED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_eval_grid_row(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_eval_grid_row(
     const DSpec s, const DGrid g, float* __restrict__ out, uint64_t* __restrict__ S, float iso) {
   constexpr int R = kRowsPerWave;
   __shared__ float4 s_slice[kRowWaves][kRowSlice];
