@@ -29,26 +29,27 @@ namespace upf {
 template <int NOISE>
 __device__ __forceinline__ void col_write_d(const RowCtx& c, const DLayer& L, int oct, float y, float z,
                                             const int* lo, const int* base, int total, float2* slice) {
-  if (c.lane < oct) {
+  const int lane = lane_of(c);
+  if (lane < oct) {
     float f = L.freq[0];
     int off = lo[0] - base[0];
 #pragma unroll
     for (int j = 1; j < UPF_MAX_OCTAVES; j++)
-      if (j < oct && c.lane == j) { f = L.freq[j]; off = lo[j] - base[j]; }
+      if (j < oct && lane == j) { f = L.freq[j]; off = lo[j] - base[j]; }
     const float qy = y * f, qz = z * f;
     const float fy = floorf(qy), fz = floorf(qz);
     const float ty = qy - fy, tz = qz - fz;
     const uint32_t y0 = (uint32_t)(int)fy * kPY, z0 = (uint32_t)(int)fz * kPZ;
-    const uint32_t sk = L.seed + (uint32_t)c.lane;
-    c.oct[3 * c.lane] = make_float4(ty, tz, fade(ty), fade(tz));
-    c.oct[3 * c.lane + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
+    const uint32_t sk = L.seed + (uint32_t)lane;
+    c.oct[3 * lane] = make_float4(ty, tz, fade(ty), fade(tz));
+    c.oct[3 * lane + 1] = make_float4(__uint_as_float(sk ^ y0 ^ z0), __uint_as_float(sk ^ (y0 + kPY) ^ z0),
                                         __uint_as_float(sk ^ y0 ^ (z0 + kPZ)),
                                         __uint_as_float(sk ^ (y0 + kPY) ^ (z0 + kPZ)));
-    c.oct[3 * c.lane + 2] = make_float4(__int_as_float(off), dfade(ty), dfade(tz), 0.0f);
+    c.oct[3 * lane + 2] = make_float4(__int_as_float(off), dfade(ty), dfade(tz), 0.0f);
   }
   wave_sync();
   for (int rd = 0; rd * 64 < total; rd++) {
-    const int t = 64 * rd + c.lane;
+    const int t = 64 * rd + lane;
     int k = 0;
 #pragma unroll
     for (int j = 1; j < UPF_MAX_OCTAVES; j++)

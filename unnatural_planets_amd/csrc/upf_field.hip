@@ -158,11 +158,13 @@ __global__ __launch_bounds__(256) void k_adjoint(const DSpec s, const DGrid g, c
 
 // Fused forward + adjoint over a grid with wave-cooperative rows: persistent waves
 // stride over 64-node row segments (so a block's θ partials are flushed once at the
-// end); lanes past the end of a row stand in for its last node with dLdf = 0.  Five
-// waves per SIMD (at most 96 VGPRs, from 109): 9.94 -> 9.68 ms on the 512^3 torus
-// workload with dL/dtheta (six waves, 80 VGPRs, spill and gain nothing).
+// end); lanes past the end of a row stand in for its last node with dLdf = 0.  Six
+// waves per SIMD (at most 80 VGPRs): on the 512^3 torus workload with dL/dtheta, 9.94 ms
+// unconstrained (109 VGPRs), 8.48 ms at five waves, 7.80 ms at six once the lane-derived
+// values are recomputed at their use (lane_of; the spills left are in the prologue of
+// these persistent waves).
 template <bool THETA>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_adjoint_rows(const DSpec s, const DGrid g, int64_t items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_adjoint_rows(const DSpec s, const DGrid g, int64_t items,
                                                       const float* __restrict__ dLdf, float* __restrict__ fout,
                                                       float* __restrict__ dfdx, float* __restrict__ dLdtheta) {
   __shared__ float4 s_slice[4][kRowSlice];
