@@ -50,10 +50,11 @@ __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ ch, int n, 
   return lo;
 }
 
-// Six waves per SIMD at most 80 VGPRs, as the single-grid row kernel (the row variant
-// needs 86 unconstrained; 2.665 -> 2.63 ms for the 64-chunk batch).
+// Seven waves per SIMD (at most 72 VGPRs), as the single-grid row kernel; with the
+// autotuner's 7 workgroups per CU, 2.12 -> 2.09 ms for the 64-chunk batch (six waves and
+// 8 workgroups per CU before).
 template <int VARIANT, bool EXT, bool COMPOSED = true>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_field(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_batch_field(
     const BChunk* __restrict__ ch, int n_chunks, int64_t n_items, float* __restrict__ field, uint64_t* __restrict__ S) {
   __shared__ float4 s_slice[4][kRowSlice];
   __shared__ float4 s_gtab[16];
@@ -384,10 +385,10 @@ int upf_batch_autotune(upf_batch* b, float* d_field, void* d_ws, size_t ws_bytes
   float best_ms = 1e30f;
   int st = UPF_OK;
   const int variants[2] = {UPF_EVAL_ROWS, UPF_EVAL_PER_LANE};
-  // 6 = the row variant's resident workgroups per CU (six waves per SIMD), 8 the
+  // 7 = the row variant's resident workgroups per CU (seven waves per SIMD), 8 the
   // per-lane variant's: persistent waves take equal shares, so a grid of exactly the
   // resident size has no tail of late waves
-  const int bpcs[4] = {2, 4, 6, 8};
+  const int bpcs[4] = {4, 6, 7, 8};
   for (int cand = 0; cand < 8 && !st; cand++) {
     const upf_batch_opts o{variants[cand / 4], bpcs[cand % 4], start.classify_z, 0};
     if ((st = upf_batch_set_opts(b, &o))) break;
