@@ -64,6 +64,8 @@ static_assert(kRowsPerWave <= kRowMaxRows, "the per-octave table holds kRowMaxRo
 // 512^3 torus workload: five waves 1.80 ms, six 1.56 ms, seven 1.45 ms once the
 // lane-derived values are recomputed at their use (lane_of; before, seven waves spilled
 // 11 VGPRs to scratch, 2.5 GB of writes per launch), eight 1.69 ms (29 VGPRs spilled).
+// The wave's LDS bases come from a scalar wave index and the node index is recomputed
+// after the layers, so nothing is spilled.
 template <bool COMPOSED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_eval_grid_row(
     const DSpec s, const DGrid g, float* __restrict__ out, uint64_t* __restrict__ S, float iso) {
@@ -74,20 +76,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   if (threadIdx.y == 0 && threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
   __syncthreads();
   const int lane = threadIdx.x;
-  const int i = blockIdx.x * kGridBX + lane;
   const int j = __builtin_amdgcn_readfirstlane((blockIdx.y * kRowWaves + threadIdx.y) * R);
   const int k = blockIdx.z;
   if (j >= g.n[1]) return;
   const int rows = min(R, g.n[1] - j);  // real rows of this wave
-  const int ic = min(i, g.n[0] - 1);
+  const int ic = min((int)blockIdx.x * kGridBX + lane, g.n[0] - 1);
   const float x = node_coord(g.o[0], g.s[0], ic);
   float y[R];
 #pragma unroll
   for (int r = 0; r < R; r++) y[r] = node_coord(g.o[1], g.s[1], j + min(r, rows - 1));
   const float z = node_coord(g.o[2], g.s[2], k);
-  const RowCtx c = make_row_ctx(s_slice[threadIdx.y], s_gtab, s_oct[threadIdx.y], lane);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);  // LDS bases in scalar registers
+  const RowCtx c = make_row_ctx(s_slice[wv], s_gtab, s_oct[wv], lane);
   float v[R];
   row_field<R, COMPOSED>(c, s, x, y, z, v);
+  const int i = (int)blockIdx.x * kGridBX + lane_of(c);  // recomputed: not live across the layers
   const bool valid = i < g.n[0];
   const size_t row0 = ((size_t)k * g.n[1] + j) * g.n[0];
 #pragma unroll
