@@ -34,12 +34,11 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, vendor figure (packed FP32)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 # HBM bytes per launch of k_eval_grid_row on the 512^3 workload, from rocprofv3 --pmc
-# FETCH_SIZE (4153 KiB, x 2.0 by the 1 GiB copy calibration) and WRITE_SIZE (634334 KiB,
-# x 1.0), averaged over the launches with and without sign words: each node (512 MiB)
-# and the sign words (16 MiB) written once, plus the part of the kernel's register spills
-# (a few values saved at each wave's start) that reaches HBM
+# FETCH_SIZE (284 KiB, x 2.0 by the 1 GiB copy calibration) and WRITE_SIZE (566886 KiB,
+# x 1.0), averaged over the launches with and without sign words: the kernel reads only
+# its arguments and writes each node (512 MiB) and the sign words (16 MiB) once
 # (profiles/field_torus_ridged_voronoi_512.txt).
-FIELD_TRAFFIC_512 = 2 * 4153 * 1024 + 634334 * 1024
+FIELD_TRAFFIC_512 = 2 * 284 * 1024 + 566886 * 1024
 
 
 def cpu_model() -> str:
