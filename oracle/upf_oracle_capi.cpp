@@ -276,6 +276,58 @@ void upfo_bake_material(const upf_material_spec* spec, const float* P, const flo
     }
 }
 
+// One generate* step of the stack on prepared tiles (for the per-step pins): the tile's
+// inputs (elevation, slope, temperature, precipitation, biome, type and the colour
+// accumulated so far) come in through `io`, the step's outputs go back there.
+// step: 0 elevation, 1 precipitation, 2 temperature, 3 slope, 4 biome, 5 type, 6 bedrock,
+// 7 cliffs, 8 mica, 9 dirt, 10 sand, 11 grass, 12 boulders, 13 water, 14 flowers, 15 ice,
+// 16 snow, 17 visualization, 18 finalization.
+int upfo_material_step(const upf_material_spec* spec, int step, const float* pos, const float* nrm, int64_t n,
+                       upf_tile* io) {
+  if (step < 0 || step > 18) return 1;
+  const mat::Spec s = mat::spec_from(*spec);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; i++) {
+    mat::Tile t;
+    upf_tile& o = io[i];
+    for (int a = 0; a < 3; a++) { t.pos[a] = pos[3 * i + a]; t.nrm[a] = nrm[3 * i + a]; t.albedo[a] = o.albedo[a]; }
+    t.roughness = o.roughness; t.metallic = o.metallic; t.height = o.height; t.opacity = o.opacity;
+    t.elevation = o.elevation; t.slope = o.slope; t.temperature = o.temperature; t.precipitation = o.precipitation;
+    t.biome = o.biome; t.type = o.type;
+    switch (step) {
+      case 0: t.elevation = s.purpose == UPF_PURPOSE_LAND ? mat::elev_sdf(s, t.pos) : elevation_eval(s.field, t.pos); break;
+      case 1: mat::precipitation(s, t); break;
+      case 2: mat::temperature(s, t); break;
+      case 3: mat::slope(s, t); break;
+      case 4: mat::biome(t); break;
+      case 5: mat::type(t); break;
+      case 6: mat::bedrock(s, t); break;
+      case 7: mat::cliffs(s, t); break;
+      case 8: mat::mica(s, t); break;
+      case 9: mat::dirt(s, t); break;
+      case 10: mat::sand(s, t); break;
+      case 11: mat::grass(s, t); break;
+      case 12: mat::boulders(s, t); break;
+      case 13: mat::water(s, t); break;
+      case 14: mat::flowers(s, t); break;
+      case 15: mat::ice(s, t); break;
+      case 16: mat::snow(s, t); break;
+      case 17: mat::visualization(t); break;
+      default: mat::finalize(t); break;
+    }
+    for (int a = 0; a < 3; a++) o.albedo[a] = t.albedo[a];
+    o.roughness = t.roughness; o.metallic = t.metallic; o.height = t.height; o.opacity = t.opacity;
+    o.elevation = t.elevation; o.slope = t.slope; o.temperature = t.temperature; o.precipitation = t.precipitation;
+    o.biome = t.biome; o.type = t.type;
+  }
+  return 0;
+}
+// Voronoi evaluate(position, normal).points[0] of the stack (the plane-projected nearest
+// point, sources/voronoi.cpp:58-72) at n positions.
+void upfo_voronoi_plane(uint32_t seed, int ppc, float cell, const float* pos, const float* nrm, int64_t n, float* out) {
+  for (int64_t i = 0; i < n; i++) mat::voronoi_plane(seed, ppc, cell, pos + 3 * i, nrm + 3 * i, out + 3 * i);
+}
+
 // single noise objects of the stack, for the pins: kind, fractal, octaves, frequency,
 // lacunarity, gain, distance, operation
 float upfo_material_noise(int kind, uint32_t seed, float freq, int fractal, int octaves, float lac, float gain, int dist,

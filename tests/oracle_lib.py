@@ -54,6 +54,8 @@ class Oracle:
             "upfo_image_to_u8": (None, [_P, ctypes.c_int64, _P]),
             "upfo_bake_material": (None, [ctypes.c_void_p, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P]),
             "upfo_material_tiles": (None, [ctypes.c_void_p, _P, _P, ctypes.c_int64, _P]),
+            "upfo_material_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P, _P, ctypes.c_int64, _P]),
+            "upfo_voronoi_plane": (None, [ctypes.c_uint32, ctypes.c_int, ctypes.c_float, _P, _P, ctypes.c_int64, _P]),
             "upfo_material_noise": (ctypes.c_float, [ctypes.c_int, ctypes.c_uint32, ctypes.c_float, ctypes.c_int,
                                                      ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                                      ctypes.c_int, _P]),
@@ -226,6 +228,23 @@ class Oracle:
         self.L.upfo_bake_material(ctypes.byref(m), _f32p(pos), _f32p(nrm), _f32p(idx), _f32p(uv), _f32p(tri), w, h,
                                   _f32p(albedo), _f32p(special), _f32p(height))
         return albedo, special, height
+
+    def material_step(self, m, step, pos, nrm, tiles):
+        """One generate* step on prepared tiles (a [n, 16] float32 array in upf_tile
+        layout, biome and type as int32 bits); returns the updated copy."""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float32).reshape(-1, 3)
+        io = np.array(tiles, dtype=np.float32, order="C", copy=True).reshape(-1, 16)
+        rc = self.L.upfo_material_step(ctypes.byref(m), step, _f32p(pos), _f32p(nrm), len(pos), _f32p(io))
+        assert rc == 0, rc
+        return io
+
+    def voronoi_plane(self, seed, ppc, cell_size, pos, nrm):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float32).reshape(-1, 3)
+        out = np.empty_like(pos)
+        self.L.upfo_voronoi_plane(seed, ppc, cell_size, _f32p(pos), _f32p(nrm), len(pos), _f32p(out))
+        return out
 
     def material_noise(self, kind, seed, freq, p, fractal=0, octaves=3, lac=2.0, gain=0.5, dist=0, ret=0):
         a = np.ascontiguousarray(p, dtype=np.float32)
