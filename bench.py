@@ -292,6 +292,43 @@ def bench_material(upf, S, torch, reps: int):
     }
 
 
+def bench_adjoint_256(upf, S, torch, timed, reps: int):
+    """The differentiation workload: torus R=1, r=0.4 + 4-octave gradient fBm + cellular
+    F1 (seed 7) on a 256^3 grid over [-1.6, 1.6]^3; the fused forward + adjoint grid
+    kernel (f, df/dx per node and the 76 dL/dtheta sums for dL/df = 1) against the plain
+    field on the same grid."""
+    L = upf.lib()
+    s, g = S.spec_torus_fbm_voronoi(7), S.uniform_grid(-1.6, 1.6, 256)
+    n = 256 ** 3
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    f = torch.empty((256, 256, 256), dtype=torch.float32, device="cuda")
+    dLdf = torch.ones_like(f)
+    dfdx = torch.empty((256, 256, 256, 3), dtype=torch.float32, device="cuda")
+    dth = torch.zeros((S.THETA_COUNT,), dtype=torch.float32, device="cuda")
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    def adj():
+        upf.check(L.upf_field_adjoint_grid(ctypes.byref(s), ctypes.byref(g), vp(dLdf), vp(f), vp(dfdx), vp(dth), sp),
+                  "adjoint 256")
+
+    def fwd():
+        upf.check(L.upf_field_eval_grid(ctypes.byref(s), ctypes.byref(g), vp(f), sp), "eval 256")
+    adj()
+    fwd()
+    t_adj, t_fwd = timed(adj, reps), timed(fwd, reps)
+    aflops = float(L.upf_adjoint_flops_per_sample(ctypes.byref(s)))
+    flops = float(L.upf_flops_per_sample(ctypes.byref(s)))
+    return {
+        "workload": "torus R=1 r=0.4 + 4-oct gradient fBm + cellular F1 (seed 7), 256^3 over [-1.6,1.6]^3, "
+                    "f + df/dx + 76 dL/dtheta",
+        "gsamples_per_s": round(n / t_adj / 1e9, 4), "ms": round(t_adj * 1e3, 4), "flops_per_sample": aflops,
+        "tflops": round(aflops * n / t_adj / 1e12, 3),
+        "frac_fp32_peak": round(aflops * n / t_adj / 1e12 / PEAK_FP32_TFLOPS, 4),
+        "plain_ms": round(t_fwd * 1e3, 4), "plain_frac_fp32_peak": round(flops * n / t_fwd / 1e12 / PEAK_FP32_TFLOPS, 4),
+        "vs_plain_time_ratio": round(t_adj / t_fwd, 3),
+    }
+
+
 def bench_batch(upf, S, torch):
     """The uneven 64-chunk batch (committed manifest, seed 0): per-chunk launches (one
     stream-ordered upf_field_mesh_async per chunk, the fastest single-grid path: fused
@@ -353,19 +390,6 @@ def bench_batch(upf, S, torch):
     }
 
 
-def nb_planets():
-    """Eight planets of mixed shapes and seeds: a 5-octave gradient fBm (f0 2.0,
-    lacunarity 2.0, gain 0.5, weight 0.05) on sphere, rounded cube, torus and capsule."""
-    from unnatural_planets_amd import spec as S
-    out = []
-    for p in range(8):
-        shape = S.BATCH_SHAPES[p % 4]
-        out.append((f"{shape}/seed {1000 + p}",
-                    S.field_spec(shape, S.BATCH_SHAPE_PARAMS[shape],
-                                 [S.layer("gradient", "fbm", 5, 2.0, 2.0, 0.5, 0.05, 1000 + p)])))
-    return out
-
-
 def bench_nb(upf, S, torch, reps: int, n: int = 1025, tex: int = 4096):
     """Narrow band: per planet a coarse pass over the 8-cell block corners of an n^3 fine
     grid over [-1.6, 1.6]^3, fine evaluation and meshing of the active blocks (analytic
@@ -374,9 +398,8 @@ def bench_nb(upf, S, torch, reps: int, n: int = 1025, tex: int = 4096):
     import numpy as np
     L = upf.lib()
     g = S.uniform_grid(-1.6, 1.6, n)
-    diag = S.NB_BLOCK * float(np.linalg.norm([g.spacing[0], g.spacing[1], g.spacing[2]]))
-    band = 2.0 * diag / 2  # |grad f| < 2; a block's points lie within diag/2 of a corner
-    planets = nb_planets()
+    band = S.nb_band(g)  # |grad f| < 2; a block's points lie within diag/2 of a corner
+    planets = S.nb_planets()
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
     # sizing pass
@@ -433,6 +456,27 @@ def bench_nb(upf, S, torch, reps: int, n: int = 1025, tex: int = 4096):
     t_mesh, t_bake = t_mesh / npl, t_bake / npl
     act = [int(st.active_blocks) for st in stats]
     blocks = int(stats[0].blocks)
+    # roofline of the mesh pass, per planet on average (host clock around each synchronised
+    # call, so both fractions are lower bounds for the kernels):
+    #   VALU: the coarse block corners and the 9^3 brick nodes of each active block, each
+    #         at the spec's flops per sample (the vertex normals' gradient work is not
+    #         counted);
+    #   HBM:  the coarse corner field (written and read), the dense block -> rank map (4 B
+    #         per block, written and read), each active block's 9^3 brick (4 B per node)
+    #         and node table (2 B per node), each written once and read once, and the mesh
+    #         (24 B per vertex, 12 B per triangle) written.
+    nb_per_axis = (n - 1) // S.NB_BLOCK
+    corners = (nb_per_axis + 1) ** 3
+    mean_flops = float(np.mean([float(L.upf_flops_per_sample(ctypes.byref(sp_))) * (corners + a * 729)
+                                for (_, sp_), a in zip(planets, act)]))
+    mean_bytes = float(np.mean([2 * 4 * corners + 2 * 4 * blocks + a * 729 * (4 + 2) * 2 + 24 * int(st.n_vertices) +
+                                12 * int(st.n_triangles) for a, st in zip(act, stats)]))
+    mesh_tflops = mean_flops / t_mesh / 1e12
+    mesh_gbs = mean_bytes / t_mesh / 1e9
+    # the bake: triangle id in + 20 B out per texel, UV and indices per triangle, position
+    # and normal per vertex (the 1024^2 bake's count), over atlas + rasterize + material
+    bake_bytes = float(np.mean([tex * tex * 24 + 36 * int(st.n_triangles) + 24 * int(st.n_vertices) for st in stats]))
+    bake_gbs = bake_bytes / t_bake / 1e9
     del pos, nrm, idx, uv, tri, albedo, bump, rough, ws
     torch.cuda.empty_cache()
 
@@ -464,6 +508,11 @@ def bench_nb(upf, S, torch, reps: int, n: int = 1025, tex: int = 4096):
         "mesh_ms_per_planet": round(t_mesh * 1e3, 3), "bake_ms_per_planet": round(t_bake * 1e3, 3),
         "latency_ms_per_planet": round((t_mesh + t_bake) * 1e3, 3),
         "effective_gsamples_per_s": round(n ** 3 / t_mesh / 1e9, 2),
+        "mesh_roofline": {"valu_tflops": round(mesh_tflops, 2), "valu_frac": round(mesh_tflops / PEAK_FP32_TFLOPS, 4),
+                          "hbm_gbs": round(mesh_gbs, 1), "hbm_frac": round(mesh_gbs / PEAK_HBM_GBS, 4),
+                          "flops_per_planet": round(mean_flops), "bytes_per_planet": round(mean_bytes)},
+        "bake_roofline": {"hbm_gbs": round(bake_gbs, 1), "hbm_frac": round(bake_gbs / PEAK_HBM_GBS, 4),
+                          "bytes_per_planet": round(bake_bytes)},
         "dense_field_count_ms_planet0": round(t_dense * 1e3, 3),
         "speedup_mesh_vs_dense_sweep": round(t_dense / t_mesh, 2),
     }
@@ -667,6 +716,9 @@ def main() -> int:
             "vs_plain_time_ratio": round(t_adj / t_field, 3),
         }
         del dLdf, dfdx
+
+    if not args.no_adjoint:
+        out["adjoint_256"] = bench_adjoint_256(upf, S, torch, timed, max(3, args.steps // 2))
 
     if not args.no_batch:
         out["batch"] = bench_batch(upf, S, torch)

@@ -676,6 +676,9 @@ void upf_batch_destroy(upf_batch* b);
 int64_t upf_batch_field_floats(const upf_batch* b);
 int64_t upf_batch_field_offset(const upf_batch* b, int32_t chunk);
 size_t upf_batch_workspace_bytes(const upf_batch* b);
+/* Changing the options drops a graph captured with other ones (upf_batch_replay then
+ * returns UPF_ERR_NULL until upf_batch_capture runs again); upf_batch_autotune changes
+ * them, so capture after tuning. */
 int upf_batch_set_opts(upf_batch* b, const upf_batch_opts* opts);
 int upf_batch_get_opts(const upf_batch* b, upf_batch_opts* opts);
 /* Evaluate + mesh every chunk.  Writes past cap_vertices / cap_triangles are dropped;

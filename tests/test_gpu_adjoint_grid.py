@@ -64,3 +64,39 @@ def test_grid_adjoint_matches_points_and_oracle(upf, oracle, case):
     # and the oracle's FP64 dual numbers for the theta sum
     ref = oracle.theta_grad(s, pts, w.reshape(-1))
     assert np.max(np.abs(th - ref)) <= 2e-3 * max(float(np.abs(ref).max()), 1e-6)
+
+
+def test_config4_grid_256_forward_adjoint(upf, oracle):
+    """The differentiation workload at size: torus R=1, r=0.4 + 4-octave gradient fBm +
+    cellular F1 (the 11-parameter theta) on a 256^3 grid over [-1.6, 1.6]^3.  dL/df is
+    nonzero at 20,000 seeded nodes, so the grid's dL/dtheta is the oracle's FP64 dual
+    sum over those nodes; f and df/dx are checked there against the FP64 oracle, and the
+    whole grid's f against the forward row kernel."""
+    import torch
+    s, g = S.spec_torus_fbm_voronoi(7), S.uniform_grid(-1.6, 1.6, 256)
+    n = 256
+    rng = np.random.default_rng(7)
+    sel = rng.choice(n ** 3, size=20000, replace=False)
+    w = np.zeros(n ** 3, dtype=np.float32)
+    w[sel] = rng.standard_normal(len(sel)).astype(np.float32)
+    print("case: config 4 grid adjoint 256^3", flush=True)
+    f, dfdx, th = upfm.field_adjoint_grid(s, g, torch.from_numpy(w.reshape(n, n, n)).cuda())
+    fwd = upfm.field_eval_grid(s, g)
+    assert float((f - fwd).abs().max()) <= 2e-6 * max(1.0, float(fwd.abs().max()))
+    f = f.cpu().numpy().reshape(-1)[sel]
+    dfdx = dfdx.cpu().numpy().reshape(-1, 3)[sel]
+    th = th.cpu().numpy()
+    k, j, i = np.unravel_index(sel, (n, n, n))
+    sp = np.float32(g.spacing[0])
+    pts = np.stack([np.float32(g.origin[0]) + i.astype(np.float32) * sp, np.float32(g.origin[1]) + j.astype(np.float32) * sp,
+                    np.float32(g.origin[2]) + k.astype(np.float32) * sp], 1).astype(np.float32)
+    ref_f = oracle.eval_points(s, pts, f64=True)
+    assert np.max(np.abs(f - ref_f) / np.maximum(1.0, np.abs(ref_f))) <= 1e-5
+    _, ref_g = oracle.grad_points(s, pts)
+    gerr = np.linalg.norm(dfdx - ref_g, axis=1) / np.maximum(np.linalg.norm(ref_g, axis=1), 1e-3)
+    assert np.mean(gerr <= 1e-4) >= 0.998, np.quantile(gerr, [0.5, 0.998, 1.0])
+    ref_th = oracle.theta_grad(s, pts, w[sel])
+    idx = [S.theta_index("amp", 0, o) for o in range(4)] + [S.theta_index("freq", 0, o) for o in range(4)] + \
+        [S.theta_index("weight", 1), S.theta_index("shape", 0), S.theta_index("shape", 1)]
+    scale = float(np.abs(ref_th[idx]).max())
+    assert np.max(np.abs(th[idx] - ref_th[idx])) <= 2e-3 * scale, (th[idx], ref_th[idx])

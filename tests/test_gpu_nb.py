@@ -163,3 +163,61 @@ def test_nb_errors(upf):
     far = S.field_spec("sphere", (10.0,), [])
     m, _, st = upfm.nb_extract(far, g, 0.01)
     assert m.n_triangles == 0 and st.active_blocks == 0
+
+
+def _edge_stats_gpu(idx):
+    """(unique edges, boundary edges, edges in > 2 triangles, consistently oriented) of a
+    triangle list on the GPU (10^7 triangles sort in milliseconds)."""
+    import torch
+    t = idx.to(torch.int64)
+    e = torch.cat([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]], 0)
+    lo, hi = torch.minimum(e[:, 0], e[:, 1]), torch.maximum(e[:, 0], e[:, 1])
+    key = lo * (1 << 32) + hi
+    fwd = (e[:, 0] < e[:, 1]).to(torch.int64)
+    key, order = torch.sort(key)
+    fwd = fwd[order]
+    uniq, counts = torch.unique_consecutive(key, return_counts=True)
+    start = torch.cumsum(counts, 0) - counts
+    dsum = torch.zeros_like(counts).index_add_(0, torch.repeat_interleave(torch.arange(len(counts), device=key.device),
+                                                                          counts), fwd)
+    oriented = bool(torch.all(dsum[counts == 2] == 1))
+    return len(uniq), int((counts == 1).sum()), int((counts > 2).sum()), oriented
+
+
+def test_nb_config5_planet0_1025_equals_dense_gpu(upf):
+    """The combined workload's fine grid (1025^3 over [-1.6, 1.6]^3) for its first planet:
+    the narrow-band mesh equals the dense GPU marching cubes of the row field bit for bit."""
+    import torch
+    name, s = S.nb_planets()[0]
+    g = S.uniform_grid(-1.6, 1.6, 1025)
+    print(f"case: config 5 {name} 1025^3 vs dense", flush=True)
+    keys, pos, nrm, idx, st = nb_numpy(s, g, S.nb_band(g))
+    f = upfm.field_eval_grid(s, g, variant=upfm.EVAL_ROWS)
+    dense = upfm.mc_extract(f, g, normals=False)
+    del f
+    torch.cuda.empty_cache()
+    sk, spos, _, stri = canonical(keys, pos, None, idx)
+    print(f"1025^3: {st.active_blocks}/{st.blocks} blocks active, V={len(pos)} T={len(idx)}")
+    assert len(stri) > 1_000_000
+    assert len(sk) == dense.n_vertices and len(stri) == dense.n_triangles
+    np.testing.assert_array_equal(spos, dense.positions.cpu().numpy())
+    np.testing.assert_array_equal(stri, canon_tris(dense.indices.cpu().numpy().astype(np.int64)))
+
+
+def test_nb_config5_all_planets_closed_with_unit_normals(upf):
+    """All eight planets of the combined workload at 1025^3: every mesh closed (each edge
+    in exactly two triangles, opposite directions), unit normals, no unresolved edges."""
+    import torch
+    g = S.uniform_grid(-1.6, 1.6, 1025)
+    ws = None
+    for name, s in S.nb_planets():
+        print(f"case: config 5 {name}", flush=True)
+        m, _, st = upfm.nb_extract(s, g, S.nb_band(g), workspace=ws)
+        ws = upfm.NbWorkspace(g, int(st.active_blocks)) if ws is None or st.active_blocks > ws.max_active else ws
+        assert st.unresolved == 0 and m.n_triangles > 1_000_000, name
+        ne, boundary, bad, oriented = _edge_stats_gpu(m.indices)
+        assert boundary == 0 and bad == 0 and oriented, (name, boundary, bad)
+        nl = torch.linalg.norm(m.normals, dim=1)
+        assert float((nl - 1).abs().max()) <= 1e-5, name
+        print(f"  V={m.n_vertices} T={m.n_triangles} Euler={m.n_vertices - ne + m.n_triangles}")
+        del m

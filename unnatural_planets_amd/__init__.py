@@ -15,10 +15,18 @@ Reference functions mirrored (paths relative to the reference checkout):
   mesh_generate_base_navigation  sources/meshGeneration.cpp:100-107
   generate_textures_land         sources/textures.cpp:45-71, 86-90 (rasterize + per-texel material)
   texture_post                   sources/textures.cpp:73-81 (dilation by 7, U8 conversion)
+
+Streams: a call given `stream=` runs with that stream current (torch.cuda.stream), so
+its outputs and temporaries are allocated on it and the caching allocator reuses a
+temporary only after the call's kernels on that stream.  The outputs belong to that
+stream: a caller reading them on another stream synchronises first, as with any torch
+tensor made on a side stream.
 """
 from __future__ import annotations
 
 import ctypes
+import functools
+import inspect
 from dataclasses import dataclass
 
 from . import spec as spec_mod
@@ -62,12 +70,37 @@ def _dev(device):
     return torch.device(device if device is not None else "cuda")
 
 
+def _on_stream(fn):
+    """Run fn with its `stream` argument (when given) as torch's current stream, so that
+    everything fn allocates is allocated on the stream its kernels run on."""
+    sig = inspect.signature(fn)
+
+    @functools.wraps(fn)
+    def wrapper(*args, **kw):
+        stream = sig.bind(*args, **kw).arguments.get("stream")
+        if stream is None:
+            return fn(*args, **kw)
+        with _torch().cuda.stream(stream):
+            return fn(*args, **kw)
+    return wrapper
+
+
 def _check_f32(t, name, shape=None):
     torch = _torch()
     if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
         raise ValueError(f"{name} must be a contiguous float32 CUDA tensor")
     if shape is not None and tuple(t.shape) != tuple(shape):
         raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
+def _check_rows3(t, name, dtype="float32"):
+    """t must be a contiguous CUDA tensor of shape [n, 3] (the kernels read 3 values per row)."""
+    torch = _torch()
+    want = torch.float32 if dtype == "float32" else torch.int32
+    if t.dtype != want or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous {dtype} CUDA tensor")
+    if t.dim() != 2 or int(t.shape[1]) != 3:
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected [n, 3]")
 
 
 def flops_per_sample(s: FieldSpec) -> float:
@@ -92,6 +125,7 @@ class EvalOpts(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
+@_on_stream
 def field_eval_grid(s: FieldSpec, g: Grid, out=None, stream=None, variant: int = EVAL_AUTO):
     """f at every node of g -> float32 tensor [nz, ny, nx] (upf_field_eval_grid_ex)."""
     torch = _torch()
@@ -106,10 +140,11 @@ def field_eval_grid(s: FieldSpec, g: Grid, out=None, stream=None, variant: int =
     return out
 
 
+@_on_stream
 def field_eval_points(s: FieldSpec, xyz, with_grad: bool = False, stream=None):
     """f (and df/dx when with_grad) at points xyz [n, 3] (upf_field_eval_points)."""
     torch = _torch()
-    _check_f32(xyz, "xyz")
+    _check_rows3(xyz, "xyz")
     n = xyz.shape[0]
     f = torch.empty((n,), dtype=torch.float32, device=xyz.device)
     g = torch.empty((n, 3), dtype=torch.float32, device=xyz.device) if with_grad else None
@@ -118,12 +153,13 @@ def field_eval_points(s: FieldSpec, xyz, with_grad: bool = False, stream=None):
     return (f, g) if with_grad else f
 
 
+@_on_stream
 def field_adjoint(s: FieldSpec, xyz, dLdf=None, want_f=True, want_dfdx=True, dtheta=None, stream=None):
     """Fused forward + adjoint at points (upf_field_adjoint).  Returns (f, dfdx, dtheta);
     dtheta is a float32 [76] tensor in upf_theta_grad order (sum over points of
     dLdf * df/dθ), None when dLdf is None."""
     torch = _torch()
-    _check_f32(xyz, "xyz")
+    _check_rows3(xyz, "xyz")
     n = xyz.shape[0]
     f = torch.empty((n,), dtype=torch.float32, device=xyz.device) if want_f else None
     g = torch.empty((n, 3), dtype=torch.float32, device=xyz.device) if want_dfdx else None
@@ -138,6 +174,7 @@ def field_adjoint(s: FieldSpec, xyz, dLdf=None, want_f=True, want_dfdx=True, dth
     return f, g, dtheta
 
 
+@_on_stream
 def field_adjoint_grid(s: FieldSpec, gr: Grid, dLdf=None, want_f=True, want_dfdx=True, stream=None):
     """Fused forward + adjoint over grid nodes (upf_field_adjoint_grid)."""
     torch = _torch()
@@ -211,6 +248,7 @@ class McWorkspace:
         self.buf = torch.empty((self.bytes,), dtype=torch.uint8, device=_dev(device))
 
 
+@_on_stream
 def mc_extract(field, g: Grid, iso: float = 0.0, normals: bool = True, workspace: McWorkspace | None = None,
                stream=None) -> Mesh:
     """Marching cubes over field [nz, ny, nx] (upf_mc_count + upf_mc_emit)."""
@@ -236,6 +274,7 @@ def mc_extract(field, g: Grid, iso: float = 0.0, normals: bool = True, workspace
     return Mesh(pos, nrm, idx)
 
 
+@_on_stream
 def mc_case_codes(field, g: Grid, iso: float = 0.0, stream=None):
     torch = _torch()
     nx, ny, nz = grid_dims(g)
@@ -246,6 +285,7 @@ def mc_case_codes(field, g: Grid, iso: float = 0.0, stream=None):
     return out
 
 
+@_on_stream
 def field_mesh_count(s: FieldSpec, g: Grid, iso: float = 0.0, field=None, workspace: McWorkspace | None = None,
                      variant: int = EVAL_AUTO, stream=None):
     """Field evaluation fused with marching-cubes classification (upf_field_mesh_count).
@@ -266,6 +306,7 @@ def field_mesh_count(s: FieldSpec, g: Grid, iso: float = 0.0, field=None, worksp
     return field, ws, int(nv.value), int(nt.value)
 
 
+@_on_stream
 def mc_emit_counted(field, g: Grid, iso: float, ws: McWorkspace, n_vertices: int, n_triangles: int,
                     normals: bool = True, stream=None) -> Mesh:
     """upf_mc_emit after upf_mc_count / upf_field_mesh_count left their state in ws."""
@@ -284,6 +325,7 @@ def mc_emit_counted(field, g: Grid, iso: float, ws: McWorkspace, n_vertices: int
     return Mesh(pos, nrm, idx)
 
 
+@_on_stream
 def field_mesh_async(s: FieldSpec, g: Grid, field, mesh: Mesh, counts, iso: float = 0.0,
                      workspace: McWorkspace | None = None, variant: int = EVAL_AUTO, stream=None) -> McWorkspace:
     """Field + marching cubes in one stream-ordered call (upf_field_mesh_async): the mesh
@@ -297,8 +339,12 @@ def field_mesh_async(s: FieldSpec, g: Grid, field, mesh: Mesh, counts, iso: floa
     ws = workspace or McWorkspace(g, field.device)
     opts = EvalOpts()
     opts.variant = variant
-    mb = MeshBuffers()
+    _check_rows3(mesh.positions, "mesh.positions")
+    _check_rows3(mesh.indices, "mesh.indices", "int32")
     cap_v, cap_t = int(mesh.positions.shape[0]), int(mesh.indices.shape[0])
+    if mesh.normals is not None:
+        _check_f32(mesh.normals, "mesh.normals", (cap_v, 3))
+    mb = MeshBuffers()
     mb.positions = mesh.positions.data_ptr() if cap_v else None
     mb.normals = mesh.normals.data_ptr() if (mesh.normals is not None and cap_v) else None
     mb.indices = mesh.indices.data_ptr() if cap_t else None
@@ -319,8 +365,13 @@ def _mesh_buffers(m: Mesh):
     return mb
 
 
+@_on_stream
 def _mesh_post(fn, m: Mesh, *lead, stream=None) -> Mesh:
     torch = _torch()
+    _check_rows3(m.positions, "positions")
+    _check_rows3(m.indices, "indices", "int32")
+    if m.normals is not None:
+        _check_f32(m.normals, "normals", tuple(m.positions.shape))
     dev = m.positions.device
     bytes_ = int(lib().upf_mesh_post_workspace_bytes(m.n_vertices, m.n_triangles))
     if bytes_ == 0:
@@ -336,12 +387,14 @@ def _mesh_post(fn, m: Mesh, *lead, stream=None) -> Mesh:
     return Mesh(pos[:nv], nrm[:nv] if nrm is not None else None, idx[:nt])
 
 
+@_on_stream
 def mesh_remove_disconnected(m: Mesh, stream=None) -> Mesh:
     """meshRemoveDisconnected (sources/meshGeneration.cpp:41): the connected component with
     the most triangles, order preserved (upf_mesh_remove_disconnected)."""
     return _mesh_post(lib().upf_mesh_remove_disconnected, m, stream=stream)
 
 
+@_on_stream
 def mesh_water_mask(s: FieldSpec, m: Mesh, threshold: float = 0.1, stream=None) -> Mesh:
     """The water-mesh vertex mask of sources/meshGeneration.cpp:62-95: vertices with raw
     elevation below threshold, grown by two triangle rings (upf_mesh_water_mask)."""
@@ -395,6 +448,7 @@ class NbWorkspace:
         self.buf = torch.empty((self.bytes,), dtype=torch.uint8, device=_dev(device))
 
 
+@_on_stream
 def nb_extract(s: FieldSpec, g: Grid, band: float, iso: float = 0.0, normals: bool = True, keys: bool = False,
                workspace: NbWorkspace | None = None, device=None, stream=None):
     """Narrow-band marching cubes of the fine grid g (upf_nb_extract).  Grows the
@@ -441,6 +495,7 @@ class Textures:
     tri: object        # int32 [h, w], covering triangle or -1
 
 
+@_on_stream
 def atlas_per_triangle(n_triangles: int, width: int, height: int, device=None, stream=None):
     """Per-triangle UV atlas (upf_atlas_per_triangle) -> float32 [T, 3, 2]."""
     torch = _torch()
@@ -450,6 +505,7 @@ def atlas_per_triangle(n_triangles: int, width: int, height: int, device=None, s
     return uv
 
 
+@_on_stream
 def bake_rasterize(uv, width: int, height: int, stream=None):
     """Covering triangle per texel (upf_bake_rasterize) -> int32 [h, w], -1 where uncovered."""
     torch = _torch()
@@ -463,15 +519,17 @@ def bake_rasterize(uv, width: int, height: int, stream=None):
     return tri
 
 
+@_on_stream
 def bake(b: BakeSpec, mesh: Mesh, uv, tri, stream=None):
     """Per-texel material (upf_bake) -> (albedo [h, w, 3], bump [h, w], roughness [h, w])."""
     torch = _torch()
     h, w = (int(x) for x in tri.shape)
     n = mesh.n_triangles
-    _check_f32(mesh.positions, "positions")
+    _check_rows3(mesh.positions, "positions")
     if mesh.normals is None:
         raise ValueError("the bake needs vertex normals")
     _check_f32(mesh.normals, "normals", tuple(mesh.positions.shape))
+    _check_rows3(mesh.indices, "indices", "int32")
     _check_f32(uv, "uv", (n, 3, 2))
     if tri.dtype != torch.int32 or not tri.is_contiguous() or not tri.is_cuda:
         raise ValueError("tri must be a contiguous int32 CUDA tensor")
@@ -486,6 +544,7 @@ def bake(b: BakeSpec, mesh: Mesh, uv, tri, stream=None):
     return albedo, bump, rough
 
 
+@_on_stream
 def generate_textures_land(b: BakeSpec, mesh: Mesh, width: int, height: int, stream=None) -> Textures:
     """Atlas + rasterize + per-texel material for a land mesh (sources/textures.cpp:45-71,
     86-90): the reference's Generator<false>::generate up to its dilation and U8 steps."""
@@ -496,6 +555,7 @@ def generate_textures_land(b: BakeSpec, mesh: Mesh, width: int, height: int, str
 
 
 # ---------------------------------------------------------------------------- texture post
+@_on_stream
 def image_dilate(img, rounds: int = 7, stream=None):
     """In-place dilation of an [h, w] or [h, w, c] float32 image (upf_image_dilate)."""
     torch = _torch()
@@ -509,6 +569,7 @@ def image_dilate(img, rounds: int = 7, stream=None):
     return img
 
 
+@_on_stream
 def image_to_u8(img, stream=None):
     """NaN -> 0, else round(saturate(v) * 255) (upf_image_to_u8)."""
     torch = _torch()
@@ -518,6 +579,7 @@ def image_to_u8(img, stream=None):
     return out
 
 
+@_on_stream
 def texture_post(tex: Textures, rounds: int = 7, stream=None):
     """The reference's images after the bake (sources/textures.cpp:73-81): albedo [h, w, 3],
     special [h, w, 2] = (roughness, metallic = 0) and height [h, w], each dilated by
@@ -532,6 +594,7 @@ def texture_post(tex: Textures, rounds: int = 7, stream=None):
     return image_to_u8(albedo, stream), image_to_u8(special, stream), image_to_u8(height, stream)
 
 
+@_on_stream
 def image_height_to_normal(height_u8, strength: float = 1.0, stream=None):
     """U8 height map [h, w] -> U8 normal map [h, w, 3] (upf_image_height_to_normal; the
     reference's imageConvertHeigthToNormal(heightMap, 1), sources/generator.cpp:138)."""
@@ -545,6 +608,7 @@ def image_height_to_normal(height_u8, strength: float = 1.0, stream=None):
     return out
 
 
+@_on_stream
 def image_special_to_gltf_pbr(special_u8, stream=None):
     """U8 (roughness, metallic) [..., 2] -> glTF metallicRoughness [..., 3] = (255, roughness,
     metallic) (upf_image_special_to_gltf_pbr; imageConvertSpecialToGltfPbr,
@@ -558,6 +622,7 @@ def image_special_to_gltf_pbr(special_u8, stream=None):
     return out
 
 
+@_on_stream
 def export_maps(albedo_u8, special_u8, height_u8, stream=None):
     """The conversions LandProcessor::chunkEntry applies before writing a chunk's images
     (sources/generator.cpp:134-139): albedo unchanged, special -> glTF PBR, height ->
@@ -566,12 +631,13 @@ def export_maps(albedo_u8, special_u8, height_u8, stream=None):
 
 
 # ---------------------------------------------------------------------------- material stack
+@_on_stream
 def material_tiles(m: MaterialSpec, positions, normals, stream=None):
     """terrainTile at points with unit normals (upf_material_tiles; the navmesh loop of
     sources/tileProperties.cpp:258-273).  Returns a float32 [n, 16] tensor of upf_tile
     rows (spec.tiles_fields names the columns)."""
     torch = _torch()
-    _check_f32(positions, "positions")
+    _check_rows3(positions, "positions")
     n = int(positions.shape[0])
     _check_f32(normals, "normals", (n, 3))
     out = torch.empty((n, 16), dtype=torch.float32, device=positions.device)
@@ -580,16 +646,18 @@ def material_tiles(m: MaterialSpec, positions, normals, stream=None):
     return out
 
 
+@_on_stream
 def bake_material(m: MaterialSpec, mesh: Mesh, uv, tri, stream=None):
     """Per-texel full material (upf_bake_material; Generator::pixel, sources/textures.cpp:
     24-43) -> (albedo [h, w, 3] or [h, w, 4] for water, special [h, w, 2], height [h, w])."""
     torch = _torch()
     h, w = (int(x) for x in tri.shape)
     n = mesh.n_triangles
-    _check_f32(mesh.positions, "positions")
+    _check_rows3(mesh.positions, "positions")
     if mesh.normals is None:
         raise ValueError("the bake needs vertex normals")
     _check_f32(mesh.normals, "normals", tuple(mesh.positions.shape))
+    _check_rows3(mesh.indices, "indices", "int32")
     _check_f32(uv, "uv", (n, 3, 2))
     if tri.dtype != torch.int32 or not tri.is_contiguous() or not tri.is_cuda:
         raise ValueError("tri must be a contiguous int32 CUDA tensor")
@@ -606,6 +674,7 @@ def bake_material(m: MaterialSpec, mesh: Mesh, uv, tri, stream=None):
     return albedo, special, height
 
 
+@_on_stream
 def generate_textures(m: MaterialSpec, mesh: Mesh, width: int, height: int, stream=None):
     """generateTexturesLand / generateTexturesWater (sources/textures.cpp:86-96) with the
     full material: atlas, rasterization and per-texel tiles, then the dilation by 7 and
@@ -617,6 +686,7 @@ def generate_textures(m: MaterialSpec, mesh: Mesh, width: int, height: int, stre
     return a, s, h, uv
 
 
+@_on_stream
 def texture_post_images(albedo, special, height, rounds: int = 7, stream=None):
     """Dilation and U8 conversion of the three images (sources/textures.cpp:73-81)."""
     ims = [albedo.clone(), special.clone(), height.clone()]

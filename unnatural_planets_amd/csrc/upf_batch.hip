@@ -321,8 +321,15 @@ int upf_batch_set_opts(upf_batch* b, const upf_batch_opts* opts) {
   upf_batch_opts o;
   const int st = normalize_opts(opts, &o);
   if (st) return st;
-  b->opts = o;
-  return upload(b);
+  // a graph captured with other options would replay the old launch shapes: drop it, so
+  // that upf_batch_replay reports UPF_ERR_NULL until upf_batch_capture runs again
+  if (o.field_variant != b->opts.field_variant || o.blocks_per_cu != b->opts.blocks_per_cu ||
+      o.classify_z != b->opts.classify_z) {
+    if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  }
+  b->opts = o;  // the options are host-side launch parameters: nothing to upload
+  return UPF_OK;
 }
 
 int upf_batch_get_opts(const upf_batch* b, upf_batch_opts* opts) {

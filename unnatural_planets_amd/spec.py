@@ -291,6 +291,26 @@ def batch_chunks(manifest):
     return arr
 
 
+def nb_planets():
+    """The narrow-band workload's eight planets of mixed shapes and seeds: a 5-octave
+    gradient fBm (f0 2.0, lacunarity 2.0, gain 0.5, weight 0.05, seed 1000 + p) on sphere,
+    rounded cube, torus and capsule, twice.  Returns [(name, FieldSpec)]."""
+    out = []
+    for p in range(8):
+        shape = BATCH_SHAPES[p % 4]
+        out.append((f"{shape}/seed {1000 + p}",
+                    field_spec(shape, BATCH_SHAPE_PARAMS[shape], [layer("gradient", "fbm", 5, 2.0, 2.0, 0.5, 0.05,
+                                                                        1000 + p)])))
+    return out
+
+
+def nb_band(g: Grid, lipschitz: float = 2.0) -> float:
+    """A band that holds the surface: every point of an 8^3-cell block lies within half the
+    block diagonal of a corner, so band = L * diagonal / 2 with L >= max |grad f|."""
+    d = NB_BLOCK * math.sqrt(float(g.spacing[0]) ** 2 + float(g.spacing[1]) ** 2 + float(g.spacing[2]) ** 2)
+    return lipschitz * d / 2
+
+
 # ---------------------------------------------------------------------------- texel bake
 class BakeSpec(ctypes.Structure):
     _fields_ = [("field", FieldSpec), ("detail", Layer), ("bands", ctypes.c_float * 3),
