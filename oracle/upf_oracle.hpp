@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../include/upf.h"
+#include "../unnatural_planets_amd/csrc/upf_math.hpp"
 
 namespace upfo {
 
@@ -93,8 +94,10 @@ inline float floor_(float a) { return std::floor(a); }
 inline double floor_(double a) { return std::floor(a); }
 inline float rint_(float a) { return std::nearbyint(a); }    // round half to even
 inline double rint_(double a) { return std::nearbyint(a); }
-// pow with a constant exponent (the elevation modes): the C library's powf / pow
-inline float pow_(float a, double e) { return std::pow(a, (float)e); }
+// pow with a constant exponent (the elevation modes, the Mandelbulb): in FP32 the
+// project's pow (upf_math.hpp: double arithmetic from basic operations, the same bits on
+// every IEEE target, pinned against Python's math.pow); in FP64 the C library's pow
+inline float pow_(float a, double e) { return upf_math::powf_(a, (float)e); }
 inline double pow_(double a, double e) { return std::pow(a, e); }
 template <int K> Dual<K> pow_(const Dual<K>& a, double e) {
   Dual<K> r(std::pow(a.v, e));
@@ -102,7 +105,7 @@ template <int K> Dual<K> pow_(const Dual<K>& a, double e) {
   for (int i = 0; i < K; i++) r.d[i] = a.d[i] * s;
   return r;
 }
-inline CountedFloat pow_(CountedFloat a, double e) { CountedFloat::count()++; return std::pow(a.v, (float)e); }
+inline CountedFloat pow_(CountedFloat a, double e) { CountedFloat::count()++; return upf_math::powf_(a.v, (float)e); }
 // sin, cos, atan2, acos, log (the further shapes): C library in float / double, chain
 // rule for the duals, one operation each for the counter
 inline float sin_(float a) { return std::sin(a); }

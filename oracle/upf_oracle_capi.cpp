@@ -347,6 +347,14 @@ void upfo_mesh_water_keep(int64_t nv, int64_t nt, const uint32_t* idx, const uin
   water_keep(nv, nt, idx, valid, keep_v, keep_t);
 }
 
+// ---- the project's pow and atan (upf_math.hpp), for the pins against Python's math
+void upfo_powf_n(const float* x, const float* e, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; i++) out[i] = upf_math::powf_(x[i], e[i]);
+}
+void upfo_atanf_n(const float* x, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; i++) out[i] = upf_math::atanf_(x[i]);
+}
+
 // ---- sources/math.cpp helpers (FP64), for closed-form checks
 double upfo_rescale(double v, double ia, double ib, double oa, double ob) { return rescale<double>(v, ia, ib, oa, ob); }
 double upfo_sharp_edge(double v, double p) { return sharp_edge<double>(v, p); }

@@ -4,7 +4,8 @@
 //
 // TEST INFRASTRUCTURE ONLY (see upf_oracle.hpp).  cage's noise kinds, colour conversions
 // and configuration defaults are not in the reference's checkout; the definitions
-// restated here are the ones documented at upf_material_spec in include/upf.h.  Noise
+// restated here are the ones documented at upf_material_spec in include/upf.h.  pow and
+// atan are the project's (upf_math.hpp), as on the device.  Noise
 // object k of the stack is seeded seed + 16 k (the reference's noiseSeed() draws), with
 // the index table below.
 #pragma once
@@ -254,7 +255,7 @@ inline void precipitation(const Spec& s, Tile& t) {  // :59-80
   p = sat(p);
   p = smootherstep(p);
   p = smootherstep(p);
-  p = std::pow(std::max(p + 0.2f, 0.0f), 3.0f) * 0.6f;
+  p = upf_math::powf_(std::max(p + 0.2f, 0.0f), 3.0f) * 0.6f;
   p *= 400.0f;
   p += std::max(120.0f - std::fabs(t.elevation), 0.0f) * 0.2f;
   t.precipitation = std::max(p, 0.0f);
@@ -268,8 +269,8 @@ inline void temperature(const Spec& s, Tile& t) {  // :82-120
   v = v * 2.0f - 1.0f;
   if (s.poles) {
     const float len = std::sqrt(t.pos[0] * t.pos[0] + t.pos[2] * t.pos[2]);
-    float polar = std::fabs(std::atan(t.pos[1] / len)) / 3.14159265358979f * 2.0f;
-    polar = std::pow(polar, 1.7f);
+    float polar = std::fabs(upf_math::atanf_(t.pos[1] / len)) / 3.14159265358979f * 2.0f;
+    polar = upf_math::powf_(polar, 1.7f);
     polar += noise({VALUE, 0.007f, 1, 3}, s.obj(O_POLAR), t.pos) * 0.1f;
     v += 0.6f - polar * 3.2f;
   }
@@ -301,7 +302,7 @@ inline void slope(const Spec& s, Tile& t) {  // :122-151
   float e1 = e[0], e2 = e[0];
   for (float v : e) { e1 = std::min(e1, v); e2 = std::max(e2, v); }
   const float md = (e2 - e1) * 0.1f;
-  t.slope = std::atan(md / radius);
+  t.slope = upf_math::atanf_(md / radius);
 }
 
 inline void biome(Tile& t) {  // :153-190
@@ -327,7 +328,7 @@ inline void bedrock(const Spec& s, Tile& t) {  // :202-261
   Cfg cr{CELLULAR, 0.017f};
   cr.dist = HYBRID;
   cr.ret = SUB;
-  const float cracks = sat(std::pow(noise(cr, s.obj(O_BED_CELLS), pf) * 0.5f + 0.5f, 0.7f));
+  const float cracks = sat(upf_math::powf_(noise(cr, s.obj(O_BED_CELLS), pf) * 0.5f + 0.5f, 0.7f));
   cr.ret = CELL;
   const float value = noise(cr, s.obj(O_BED_CELLS), pf);
   const float satn = sat(noise({CUBIC, 0.0022f, 1, 2}, s.obj(O_BED_SAT), t.pos) * 0.35f + 0.3f);
@@ -432,7 +433,7 @@ inline void grass(const Spec& s, Tile& t) {  // :467-533
   cells.ret = CELL;
   const float pm = sharp_edge_f(noise(cells, s.obj(O_GRASS_CELLS), t.pos) * 0.5f + 0.2f);
   cells.ret = SUB;
-  const float patches = std::pow(noise(cells, s.obj(O_GRASS_CELLS), t.pos) * 0.5f + 0.5f, 0.5f) * pm * (1.0f - dry * 0.8f);
+  const float patches = upf_math::powf_(noise(cells, s.obj(O_GRASS_CELLS), t.pos) * 0.5f + 0.5f, 0.5f) * pm * (1.0f - dry * 0.8f);
   for (float& c : col) c *= 1.0f - patches * 0.09f;
   Cfg scr{CELLULAR, 0.3f};
   scr.dist = HYBRID;
@@ -525,7 +526,7 @@ inline void ice(const Spec& s, Tile& t) {  // :746-817
   cc.ret = CELL;
   const float thick = sharp_edge_f(sat(noise(cc, s.obj(O_ICE_CELLS), pf) * 0.5f + 0.8f), 0.1f) * 0.6f + 0.4f;
   cc.ret = SUB;
-  const float crack = 1.0f - std::pow(noise(cc, s.obj(O_ICE_CELLS), pf) * 0.5f + 0.5f, 0.3f);
+  const float crack = 1.0f - upf_math::powf_(noise(cc, s.obj(O_ICE_CELLS), pf) * 0.5f + 0.5f, 0.3f);
   float col[3] = {std::max(122.0f / 255 - crack * 0.3f, 0.0f), std::max(162.0f / 255 - crack * 0.3f, 0.0f),
                   std::max(164.0f / 255 - crack * 0.3f, 0.0f)};
   mix_color(t.albedo, col, thick, col);

@@ -69,6 +69,8 @@ class Oracle:
             "upfo_mesh_water_keep": (None, [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P]),
             "upfo_num_threads": (ctypes.c_int, []),
             "upfo_set_num_threads": (None, [ctypes.c_int]),
+            "upfo_powf_n": (None, [_P, _P, _P, ctypes.c_int64]),
+            "upfo_atanf_n": (None, [_P, _P, ctypes.c_int64]),
             "upfo_rescale": (ctypes.c_double, [ctypes.c_double] * 5),
             "upfo_sharp_edge": (ctypes.c_double, [ctypes.c_double] * 2),
             "upfo_terrace": (ctypes.c_double, [ctypes.c_double] * 2),
@@ -280,6 +282,20 @@ class Oracle:
         n = special_u8.size // 2
         out = np.empty(special_u8.shape[:-1] + (3,), dtype=np.uint8)
         self.L.upfo_image_special_to_pbr(_f32p(special_u8), n, _f32p(out))
+        return out
+
+    # ---- the project's pow and atan (upf_math.hpp, shared by the oracle and the device)
+    def powf(self, x, e):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(e, dtype=np.float32), x.shape))
+        out = np.empty_like(x)
+        self.L.upfo_powf_n(_f32p(x), _f32p(e), _f32p(out), len(x))
+        return out
+
+    def atanf(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        out = np.empty_like(x)
+        self.L.upfo_atanf_n(_f32p(x), _f32p(out), len(x))
         return out
 
     # ---- mesh post-processing: keep flags and the order-preserving compaction

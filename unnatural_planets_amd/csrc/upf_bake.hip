@@ -138,8 +138,10 @@ __global__ __launch_bounds__(256) void k_bake(const DBake b, const float* __rest
   float cosang = 1.0f;
   if (nl > 0.0f && pl > 0.0f) cosang = (n[0] * p[0] + n[1] * p[1] + n[2] * p[2]) / (nl * pl);
   const float slope = 1.0f - cosang;
-  const float elev = shape_eval<EXT>(b.field, p[0], p[1], p[2]) / b.field.escale;
-  const float d = b.detail.weight * layer_value(b.detail, p[0], p[1], p[2]);
+  // the shape and the detail noise round like the CPU restatement (X = true): the bands'
+  // smoothsteps would amplify a few ulps of elevation by up to 150x
+  const float elev = shape_eval<EXT, true, true>(b.field, p[0], p[1], p[2]) / b.field.escale;
+  const float d = b.detail.weight * layer_value<true>(b.detail, p[0], p[1], p[2]);
   const float s1 = smoothstep_(b.bands[0] - b.band_width, b.bands[0] + b.band_width, elev);
   const float s2 = smoothstep_(b.bands[1] - b.band_width, b.bands[1] + b.band_width, elev);
   const float s3 = smoothstep_(b.bands[2] - b.band_width, b.bands[2] + b.band_width, elev);

@@ -83,8 +83,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
     float v[2];
     if (VARIANT == UPF_EVAL_PER_LANE) {
-      v[0] = field_eval<EXT>(C.spec, x, y[0], z);
-      v[1] = two ? field_eval<EXT>(C.spec, x, y[1], z) : v[0];
+      // a chunk rounds as its own single-grid evaluation would: the exact form for an
+      // elevation mode or a further shape, the fused form otherwise (the choice is
+      // uniform across the wave: one chunk per row pair)
+      if (EXT && spec_ext(C.spec)) {
+        v[0] = field_eval<true, true>(C.spec, x, y[0], z);
+        v[1] = two ? field_eval<true, true>(C.spec, x, y[1], z) : v[0];
+      } else {
+        v[0] = field_eval<EXT, false>(C.spec, x, y[0], z);
+        v[1] = two ? field_eval<EXT, false>(C.spec, x, y[1], z) : v[0];
+      }
     } else {
       row_field<2, COMPOSED>(ctx, C.spec, x, y, z, v);
     }

@@ -2,9 +2,13 @@
 // base shapes, layer stack and field composition (forward values).
 //
 // Everything is FP32 VALU work; the integer lattice (floor, hash, gradient selection)
-// is bit-exact with the CPU restatement, and the floating-point parts agree to a few
-// ulps.  This file is compiled with -ffp-contract=off: every fused multiply-add is
-// written as fmaf() on purpose, everything else rounds per operation.
+// is bit-exact with the CPU restatement.  This file is compiled with -ffp-contract=off:
+// every fused multiply-add is written as fmaf() on purpose, everything else rounds per
+// operation.  The field kernels' default form fuses and uses the hardware square root
+// (a few ulps from the restatement); the functions that take `X = true` instead round
+// each operation separately, in the restatement's order, with the correctly rounded
+// square root, so that their results equal the CPU restatement's bit for bit.  The texel
+// bake, the material stack's elevation and the elevation modes run that form.
 //
 // Reference anchors: field composition sources/modesConfiguration.cpp:230-279, base
 // shapes sources/sdf.cpp:20-48,81-84, layer configs sources/elevationModes.cpp:16-27.
@@ -25,6 +29,11 @@ namespace upf {
 // the row field kernel's instructions on the torus + noise workload.  1 ulp is far
 // inside the field's 1e-5 tolerance; the compositions of upf_shapes.hpp keep sqrt_f.
 __device__ __forceinline__ float sqrt_hw(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+// a * b + c fused (X = false) or with two roundings (X = true); the square root of the
+// hot distances in hardware (X = false) or correctly rounded (X = true)
+template <bool X> __device__ __forceinline__ float mad_x(float a, float b, float c) { return X ? a * b + c : fmaf(a, b, c); }
+template <bool X> __device__ __forceinline__ float sqrt_x(float a) { return X ? sqrt_f(a) : sqrt_hw(a); }
 
 // Kernel-side copy of a layer: the spec's layer plus the cellular constants
 // js = jitter/1023 and j0 = jitter*0.5, precomputed on the host in FP32.
@@ -76,9 +85,9 @@ __device__ __forceinline__ float node_coord(float o, float s, int i) {
 }
 
 // Quintic fade t^3 (t (6t - 15) + 10).
-__device__ __forceinline__ float fade(float t) {
-  const float a = fmaf(t, 6.0f, -15.0f);
-  const float b = fmaf(t, a, 10.0f);
+template <bool X = false> __device__ __forceinline__ float fade(float t) {
+  const float a = mad_x<X>(t, 6.0f, -15.0f);
+  const float b = mad_x<X>(t, a, 10.0f);
   return (t * t) * t * b;
 }
 
@@ -88,7 +97,7 @@ __device__ __forceinline__ float dfade(float t) {
   return 30.0f * a * a;
 }
 
-__device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
+template <bool X = false> __device__ __forceinline__ float lerpf(float a, float b, float t) { return mad_x<X>(t, b - a, a); }
 
 // Dot product with gradient g = h >> 28 of Perlin's 16-entry improved-noise set:
 // u = g<8 ? x : y;  v = g<4 ? y : (g==12 || g==14 ? x : z);  ±u ± v by bits 0 and 1.
@@ -129,36 +138,39 @@ __device__ __forceinline__ void cell_hashes(uint32_t seed, int ix, int iy, int i
 }
 
 // ------------------------------------------------------------------ noise (values)
+template <bool X = false>
 __device__ __forceinline__ float noise_gradient(uint32_t seed, float qx, float qy, float qz) {
   const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
   uint32_t h[8];
   cell_hashes(seed, (int)fx, (int)fy, (int)fz, h);
   const float tx = qx - fx, ty = qy - fy, tz = qz - fz;
   const float sx = tx - 1.0f, sy = ty - 1.0f, sz = tz - 1.0f;
-  const float ux = fade(tx), uy = fade(ty), uz = fade(tz);
-  const float x00 = lerpf(gdot(h[0], tx, ty, tz), gdot(h[1], sx, ty, tz), ux);
-  const float x10 = lerpf(gdot(h[2], tx, sy, tz), gdot(h[3], sx, sy, tz), ux);
-  const float x01 = lerpf(gdot(h[4], tx, ty, sz), gdot(h[5], sx, ty, sz), ux);
-  const float x11 = lerpf(gdot(h[6], tx, sy, sz), gdot(h[7], sx, sy, sz), ux);
-  return lerpf(lerpf(x00, x10, uy), lerpf(x01, x11, uy), uz);
+  const float ux = fade<X>(tx), uy = fade<X>(ty), uz = fade<X>(tz);
+  const float x00 = lerpf<X>(gdot(h[0], tx, ty, tz), gdot(h[1], sx, ty, tz), ux);
+  const float x10 = lerpf<X>(gdot(h[2], tx, sy, tz), gdot(h[3], sx, sy, tz), ux);
+  const float x01 = lerpf<X>(gdot(h[4], tx, ty, sz), gdot(h[5], sx, ty, sz), ux);
+  const float x11 = lerpf<X>(gdot(h[6], tx, sy, sz), gdot(h[7], sx, sy, sz), ux);
+  return lerpf<X>(lerpf<X>(x00, x10, uy), lerpf<X>(x01, x11, uy), uz);
 }
 
 __device__ __forceinline__ float lattice_value(uint32_t h) { return (float)(int32_t)h * 4.656612873077393e-10f; }
 
+template <bool X = false>
 __device__ __forceinline__ float noise_value(uint32_t seed, float qx, float qy, float qz) {
   const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
   uint32_t h[8];
   cell_hashes(seed, (int)fx, (int)fy, (int)fz, h);
-  const float ux = fade(qx - fx), uy = fade(qy - fy), uz = fade(qz - fz);
-  const float x00 = lerpf(lattice_value(h[0]), lattice_value(h[1]), ux);
-  const float x10 = lerpf(lattice_value(h[2]), lattice_value(h[3]), ux);
-  const float x01 = lerpf(lattice_value(h[4]), lattice_value(h[5]), ux);
-  const float x11 = lerpf(lattice_value(h[6]), lattice_value(h[7]), ux);
-  return lerpf(lerpf(x00, x10, uy), lerpf(x01, x11, uy), uz);
+  const float ux = fade<X>(qx - fx), uy = fade<X>(qy - fy), uz = fade<X>(qz - fz);
+  const float x00 = lerpf<X>(lattice_value(h[0]), lattice_value(h[1]), ux);
+  const float x10 = lerpf<X>(lattice_value(h[2]), lattice_value(h[3]), ux);
+  const float x01 = lerpf<X>(lattice_value(h[4]), lattice_value(h[5]), ux);
+  const float x11 = lerpf<X>(lattice_value(h[6]), lattice_value(h[7]), ux);
+  return lerpf<X>(lerpf<X>(x00, x10, uy), lerpf<X>(x01, x11, uy), uz);
 }
 
 // Cellular F1 over the 27 cells around rint(q); returns the squared distance and the
 // offset (q - nearest point) for the gradient.
+template <bool X = false>
 __device__ __forceinline__ float cellular_d2(uint32_t seed, float qx, float qy, float qz, float js,
                                              float j0, float& ex, float& ey, float& ez) {
   const float cxf = rintf(qx), cyf = rintf(qy), czf = rintf(qz);
@@ -179,11 +191,11 @@ __device__ __forceinline__ float cellular_d2(uint32_t seed, float qx, float qy, 
         const float kx = (float)(cx - 1 + dx) - j0;
         const uint32_t h = (yzs ^ (xb + (uint32_t)dx * kPX)) * kHM;
         const uint32_t m = h ^ (h >> 15);
-        const float px = fmaf((float)(m & 1023u), js, kx);
-        const float py = fmaf((float)((m >> 10) & 1023u), js, ky);
-        const float pz = fmaf((float)((m >> 20) & 1023u), js, kz);
+        const float px = mad_x<X>((float)(m & 1023u), js, kx);
+        const float py = mad_x<X>((float)((m >> 10) & 1023u), js, ky);
+        const float pz = mad_x<X>((float)((m >> 20) & 1023u), js, kz);
         const float dxv = px - qx, dyv = py - qy, dzv = pz - qz;
-        const float d2 = fmaf(dzv, dzv, fmaf(dyv, dyv, dxv * dxv));
+        const float d2 = mad_x<X>(dzv, dzv, mad_x<X>(dyv, dyv, dxv * dxv));
         if (d2 < best) { best = d2; ex = -dxv; ey = -dyv; ez = -dzv; }
       }
     }
@@ -191,9 +203,10 @@ __device__ __forceinline__ float cellular_d2(uint32_t seed, float qx, float qy, 
   return best;
 }
 
+template <bool X = false>
 __device__ __forceinline__ float noise_cellular(uint32_t seed, float qx, float qy, float qz, float js, float j0) {
   float ex, ey, ez;
-  return sqrt_hw(cellular_d2(seed, qx, qy, qz, js, j0, ex, ey, ez));
+  return sqrt_x<X>(cellular_d2<X>(seed, qx, qy, qz, js, j0, ex, ey, ez));
 }
 
 // 3D simplex noise (Perlin 2001, in the form of Gustavson's "Simplex noise demystified",
@@ -274,26 +287,26 @@ __device__ __noinline__ float shape_more(int shape, float p0, float p1, float p2
   return shape_more_t<float>(shape, sp, x, y, z, 0.0f);
 }
 
-template <bool EXT = true, bool COMPOSED = true>
+template <bool EXT = true, bool COMPOSED = true, bool X = false>
 __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, float z) {
   switch (s.shape) {
     case UPF_SHAPE_SPHERE:
-      return sqrt_hw(fmaf(z, z, fmaf(y, y, x * x))) - s.sp[0];
+      return sqrt_x<X>(mad_x<X>(z, z, mad_x<X>(y, y, x * x))) - s.sp[0];
     case UPF_SHAPE_BOX: {
       const float qx = fabsf(x) - s.sp[0], qy = fabsf(y) - s.sp[1], qz = fabsf(z) - s.sp[2];
       const float mx = fmaxf(qx, 0.0f), my = fmaxf(qy, 0.0f), mz = fmaxf(qz, 0.0f);
-      const float outside = sqrt_hw(fmaf(mz, mz, fmaf(my, my, mx * mx)));
+      const float outside = sqrt_x<X>(mad_x<X>(mz, mz, mad_x<X>(my, my, mx * mx)));
       const float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
       return outside + inside - s.sp[3];
     }
     case UPF_SHAPE_TORUS: {
-      const float ring = sqrt_hw(fmaf(z, z, x * x)) - s.sp[0];
-      return sqrt_hw(fmaf(y, y, ring * ring)) - s.sp[1];
+      const float ring = sqrt_x<X>(mad_x<X>(z, z, x * x)) - s.sp[0];
+      return sqrt_x<X>(mad_x<X>(y, y, ring * ring)) - s.sp[1];
     }
     case UPF_SHAPE_CAPSULE: {
       const float h2 = s.sp[0] * 0.5f;
       const float zc = z - fminf(fmaxf(z, -h2), h2);
-      return sqrt_hw(fmaf(zc, zc, fmaf(y, y, x * x))) - s.sp[1];
+      return sqrt_x<X>(mad_x<X>(zc, zc, mad_x<X>(y, y, x * x))) - s.sp[1];
     }
     case UPF_SHAPE_NONE:
       return 0.0f;
@@ -306,26 +319,28 @@ __device__ __forceinline__ float shape_eval(const DSpec& s, float x, float y, fl
 }
 
 // ------------------------------------------------------------------ layers
+template <bool X = false>
 __device__ __forceinline__ float octave_noise(const DLayer& L, int k, float x, float y, float z) {
   const float f = L.freq[k];
   const float qx = x * f, qy = y * f, qz = z * f;
   const uint32_t seed = L.seed + (uint32_t)k;
-  if (L.noise == UPF_NOISE_GRADIENT) return noise_gradient(seed, qx, qy, qz);
-  if (L.noise == UPF_NOISE_VALUE) return noise_value(seed, qx, qy, qz);
+  if (L.noise == UPF_NOISE_GRADIENT) return noise_gradient<X>(seed, qx, qy, qz);
+  if (L.noise == UPF_NOISE_VALUE) return noise_value<X>(seed, qx, qy, qz);
   if (L.noise == UPF_NOISE_SIMPLEX) return noise_simplex(seed, qx, qy, qz);
-  return noise_cellular(seed, qx, qy, qz, L.js, L.j0);
+  return noise_cellular<X>(seed, qx, qy, qz, L.js, L.j0);
 }
 
+template <bool X = false>
 __device__ __forceinline__ float layer_value(const DLayer& L, float x, float y, float z) {
-  if (L.fractal == UPF_FRACTAL_NONE) return octave_noise(L, 0, x, y, z);
+  if (L.fractal == UPF_FRACTAL_NONE) return octave_noise<X>(L, 0, x, y, z);
   float acc = 0.0f;
   if (L.fractal == UPF_FRACTAL_RIDGED) {
     for (int k = 0; k < L.octaves; k++) {
-      const float n = octave_noise(L, k, x, y, z);
-      acc = fmaf(L.amp[k], fmaf(-2.0f, fabsf(n), 1.0f), acc);
+      const float n = octave_noise<X>(L, k, x, y, z);
+      acc = mad_x<X>(L.amp[k], mad_x<X>(-2.0f, fabsf(n), 1.0f), acc);
     }
   } else {
-    for (int k = 0; k < L.octaves; k++) acc = fmaf(L.amp[k], octave_noise(L, k, x, y, z), acc);
+    for (int k = 0; k < L.octaves; k++) acc = mad_x<X>(L.amp[k], octave_noise<X>(L, k, x, y, z), acc);
   }
   return acc;
 }
@@ -335,7 +350,7 @@ __device__ __forceinline__ float layer_value(const DLayer& L, float x, float y, 
 // Voronoi parameters (seed, points per cell in octaves, cell size in freq[0]).
 // EXT = false compiles the modes and the further shapes out (the kernels' common
 // instantiation); specs with either run the EXT = true instantiations.
-template <bool EXT = true>
+template <bool EXT = true, bool X = false>
 __device__ __forceinline__ float elevation_eval(const DSpec& s, float x, float y, float z) {
   const int mode = EXT ? s.mode : UPF_ELEV_LAYERS;
   ModeState<float> st;
@@ -345,7 +360,7 @@ __device__ __forceinline__ float elevation_eval(const DSpec& s, float x, float y
     const DLayer& L = s.L[l];
     float px = x, py = y, pz = z;
     if (mode != UPF_ELEV_LAYERS) mode_position(mode, l, st, x, y, z, 0.0f, px, py, pz);
-    const float raw = layer_value(L, px, py, pz);
+    const float raw = layer_value<X>(L, px, py, pz);
     if (mode != UPF_ELEV_LAYERS) {
       if (mode_after(mode, l, raw, 0.0f, st)) break;
       continue;
@@ -353,18 +368,21 @@ __device__ __forceinline__ float elevation_eval(const DSpec& s, float x, float y
     const float v = L.weight * raw;
     if (L.blend == UPF_BLEND_ADD) acc = acc + v;
     else if (L.blend == UPF_BLEND_MASK) m = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
-    else acc = fmaf(m, v - acc, acc);
+    else acc = mad_x<X>(m, v - acc, acc);
   }
   if (mode != UPF_ELEV_LAYERS) return mode_finish(mode, st, x, y, z, 0.0f);
   return acc;
 }
 
-template <bool EXT = true>
+// X = EXT by default: the per-lane kernels of the elevation modes and the further shapes
+// round like the CPU restatement (the modes' pow is upf_math.hpp's on both sides), so
+// their fields equal it bit for bit; the common kernels keep the fused form.
+template <bool EXT = true, bool X = EXT>
 __device__ __forceinline__ float field_eval(const DSpec& s, float x, float y, float z) {
   const bool need_elev = s.compose == UPF_COMPOSE_LAND || s.compose == UPF_COMPOSE_NAVIGATION ||
                          s.compose == UPF_COMPOSE_ELEVATION_RAW;
-  const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<EXT>(s, x, y, z);
-  const float elev = need_elev ? elevation_eval<EXT>(s, x, y, z) : 0.0f;
+  const float shape = s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape_eval<EXT, true, X>(s, x, y, z);
+  const float elev = need_elev ? elevation_eval<EXT, X>(s, x, y, z) : 0.0f;
   switch (s.compose) {
     case UPF_COMPOSE_LAND: return shape - s.escale * elev;
     case UPF_COMPOSE_WATER: return shape;

@@ -15,7 +15,8 @@
 //   SimplexReduced -> simplex noise; Cellular -> F1 / F2 over the 27 cells around the
 //   sample with a distance (Euclidean, Manhattan, Hybrid) and an operation (Cell value,
 //   Subtract, Multiply, Divide).
-// Everything is plain FP32 (no fused operations): the CPU restatement rounds alike.
+// Everything is plain FP32 (no fused operations) and pow / atan are the library's own
+// (upf_math.hpp): the CPU restatement rounds alike.
 #pragma once
 
 #include "upf_device.hpp"
@@ -360,7 +361,7 @@ __device__ __forceinline__ void m_precipitation(const DMaterial& m, MTile& t) { 
   p = m_sat(p);
   p = m_smootherstep(p);
   p = m_smootherstep(p);
-  p = powf(fmaxf(p + 0.2f, 0.0f), 3.0f) * 0.6f;
+  p = upf_math::powf_(fmaxf(p + 0.2f, 0.0f), 3.0f) * 0.6f;
   p = p * 400.0f;
   p = p + fmaxf(120.0f - fabsf(t.elevation), 0.0f) * 0.2f;
   t.precipitation = fmaxf(p, 0.0f);
@@ -372,8 +373,8 @@ __device__ __forceinline__ void m_temperature(const DMaterial& m, MTile& t) {  /
   v = m_smoothstep(v);
   v = v * 2.0f - 1.0f;
   if (m.poles) {
-    float polar = fabsf(atanf(t.py / sqrtf(t.px * t.px + t.pz * t.pz))) / kPi * 2.0f;
-    polar = powf(polar, 1.7f);
+    float polar = fabsf(upf_math::atanf_(t.py / sqrtf(t.px * t.px + t.pz * t.pz))) / kPi * 2.0f;
+    polar = upf_math::powf_(polar, 1.7f);
     polar = polar + mnoise(kMNoValue, ms(m, 3), 0.007f, 1, 3, 2.0f, 0.5f, 0, 0, t.px, t.py, t.pz) * 0.1f;
     v = v + (0.6f - polar * 3.2f);
   }
@@ -402,7 +403,7 @@ __device__ __forceinline__ void m_slope(const DMaterial& m, MTile& t) {  // :122
 #pragma unroll
   for (int i = 0; i < 8; i++) { e1 = fminf(e1, e[i]); e2 = fmaxf(e2, e[i]); }
   const float md = (e2 - e1) * 0.1f;
-  t.slope = atanf(md / radius);
+  t.slope = upf_math::atanf_(md / radius);
 }
 
 __device__ __forceinline__ void m_biome(MTile& t) {  // :153-190
@@ -428,7 +429,7 @@ __device__ __forceinline__ void m_bedrock(const DMaterial& m, MTile& t) {  // :2
   const float freq = mnoise(kMNoCubic, ms(m, 6), 0.013f, 0, 1, 2.0f, 0.5f, 0, 0, t.px, t.py, t.pz) * 0.15f + 1.0f;
   const float fx = t.px * freq, fy = t.py * freq, fz = t.pz * freq;
   const float cr = mnoise(kMNoCellular, ms(m, 4), 0.017f, 0, 1, 2.0f, 0.5f, kDistHybrid, kRetSub, fx, fy, fz);
-  const float cracks = m_sat(powf(cr * 0.5f + 0.5f, 0.7f));
+  const float cracks = m_sat(upf_math::powf_(cr * 0.5f + 0.5f, 0.7f));
   const float value = mnoise(kMNoCellular, ms(m, 4), 0.017f, 0, 1, 2.0f, 0.5f, kDistHybrid, kRetCell, fx, fy, fz);
   const float sat = m_sat(mnoise(kMNoCubic, ms(m, 7), 0.0022f, 1, 2, 2.0f, 0.5f, 0, 0, t.px, t.py, t.pz) * 0.35f + 0.3f);
   m_hsv_to_rgb(0.07f, sat, m_sat(value * 0.1f + 0.7f) * (cracks * 0.4f + 0.6f), t.r, t.g, t.b);
@@ -528,7 +529,7 @@ __device__ __forceinline__ void m_grass(const DMaterial& m, MTile& t) {  // :467
   const float pm = m_sharp_edge(
       mnoise(kMNoCellular, ms(m, 18), 0.1f, 0, 1, 2.0f, 0.5f, kDistHybrid, kRetCell, t.px, t.py, t.pz) * 0.5f + 0.2f);
   const float patches =
-      powf(mnoise(kMNoCellular, ms(m, 18), 0.1f, 0, 1, 2.0f, 0.5f, kDistHybrid, kRetSub, t.px, t.py, t.pz) * 0.5f + 0.5f,
+      upf_math::powf_(mnoise(kMNoCellular, ms(m, 18), 0.1f, 0, 1, 2.0f, 0.5f, kDistHybrid, kRetSub, t.px, t.py, t.pz) * 0.5f + 0.5f,
            0.5f) *
       pm * (1.0f - dry * 0.8f);
   const float k1 = 1.0f - patches * 0.09f;
@@ -626,7 +627,7 @@ __device__ __forceinline__ void m_ice(const DMaterial& m, MTile& t) {  // :746-8
       m_sharp_edge(m_sat(mnoise(kMNoCellular, ms(m, 37), 0.05f, 1, 2, 2.0f, 0.5f, kDistHybrid, kRetCell, fx, fy, fz) * 0.5f + 0.8f),
                    0.1f) * 0.6f + 0.4f;
   const float crack =
-      1.0f - powf(mnoise(kMNoCellular, ms(m, 37), 0.05f, 1, 2, 2.0f, 0.5f, kDistHybrid, kRetSub, fx, fy, fz) * 0.5f + 0.5f, 0.3f);
+      1.0f - upf_math::powf_(mnoise(kMNoCellular, ms(m, 37), 0.05f, 1, 2, 2.0f, 0.5f, kDistHybrid, kRetSub, fx, fy, fz) * 0.5f + 0.5f, 0.3f);
   float r = fmaxf(122.0f / 255.0f - crack * 0.3f, 0.0f);
   float g = fmaxf(162.0f / 255.0f - crack * 0.3f, 0.0f);
   float b = fmaxf(164.0f / 255.0f - crack * 0.3f, 0.0f);
@@ -691,7 +692,7 @@ __device__ __forceinline__ void material_tile(const DMaterial& m, MTile& t) {
   t.type = kTypeWater + 1;
   // generateElevation (:51-57)
   t.elevation = m.purpose == UPF_PURPOSE_LAND ? m_elev_sdf(m, t.px, t.py, t.pz)
-                                               : elevation_eval(m.field, t.px, t.py, t.pz);
+                                               : elevation_eval<true, true>(m.field, t.px, t.py, t.pz);
   const bool water = m.purpose == UPF_PURPOSE_WATER;
   if (m.coloring == UPF_COLORING_BARREN) {  // :932-952
     m_slope(m, t);

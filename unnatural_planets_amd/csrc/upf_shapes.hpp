@@ -13,6 +13,8 @@
 
 #include "../../include/upf.h"
 
+#include "upf_math.hpp"
+
 #if defined(__HIPCC__)
 #define UPF_HD __host__ __device__ __forceinline__
 #else
@@ -64,12 +66,12 @@ UPF_HD Grad sq_root(Grad a) {
   return {r, a.dx * k, a.dy * k, a.dz * k};
 }
 UPF_HD float sq_root(float a) { return sqrt_f(a); }
-// transcendental functions of the scalar type (the C library's for float; forward-mode
-// derivatives for Grad)
-UPF_HD float pow_v(float a, float e) { return powf(a, e); }
+// transcendental functions of the scalar type (the C library's for float, except pow:
+// the library's own, upf_math.hpp; forward-mode derivatives for Grad)
+UPF_HD float pow_v(float a, float e) { return upf_math::powf_(a, e); }
 UPF_HD Grad pow_v(const Grad& a, float e) {
-  const float v = powf(a.v, e);
-  const float d = a.v != 0.0f ? e * powf(a.v, e - 1.0f) : 0.0f;  // d/da a^e, 0 at a = 0
+  const float v = upf_math::powf_(a.v, e);
+  const float d = a.v != 0.0f ? e * upf_math::powf_(a.v, e - 1.0f) : 0.0f;  // d/da a^e, 0 at a = 0
   return {v, d * a.dx, d * a.dy, d * a.dz};
 }
 UPF_HD float floor_v(float a) { return floorf(a); }
