@@ -395,6 +395,20 @@ int upf_field_mesh_async(const upf_field_spec* spec, const upf_grid* grid, float
                          const upf_eval_opts* opts, upf_mesh_buffers* out, int64_t* d_counts,
                          void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* upf_field_mesh_async without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The workspace is a library-owned device
+ * array of upf_static_scratch_bytes() = 64 MiB per GPU, held until the process ends,
+ * which fits the grids whose upf_mc_workspace_bytes is at most that (up to 512^3 nodes);
+ * larger grids return UPF_ERR_WORKSPACE.  The call allocates nothing and clears nothing:
+ * every word of the workspace is written before it is read.  All calls on one GPU share
+ * that GPU's array, so calls that overlap on one GPU (other streams, other threads)
+ * race: order them on one stream.  Mesh and counts equal upf_field_mesh_async's. */
+size_t upf_static_scratch_bytes(void);
+int upf_field_mesh_async_static(const upf_field_spec* spec, const upf_grid* grid, float iso, float* d_field,
+                                const upf_eval_opts* opts, upf_mesh_buffers* out, int64_t* d_counts,
+                                void* stream);
+
 /* Case code of every cell -> d_codes[dims[2]-1][dims[1]-1][dims[0]-1] (uint8). */
 int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,
                       uint8_t* d_codes, void* stream);

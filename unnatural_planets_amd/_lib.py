@@ -57,6 +57,10 @@ SIGNATURES = {
     "upf_field_mesh_async": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_float,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MeshBuffers),
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "upf_static_scratch_bytes": (ctypes.c_size_t, []),
+    "upf_field_mesh_async_static": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_float,
+                                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MeshBuffers),
+                                                   ctypes.c_void_p, ctypes.c_void_p]),
     "upf_mc_emit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float,
                                    ctypes.POINTER(MeshBuffers), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "upf_batch_create": (ctypes.c_int, [ctypes.POINTER(Chunk), ctypes.c_int32, ctypes.POINTER(BatchOpts),

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "upf_internal.hpp"
 #include "upf_mc.hpp"
@@ -290,6 +291,34 @@ int64_t mc_scan_tiles(int64_t groups) { return (groups + kScanTile - 1) / kScanT
 
 constexpr int kEmitBlocksPerCu = 32;
 
+// ------------------------------------------------------------------ allocation-free scratch
+// The workspace of upf_field_mesh_async_static: a file-scope device array, one copy per
+// GPU, held until the process ends.  64 MiB holds the marching-cubes workspace of grids
+// up to 512^3 nodes (50.4 MB there: 24 B per 64-node group and the scan tiles).  In host
+// code `scratch` is only a stand-in for the symbol; each device's address comes from
+// hipGetSymbolAddress, looked up once per device on the entry point's first call there.
+constexpr size_t kStaticScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's address of `scratch` (the current device's), or an error.
+int static_scratch(void** out) {
+  int dev = -1;
+  UPF_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return UPF_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return UPF_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return UPF_OK;
+}
+
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
                    float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s, int64_t* counts_out) {
   const McLayout L = mc_layout(g);
@@ -377,6 +406,21 @@ int upf_field_mesh_async(const upf_field_spec* spec, const upf_grid* grid, float
   // for the largest possible list; the kernel also stores the totals in d_counts
   return mc_emit_launch(d_field, g, iso, d_ws, L.d.groups, out->positions, out->normals, out->indices,
                         out->cap_vertices, out->cap_triangles, s, d_counts);
+}
+
+size_t upf_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+int upf_field_mesh_async_static(const upf_field_spec* spec, const upf_grid* grid, float iso, float* d_field,
+                                const upf_eval_opts* opts, upf_mesh_buffers* out, int64_t* d_counts, void* stream) {
+  DGrid g;
+  int st = to_device(grid, &g);
+  if (st || (st = mc_grid_ok(g))) return st;
+  if (mc_layout(g).bytes > kStaticScratchBytes) return UPF_ERR_WORKSPACE;
+  void* ws = nullptr;
+  if ((st = static_scratch(&ws))) return st;
+  // the same launches as upf_field_mesh_async; every word of the workspace is written
+  // before it is read within a call, so nothing is cleared
+  return upf_field_mesh_async(spec, grid, iso, d_field, opts, out, d_counts, ws, kStaticScratchBytes, stream);
 }
 
 int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out, void* d_ws,
