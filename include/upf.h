@@ -122,7 +122,14 @@ enum upf_shape_id {
   UPF_SHAPE_SINE_NET = 29,     /* params: scale, d1, d2; weights: shape_weights (below).
                                   The architecture of sdfBunny / sdfMonkeyHead
                                   (sdf.cpp:252-303), whose weight tables the caller passes */
-  UPF_SHAPE_COUNT = 30
+  UPF_SHAPE_KNOT = 30,         /* no params; sdfKnot = knot(p * (1, 1, 0.3)) / |(1, 1, 0.3)|
+                                  (sdf.cpp:86-90) with cage's knot(q, 1600, 1.5) taken as
+                                  the (3, 2) torus knot of ratio 1.5 about the z axis: core
+                                  radius 1600, strands 200 off the core circle at angle
+                                  1.5 t, tube radius 100; distance in the half-plane of q's
+                                  azimuth phi to the two strands crossing it (t = phi,
+                                  phi + 2 pi), minus the tube radius                        */
+  UPF_SHAPE_COUNT = 31
 };
 
 /* UPF_SHAPE_SINE_NET: a sine network of width 16 evaluated at q = (x, z, y) * scale:

@@ -522,6 +522,20 @@ template <class T> T shape_further(int shape, const T* a, const T p[3]) {
       const T ang = y * T(0.001f), c = cos_(ang), s = sin_(ang);
       return hex_prism(c * x - s * z, s * x + c * z, y, 650.0f, 450.0f) - T(200.0f);
     }
+    case UPF_SHAPE_KNOT: {                                             // sdf.cpp:86-90
+      // sdfKnot(pos) = knot(pos * (1, 1, 0.3), 1600, 1.5) / |(1, 1, 0.3)|; the knot is the
+      // (3, 2) torus knot t -> ((1600 + 200 cos 1.5t) cos t, (1600 + 200 cos 1.5t) sin t,
+      // 200 sin 1.5t), t in [0, 4 pi), with tube radius 100: in the half-plane of the
+      // azimuth phi the curve passes at t = phi and phi + 2 pi
+      const T qz = z * T(0.3f);
+      const T rho = hyp2(x, y), phi = atan2_(y, x);
+      T best(1e30f);
+      for (int j = 0; j < 2; j++) {
+        const T a = (phi + T(6.2831853f * (float)j)) * T(1.5f);
+        best = min_(best, hyp2(rho - (T(1600.0f) + cos_(a) * T(200.0f)), qz - sin_(a) * T(200.0f)));
+      }
+      return (best - T(100.0f)) / T(1.0440307f);
+    }
     case UPF_SHAPE_TETRAHEDRON: {                                      // sdf.cpp:50-53
       const float k = 346.41016f;
       const float v[4][3] = {{k, k, k}, {k, -k, -k}, {-k, k, -k}, {-k, -k, k}};
@@ -696,10 +710,17 @@ inline void voronoi_cell_points(uint32_t seed, int ppc, float cell_size, const i
   }
 }
 
+// Vec3i(position * frequency) of :51, with the float clamped to +-(2^31 - 128) before the
+// truncation so that it stays defined for any finite input (as the library does).
+inline int32_t cell_index(float v) {
+  const float m = 2147483520.0f;
+  return (int32_t)(v < -m ? -m : (v > m ? m : v));
+}
+
 inline void voronoi_nearest(uint32_t seed, int ppc, float cell_size, const float pos[3], float out[3]) {
   const float frequency = 1.0f / cell_size;
-  const int32_t cell[3] = {(int32_t)(pos[0] * frequency), (int32_t)(pos[1] * frequency),
-                           (int32_t)(pos[2] * frequency)};                               // :51
+  const int32_t cell[3] = {cell_index(pos[0] * frequency), cell_index(pos[1] * frequency),
+                           cell_index(pos[2] * frequency)};                               // :51
   float best = INFINITY;
   std::vector<float> pts(3 * (size_t)ppc);
   for (int z = -1; z < 2; z++)

@@ -32,7 +32,9 @@ inline int atlas_side(int64_t n, int w, int h) {
 }
 
 inline void atlas(int64_t n, int w, int h, float* uv) {
-  const int side = atlas_side(n, w, h), cols = w / side;
+  const int side = atlas_side(n, w, h);
+  if (side < 2) return;  // no cell of 2 x 2 texels fits (the library reports UPF_ERR_CAPACITY)
+  const int cols = w / side;
   const float iw = 1.0f / (float)w, ih = 1.0f / (float)h;
   for (int64_t t = 0; t < n; t++) {
     const int64_t c = t / 2;

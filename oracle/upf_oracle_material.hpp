@@ -211,7 +211,7 @@ inline float steepness(float slope_rad, float threshold, float smoothing) {
 // the plane through the position (sources/voronoi.cpp:58-66), then the nearest.
 inline void voronoi_plane(uint32_t seed, int ppc, float cell, const float pos[3], const float n[3], float out[3]) {
   const float frequency = 1.0f / cell;
-  const int32_t cc[3] = {(int32_t)(pos[0] * frequency), (int32_t)(pos[1] * frequency), (int32_t)(pos[2] * frequency)};
+  const int32_t cc[3] = {cell_index(pos[0] * frequency), cell_index(pos[1] * frequency), cell_index(pos[2] * frequency)};
   std::vector<float> pts(3 * (size_t)ppc);
   float best = INFINITY;
   out[0] = out[1] = out[2] = 0.0f;

@@ -22,7 +22,8 @@ CASES = [("cylinder", (2000.0, 800.0), 1500), ("plane", (0.0, 0.6, -0.8, 100.0),
          ("h3o", (), 1800), ("h4o", (), 1800), ("gear", (), 2000), ("mandelbulb", (), 3200), ("mobius", (), 1900),
          ("fibers", S.SHAPE_REFERENCE_PARAMS["fibers"], 2200), ("asteroid", S.SHAPE_REFERENCE_PARAMS["asteroid"], 2200),
          ("twisted_plane", (), 1500), ("belt", (), 2500), ("triangular_prism", (), 1100),
-         ("hexagonal_prism", (), 700), ("twisted_hex_prism", (), 1200), ("tetrahedron", (), 800)]
+         ("hexagonal_prism", (), 700), ("twisted_hex_prism", (), 1200), ("tetrahedron", (), 800),
+         ("knot", (), 2100)]
 
 
 def spec_of(name, params):

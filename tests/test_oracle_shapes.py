@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from unnatural_planets_amd import spec as S
+from oracle_lib import mesh_edge_stats
 
 EXT = ["cylinder", "plane", "octahedron", "double_torus", "torus_cross", "h2o", "bowl", "inside_cube",
        "wormhole", "pipe", "hemispheres"]
@@ -82,15 +83,15 @@ def test_flop_table_counts_extended_shapes(oracle, upf, name):
 
 def test_shape_ids_validated(upf):
     s = S.field_spec("sphere", (1.0,), [])
-    s.shape = 30
+    s.shape = 31
     assert upf.lib().upf_spec_validate(ctypes.byref(s)) == S.ERR_BAD_SPEC
     for name in FURTHER:
         assert upf.lib().upf_spec_validate(ctypes.byref(shape_spec(name))) == S.OK
 
 
-# ---- the further shapes (ids 16 .. 28)
+# ---- the further shapes (ids 16 .. 28, and the knot, 30)
 FURTHER = ["h3o", "h4o", "gear", "mandelbulb", "mobius", "fibers", "asteroid", "twisted_plane", "belt",
-           "triangular_prism", "hexagonal_prism", "twisted_hex_prism", "tetrahedron"]
+           "triangular_prism", "hexagonal_prism", "twisted_hex_prism", "tetrahedron", "knot"]
 
 
 def test_further_closed_forms(oracle):
@@ -181,3 +182,51 @@ def test_further_gradients_match_central_differences(oracle, name):
         fd[:, a] = (f64(oracle, name, pts + e) - f64(oracle, name, pts - e)) / (2 * h)
     err = np.linalg.norm(g - fd, axis=1) / np.maximum(np.linalg.norm(g, axis=1), 1e-3)
     assert np.median(err) < 1e-3 and np.mean(err < 1e-2) > 0.9, (name, np.quantile(err, [0.5, 0.9]))
+
+
+# ---- the knot (sources/sdf.cpp:86-90)
+KNOT_NORM = np.sqrt(1.09)
+
+
+def _knot_strand(phi, j=0):
+    """Cross-section position (rho, z) of strand j at azimuth phi, z in the unscaled frame."""
+    a = 1.5 * (phi + 2 * np.pi * j)
+    return 1600 + 200 * np.cos(a), 200 * np.sin(a) / 0.3
+
+
+def test_knot_strand_centres_and_tube_surface(oracle):
+    phi = np.linspace(-np.pi + 0.05, np.pi - 0.05, 41)
+    for j in (0, 1):
+        rho, z = _knot_strand(phi, j)
+        p = np.stack([rho * np.cos(phi), rho * np.sin(phi), z], 1)
+        # on a strand: f = -100 / |(1, 1, 0.3)| (the other strand is 400 away)
+        np.testing.assert_allclose(f64(oracle, "knot", p), -100 / KNOT_NORM, atol=2e-3)
+        # 100 from the strand in the half-plane, radially or along (scaled) z: the surface
+        for d in ((100.0, 0.0), (-100.0, 0.0), (0.0, 100.0 / 0.3)):
+            q = np.stack([(rho + d[0]) * np.cos(phi), (rho + d[0]) * np.sin(phi), z + d[1]], 1)
+            np.testing.assert_allclose(f64(oracle, "knot", q), 0.0, atol=2e-3)
+
+
+def test_knot_symmetries(oracle):
+    # the (3, 2) torus knot is invariant under a turn of 2 pi / 3 about its axis and under
+    # the half turn about the x axis (y, z) -> (-y, -z)
+    rng = np.random.default_rng(12)
+    p = rng.uniform(-2000, 2000, size=(500, 3))
+    c, s_ = np.cos(2 * np.pi / 3), np.sin(2 * np.pi / 3)
+    rot = np.stack([c * p[:, 0] - s_ * p[:, 1], s_ * p[:, 0] + c * p[:, 1], p[:, 2]], 1)
+    base = f64(oracle, "knot", p)
+    np.testing.assert_allclose(f64(oracle, "knot", rot), base, rtol=0, atol=2e-3)
+    np.testing.assert_allclose(f64(oracle, "knot", p * np.array([1, -1, -1])), base, rtol=0, atol=2e-3)
+    # on the axis both strands are 1600 + 200 cos(1.5 phi) away in rho: closed form at z = 0
+    # (atan2(0, 0) = 0: the strands at angles 0 and 3 pi)
+    np.testing.assert_allclose(f64(oracle, "knot", [[0.0, 0.0, 0.0]]), (1400 - 100) / KNOT_NORM, atol=2e-3)
+
+
+def test_knot_mesh_is_a_closed_torus(oracle):
+    """The knotted tube is a torus: closed, oriented, Euler characteristic 0."""
+    g = S.uniform_grid(-2100.0, 2100.0, 100)
+    f = oracle.eval_grid(shape_spec("knot"), g)
+    pos, _, idx = oracle.mc(f, g, normals=False)
+    ne, boundary, bad, oriented = mesh_edge_stats(idx)
+    assert boundary == 0 and bad == 0 and oriented and len(idx) > 5000
+    assert len(pos) - ne + len(idx) == 0

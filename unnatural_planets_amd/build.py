@@ -56,5 +56,26 @@ def build(verbose: bool = False, extra_flags=()) -> str:
     return LIB
 
 
+ROOT = os.path.dirname(HERE)
+CALLER_SRC = os.path.join(ROOT, "tests", "integration", "mesh_generate_generic.cpp")
+CALLER_BIN = os.path.join(ROOT, "tests", "integration", "mesh_generate_generic")
+
+
+def build_caller(verbose: bool = False) -> str:
+    """Compile the integration guide's C++ caller (host code against include/upf.h,
+    linked with libupf.so) next to its source."""
+    cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", CALLER_BIN + ".tmp", CALLER_SRC,
+           "-L", HERE, "-lupf", "-Wl,-rpath,$ORIGIN/../../unnatural_planets_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building the integration caller")
+    os.replace(CALLER_BIN + ".tmp", CALLER_BIN)
+    return CALLER_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_caller(verbose=True))

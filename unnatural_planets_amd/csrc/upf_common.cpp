@@ -27,7 +27,8 @@ int shape_params_used(int shape) {
     case UPF_SHAPE_PIPE: case UPF_SHAPE_HEMISPHERES: return 0;
     case UPF_SHAPE_H3O: case UPF_SHAPE_H4O: case UPF_SHAPE_GEAR: case UPF_SHAPE_MANDELBULB: case UPF_SHAPE_MOBIUS:
     case UPF_SHAPE_TWISTED_PLANE: case UPF_SHAPE_BELT: case UPF_SHAPE_TRIANGULAR_PRISM:
-    case UPF_SHAPE_HEXAGONAL_PRISM: case UPF_SHAPE_TWISTED_HEX_PRISM: case UPF_SHAPE_TETRAHEDRON: return 0;
+    case UPF_SHAPE_HEXAGONAL_PRISM: case UPF_SHAPE_TWISTED_HEX_PRISM: case UPF_SHAPE_TETRAHEDRON:
+    case UPF_SHAPE_KNOT: return 0;
     case UPF_SHAPE_FIBERS: return 3;
     case UPF_SHAPE_ASTEROID: return 2;
     case UPF_SHAPE_SINE_NET: return 3;

@@ -274,7 +274,7 @@ __device__ __forceinline__ void m_voronoi_plane(uint32_t seed, int ppc, float ce
                                 float ny, float nz, float c[3]) {
   const uint32_t h0 = fmix32(seed), h1 = fmix32(h0), h2 = fmix32(h1);
   const float freq = 1.0f / cell;
-  const int32_t cx = (int32_t)(px * freq), cy = (int32_t)(py * freq), cz = (int32_t)(pz * freq);
+  const int32_t cx = cell_index(px * freq), cy = cell_index(py * freq), cz = cell_index(pz * freq);
   float best = __builtin_inff();
   c[0] = c[1] = c[2] = 0.0f;
   for (int dz = -1; dz <= 1; dz++)

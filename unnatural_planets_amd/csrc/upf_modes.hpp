@@ -49,10 +49,18 @@ UPF_HD uint32_t fmix32(uint32_t k) {
 // `ppc` points (sources/voronoi.cpp:17-56, 67-77).  Integer arithmetic wraps mod 2^32;
 // the point of a state s is ((s % 65536) / 65535 + cell) * cell_size with C's truncating
 // %; ties keep the first point in generation order (z, y, x outer to inner).
+// Truncation of a cell coordinate to int32, clamped to +-(2^31 - 128) first, so that the
+// conversion (and the neighbours' +-1) stays defined for any finite position and cell size
+// (identical for every position whose cell index fits).
+UPF_HD int32_t cell_index(float v) {
+  const float m = 2147483520.0f;
+  return (int32_t)(v < -m ? -m : (v > m ? m : v));
+}
+
 UPF_HD void voronoi_nearest(uint32_t seed, int ppc, float cell_size, float px, float py, float pz, float c[3]) {
   const uint32_t h0 = fmix32(seed), h1 = fmix32(h0), h2 = fmix32(h1);
   const float freq = 1.0f / cell_size;
-  const int32_t cx = (int32_t)(px * freq), cy = (int32_t)(py * freq), cz = (int32_t)(pz * freq);
+  const int32_t cx = cell_index(px * freq), cy = cell_index(py * freq), cz = cell_index(pz * freq);
   float best = __builtin_inff();
   c[0] = c[1] = c[2] = 0.0f;
   for (int dz = -1; dz <= 1; dz++)

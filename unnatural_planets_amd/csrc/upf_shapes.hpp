@@ -413,6 +413,22 @@ template <class T> UPF_HD T shape_more_t(int shape, const float* spf, T x, T y, 
       const T rx = c * x - s * z, rz = s * x + c * z;
       return sd_hex_prism(rx, rz, y, 650.0f, 450.0f, zero) - 200.0f;
     }
+    case UPF_SHAPE_KNOT: {  // :86-90: knot(q) / |(1, 1, 0.3)| at q = (x, y, 0.3 z)
+      // the (3, 2) torus knot about z (the form documented at UPF_SHAPE_KNOT): in the
+      // half-plane of q's azimuth the two strands that cross it sit 200 from the core
+      // circle of radius 1600 at angles 1.5 phi and 1.5 (phi + 2 pi)
+      const T qz = z * 0.3f;
+      const T rho = len2(x, y);
+      const T phi = atan2_v(y, x);
+      T best = lift(zero, 1e30f);
+      for (int j = 0; j < 2; j++) {
+        const T a = (phi + 6.2831853f * (float)j) * 1.5f;
+        const T dr = rho - (1600.0f + cos_v(a) * 200.0f);
+        const T dz = qz - sin_v(a) * 200.0f;
+        best = min_v(best, len2(dr, dz));
+      }
+      return (best - 100.0f) / 1.0440307f;  // |(1, 1, 0.3)| = sqrt(1.09)
+    }
     case UPF_SHAPE_TETRAHEDRON: {  // :50-53: circumradius 600, vertices s (1,1,1), s (1,-1,-1), ...
       const float k = 346.41016f;  // 600 / sqrt(3)
       const float v[4][3] = {{k, k, k}, {k, -k, -k}, {-k, k, -k}, {-k, -k, k}};

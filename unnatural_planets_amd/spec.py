@@ -26,7 +26,7 @@ SHAPE = {"none": 0, "sphere": 1, "box": 2, "torus": 3, "capsule": 4, "cylinder":
          "double_torus": 8, "torus_cross": 9, "h2o": 10, "bowl": 11, "inside_cube": 12, "wormhole": 13,
          "pipe": 14, "hemispheres": 15, "h3o": 16, "h4o": 17, "gear": 18, "mandelbulb": 19, "mobius": 20,
          "fibers": 21, "asteroid": 22, "twisted_plane": 23, "belt": 24, "triangular_prism": 25,
-         "hexagonal_prism": 26, "twisted_hex_prism": 27, "tetrahedron": 28, "sine_net": 29}
+         "hexagonal_prism": 26, "twisted_hex_prism": 27, "tetrahedron": 28, "sine_net": 29, "knot": 30}
 SINE_NET_WEIGHTS = 625
 # the reference's parameters (sources/sdf.cpp, metres) for the parametric extended shapes
 SHAPE_REFERENCE_PARAMS = {"cylinder": (10000.0, 800.0), "octahedron": (800.0, 100.0),
