@@ -123,13 +123,18 @@ def bench_bake(upf, S, torch, reps: int):
     albedo = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
     bump = torch.empty((h, w), dtype=torch.float32, device="cuda")
     rough = torch.empty((h, w), dtype=torch.float32, device="cuda")
+    rbytes = L.upf_bake_rasterize_workspace_bytes(T, w, h)
+    rws = torch.empty(rbytes, dtype=torch.uint8, device="cuda")
     P = [ctypes.c_void_p(t.data_ptr()) for t in (mesh.positions, mesh.normals, mesh.indices, uv, tri, albedo, bump,
-                                                 rough)]
+                                                 rough, rws)]
 
     def atlas():
         upf.check(L.upf_atlas_per_triangle(T, w, h, P[3], sp), "atlas")
 
     def raster():
+        upf.check(L.upf_bake_rasterize_ws(P[3], T, w, h, P[4], P[8], rbytes, sp), "rasterize")
+
+    def raster_atomic():
         upf.check(L.upf_bake_rasterize(P[3], T, w, h, P[4], sp), "rasterize")
 
     def material():
@@ -145,7 +150,7 @@ def bench_bake(upf, S, torch, reps: int):
         e1.synchronize()
         return e0.elapsed_time(e1) / reps / 1e3
 
-    t_atlas, t_raster, t_mat = timed(atlas), timed(raster), timed(material)
+    t_atlas, t_raster_atomic, t_raster, t_mat = timed(atlas), timed(raster_atomic), timed(raster), timed(material)
     covered = int((tri >= 0).sum())
     # algorithmic bytes of k_bake: triangle id in + 20 B out per texel, the triangle's UV
     # and indices, and the position and normal of its vertices
@@ -155,6 +160,7 @@ def bench_bake(upf, S, torch, reps: int):
         "workload": "128^3 sphere r=1 + 4-oct gradient fBm (seed 1234) -> marching cubes -> 1024^2 atlas",
         "triangles": T, "vertices": V, "texels": w * h, "covered_texels": covered,
         "atlas_us": round(t_atlas * 1e6, 2), "rasterize_us": round(t_raster * 1e6, 2),
+        "rasterize_atomic_us": round(t_raster_atomic * 1e6, 2),
         "material_us": round(t_mat * 1e6, 2),
         "gtexels_per_s": round(w * h / (t_atlas + t_raster + t_mat) / 1e9, 3),
         "roofline": {"bound": "hbm", "kernel": "k_bake", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
@@ -415,6 +421,8 @@ def bench_nb(upf, S, torch, reps: int, n: int = 1025, tex: int = 4096):
     idx = torch.empty((max_t, 3), dtype=torch.int32, device="cuda")
     uv = torch.empty((max_t, 3, 2), dtype=torch.float32, device="cuda")
     tri = torch.empty((tex, tex), dtype=torch.int32, device="cuda")
+    rbytes = L.upf_bake_rasterize_workspace_bytes(max_t, tex, tex)
+    rws = torch.empty(rbytes, dtype=torch.uint8, device="cuda")
     albedo = torch.empty((tex, tex, 3), dtype=torch.float32, device="cuda")
     bump = torch.empty((tex, tex), dtype=torch.float32, device="cuda")
     rough = torch.empty((tex, tex), dtype=torch.float32, device="cuda")
@@ -432,7 +440,7 @@ def bench_nb(upf, S, torch, reps: int, n: int = 1025, tex: int = 4096):
     def bake_one(i):
         T, V = int(stats[i].n_triangles), int(stats[i].n_vertices)
         upf.check(L.upf_atlas_per_triangle(T, tex, tex, vp(uv), sp), "atlas")
-        upf.check(L.upf_bake_rasterize(vp(uv), T, tex, tex, vp(tri), sp), "rasterize")
+        upf.check(L.upf_bake_rasterize_ws(vp(uv), T, tex, tex, vp(tri), vp(rws), rbytes, sp), "rasterize")
         upf.check(L.upf_bake(ctypes.byref(bakes[i]), vp(pos), vp(nrm), V, vp(idx), T, vp(uv), vp(tri), tex, tex,
                              vp(albedo), vp(bump), vp(rough), sp), "bake")
 
@@ -477,7 +485,7 @@ def bench_nb(upf, S, torch, reps: int, n: int = 1025, tex: int = 4096):
     # and normal per vertex (the 1024^2 bake's count), over atlas + rasterize + material
     bake_bytes = float(np.mean([tex * tex * 24 + 36 * int(st.n_triangles) + 24 * int(st.n_vertices) for st in stats]))
     bake_gbs = bake_bytes / t_bake / 1e9
-    del pos, nrm, idx, uv, tri, albedo, bump, rough, ws
+    del pos, nrm, idx, uv, tri, albedo, bump, rough, ws, rws
     torch.cuda.empty_cache()
 
     # the dense sweep of the same fine grid for planet 0, for comparison
@@ -655,7 +663,17 @@ def main() -> int:
     V, T = int(nv.value), int(nt.value)
     t_field = timed(eval_field, reps)
     t_field_lane = timed(lambda: eval_field(upf.EVAL_PER_LANE), reps)
-    t_mc = timed(lambda: (mc_count(), mc_emit()), reps)
+    mc_counts = torch.full((2,), -1, dtype=torch.int64, device=dev)
+
+    def mc_async():  # stored field -> mesh in stream order, no count read-back
+        upf.check(L.upf_mc_extract_async(ctypes.c_void_p(field.data_ptr()), grid_p, 0.0, ctypes.byref(mb),
+                                         ctypes.c_void_p(mc_counts.data_ptr()), ctypes.c_void_p(ws.buf.data_ptr()),
+                                         ws.bytes, sp), "mc_extract_async")
+
+    t_mc_sync = timed(lambda: (mc_count(), mc_emit()), reps)
+    t_mc = timed(mc_async, reps)
+    if mc_counts.tolist() != [nv.value, nt.value]:
+        raise RuntimeError(f"async mesh counts {mc_counts.tolist()} != {[nv.value, nt.value]}")
     t_fused = timed(field_count, reps)
 
     flops = float(L.upf_flops_per_sample(spec_p))
@@ -685,12 +703,13 @@ def main() -> int:
         "flops_per_sample": flops,
         "mc_cells_per_s": round(cells / t_mc, 1),
         "mc_ms": round(t_mc * 1e3, 4),
+        "mc_count_emit_ms": round(t_mc_sync * 1e3, 4),
         "field_plus_classify_ms": round(t_fused * 1e3, 4),
         "step_two_calls_ms": round(t_step_two * 1e3, 4),
         "roofline": {"bound": "valu", "kernel": "k_eval_grid_row", "achieved": round(field_tflops, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(field_tflops / PEAK_FP32_TFLOPS, 4),
                      "traffic": FIELD_TRAFFIC_512 if nx == ny == nz == 512 else None},
-        "roofline_mc": {"bound": "hbm", "kernel": "marching cubes (classify+scan+emit)", "achieved": round(mc_gbs, 1),
+        "roofline_mc": {"bound": "hbm", "kernel": "marching cubes (signs+classify+scan+emit, upf_mc_extract_async)", "achieved": round(mc_gbs, 1),
                         "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(mc_gbs / PEAK_HBM_GBS, 4),
                         "algorithmic_bytes": mc_bytes},
     }

@@ -376,6 +376,16 @@ int upf_mc_count(const float* d_field, const upf_grid* grid, float iso, void* d_
 int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out,
                 void* d_workspace, size_t workspace_bytes, void* stream);
 
+/* upf_mc_extract with no host synchronisation, for a stored field: classification, scan
+ * and emission in stream order, the mesh into `out` within its capacities and the
+ * required counts into d_counts[0] (vertices), d_counts[1] (triangles), device memory,
+ * exactly as upf_field_mesh_async does after its field pass (same truncation rule when a
+ * count exceeds its capacity).  Mesh and counts equal upf_mc_extract's bit for bit.
+ * Replaces cubes->makeMesh() (sources/meshGeneration.cpp:39) for callers that keep
+ * buffers sized by an earlier call. */
+int upf_mc_extract_async(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out,
+                         int64_t* d_counts, void* d_workspace, size_t workspace_bytes, void* stream);
+
 /* Field evaluation fused with the counting half of marching cubes: evaluates the field
  * into d_field (as upf_field_eval_grid_ex) while the field kernel stores each 64-node
  * group's sign mask, then classifies from the sign masks alone (no second read of the
@@ -527,6 +537,16 @@ int upf_atlas_per_triangle(int64_t n_triangles, int32_t width, int32_t height, f
 /* UV rasterization: d_tri[height][width] = covering triangle id or -1. */
 int upf_bake_rasterize(const float* d_uv, int64_t n_triangles, int32_t width, int32_t height,
                        int32_t* d_tri, void* stream);
+/* The same map, tile-binned: triangles are binned into 64 x 16 texel tiles, each tile is
+ * resolved in on-chip memory and every texel is stored once (no global atomics for
+ * triangles over at most four tiles while a tile's bin of 1024 has room, no separate -1
+ * fill).  d_tri equals upf_bake_rasterize's bit for bit.  The workspace (size from
+ * upf_bake_rasterize_workspace_bytes, 0 for invalid sizes) holds the bins, 4 bytes per
+ * texel, and a list of 16 bytes per triangle.  n_triangles < 2^30, width and height at
+ * most 2^24.  Overlapping calls must use different workspaces. */
+size_t upf_bake_rasterize_workspace_bytes(int64_t n_triangles, int32_t width, int32_t height);
+int upf_bake_rasterize_ws(const float* d_uv, int64_t n_triangles, int32_t width, int32_t height,
+                          int32_t* d_tri, void* d_ws, size_t ws_bytes, void* stream);
 /* Per-texel material.  Mesh: positions/normals float[V][3], indices uint32[T][3]; d_uv
  * and d_tri from above.  Outputs: albedo float[h][w][3], bump and roughness float[h][w].
  * A texel whose triangle id is outside [0, T) or whose triangle names a vertex outside

@@ -13,26 +13,95 @@ from unnatural_planets_amd import spec as S
 
 pytestmark = pytest.mark.gpu
 
+METHODS = ["tiles", "atomic"]
+
+
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("n,w,h", [(90001, 1024, 1024), (1001, 333, 200), (1, 2, 2)])
-def test_atlas_and_rasterize_bit_exact(upf, oracle, n, w, h):
+def test_atlas_and_rasterize_bit_exact(upf, oracle, n, w, h, method):
     uv = upfm.atlas_per_triangle(n, w, h)
     ref_uv = oracle.atlas(n, w, h)
     np.testing.assert_array_equal(uv.cpu().numpy(), ref_uv)
-    tri = upfm.bake_rasterize(uv, w, h)
+    tri = upfm.bake_rasterize(uv, w, h, method=method)
     np.testing.assert_array_equal(tri.cpu().numpy(), oracle.rasterize(ref_uv, w, h))
 
 
-def test_rasterize_overlapping_random_triangles(upf, oracle):
+@pytest.mark.parametrize("method", METHODS)
+def test_rasterize_overlapping_random_triangles(upf, oracle, method):
     import torch
     rng = np.random.default_rng(3)
     w, h, T = 320, 192, 4000
     c = rng.uniform(-0.1, 1.1, size=(T, 1, 2))
     uv = (c + rng.normal(0, 0.03, size=(T, 3, 2))).astype(np.float32)
     uv[::101, 2] = uv[::101, 1]  # degenerate
-    got = upfm.bake_rasterize(torch.from_numpy(uv).cuda(), w, h).cpu().numpy()
+    got = upfm.bake_rasterize(torch.from_numpy(uv).cuda(), w, h, method=method).cpu().numpy()
     ref = oracle.rasterize(uv, w, h)
     np.testing.assert_array_equal(got, ref)
     assert (got >= 0).mean() > 0.5
+
+
+def _tile_cases():
+    rng = np.random.default_rng(11)
+    f32 = np.float32
+    # 3000 small triangles inside one 64 x 16 tile: its bin of 1024 fills and the rest
+    # go through the listed (triangle, tile) pairs
+    c = rng.uniform([0.02, 0.1], [0.45, 0.4], size=(3000, 1, 2))
+    crowded = (c + rng.normal(0, 0.02, size=(3000, 3, 2))).astype(f32)
+    # big triangles over many tiles, mixed with small ones, the image sides not multiples
+    # of the tile's
+    big = rng.uniform(-0.2, 1.2, size=(40, 3, 2)).astype(f32)
+    small = (rng.uniform(0, 1, size=(2000, 1, 2)) + rng.normal(0, 0.01, size=(2000, 3, 2))).astype(f32)
+    mixed = np.concatenate([small[:1000], big, small[1000:]])
+    # non-finite and huge coordinates, zero area, a triangle over the whole image
+    odd = (rng.uniform(0, 1, size=(64, 1, 2)) + rng.normal(0, 0.05, size=(64, 3, 2))).astype(f32)
+    odd[3, 1, 0] = np.nan
+    odd[5, 0, 1] = np.inf
+    odd[7] = [[-1e30, -1e30], [1e30, -1e30], [0.5, 1e30]]
+    odd[9] = [[0.2, 0.2], [0.4, 0.4], [0.6, 0.6]]
+    odd[11] = [[-4, -4], [6, -4], [-4, 6]]
+    odd[13] = [[3e7, 0.5], [3e7 + 1, 0.5], [3e7, 0.6]]
+    one = crowded[:50].copy()
+    one[20] = [[-0.5, -0.5], [2.0, -0.5], [-0.5, 2.0]]  # covers the single texel's centre
+    return [("crowded", crowded, 64, 16), ("mixed", mixed, 301, 77), ("odd", odd, 97, 53),
+            ("one_texel", one, 1, 1)]
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_rasterize_tiles_full_bins_big_and_odd_triangles(upf, oracle, case):
+    import torch
+    name, uv, w, h = _tile_cases()[case]
+    print("case:", name, uv.shape[0], w, h, flush=True)
+    d = torch.from_numpy(uv).cuda()
+    got = upfm.bake_rasterize(d, w, h, method="tiles").cpu().numpy()
+    ref = oracle.rasterize(uv, w, h)
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_array_equal(upfm.bake_rasterize(d, w, h, method="atomic").cpu().numpy(), ref)
+
+
+def test_rasterize_tiles_workspace_reuse_and_errors(upf, oracle):
+    """The workspace of a larger call serves a smaller one (the counts are cleared each
+    call); a short workspace, a NULL map and sides over 2^24 are refused."""
+    import ctypes
+    import torch
+    L = upfm.lib()
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    big = upfm.atlas_per_triangle(20000, 512, 512)
+    nbytes = L.upf_bake_rasterize_workspace_bytes(20000, 512, 512)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    for n, w, h in ((20000, 512, 512), (999, 200, 100), (0, 40, 9)):
+        uv = upfm.atlas_per_triangle(n, w, h)
+        tri = torch.empty((h, w), dtype=torch.int32, device="cuda")
+        assert L.upf_bake_rasterize_workspace_bytes(n, w, h) <= nbytes
+        upfm.check(L.upf_bake_rasterize_ws(p(uv) if n else None, n, w, h, p(tri), p(ws), nbytes, sp), "ws")
+        ref = oracle.rasterize(uv.cpu().numpy(), w, h)
+        np.testing.assert_array_equal(tri.cpu().numpy(), ref)
+    tri = torch.empty((16, 16), dtype=torch.int32, device="cuda")
+    need = L.upf_bake_rasterize_workspace_bytes(100, 16, 16)
+    assert L.upf_bake_rasterize_ws(p(big), 100, 16, 16, p(tri), p(ws), need - 1, sp) == S.ERR_WORKSPACE
+    assert L.upf_bake_rasterize_ws(p(big), 100, 16, 16, None, p(ws), nbytes, sp) == S.ERR_NULL
+    assert L.upf_bake_rasterize_workspace_bytes(100, (1 << 24) + 1, 1) == 0
+    assert L.upf_bake_rasterize_ws(p(big), 100, (1 << 24) + 1, 1, p(tri), p(ws), nbytes, sp) == S.ERR_BAD_GRID
 
 
 def _compare_material(got, ref):

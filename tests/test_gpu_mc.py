@@ -243,3 +243,38 @@ def test_field_mesh_async_capacity_overflow(upf):
     assert torch.equal(pos[:cap_v], ref.positions[:cap_v]) and torch.equal(nrm[:cap_v], ref.normals[:cap_v])
     assert torch.equal(idx[:cap_t], ref.indices[:cap_t])
     assert bool((pos[cap_v:] == 7.0).all()) and bool((nrm[cap_v:] == 9.0).all()) and bool((idx[cap_t:] == -5).all())
+
+
+@pytest.mark.parametrize("dims", [(128, 128, 128), (97, 70, 33), (2, 2, 2)])
+def test_mc_extract_async_equals_extract(upf, dims):
+    """upf_mc_extract_async on a stored field (no host synchronisation) = upf_mc_extract
+    bit for bit, twice in a row on the same buffers; rows past the counts stay untouched;
+    with capacities below the counts the counts still report the full mesh and the rows
+    within the capacities equal the full mesh's."""
+    torch = _torch()
+    s = S.spec_sphere_fbm(1234) if dims[0] == 128 else S.spec_torus_ridged_voronoi(42)
+    g = _grid(dims, -1.3, 1.3)
+    f = upf.field_eval_grid(s, g)
+    ref = upf.mc_extract(f, g)
+    nv, nt = ref.n_vertices, ref.n_triangles
+    mesh = upf.Mesh(torch.full((nv + 50, 3), 7.0, device="cuda"), torch.full((nv + 50, 3), 9.0, device="cuda"),
+                    torch.full((nt + 50, 3), -5, dtype=torch.int32, device="cuda"))
+    ws = None
+    for call in range(2):
+        counts = torch.full((2,), -1, dtype=torch.int64, device="cuda")
+        ws = upf.mc_extract_async(f, g, mesh, counts, workspace=ws)
+        torch.cuda.synchronize()
+        print(f"case: mc async {dims} call {call}: counts {counts.tolist()} vs {(nv, nt)}", flush=True)
+        assert counts.tolist() == [nv, nt]
+        assert torch.equal(mesh.positions[:nv], ref.positions) and torch.equal(mesh.normals[:nv], ref.normals)
+        assert torch.equal(mesh.indices[:nt], ref.indices)
+        assert bool((mesh.positions[nv:] == 7.0).all()) and bool((mesh.indices[nt:] == -5).all())
+    if nt > 4:
+        cap_v, cap_t = nv // 3, nt // 2
+        small = upf.Mesh(torch.full((cap_v, 3), 7.0, device="cuda"), torch.full((cap_v, 3), 9.0, device="cuda"),
+                         torch.full((cap_t, 3), -5, dtype=torch.int32, device="cuda"))
+        counts = torch.full((2,), -1, dtype=torch.int64, device="cuda")
+        upf.mc_extract_async(f, g, small, counts, workspace=ws)
+        torch.cuda.synchronize()
+        assert counts.tolist() == [nv, nt]
+        assert torch.equal(small.positions, ref.positions[:cap_v]) and torch.equal(small.indices, ref.indices[:cap_t])

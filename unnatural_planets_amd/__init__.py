@@ -41,7 +41,7 @@ __all__ = [
     "terrain_sdf_elevation", "terrain_sdf_elevation_raw", "mesh_generate_generic",
     "mesh_generate_base_land", "mesh_generate_base_water", "mesh_generate_base_navigation",
     "reference_grid", "flops_per_sample", "adjoint_flops_per_sample", "BakeSpec", "bake_spec",
-    "Textures", "atlas_per_triangle", "bake_rasterize", "bake", "generate_textures_land",
+    "Textures", "mc_extract_async", "atlas_per_triangle", "bake_rasterize", "bake", "generate_textures_land",
     "NbWorkspace", "nb_extract", "field_mesh_count", "mc_emit_counted", "image_dilate", "image_to_u8",
     "image_height_to_normal", "image_special_to_gltf_pbr", "export_maps",
     "texture_post", "mesh_remove_disconnected", "mesh_water_mask",
@@ -355,6 +355,42 @@ def field_mesh_async(s: FieldSpec, g: Grid, field, mesh: Mesh, counts, iso: floa
     return ws
 
 
+def _cap_buffers(mesh: Mesh):
+    _check_rows3(mesh.positions, "mesh.positions")
+    _check_rows3(mesh.indices, "mesh.indices", "int32")
+    cap_v, cap_t = int(mesh.positions.shape[0]), int(mesh.indices.shape[0])
+    if mesh.normals is not None:
+        _check_f32(mesh.normals, "mesh.normals", (cap_v, 3))
+    mb = MeshBuffers()
+    mb.positions = mesh.positions.data_ptr() if cap_v else None
+    mb.normals = mesh.normals.data_ptr() if (mesh.normals is not None and cap_v) else None
+    mb.indices = mesh.indices.data_ptr() if cap_t else None
+    mb.cap_vertices, mb.cap_triangles = cap_v, cap_t
+    return mb
+
+
+def _check_counts(counts):
+    if counts.dtype != _torch().int64 or counts.numel() < 2 or not counts.is_contiguous() or not counts.is_cuda:
+        raise ValueError("counts must be a contiguous int64 CUDA tensor of 2 elements")
+
+
+@_on_stream
+def mc_extract_async(field, g: Grid, mesh: Mesh, counts, iso: float = 0.0, workspace: McWorkspace | None = None,
+                     stream=None) -> McWorkspace:
+    """Marching cubes over a stored field in one stream-ordered call (upf_mc_extract_async):
+    the mesh goes into the preallocated `mesh` tensors (their row counts are the
+    capacities), the required counts into `counts` (int64 [2] on the device).  Nothing
+    synchronises; the mesh is complete when counts <= the capacities."""
+    nx, ny, nz = grid_dims(g)
+    _check_f32(field, "field", (nz, ny, nx))
+    _check_counts(counts)
+    ws = workspace or McWorkspace(g, field.device)
+    mb = _cap_buffers(mesh)
+    check(lib().upf_mc_extract_async(_ptr(field), ctypes.byref(g), iso, ctypes.byref(mb), _ptr(counts), _ptr(ws.buf),
+                                      ws.bytes, _stream(stream)), "upf_mc_extract_async")
+    return ws
+
+
 def _mesh_buffers(m: Mesh):
     mb = MeshBuffers()
     mb.positions = m.positions.data_ptr() if m.n_vertices else None
@@ -506,16 +542,31 @@ def atlas_per_triangle(n_triangles: int, width: int, height: int, device=None, s
 
 
 @_on_stream
-def bake_rasterize(uv, width: int, height: int, stream=None):
-    """Covering triangle per texel (upf_bake_rasterize) -> int32 [h, w], -1 where uncovered."""
+def bake_rasterize(uv, width: int, height: int, stream=None, method: str = "tiles"):
+    """Covering triangle per texel -> int32 [h, w], -1 where uncovered.
+
+    method "tiles": upf_bake_rasterize_ws (tile-binned, the map stored once; sides up to
+    2^24 texels); "atomic": upf_bake_rasterize (one thread per triangle, atomicMax into a
+    -1 filled map).  Both give the same map."""
     torch = _torch()
     _check_f32(uv, "uv")
     if uv.dim() != 3 or tuple(uv.shape[1:]) != (3, 2):
         raise ValueError("uv must have shape [T, 3, 2]")
     tri = torch.empty((height, width), dtype=torch.int32, device=uv.device)
     n = int(uv.shape[0])
-    check(lib().upf_bake_rasterize(_ptr(uv) if n else None, n, width, height, _ptr(tri), _stream(stream)),
-          "upf_bake_rasterize")
+    L = lib()
+    if method == "atomic":
+        check(L.upf_bake_rasterize(_ptr(uv) if n else None, n, width, height, _ptr(tri), _stream(stream)),
+              "upf_bake_rasterize")
+    elif method == "tiles":
+        nbytes = L.upf_bake_rasterize_workspace_bytes(n, width, height)
+        if nbytes == 0:
+            check(5, "upf_bake_rasterize_workspace_bytes")  # UPF_ERR_BAD_GRID
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=uv.device)
+        check(L.upf_bake_rasterize_ws(_ptr(uv) if n else None, n, width, height, _ptr(tri), _ptr(ws), nbytes,
+                                      _stream(stream)), "upf_bake_rasterize_ws")
+    else:
+        raise ValueError(f"unknown method {method!r}")
     return tri
 
 

@@ -57,6 +57,9 @@ SIGNATURES = {
     "upf_field_mesh_async": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_float,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MeshBuffers),
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "upf_mc_extract_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Grid), ctypes.c_float,
+                                            ctypes.POINTER(MeshBuffers), ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_size_t, ctypes.c_void_p]),
     "upf_static_scratch_bytes": (ctypes.c_size_t, []),
     "upf_field_mesh_async_static": (ctypes.c_int, [ctypes.POINTER(FieldSpec), ctypes.POINTER(Grid), ctypes.c_float,
                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(MeshBuffers),
@@ -106,6 +109,9 @@ SIGNATURES = {
                                               ctypes.c_void_p]),
     "upf_bake_rasterize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "upf_bake_rasterize_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]),
+    "upf_bake_rasterize_ws": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "upf_bake": (ctypes.c_int, [ctypes.POINTER(BakeSpec), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                 ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -10,6 +10,7 @@
 // material is light; the detail noise adds about 60 flops per octave.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "upf_device.hpp"
@@ -59,32 +60,240 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Texel-space set-up of triangle t: its corners, the sign that orients its edge
+// functions, its box and the box's texels clipped to the image.  False when it covers
+// nothing (zero or non-finite area, a box outside the image).  TIGHT: the box's texels
+// are exactly those whose centres lie in the box (a texel range of at most 2^24 per
+// side); otherwise the range is padded by up to one texel per side.  Either way
+// rast_covers tests the box, so the covered set is the same.
+struct RastTri {
+  float ax, ay, bx, by, cx, cy, sgn, lx, hx, ly, hy;
+  int x0, x1, y0, y1;
+};
+
+// Smallest x in [0, n] whose centre x + 0.5f is >= v (n when there is none), n <= 2^24.
+// floorf(v - 0.5f) is within one texel of it (v - 0.5f rounds by at most half a unit
+// where the centres are integers plus a half, by at most one above 2^23); two checked
+// steps each way reach it without a data-dependent loop.
+__device__ __forceinline__ int texel_first(float v, int n) {
+  int x = min((int)fminf(fmaxf(floorf(v - 0.5f), 0.0f), 16777216.0f), n);
+#pragma unroll
+  for (int step = 0; step < 2; step++)
+    if (x > 0 && (float)(x - 1) + 0.5f >= v) x--;
+#pragma unroll
+  for (int step = 0; step < 2; step++)
+    if (x < n && (float)x + 0.5f < v) x++;
+  return x;
+}
+
+// Largest x in [-1, n - 1] whose centre x + 0.5f is <= v (-1 when there is none).
+__device__ __forceinline__ int texel_last(float v, int n) {
+  int x = min((int)fminf(fmaxf(floorf(v - 0.5f), -1.0f), 16777216.0f), n - 1);
+#pragma unroll
+  for (int step = 0; step < 2; step++)
+    if (x < n - 1 && (float)(x + 1) + 0.5f <= v) x++;
+#pragma unroll
+  for (int step = 0; step < 2; step++)
+    if (x >= 0 && (float)x + 0.5f > v) x--;
+  return x;
+}
+
+template <bool TIGHT>
+__device__ __forceinline__ bool rast_setup(const float* __restrict__ uv, int64_t t, int w, int h, RastTri& r) {
+  r.ax = uv[6 * t] * (float)w; r.ay = uv[6 * t + 1] * (float)h;
+  r.bx = uv[6 * t + 2] * (float)w; r.by = uv[6 * t + 3] * (float)h;
+  r.cx = uv[6 * t + 4] * (float)w; r.cy = uv[6 * t + 5] * (float)h;
+  const float area = edge_fn(r.ax, r.ay, r.bx, r.by, r.cx, r.cy);
+  if (!(area != 0.0f) || !isfinite(area)) return false;
+  r.sgn = area > 0.0f ? 1.0f : -1.0f;
+  r.lx = fminf(r.ax, fminf(r.bx, r.cx)); r.hx = fmaxf(r.ax, fmaxf(r.bx, r.cx));
+  r.ly = fminf(r.ay, fminf(r.by, r.cy)); r.hy = fmaxf(r.ay, fmaxf(r.by, r.cy));
+  if (TIGHT) {
+    r.x0 = texel_first(r.lx, w); r.x1 = texel_last(r.hx, w);
+    r.y0 = texel_first(r.ly, h); r.y1 = texel_last(r.hy, h);
+  } else {
+    r.x0 = max(0, (int)floorf(r.lx - 0.5f)); r.x1 = min(w - 1, (int)ceilf(r.hx - 0.5f));
+    r.y0 = max(0, (int)floorf(r.ly - 0.5f)); r.y1 = min(h - 1, (int)ceilf(r.hy - 0.5f));
+  }
+  return r.x0 <= r.x1 && r.y0 <= r.y1;
+}
+
+// Whether the centre of texel (x, y) is inside (or on) the triangle; texel centres
+// outside the triangle's box are outside, whatever the edge functions round to.
+__device__ __forceinline__ bool rast_covers(const RastTri& r, int x, int y) {
+  const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  if (px < r.lx || px > r.hx || py < r.ly || py > r.hy) return false;
+  const float w0 = r.sgn * edge_fn(r.bx, r.by, r.cx, r.cy, px, py);
+  const float w1 = r.sgn * edge_fn(r.cx, r.cy, r.ax, r.ay, px, py);
+  const float w2 = r.sgn * edge_fn(r.ax, r.ay, r.bx, r.by, px, py);
+  return w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f;
+}
+
 // One thread per triangle: every texel centre of its bounding box inside (or on) the UV
 // triangle gets max(current, t).
 __global__ __launch_bounds__(256) void k_rasterize(const float* __restrict__ uv, int64_t n, int w, int h,
                                                    int32_t* __restrict__ tri) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= n) return;
-  const float ax = uv[6 * t] * (float)w, ay = uv[6 * t + 1] * (float)h;
-  const float bx = uv[6 * t + 2] * (float)w, by = uv[6 * t + 3] * (float)h;
-  const float cx = uv[6 * t + 4] * (float)w, cy = uv[6 * t + 5] * (float)h;
-  const float area = edge_fn(ax, ay, bx, by, cx, cy);
-  if (!(area != 0.0f) || !isfinite(area)) return;
-  const float sgn = area > 0.0f ? 1.0f : -1.0f;
-  const float lx = fminf(ax, fminf(bx, cx)), hx = fmaxf(ax, fmaxf(bx, cx));
-  const float ly = fminf(ay, fminf(by, cy)), hy = fmaxf(ay, fmaxf(by, cy));
-  const int x0 = max(0, (int)floorf(lx - 0.5f)), x1 = min(w - 1, (int)ceilf(hx - 0.5f));
-  const int y0 = max(0, (int)floorf(ly - 0.5f)), y1 = min(h - 1, (int)ceilf(hy - 0.5f));
-  for (int y = y0; y <= y1; y++)
-    for (int x = x0; x <= x1; x++) {
-      const float px = (float)x + 0.5f, py = (float)y + 0.5f;
-      // texel centres outside the bounding box are outside, whatever the edge functions round to
-      if (px < lx || px > hx || py < ly || py > hy) continue;
-      const float w0 = sgn * edge_fn(bx, by, cx, cy, px, py);
-      const float w1 = sgn * edge_fn(cx, cy, ax, ay, px, py);
-      const float w2 = sgn * edge_fn(ax, ay, bx, by, px, py);
-      if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f) atomicMax(tri + (int64_t)y * w + x, (int32_t)t);
+  RastTri r;
+  if (!rast_setup<false>(uv, t, w, h, r)) return;
+  for (int y = r.y0; y <= r.y1; y++)
+    for (int x = r.x0; x <= r.x1; x++)
+      if (rast_covers(r, x, y)) atomicMax(tri + (int64_t)y * w + x, (int32_t)t);
+}
+
+// ------------------------------------------------------------------ tile-binned rasterization
+// The image is cut into tiles of 64 x 16 texels (a wave stores one 256-byte line).  The
+// bin pass gives each tile that a triangle's texel box meets (at most kRastSlots of them)
+// a slot in that tile's bin of kRastBin entries: consecutive triangles of an atlas share
+// tiles, so the lanes of a wave with equal tiles in a row take their slots with one
+// atomic, all runs of the wave at once.  The tile pass then resolves the largest
+// covering id of each of a tile's 1024 texels in LDS, binned triangles spread over the
+// workgroup's threads, and stores every texel once, -1 where nothing covers it: the
+// image is written in whole lines, with no global atomics and no separate fill.  What
+// does not fit, a (triangle, tile) pair beyond a full bin or a triangle over more than
+// kRastSlots tiles, is listed as t * 4 + slot and added after the tile pass with global
+// atomicMax over that tile (over the whole box for a triangle over more tiles), as
+// k_rasterize does.
+constexpr int kRastTW = 64, kRastTH = 16;
+constexpr int kRastBin = 1024;
+constexpr int kRastSlots = 4;
+constexpr int kRastMaxSide = 1 << 24;
+
+struct RastTiles {
+  int tx0, ty0, cols, n;
+};
+
+__device__ __forceinline__ RastTiles rast_tiles(const RastTri& r) {
+  RastTiles q;
+  q.tx0 = r.x0 / kRastTW; q.ty0 = r.y0 / kRastTH;
+  q.cols = r.x1 / kRastTW - q.tx0 + 1;
+  q.n = q.cols * (r.y1 / kRastTH - q.ty0 + 1);
+  return q;
+}
+
+__device__ __forceinline__ int rast_tile_key(const RastTiles& q, int tiles_x, int j) {
+  return (q.ty0 + j / q.cols) * tiles_x + q.tx0 + j % q.cols;
+}
+
+__global__ __launch_bounds__(256) void k_rast_bin(const float* __restrict__ uv, int64_t n, int w, int h, int tiles_x,
+                                                  uint32_t* __restrict__ count, uint32_t* __restrict__ bins,
+                                                  uint32_t* __restrict__ over, uint32_t* __restrict__ nover) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  RastTri r;
+  const bool ok = t < n && rast_setup<true>(uv, t, w, h, r);
+  RastTiles q = {0, 0, 1, 0};
+  if (ok) q = rast_tiles(r);
+  const bool small = q.n <= kRastSlots;
+  if (ok && !small) over[atomicAdd(nover, 1u)] = (uint32_t)t * kRastSlots;
+  const uint64_t below = (1ull << lane) - 1ull;  // lanes under this one
+#pragma unroll
+  for (int j = 0; j < kRastSlots; j++) {
+    const int key = (small && j < q.n) ? rast_tile_key(q, tiles_x, j) : -1;
+    if (__ballot(key >= 0) == 0) break;  // no lane has a j-th tile (uniform)
+    // runs of equal keys among consecutive lanes: the first lane of a run takes the
+    // run's slots, the others their offsets in it
+    const int prev = __shfl_up(key, 1);
+    const bool head = lane == 0 || key != prev;
+    const uint64_t heads = __ballot(head);
+    const uint64_t after = heads & ~(below | (1ull << lane));  // heads above this lane
+    const int next = after ? __ffsll((unsigned long long)after) - 1 : 64;
+    uint32_t base = 0;
+    if (head && key >= 0) base = atomicAdd(count + key, (uint32_t)(next - lane));
+    const int my_head = 63 - __clzll((unsigned long long)(heads & (below | (1ull << lane))));
+    const uint32_t pos = (uint32_t)__shfl((int)base, my_head) + (uint32_t)(lane - my_head);
+    if (key >= 0) {
+      if (pos < (uint32_t)kRastBin)
+        bins[(int64_t)key * kRastBin + pos] = (uint32_t)t;
+      else
+        over[atomicAdd(nover, 1u)] = (uint32_t)t * kRastSlots + (uint32_t)j;
     }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rast_tiles(const float* __restrict__ uv, int w, int h, int tiles_x,
+                                                    const uint32_t* __restrict__ count,
+                                                    const uint32_t* __restrict__ bins, int32_t* __restrict__ tri) {
+  constexpr int kTexels = kRastTW * kRastTH;
+  __shared__ int s_best[kTexels];
+  const int tile = blockIdx.x;
+  const int bx0 = (tile % tiles_x) * kRastTW, by0 = (tile / tiles_x) * kRastTH;
+  const uint32_t c = min(count[tile], (uint32_t)kRastBin);
+  uint32_t ids[kRastBin / 256];
+#pragma unroll
+  for (int e = 0; e < kRastBin / 256; e++) {  // the bin's loads in flight together
+    const uint32_t slot = threadIdx.x + 256 * e;
+    ids[e] = slot < c ? bins[(int64_t)tile * kRastBin + slot] : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (int e = 0; e < kTexels / 256; e++) s_best[threadIdx.x + 256 * e] = -1;
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kRastBin / 256; e++) {
+    if (ids[e] == 0xFFFFFFFFu) continue;
+    const int64_t t = ids[e];
+    RastTri r;
+    if (!rast_setup<true>(uv, t, w, h, r)) continue;  // true for every binned triangle
+    const int x0 = max(r.x0, bx0), x1 = min(r.x1, bx0 + kRastTW - 1);
+    const int y0 = max(r.y0, by0), y1 = min(r.y1, by0 + kRastTH - 1);
+    for (int y = y0; y <= y1; y++)
+      for (int x = x0; x <= x1; x++)
+        if (rast_covers(r, x, y)) atomicMax(&s_best[(y - by0) * kRastTW + (x - bx0)], (int)t);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < kTexels / 256; e++) {
+    const int u = threadIdx.x + 256 * e;
+    const int x = bx0 + (u & (kRastTW - 1)), y = by0 + u / kRastTW;
+    if (x < w && y < h) tri[(int64_t)y * w + x] = s_best[u];
+  }
+}
+
+// The listed (triangle, tile) pairs and triangles over more tiles, after the tile pass.
+__global__ __launch_bounds__(256) void k_rast_over(const float* __restrict__ uv, int w, int h, int tiles_x,
+                                                   const uint32_t* __restrict__ over,
+                                                   const uint32_t* __restrict__ nover, int32_t* __restrict__ tri) {
+  const uint32_t no = *nover;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < no; i += gridDim.x * 256) {
+    const int64_t t = over[i] / kRastSlots;
+    const int j = (int)(over[i] % kRastSlots);
+    RastTri r;
+    if (!rast_setup<true>(uv, t, w, h, r)) continue;
+    const RastTiles q = rast_tiles(r);
+    int x0 = r.x0, x1 = r.x1, y0 = r.y0, y1 = r.y1;
+    if (q.n <= kRastSlots) {  // one tile of a binned triangle
+      const int key = rast_tile_key(q, tiles_x, j);
+      const int bx0 = (key % tiles_x) * kRastTW, by0 = (key / tiles_x) * kRastTH;
+      x0 = max(x0, bx0); x1 = min(x1, bx0 + kRastTW - 1);
+      y0 = max(y0, by0); y1 = min(y1, by0 + kRastTH - 1);
+    }
+    for (int y = y0; y <= y1; y++)
+      for (int x = x0; x <= x1; x++)
+        if (rast_covers(r, x, y)) atomicMax(tri + (int64_t)y * w + x, (int32_t)t);
+  }
+}
+
+struct RastLayout {
+  int tiles_x, tiles_y;
+  int64_t tiles;
+  size_t off_count, off_bins, off_over, bytes;
+};
+
+// count: tiles + 1 words (the last counts the list), bins: kRastBin words per tile,
+// list: kRastSlots words per triangle (its upper bound).
+RastLayout rast_layout(int64_t n, int w, int h) {
+  RastLayout L;
+  L.tiles_x = (w + kRastTW - 1) / kRastTW;
+  L.tiles_y = (h + kRastTH - 1) / kRastTH;
+  L.tiles = (int64_t)L.tiles_x * L.tiles_y;
+  const auto a256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t o = 0;
+  L.off_count = o; o = a256(o + 4 * (size_t)(L.tiles + 1));
+  L.off_bins = o; o = a256(o + 4 * (size_t)kRastBin * (size_t)L.tiles);
+  L.off_over = o; o = a256(o + 4 * (size_t)kRastSlots * (size_t)n);
+  L.bytes = o;
+  return L;
 }
 
 __device__ __forceinline__ float sat(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
@@ -278,6 +487,41 @@ int upf_bake_rasterize(const float* d_uv, int64_t n, int32_t w, int32_t h, int32
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((texels + 255) / 256)), dim3(256), 0, s, d_tri, texels, -1);
   if (n > 0) hipLaunchKernelGGL(k_rasterize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_uv, n, w, h, d_tri);
+  UPF_LAUNCH_CHECK();
+  return UPF_OK;
+}
+
+size_t upf_bake_rasterize_workspace_bytes(int64_t n, int32_t w, int32_t h) {
+  if (n < 0 || n >= ((int64_t)1 << 30) || w < 1 || h < 1 || w > kRastMaxSide || h > kRastMaxSide) return 0;
+  return rast_layout(n, w, h).bytes;
+}
+
+int upf_bake_rasterize_ws(const float* d_uv, int64_t n, int32_t w, int32_t h, int32_t* d_tri, void* d_ws,
+                          size_t ws_bytes, void* stream) {
+  if (n < 0 || w < 1 || h < 1) return UPF_ERR_BAD_GRID;
+  if (!d_tri || (n > 0 && !d_uv)) return UPF_ERR_NULL;
+  // list entries are t * 4 + slot in 32 bits; tight boxes need sides of at most 2^24
+  if (n >= ((int64_t)1 << 30) || w > kRastMaxSide || h > kRastMaxSide) return UPF_ERR_BAD_GRID;
+  hipStream_t s = as_stream(stream);
+  const RastLayout L = rast_layout(n, w, h);
+  if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
+  if (L.tiles >= ((int64_t)1 << 31)) return UPF_ERR_BAD_GRID;
+  char* ws = static_cast<char*>(d_ws);
+  uint32_t* count = reinterpret_cast<uint32_t*>(ws + L.off_count);
+  uint32_t* nover = count + L.tiles;
+  uint32_t* bins = reinterpret_cast<uint32_t*>(ws + L.off_bins);
+  uint32_t* over = reinterpret_cast<uint32_t*>(ws + L.off_over);
+  // the counts and the list length start at zero (a fill kernel: no memset nodes)
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((L.tiles + 1 + 255) / 256)), dim3(256), 0, s,
+                     reinterpret_cast<int32_t*>(count), L.tiles + 1, 0);
+  if (n > 0)
+    hipLaunchKernelGGL(k_rast_bin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_uv, n, w, h, L.tiles_x,
+                       count, bins, over, nover);
+  hipLaunchKernelGGL(k_rast_tiles, dim3((unsigned)L.tiles), dim3(256), 0, s, d_uv, w, h, L.tiles_x, count, bins,
+                     d_tri);
+  if (n > 0)
+    hipLaunchKernelGGL(k_rast_over, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 1024)), dim3(256), 0, s, d_uv,
+                       w, h, L.tiles_x, over, nover, d_tri);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }

@@ -228,7 +228,38 @@ __device__ __forceinline__ float row_layer_grad(const RowCtx& c, const DLayer& L
   return layer_grad<THETA>(L, l, x, y, z, coeff, coeff_theta, g, acc);  // simplex: per lane
 }
 
-// Row version of field_grad: the same passes, with the row layer functions.
+// Row version of field_grad with the layer passes ordered so that no layer is evaluated
+// twice except a mask's value.  elev = sum over layers of coefficient * weight * value.
+// The coefficient of an add layer is abar(l), of a lerp layer abar(l) * m_l, where m_l
+// is the mask in effect at l and abar(l) the product of (1 - m) over the lerp layers
+// after l: both depend only on the masks' values.  A mask's coefficient is the sum over
+// the lerp layers it drives of abar(q) * (v_q - acc before q), where acc is the running
+// blend.  So: the masks' values first (one octave each in practice), then the gradients
+// of the add and lerp layers with their coefficients known, whose values complete the
+// masks' coefficients, then the masks' gradients.  NAVIGATION needs the sign of elev
+// before any coefficient, so it takes every layer's value first, as field_grad does.
+// Per-layer scalars live in float4 registers (lget / lset): indexing a local array with
+// the (uniform, not constant) layer index would put it in scratch memory.
+static_assert(UPF_MAX_LAYERS == 4, "per-layer scalars live in a float4");
+
+__device__ __forceinline__ float lget(const float4& v, int l) {
+  return l == 0 ? v.x : (l == 1 ? v.y : (l == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ void lset(float4& v, int l, float x) {
+  v.x = l == 0 ? x : v.x;
+  v.y = l == 1 ? x : v.y;
+  v.z = l == 2 ? x : v.z;
+  v.w = l == 3 ? x : v.w;
+}
+
+// abar(l): the product of (1 - m) over the lerp layers after l.
+__device__ __forceinline__ float lerp_tail(const DSpec& s, const float4& min_, int l) {
+  float a = 1.0f;
+  for (int q = l + 1; q < s.n_layers; q++)
+    if (s.L[q].blend == UPF_BLEND_LERP) a *= 1.0f - lget(min_, q);
+  return a;
+}
+
 template <bool THETA>
 __device__ __forceinline__ float row_field_grad(const RowCtx& c, const DSpec& s, float x, float y, float z,
                                                 float dLdf, float g[3], ThetaAcc& acc) {
@@ -253,56 +284,80 @@ __device__ __forceinline__ float row_field_grad(const RowCtx& c, const DSpec& s,
     return s.compose == UPF_COMPOSE_ELEVATION ? shape / s.escale
            : (s.compose == UPF_COMPOSE_ELEVATION_RAW ? 0.0f : shape);
   }
-  float c_[UPF_MAX_LAYERS], cw[UPF_MAX_LAYERS];
+  const bool nav = s.compose == UPF_COMPOSE_NAVIGATION;
+  const bool blended = !s.all_add;
+  // min_: the mask in effect at each layer; mcoef: the masks' coefficients (before fe
+  // and the weight); inband: bit l set when mask l is strictly inside its clamp
+  float4 min_ = make_float4(0.0f, 0.0f, 0.0f, 0.0f), mcoef = min_;
+  uint32_t inband = 0u;
   float elev = 0.0f;
-  const bool pass1 = !s.all_add || s.compose == UPF_COMPOSE_NAVIGATION;
-  if (pass1) {
-    float vv[UPF_MAX_LAYERS], accin[UPF_MAX_LAYERS], min_[UPF_MAX_LAYERS];
+  if (nav || blended) {
     float acc0 = 0.0f, m = 0.0f;
+    float4 dl = min_;  // NAVIGATION: v - acc before, per lerp layer
     for (int l = 0; l < s.n_layers; l++) {
       const DLayer& L = s.L[l];
-      float lv;
-      row_layer_value<1>(c, L, x, &y, z, &lv);
-      const float v = L.weight * lv;
-      vv[l] = v; accin[l] = acc0; min_[l] = m;
-      if (L.blend == UPF_BLEND_ADD) acc0 = acc0 + v;
-      else if (L.blend == UPF_BLEND_MASK) m = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
-      else acc0 = fmaf(m, v - acc0, acc0);
-    }
-    elev = acc0;
-    float abar = 1.0f, mbar = 0.0f;
-    for (int l = s.n_layers - 1; l >= 0; l--) {
-      const DLayer& L = s.L[l];
-      if (L.blend == UPF_BLEND_ADD) cw[l] = abar;
-      else if (L.blend == UPF_BLEND_MASK) {
-        const float t = vv[l] + 0.5f;
-        cw[l] = (t > 0.0f && t < 1.0f) ? mbar : 0.0f;
-        mbar = 0.0f;
-      } else {
-        const float mm = min_[l];
-        cw[l] = abar * mm;
-        mbar = fmaf(abar, vv[l] - accin[l], mbar);
-        abar = abar * (1.0f - mm);
+      lset(min_, l, m);
+      if (nav || L.blend == UPF_BLEND_MASK) {
+        float lv;
+        row_layer_value<1>(c, L, x, &y, z, &lv);
+        const float wv = L.weight * lv;
+        if (L.blend == UPF_BLEND_ADD) acc0 = acc0 + wv;
+        else if (L.blend == UPF_BLEND_MASK) {
+          m = __builtin_amdgcn_fmed3f(wv + 0.5f, 0.0f, 1.0f);
+          inband |= (wv + 0.5f > 0.0f && wv + 0.5f < 1.0f) ? 1u << l : 0u;
+        } else {
+          lset(dl, l, wv - acc0);
+          acc0 = fmaf(m, wv - acc0, acc0);
+        }
       }
-      c_[l] = cw[l] * L.weight;
     }
-  } else {
-    for (int l = 0; l < s.n_layers; l++) { cw[l] = 1.0f; c_[l] = s.L[l].weight; }
+    elev = acc0;  // complete under NAVIGATION only
+    if (nav && blended) {
+      int drv = -1;
+      for (int l = 0; l < s.n_layers; l++) {
+        const int bl = s.L[l].blend;
+        if (bl == UPF_BLEND_MASK) drv = l;
+        else if (bl == UPF_BLEND_LERP && drv >= 0)
+          lset(mcoef, drv, fmaf(lerp_tail(s, min_, l), lget(dl, l), lget(mcoef, drv)));
+      }
+    }
   }
   float fe = 1.0f;
   if (s.compose == UPF_COMPOSE_LAND) fe = -s.escale;
-  else if (s.compose == UPF_COMPOSE_NAVIGATION) fe = elev > 0.0f ? -s.escale : 0.0f;
-  float elev2 = 0.0f;
-  for (int l = 0; l < s.n_layers; l++) {
+  else if (nav) fe = elev > 0.0f ? -s.escale : 0.0f;
+  // one loop with a single call site of the layer gradient (inlined once): first the add
+  // and lerp layers, then the masks
+  float acc0 = 0.0f, elev2 = 0.0f;
+  int drv = -1;
+  const int passes = blended ? 2 * s.n_layers : s.n_layers;
+  for (int it = 0; it < passes; it++) {
+    const bool second = it >= s.n_layers;
+    const int l = second ? it - s.n_layers : it;
     const DLayer& L = s.L[l];
-    const float k = fe * c_[l];
+    if (L.blend == UPF_BLEND_MASK) drv = l;
+    if ((L.blend == UPF_BLEND_MASK) != second) continue;
+    float cwl;
+    if (second) cwl = (inband >> l) & 1u ? lget(mcoef, l) : 0.0f;
+    else if (!blended) cwl = 1.0f;
+    else cwl = lerp_tail(s, min_, l) * (L.blend == UPF_BLEND_LERP ? lget(min_, l) : 1.0f);
+    const float k = fe * cwl * L.weight;
     const float v = row_layer_grad<THETA>(c, L, l, x, y, z, k, dLdf * k, g, acc);
+    if (THETA) acc.add(kThWeight + l, dLdf * fe * cwl * v);
+    if (second) continue;
+    const float wv = L.weight * v;
     elev2 = fmaf(L.weight, v, elev2);
-    if (THETA) acc.add(kThWeight + l, dLdf * fe * cw[l] * v);
+    if (blended) {
+      if (L.blend == UPF_BLEND_LERP) {
+        if (!nav && drv >= 0) lset(mcoef, drv, fmaf(lerp_tail(s, min_, l), wv - acc0, lget(mcoef, drv)));
+        acc0 = fmaf(lget(min_, l), wv - acc0, acc0);
+      } else {
+        acc0 = acc0 + wv;
+      }
+    }
   }
-  if (!pass1) elev = elev2;
+  if (!nav) elev = blended ? acc0 : elev2;
   if (s.compose == UPF_COMPOSE_LAND) return shape - s.escale * elev;
-  if (s.compose == UPF_COMPOSE_NAVIGATION) return shape - s.escale * fmaxf(elev, 0.0f);
+  if (nav) return shape - s.escale * fmaxf(elev, 0.0f);
   return elev;  // ELEVATION_RAW
 }
 

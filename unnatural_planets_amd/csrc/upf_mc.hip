@@ -76,10 +76,14 @@ __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, co
     counts_out[threadIdx.x] = (int64_t)(threadIdx.x ? total->y : total->x);
   // each wave takes a contiguous run of the active-group list (neighbouring groups share
   // sign words and field values in cache) in passes of up to kEmitPassGroups groups; the
-  // grid is sized to keep the CUs full, the list's length is known only on the device
+  // grid is sized to keep the CUs full, the list's length is known only on the device.
+  // The first workgroups take the end of the list: the kernel before (the field kernel
+  // or k_mc_signs) swept the grid plane by plane, so the last planes are the ones still
+  // in the caches.  (Runs renumbered XCD by XCD, as k_batch_emit does, were slower here:
+  // 40 -> 70 us after the field kernel, 103 -> 110 us on a stored field, at 512^3.)
   const int64_t nact = (int64_t)total->z, nw = (int64_t)gridDim.x * 4;
   const int64_t per = (nact + nw - 1) / nw;
-  int64_t a = ((int64_t)blockIdx.x * 4 + wave) * per;
+  int64_t a = (nw - 1 - ((int64_t)blockIdx.x * 4 + wave)) * per;
   const int64_t a1 = min(a + per, nact);
   while (a < a1)
     a += mc_emit_pass(f, d, g, iso, S, voff, act, a, (int)min<int64_t>(kEmitPassGroups, a1 - a), 0u, 0xFFFFFFFFu, 0u,
@@ -441,6 +445,25 @@ int upf_mc_emit(const float* d_field, const upf_grid* grid, float iso, upf_mesh_
   const int64_t bound = std::min<int64_t>(L.d.groups, out->n_vertices + out->n_triangles);
   return mc_emit_launch(d_field, g, iso, d_ws, bound, out->positions, out->normals, out->indices, out->cap_vertices,
                         out->cap_triangles, as_stream(stream));
+}
+
+int upf_mc_extract_async(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out,
+                         int64_t* d_counts, void* d_ws, size_t ws_bytes, void* stream) {
+  if (!d_field || !out || !d_counts) return UPF_ERR_NULL;
+  if (!std::isfinite(iso)) return UPF_ERR_INVALID_VALUE;
+  if (out->cap_vertices < 0 || out->cap_triangles < 0) return UPF_ERR_CAPACITY;
+  if ((out->cap_vertices && !out->positions) || (out->cap_triangles && !out->indices)) return UPF_ERR_NULL;
+  DGrid g;
+  int st = to_device(grid, &g);
+  if (st || (st = mc_grid_ok(g))) return st;
+  const McLayout L = mc_layout(g);
+  if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
+  hipStream_t s = as_stream(stream);
+  if ((st = mc_classify_scan(d_field, g, iso, d_ws, ws_bytes, s))) return st;
+  // as upf_field_mesh_async: the persistent emit grid for the largest possible list, the
+  // totals stored in d_counts by the kernel
+  return mc_emit_launch(d_field, g, iso, d_ws, L.d.groups, out->positions, out->normals, out->indices,
+                        out->cap_vertices, out->cap_triangles, s, d_counts);
 }
 
 int upf_mc_extract(const float* d_field, const upf_grid* grid, float iso, upf_mesh_buffers* out, void* d_ws,

@@ -43,6 +43,16 @@ __constant__ __align__(16) mc_ct::Tris16 c_tris16 = mc_ct::pad_tris();
 constexpr uint64_t kEdgeOwner = mc_ct::pack_owner();
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+
+// The workgroup's index with the grid renumbered XCD by XCD.  The hardware hands the
+// workgroups of a launch to the 8 XCDs round robin (workgroup b to XCD b % 8), and each
+// XCD has its own L2.  Numbered this way, XCD x runs the x-th eighth of the work (XCDs
+// below G % 8 one workgroup more), so work items next to each other in the numbering,
+// which read the same field rows and sign words, share an L2.  A bijection on [0, G).
+__device__ __forceinline__ int64_t xcd_block() {
+  const uint32_t G = gridDim.x, b = blockIdx.x, x = b & 7u, q = G >> 3, r = G & 7u;
+  return (int64_t)x * q + (int64_t)min(x, r) + (int64_t)(b >> 3);
+}
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
