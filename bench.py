@@ -33,12 +33,13 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector, vendor figure (packed FP32)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
-# HBM bytes per launch of k_eval_grid_row on the 512^3 workload, from rocprofv3 --pmc
-# FETCH_SIZE (284 KiB, x 2.0 by the 1 GiB copy calibration) and WRITE_SIZE (566886 KiB,
-# x 1.0), averaged over the launches with and without sign words: the kernel reads only
-# its arguments and writes each node (512 MiB) and the sign words (16 MiB) once
-# (profiles/field_torus_ridged_voronoi_512.txt).
-FIELD_TRAFFIC_512 = 2 * 284 * 1024 + 566886 * 1024
+# HBM bytes per launch of k_eval_grid_row on the 512^3 workload as timed for `roofline`
+# (upf_field_eval_grid_ex, no sign words), from rocprofv3 --pmc: FETCH_SIZE 302 KiB (x 2.0
+# by the 1 GiB copy calibration: the kernel reads only its arguments) and WRITE_SIZE
+# 524288 KiB (x 1.0: each node written once, 512 MiB).  Launches that also store the
+# sign words read 576 MiB of writes: 16 MiB of 8-byte words counted as 64 MiB, each word
+# a partial line written by another workgroup (profiles/field_torus_ridged_voronoi_512.txt).
+FIELD_TRAFFIC_512 = 2 * 302 * 1024 + 524288 * 1024
 
 
 def cpu_model() -> str:
