@@ -155,9 +155,7 @@ __global__ __launch_bounds__(256) void k_batch_emit(const BChunk* __restrict__ c
   // contiguous runs of the active-group list per wave, in passes of one chunk each
   const int64_t n_active = (int64_t)total->z, nw = (int64_t)gridDim.x * 4;
   const int64_t per = (n_active + nw - 1) / nw;
-  // an XCD's runs are contiguous, and the first workgroups take the end of the list: the
-  // field kernel swept the items in list order, so the last chunks are still in the caches
-  int64_t a = (nw - 1 - (xcd_block() * 4 + wave)) * per;
+  int64_t a = ((int64_t)blockIdx.x * 4 + wave) * per;
   const int64_t a1 = min(a + per, n_active);
   while (a < a1) {
     const int64_t g = (int64_t)(uint32_t)uniform((int)act[a].x);

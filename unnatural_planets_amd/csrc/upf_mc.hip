@@ -77,13 +77,13 @@ __global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, co
   // each wave takes a contiguous run of the active-group list (neighbouring groups share
   // sign words and field values in cache) in passes of up to kEmitPassGroups groups; the
   // grid is sized to keep the CUs full, the list's length is known only on the device.
-  // The first workgroups take the end of the list: the kernel before (the field kernel
-  // or k_mc_signs) swept the grid plane by plane, so the last planes are the ones still
-  // in the caches.  (Runs renumbered XCD by XCD, as k_batch_emit does, were slower here:
-  // 40 -> 70 us after the field kernel, 103 -> 110 us on a stored field, at 512^3.)
+  // Counters at 512^3 (185,747 active groups): ~1,480 VALU instructions per pass, so the
+  // kernel is VALU-bound (49 M wave instructions, ~80 us of issue at four waves per
+  // SIMD); a software-pipelined variant that issued the next pass's record loads before
+  // the current pass's field reads ran no faster (112 against 106 us).
   const int64_t nact = (int64_t)total->z, nw = (int64_t)gridDim.x * 4;
   const int64_t per = (nact + nw - 1) / nw;
-  int64_t a = (nw - 1 - ((int64_t)blockIdx.x * 4 + wave)) * per;
+  int64_t a = ((int64_t)blockIdx.x * 4 + wave) * per;
   const int64_t a1 = min(a + per, nact);
   while (a < a1)
     a += mc_emit_pass(f, d, g, iso, S, voff, act, a, (int)min<int64_t>(kEmitPassGroups, a1 - a), 0u, 0xFFFFFFFFu, 0u,
@@ -293,7 +293,6 @@ int mc_classify_signs_scan(const DGrid& g, void* d_ws, size_t ws_bytes, hipStrea
 
 int64_t mc_scan_tiles(int64_t groups) { return (groups + kScanTile - 1) / kScanTile; }
 
-constexpr int kEmitBlocksPerCu = 32;
 
 // ------------------------------------------------------------------ allocation-free scratch
 // The workspace of upf_field_mesh_async_static: a file-scope device array, one copy per
@@ -322,6 +321,8 @@ int static_scratch(void** out) {
   *out = g_scratch_ptr[dev];
   return UPF_OK;
 }
+
+constexpr int kEmitBlocksPerCu = 32;
 
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
                    float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s, int64_t* counts_out) {

@@ -44,15 +44,6 @@ constexpr uint64_t kEdgeOwner = mc_ct::pack_owner();
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 
-// The workgroup's index with the grid renumbered XCD by XCD.  The hardware hands the
-// workgroups of a launch to the 8 XCDs round robin (workgroup b to XCD b % 8), and each
-// XCD has its own L2.  Numbered this way, XCD x runs the x-th eighth of the work (XCDs
-// below G % 8 one workgroup more), so work items next to each other in the numbering,
-// which read the same field rows and sign words, share an L2.  A bijection on [0, G).
-__device__ __forceinline__ int64_t xcd_block() {
-  const uint32_t G = gridDim.x, b = blockIdx.x, x = b & 7u, q = G >> 3, r = G & 7u;
-  return (int64_t)x * q + (int64_t)min(x, r) + (int64_t)(b >> 3);
-}
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int src) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
@@ -256,45 +247,56 @@ __device__ __forceinline__ void emit_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ int mc_emit_pass(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
-                                            const uint64_t* __restrict__ S, const uint32_t* __restrict__ voff,
-                                            const uint2* __restrict__ act, int64_t a, int ng, uint32_t gbase,
-                                            uint32_t glimit, uint32_t vbase, EmitPass& P, float* __restrict__ pos,
-                                            float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
-                                            int64_t cap_t) {
+// Step 1 for lane 8 b + s: the crossing record of neighbour slot s of group b of a pass,
+// from the group's active-list entry e (global id e.x, first triangle e.y).  Lanes past
+// the pass's groups return an empty record.
+struct EmitRec {
+  Cross c;
+  uint32_t toff;
+  int xb, j, k;
+};
+
+__device__ __forceinline__ EmitRec emit_record(const uint64_t* __restrict__ S, const uint32_t* __restrict__ voff,
+                                               const McDims& d, uint2 e, bool in, uint32_t gbase) {
   const int lane = threadIdx.x & 63;
-  // entries of another grid end the pass (the list is sorted, so a grid's entries are
-  // contiguous); entry a is the caller's grid, so every pass takes at least one group
-  {
-    const bool mine = lane < ng && act[a + lane].x < glimit;
-    ng = max(__popcll(ballot64(mine) & 0xFFull), 1);
+  const int sl = lane & 7;
+  EmitRec r;
+  r.c.bx = r.c.by = r.c.bz = r.c.w = 0ull;
+  r.c.voff = r.c.pad = 0u;
+  r.toff = e.y;
+  r.xb = r.j = r.k = 0;
+  if (in) {
+    const uint32_t gid = e.x - gbase;
+    const uint32_t row = gid / (uint32_t)d.gpr;
+    r.xb = (int)(gid - row * (uint32_t)d.gpr);
+    r.k = (int)(row / (uint32_t)d.ny);
+    r.j = (int)(row - (uint32_t)r.k * (uint32_t)d.ny);
+    const int gx = sl & 1, dy = (sl >> 1) & 1, dz = sl >> 2;
+    if (r.xb + gx < d.gpr && r.j + dy < d.ny && r.k + dz < d.nz)
+      r.c = group_crossings(S, voff, d, r.xb + gx, r.j + dy, r.k + dz);
   }
-  // ---- 1. neighbour crossing records
-  {
-    const int b = lane >> 3, sl = lane & 7;
-    if (b < ng) {
-      const uint2 e = act[a + b];
-      const uint32_t gid = e.x - gbase;
-      const uint32_t row = gid / (uint32_t)d.gpr;
-      const int xb = (int)(gid - row * (uint32_t)d.gpr);
-      const int k = (int)(row / (uint32_t)d.ny);
-      const int j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
-      const int gx = sl & 1, dy = (sl >> 1) & 1, dz = sl >> 2;
-      Cross c;
-      c.bx = c.by = c.bz = c.w = 0ull;
-      c.voff = c.pad = 0u;
-      if (xb + gx < d.gpr && j + dy < d.ny && k + dz < d.nz) c = group_crossings(S, voff, d, xb + gx, j + dy, k + dz);
-      P.nb[b][sl] = c;
-      if (sl == 0) {
-        P.toff[b] = e.y;
-        P.xb[b] = xb;
-        P.j[b] = j;
-        P.k[b] = k;
-      }
+  return r;
+}
+
+__device__ __forceinline__ void emit_put(EmitPass& P, const EmitRec& r, int ng) {
+  const int lane = threadIdx.x & 63;
+  const int b = lane >> 3, sl = lane & 7;
+  if (b < ng) {
+    P.nb[b][sl] = r.c;
+    if (sl == 0) {
+      P.toff[b] = r.toff;
+      P.xb[b] = r.xb;
+      P.j[b] = r.j;
+      P.k[b] = r.k;
     }
   }
-  emit_sync();
-  // ---- 2. cells with work per group (lane b), and how many groups this pass takes
+}
+
+// Steps 2 and 3: the cells with work of each of the pass's ng groups (lane b), how many
+// groups the pass takes (returned) and the list of its cells (P.item, P.pre); *nitems =
+// the number of listed cells.  The records of the ng groups are in P.
+__device__ __forceinline__ int emit_plan(EmitPass& P, int ng, const McDims& d, int* nitems_out) {
+  const int lane = threadIdx.x & 63;
   uint64_t items = 0ull;
   int nit = 0;
   if (lane < ng) {
@@ -316,16 +318,24 @@ __device__ __forceinline__ int mc_emit_pass(const float* __restrict__ f, const M
   const int incl = wave_inclusive_sum<kEmitPassGroups>(nit);
   const int pre = incl - nit;
   const int npass = __popcll(ballot64(lane < ng && incl <= 64));  // >= 1
-  const int nitems = __builtin_amdgcn_readlane(incl, npass - 1);
+  *nitems_out = __builtin_amdgcn_readlane(incl, npass - 1);
   if (lane < npass) P.pre[lane] = pre;
-  // ---- 3. the list of cells
   for (int b = 0; b < npass; b++) {
     const uint64_t it = readlane64(items, b);
     const int p = __builtin_amdgcn_readlane(pre, b);
     if ((it >> lane) & 1ull) P.item[p + __popcll(it & ((1ull << lane) - 1))] = (uint16_t)((b << 6) | lane);
   }
   emit_sync();
-  // ---- 4. one listed cell per lane
+  return npass;
+}
+
+// Step 4: one listed cell per lane (the vertices on the node's own edges, the cell's
+// triangles), from the pass's records and list in P.
+__device__ __forceinline__ void emit_cells(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
+                                           const EmitPass& P, int nitems, uint32_t vbase, float* __restrict__ pos,
+                                           float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
+                                           int64_t cap_t) {
+  const int lane = threadIdx.x & 63;
   const bool act_lane = lane < nitems;
   const int e = act_lane ? (int)P.item[lane] : 0;
   const int b = e >> 6, l = e & 63;
@@ -456,6 +466,30 @@ __device__ __forceinline__ int mc_emit_pass(const float* __restrict__ f, const M
       }
     }
   }
+}
+
+__device__ __forceinline__ int mc_emit_pass(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
+                                            const uint64_t* __restrict__ S, const uint32_t* __restrict__ voff,
+                                            const uint2* __restrict__ act, int64_t a, int ng, uint32_t gbase,
+                                            uint32_t glimit, uint32_t vbase, EmitPass& P, float* __restrict__ pos,
+                                            float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
+                                            int64_t cap_t) {
+  const int lane = threadIdx.x & 63;
+  // entries of another grid end the pass (the list is sorted, so a grid's entries are
+  // contiguous); entry a is the caller's grid, so every pass takes at least one group
+  {
+    const bool mine = lane < ng && act[a + lane].x < glimit;
+    ng = max(__popcll(ballot64(mine) & 0xFFull), 1);
+  }
+  {
+    const int b = lane >> 3;
+    const uint2 e = b < ng ? act[a + b] : make_uint2(0u, 0u);
+    emit_put(P, emit_record(S, voff, d, e, b < ng, gbase), ng);
+  }
+  emit_sync();
+  int nitems;
+  const int npass = emit_plan(P, ng, d, &nitems);
+  emit_cells(f, d, g, iso, P, nitems, vbase, pos, nrm, idx, cap_v, cap_t);
   emit_sync();  // this pass's reads of P before the next pass's writes
   return npass;
 }
