@@ -50,6 +50,8 @@ __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ ch, int n, 
   return lo;
 }
 
+constexpr int kFieldSpan = 8;  // consecutive field items per wave step
+
 // Seven waves per SIMD (at most 72 VGPRs), as the single-grid row kernel; with the
 // autotuner's 7 workgroups per CU, 2.12 -> 2.09 ms for the 64-chunk batch (six waves and
 // 8 workgroups per CU before).
@@ -64,50 +66,72 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const int wave = uniform(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], lane);
-  const int64_t stride = (int64_t)gridDim.x * 4;
-  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < n_items; r += stride) {
-    const int c = uniform(find_chunk<&BChunk::row_off>(ch, n_chunks, r));
-    const BChunk& C = ch[c];
-    const int64_t local = r - C.row_off;
-    const int xb = (int)(local % C.gx);
-    const int64_t rest = local / C.gx;
-    const int pairs = (C.grid.n[1] + 1) / 2;
-    const int j = 2 * (int)(rest % pairs);
-    const int k = (int)(rest / pairs);
-    const bool two = j + 1 < C.grid.n[1];
-    const int i = xb * 64 + lane;
-    const int ic = min(i, C.grid.n[0] - 1);
-    const float x = node_coord(C.grid.o[0], C.grid.s[0], ic);
-    const float y[2] = {node_coord(C.grid.o[1], C.grid.s[1], j),
-                        node_coord(C.grid.o[1], C.grid.s[1], two ? j + 1 : j)};
-    const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
-    float v[2];
-    if (VARIANT == UPF_EVAL_PER_LANE) {
-      // a chunk rounds as its own single-grid evaluation would: the exact form for an
-      // elevation mode or a further shape, the fused form otherwise (the choice is
-      // uniform across the wave: one chunk per row pair)
-      if (EXT && spec_ext(C.spec)) {
-        v[0] = field_eval<true, true>(C.spec, x, y[0], z);
-        v[1] = two ? field_eval<true, true>(C.spec, x, y[1], z) : v[0];
-      } else {
-        v[0] = field_eval<EXT, false>(C.spec, x, y[0], z);
-        v[1] = two ? field_eval<EXT, false>(C.spec, x, y[1], z) : v[0];
+  // Each wave takes spans of kFieldSpan consecutive items, span after span with the
+  // grid's stride: the chunk is found once and then advanced (the item index only
+  // grows), and within a span the row pair is decoded once and then stepped.
+  const int64_t step = (int64_t)gridDim.x * 4 * kFieldSpan;
+  int64_t s0 = ((int64_t)blockIdx.x * 4 + wave) * kFieldSpan;
+  int c = s0 < n_items ? uniform(find_chunk<&BChunk::row_off>(ch, n_chunks, s0)) : 0;
+  for (; s0 < n_items; s0 += step) {
+    const int64_t s1 = min(s0 + kFieldSpan, n_items);
+    while (c + 1 < n_chunks && ch[c + 1].row_off <= s0) c++;
+    c = uniform(c);
+    int64_t r = s0;
+    while (r < s1) {
+      const BChunk& C = ch[c];
+      const int64_t e = c + 1 < n_chunks ? min(s1, ch[c + 1].row_off) : s1;  // the span's part in chunk c
+      const int pairs = (C.grid.n[1] + 1) / 2;
+      const uint32_t local = (uint32_t)(r - C.row_off);  // < 2^31: a chunk's groups fit 31 bits
+      int xb = (int)(local % (uint32_t)C.gx);
+      const uint32_t rest = local / (uint32_t)C.gx;
+      int jp = (int)(rest % (uint32_t)pairs);
+      int k = (int)(rest / (uint32_t)pairs);
+      for (; r < e; r++) {
+        const int j = 2 * jp;
+        const bool two = j + 1 < C.grid.n[1];
+        const int i = xb * 64 + lane;
+        const int ic = min(i, C.grid.n[0] - 1);
+        const float x = node_coord(C.grid.o[0], C.grid.s[0], ic);
+        const float y[2] = {node_coord(C.grid.o[1], C.grid.s[1], j),
+                            node_coord(C.grid.o[1], C.grid.s[1], two ? j + 1 : j)};
+        const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
+        float v[2];
+        if (VARIANT == UPF_EVAL_PER_LANE) {
+          // a chunk rounds as its own single-grid evaluation would: the exact form for an
+          // elevation mode or a further shape, the fused form otherwise (the choice is
+          // uniform across the wave: one chunk per row pair)
+          if (EXT && spec_ext(C.spec)) {
+            v[0] = field_eval<true, true>(C.spec, x, y[0], z);
+            v[1] = two ? field_eval<true, true>(C.spec, x, y[1], z) : v[0];
+          } else {
+            v[0] = field_eval<EXT, false>(C.spec, x, y[0], z);
+            v[1] = two ? field_eval<EXT, false>(C.spec, x, y[1], z) : v[0];
+          }
+        } else {
+          row_field<2, COMPOSED>(ctx, C.spec, x, y, z, v);
+        }
+        const bool valid = i < C.grid.n[0];
+        if (valid) {
+          const int64_t o = C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + i;
+          field[o] = v[0];
+          if (two) field[o + C.grid.n[0]] = v[1];
+        }
+        // the rows' sign words (bit = node exists and f < iso), as the single-grid kernels
+        const uint64_t m0 = ballot64(valid && v[0] < C.iso), m1 = ballot64(valid && v[1] < C.iso);
+        if (lane == 0) {
+          const int64_t w = C.group_off + ((int64_t)k * C.grid.n[1] + j) * C.gx + xb;
+          S[w] = m0;
+          if (two) S[w + C.gx] = m1;
+        }
+        if (++xb == C.gx) {
+          xb = 0;
+          if (++jp == pairs) {
+            jp = 0;
+            k++;
+          }
+        }
       }
-    } else {
-      row_field<2, COMPOSED>(ctx, C.spec, x, y, z, v);
-    }
-    const bool valid = i < C.grid.n[0];
-    if (valid) {
-      const int64_t o = C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + i;
-      field[o] = v[0];
-      if (two) field[o + C.grid.n[0]] = v[1];
-    }
-    // the rows' sign words (bit = node exists and f < iso), as the single-grid kernels
-    const uint64_t m0 = ballot64(valid && v[0] < C.iso), m1 = ballot64(valid && v[1] < C.iso);
-    if (lane == 0) {
-      const int64_t w = C.group_off + ((int64_t)k * C.grid.n[1] + j) * C.gx + xb;
-      S[w] = m0;
-      if (two) S[w + C.gx] = m1;
+      if (r < s1) c++;  // the span continues in the next chunk
     }
   }
 }
@@ -123,7 +147,10 @@ __global__ __launch_bounds__(256) void k_batch_classify(const BChunk* __restrict
   __syncthreads();
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool in = g < groups;
-  const int c = find_chunk<&BChunk::group_off>(ch, n_chunks, in ? g : groups - 1);
+  // the chunk of the workgroup's first group (one search per workgroup), then a short
+  // walk per lane: a workgroup's 256 groups rarely cross a chunk boundary
+  int c = uniform(find_chunk<&BChunk::group_off>(ch, n_chunks, (int64_t)blockIdx.x * 256));
+  while (c + 1 < n_chunks && ch[c + 1].group_off <= (in ? g : groups - 1)) c++;
   const BChunk& C = ch[c];
   const uint32_t cnt = classify_signs_group(S + C.group_off, C.md, g - C.group_off, in, s_ntris, s_pass[threadIdx.x >> 6]);
   if (in) counts[g] = cnt;
@@ -157,9 +184,12 @@ __global__ __launch_bounds__(256) void k_batch_emit(const BChunk* __restrict__ c
   const int64_t per = (n_active + nw - 1) / nw;
   int64_t a = ((int64_t)blockIdx.x * 4 + wave) * per;
   const int64_t a1 = min(a + per, n_active);
+  int c = a < a1 ? uniform(find_chunk<&BChunk::group_off>(ch, n_chunks, (int64_t)(uint32_t)uniform((int)act[a].x))) : 0;
   while (a < a1) {
+    // the list is sorted, so the chunk only advances
     const int64_t g = (int64_t)(uint32_t)uniform((int)act[a].x);
-    const int c = uniform(find_chunk<&BChunk::group_off>(ch, n_chunks, g));
+    while (c + 1 < n_chunks && ch[c + 1].group_off <= g) c++;
+    c = uniform(c);
     const BChunk& C = ch[c];
     const uint32_t vbase = voff[C.group_off];
     a += mc_emit_pass(field + C.node_off, C.md, C.grid, C.iso, S + C.group_off, voff + C.group_off, act, a,
