@@ -46,6 +46,38 @@ def test_mesh_async_checks_every_output_buffer(upf):
             upf.field_mesh_async(_spec(), g, field, m, counts)
 
 
+def test_mc_extract_async_checks_buffers_and_counts(upf):
+    g = S.grid_sphere_fbm(32)
+    field = torch.zeros((32, 32, 32), dtype=torch.float32, device="cuda")
+    counts = torch.zeros(2, dtype=torch.int64, device="cuda")
+    good_p = torch.empty((100, 3), dtype=torch.float32, device="cuda")
+    good_i = torch.empty((200, 3), dtype=torch.int32, device="cuda")
+    for m in (upf.Mesh(good_p, torch.empty((99, 3), dtype=torch.float32, device="cuda"), good_i),
+              upf.Mesh(good_p, None, torch.empty((200, 3), dtype=torch.int64, device="cuda")),
+              upf.Mesh(torch.empty((300,), dtype=torch.float32, device="cuda"), None, good_i)):
+        with pytest.raises(ValueError):
+            upf.mc_extract_async(field, g, m, counts)
+    ok = upf.Mesh(good_p, None, good_i)
+    for c in (torch.zeros(2, dtype=torch.int32, device="cuda"), torch.zeros(1, dtype=torch.int64, device="cuda"),
+              torch.zeros(2, dtype=torch.int64)):
+        with pytest.raises(ValueError):
+            upf.mc_extract_async(field, g, ok, c)
+    with pytest.raises(ValueError):
+        upf.mc_extract_async(field[:, :, :31], g, ok, counts)
+
+
+def test_bake_rasterize_checks_uv_and_method(upf):
+    uv = upf.atlas_per_triangle(10, 32, 32)
+    for bad in (uv.reshape(10, 6), uv.reshape(10, 2, 3), uv.double()):
+        with pytest.raises(ValueError):
+            upf.bake_rasterize(bad, 32, 32)
+    with pytest.raises(ValueError):
+        upf.bake_rasterize(uv, 32, 32, method="bogus")
+    with pytest.raises(upf.UpfError) as e:
+        upf.bake_rasterize(uv, (1 << 24) + 1, 1)
+    assert e.value.status == S.ERR_BAD_GRID
+
+
 def test_bake_checks_indices(upf):
     s = _spec()
     g = S.grid_sphere_fbm(24)

@@ -205,10 +205,14 @@ __device__ __forceinline__ Cross group_crossings(const uint64_t* __restrict__ S,
 
 // One axis of the central-difference gradient at a node (one-sided on the grid faces),
 // given f one step up (fp), at the node (f0) and one step down (fm).
+// One division with selected operands: written as three returns, the three divisions
+// were all computed and selected per lane (the branches diverge), three times the
+// division sequence for the same result.
 __device__ __forceinline__ float grad_axis(int idx, int nmax, float fp, float f0, float fm, float sp) {
-  if (idx == 0) return (fp - f0) / sp;
-  if (idx == nmax - 1) return (f0 - fm) / sp;
-  return (fp - fm) / (2.0f * sp);
+  const bool lo = idx == 0, hi = idx == nmax - 1;
+  const float num = lo ? fp - f0 : (hi ? f0 - fm : fp - fm);
+  const float den = (lo || hi) ? sp : 2.0f * sp;
+  return num / den;
 }
 
 // Emission, one pass of a wave: up to kEmitPassGroups consecutive entries of the
