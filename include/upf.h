@@ -543,7 +543,8 @@ int upf_bake_rasterize(const float* d_uv, int64_t n_triangles, int32_t width, in
  * fill).  d_tri equals upf_bake_rasterize's bit for bit.  The workspace (size from
  * upf_bake_rasterize_workspace_bytes, 0 for invalid sizes) holds the bins, 4 bytes per
  * texel, and a list of 16 bytes per triangle.  n_triangles < 2^30, width and height at
- * most 2^24.  Overlapping calls must use different workspaces. */
+ * most 2^24 and fewer than 2^31 tiles (UPF_ERR_BAD_GRID otherwise).  Overlapping calls
+ * must use different workspaces. */
 size_t upf_bake_rasterize_workspace_bytes(int64_t n_triangles, int32_t width, int32_t height);
 int upf_bake_rasterize_ws(const float* d_uv, int64_t n_triangles, int32_t width, int32_t height,
                           int32_t* d_tri, void* d_ws, size_t ws_bytes, void* stream);

@@ -493,7 +493,8 @@ int upf_bake_rasterize(const float* d_uv, int64_t n, int32_t w, int32_t h, int32
 
 size_t upf_bake_rasterize_workspace_bytes(int64_t n, int32_t w, int32_t h) {
   if (n < 0 || n >= ((int64_t)1 << 30) || w < 1 || h < 1 || w > kRastMaxSide || h > kRastMaxSide) return 0;
-  return rast_layout(n, w, h).bytes;
+  const RastLayout L = rast_layout(n, w, h);
+  return L.tiles < ((int64_t)1 << 31) ? L.bytes : 0;  // one workgroup per tile
 }
 
 int upf_bake_rasterize_ws(const float* d_uv, int64_t n, int32_t w, int32_t h, int32_t* d_tri, void* d_ws,
