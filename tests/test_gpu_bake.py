@@ -78,6 +78,37 @@ def test_rasterize_tiles_full_bins_big_and_odd_triangles(upf, oracle, case):
     np.testing.assert_array_equal(upfm.bake_rasterize(d, w, h, method="atomic").cpu().numpy(), ref)
 
 
+def _atlas_coverage(n, w, h, s):
+    """Closed form of the per-triangle atlas's coverage (see test_oracle_bake.py), vectorised:
+    texel (x, y) of cell c = (y // s) * cols + x // s belongs to triangle 2c below the
+    cell's diagonal and to 2c + 1 on and above it (to 2c up to the diagonal when 2c + 1
+    does not exist).  Exact when the image sides are powers of two (the UV corners and
+    the texel centres then carry no rounding); elsewhere diagonal centres may round across."""
+    cols, rows = w // s, h // s
+    y, x = np.mgrid[0:h, 0:w]
+    c = (y // s).astype(np.int64) * cols + x // s
+    diag = (x % s) + (y % s)
+    t = np.where(diag < s - 1, 2 * c, 2 * c + 1)
+    t = np.where(2 * c + 1 >= n, np.where(diag <= s - 1, 2 * c, -1), t)
+    t = np.where((x >= cols * s) | (y >= rows * s) | (2 * c >= n), -1, t)
+    return t.astype(np.int32)
+
+
+def test_rasterize_tiles_at_4096_equals_closed_form(upf, oracle):
+    """The 4096^2 bake of the largest narrow-band planet (4,724,992 triangles, 2 x 2 texel
+    cells): the tile-binned map equals the atomic rasterizer's and the atlas's closed-form
+    coverage, texel for texel (the oracle itself is checked against the same closed form
+    at smaller sizes)."""
+    n, w = 4724992, 4096
+    uv = upfm.atlas_per_triangle(n, w, w)
+    side = oracle.atlas_side(n, w, w)
+    print("case: atlas", n, w, "side", side, flush=True)
+    tiles = upfm.bake_rasterize(uv, w, w, method="tiles")
+    atomic = upfm.bake_rasterize(uv, w, w, method="atomic")
+    np.testing.assert_array_equal(tiles.cpu().numpy(), atomic.cpu().numpy())
+    np.testing.assert_array_equal(tiles.cpu().numpy(), _atlas_coverage(n, w, w, side))
+
+
 def test_rasterize_tiles_workspace_reuse_and_errors(upf, oracle):
     """The workspace of a larger call serves a smaller one (the counts are cleared each
     call); a short workspace, a NULL map and sides over 2^24 are refused."""
