@@ -275,8 +275,11 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
   int st = mc_scan_launch(counts, b->groups, tiles, total, voff, act, toff, s);
   if (st) return st;
   hipLaunchKernelGGL(k_batch_meshes, dim3((b->n + 63) / 64), dim3(64), 0, s, b->dev, b->n, voff, toff, total, meshes);
-  hipLaunchKernelGGL(k_batch_emit, dim3(grid), dim3(256), 0, s, b->dev, b->n, field, S, voff, act, total, pos,
-                     nrm, idx, cap_v, cap_t);
+  // the emission runs at fewer waves per SIMD than the field pass (its registers), so it
+  // gets its own grid: 32 small workgroups per CU, several rounds of short runs, instead
+  // of one and a fraction of the field pass's workgroups
+  hipLaunchKernelGGL(k_batch_emit, dim3((unsigned)(b->cus * 32)), dim3(256), 0, s, b->dev, b->n, field, S, voff, act,
+                     total, pos, nrm, idx, cap_v, cap_t);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
