@@ -213,6 +213,7 @@ struct upf_batch {
   int device = 0;
   int n = 0;
   int cus = 256;
+  int emit_blocks = 256 * 32;  // k_batch_emit's grid
   int64_t nodes = 0, groups = 0, rows = 0;
   bool any_ext = false;  // a chunk with an elevation mode or a further shape: the field runs per lane
   bool any_composed = false;  // a chunk whose shape is a composition (spec_composed)
@@ -276,10 +277,10 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
   if (st) return st;
   hipLaunchKernelGGL(k_batch_meshes, dim3((b->n + 63) / 64), dim3(64), 0, s, b->dev, b->n, voff, toff, total, meshes);
   // the emission runs at fewer waves per SIMD than the field pass (its registers), so it
-  // gets its own grid: 32 small workgroups per CU, several rounds of short runs, instead
-  // of one and a fraction of the field pass's workgroups
-  hipLaunchKernelGGL(k_batch_emit, dim3((unsigned)(b->cus * 32)), dim3(256), 0, s, b->dev, b->n, field, S, voff, act,
-                     total, pos, nrm, idx, cap_v, cap_t);
+  // gets its own grid: about 32 small workgroups per CU in whole rounds of its resident
+  // ones, several rounds of short runs, instead of one and a fraction of the field pass's
+  hipLaunchKernelGGL(k_batch_emit, dim3((unsigned)b->emit_blocks), dim3(256), 0, s, b->dev, b->n, field, S, voff,
+                     act, total, pos, nrm, idx, cap_v, cap_t);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
@@ -337,6 +338,10 @@ int upf_batch_create(const upf_chunk* chunks, int32_t n_chunks, const upf_batch_
       hipMalloc(&b->dev, sizeof(BChunk) * n_chunks) != hipSuccess) {
     delete b;
     return UPF_ERR_HIP;
+  }
+  {
+    const int per_cu = std::max(1, resident_blocks(reinterpret_cast<const void*>(&k_batch_emit), 256) / b->cus);
+    b->emit_blocks = b->cus * per_cu * std::max(1, 32 / per_cu);
   }
   if ((st = upload(b))) { upf_batch_destroy(b); return st; }
   *out = b;

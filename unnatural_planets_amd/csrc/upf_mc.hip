@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 
@@ -322,15 +323,34 @@ int static_scratch(void** out) {
   return UPF_OK;
 }
 
+// About 32 workgroups of k_mc_emit per CU, rounded down to whole rounds of the resident
+// ones (occupancy API, cached per device): the runs are equal, so the workgroups finish
+// in rounds, and a last round only partly filled costs as long as a full one.
 constexpr int kEmitBlocksPerCu = 32;
+constexpr int kEmitMaxDevices = 64;
+
+int64_t emit_grid_waves() {
+  static std::atomic<int64_t> cache[kEmitMaxDevices];
+  int dev = 0;
+  const bool cached = hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kEmitMaxDevices;
+  if (cached) {
+    const int64_t w = cache[dev].load(std::memory_order_relaxed);
+    if (w > 0) return w;
+  }
+  const int cus = device_cus();
+  const int per_cu = std::max(1, resident_blocks(reinterpret_cast<const void*>(&k_mc_emit), 256) / cus);
+  const int64_t w = (int64_t)cus * per_cu * std::max(1, kEmitBlocksPerCu / per_cu) * 4;
+  if (cached) cache[dev].store(w, std::memory_order_relaxed);
+  return w;
+}
 
 int mc_emit_launch(const float* d_field, const DGrid& g, float iso, void* d_ws, int64_t max_active, float* pos,
                    float* nrm, uint32_t* idx, int64_t cap_v, int64_t cap_t, hipStream_t s, int64_t* counts_out) {
   const McLayout L = mc_layout(g);
   const McPtrs P = mc_ptrs(L, d_ws);
   if (max_active <= 0) return UPF_OK;
-  // 32 blocks of 4 waves per CU (more than stay resident), each looping over the list
-  max_active = std::min<int64_t>(max_active, (int64_t)device_cus() * 4 * kEmitBlocksPerCu);
+  // whole rounds of resident workgroups of 4 waves, each wave looping over its run
+  max_active = std::min<int64_t>(max_active, emit_grid_waves());
   hipLaunchKernelGGL(k_mc_emit, dim3((unsigned)((max_active + 3) / 4)), dim3(256), 0, s, d_field, L.d, g, iso, P.S,
                      P.voff, P.act, P.total, pos, nrm, idx, cap_v, cap_t, counts_out);
   UPF_LAUNCH_CHECK();
