@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __res
 // ------------------------------------------------------------------ scan
 // Three running sums at once: x = vertices, y = triangles, z = active groups.
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
 __device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, 0); }
