@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_mc_signs(const float* __restrict__ f, c
   if (lane < kSignGroups && g0 + lane < d.gpr) S[row * d.gpr + g0 + lane] = mine;
 }
 
-__global__ __launch_bounds__(256) void k_mc_emit(const float* __restrict__ f, const McDims d, const DGrid g,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_mc_emit(const float* __restrict__ f, const McDims d, const DGrid g,
                                                  float iso, const uint64_t* __restrict__ S,
                                                  const uint32_t* __restrict__ voff, const uint2* __restrict__ act,
                                                  const uint4* __restrict__ total, float* __restrict__ pos,

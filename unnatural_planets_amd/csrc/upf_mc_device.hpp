@@ -13,8 +13,7 @@ __constant__ int8_t c_edge_owner[12][4] = {
     {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
 
 // The same tables as compile-time data for the emit: the triangulation with rows padded
-// to 16 bytes (one 16-byte load per cell), and the edge owners packed 5 bits per edge
-// (dx | dy << 1 | dz << 2 | axis << 3) into one 64-bit immediate.
+// to 16 bytes (one 16-byte load per cell), and the edge owners to check the emit's order.
 namespace mc_ct {
 #define UPF_MC_QUAL static constexpr
 #include "mc_table.inc"
@@ -32,15 +31,19 @@ constexpr Tris16 pad_tris() {
   }
   return t;
 }
-constexpr uint64_t pack_owner() {
-  uint64_t r = 0;
+// true when edge e is owned by corner (x, y, z) along axis a as the emit's id table
+// assumes: x edges e = 0, 2, 4, 6 at corners 000, 010, 001, 011; y edges 1, 3, 5, 7 at
+// 100, 000, 101, 001; z edges 8..11 at 000, 100, 110, 010
+constexpr bool owner_order_ok() {
+  constexpr int8_t want[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
+                                  {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
   for (int e = 0; e < 12; e++)
-    r |= (uint64_t)(owner[e][0] | (owner[e][1] << 1) | (owner[e][2] << 2) | (owner[e][3] << 3)) << (5 * e);
-  return r;
+    for (int q = 0; q < 4; q++)
+      if (owner[e][q] != want[e][q]) return false;
+  return true;
 }
 }  // namespace mc_ct
 __constant__ __align__(16) mc_ct::Tris16 c_tris16 = mc_ct::pad_tris();
-constexpr uint64_t kEdgeOwner = mc_ct::pack_owner();
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 
@@ -243,6 +246,7 @@ struct EmitPass {  // one wave's LDS
   uint32_t toff[kEmitPassGroups];
   int xb[kEmitPassGroups], j[kEmitPassGroups], k[kEmitPassGroups], pre[kEmitPassGroups];
   uint16_t item[64];  // (group << 6) | cell
+  uint32_t eid[12][64];  // lane t: the vertex id (less vbase) on edge e of listed cell t
 };
 
 __device__ __forceinline__ void emit_sync() {
@@ -336,7 +340,7 @@ __device__ __forceinline__ int emit_plan(EmitPass& P, int ng, const McDims& d, i
 // Step 4: one listed cell per lane (the vertices on the node's own edges, the cell's
 // triangles), from the pass's records and list in P.
 __device__ __forceinline__ void emit_cells(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
-                                           const EmitPass& P, int nitems, uint32_t vbase, float* __restrict__ pos,
+                                           EmitPass& P, int nitems, uint32_t vbase, float* __restrict__ pos,
                                            float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
                                            int64_t cap_t) {
   const int lane = threadIdx.x & 63;
@@ -443,9 +447,53 @@ __device__ __forceinline__ void emit_cells(const float* __restrict__ f, const Mc
   // the triangles of the pass's earlier groups: exc at the first cell of this lane's group
   // (b is the lane's own group, so the source lane differs between lanes)
   const int seg = __shfl(exc, P.pre[b], 64);
+  // The vertex ids of the cell's 12 edges, once per lane, into the lane's column of
+  // P.eid: an edge's vertex is owned by corner (dx, dy, dz) of the cell (its x, y and z
+  // crossings, in that order, after those of the lower nodes of the corner's row
+  // segment).  Edges without a crossing get an id that no triangle uses.
+  {
+    struct Node { uint32_t base, xb, yb; };
+    auto node = [&](int dx, int dy, int dz) {
+      const int ol = l + dx;
+      const Cross& cc = nb[(ol >> 6) | (dy << 1) | (dz << 2)];
+      const int lo = ol & 63;
+      const uint64_t m = (1ull << lo) - 1;
+      Node n;
+      n.base = cc.voff - vbase + (uint32_t)(__popcll(cc.bx & m) + __popcll(cc.by & m) + __popcll(cc.bz & m));
+      n.xb = (uint32_t)((cc.bx >> lo) & 1ull);
+      n.yb = (uint32_t)((cc.by >> lo) & 1ull);
+      return n;
+    };
+    static_assert(mc_ct::owner_order_ok(), "edge order of the id table");
+    // corner by corner, each corner's edges stored as soon as it is known
+    {
+      const Node n = node(0, 0, 0);
+      P.eid[0][lane] = n.base;               // x
+      P.eid[3][lane] = n.base + n.xb;        // y
+      P.eid[8][lane] = n.base + n.xb + n.yb; // z
+    }
+    {
+      const Node n = node(1, 0, 0);
+      P.eid[1][lane] = n.base + n.xb;        // y
+      P.eid[9][lane] = n.base + n.xb + n.yb; // z
+    }
+    {
+      const Node n = node(0, 1, 0);
+      P.eid[2][lane] = n.base;                // x
+      P.eid[11][lane] = n.base + n.xb + n.yb; // z
+    }
+    {
+      const Node n = node(0, 0, 1);
+      P.eid[4][lane] = n.base;         // x
+      P.eid[7][lane] = n.base + n.xb;  // y
+    }
+    P.eid[5][lane] = [&] { const Node n = node(1, 0, 1); return n.base + n.xb; }();           // y
+    P.eid[6][lane] = node(0, 1, 1).base;                                                     // x
+    P.eid[10][lane] = [&] { const Node n = node(1, 1, 0); return n.base + n.xb + n.yb; }();  // z
+  }
   if (nt) {
     const uint32_t tid0 = P.toff[b] + (uint32_t)(exc - seg);
-    // the cell's triangulation row in one 16-byte load; edge owners from the immediate
+    // the cell's triangulation row in one 16-byte load
     const int4 row = *reinterpret_cast<const int4*>(&c_tris16.v[code][0]);
     const uint32_t rw[4] = {(uint32_t)row.x, (uint32_t)row.y, (uint32_t)row.z, (uint32_t)row.w};
 #pragma unroll
@@ -455,18 +503,8 @@ __device__ __forceinline__ void emit_cells(const float* __restrict__ f, const Mc
 #pragma unroll
       for (int v = 0; v < 3; v++) {
         const int q = 3 * t + v;
-        const int ed = (int)(int8_t)(uint8_t)(rw[q >> 2] >> (8 * (q & 3)));
-        const int own = (int)((kEdgeOwner >> (5 * ed)) & 31u);
-        const int ol = l + (own & 1);
-        const int slot = (ol >> 6) | (((own >> 1) & 1) << 1) | (((own >> 2) & 1) << 2);
-        const int axis = own >> 3;
-        const Cross cc = nb[slot];
-        const int lo = ol & 63;
-        const uint64_t m = (1ull << lo) - 1;
-        uint32_t id = cc.voff + (uint32_t)(__popcll(cc.bx & m) + __popcll(cc.by & m) + __popcll(cc.bz & m));
-        if (axis >= 1) id += (uint32_t)((cc.bx >> lo) & 1ull);
-        if (axis == 2) id += (uint32_t)((cc.by >> lo) & 1ull);
-        if ((int64_t)tid < cap_t) idx[3 * (size_t)tid + v] = id - vbase;
+        const int ed = (int)((rw[q >> 2] >> (8 * (q & 3))) & 0xFu);  // 0..11 on a triangle row
+        if ((int64_t)tid < cap_t) idx[3 * (size_t)tid + v] = P.eid[ed][lane];
       }
     }
   }
