@@ -247,6 +247,7 @@ struct EmitPass {  // one wave's LDS
   int xb[kEmitPassGroups], j[kEmitPassGroups], k[kEmitPassGroups], pre[kEmitPassGroups];
   uint16_t item[64];  // (group << 6) | cell
   uint32_t eid[12][64];  // lane t: the vertex id (less vbase) on edge e of listed cell t
+  uint16_t vitem[192];   // the pass's vertices in id order: (listed cell << 2) | axis
 };
 
 __device__ __forceinline__ void emit_sync() {
@@ -337,8 +338,93 @@ __device__ __forceinline__ int emit_plan(EmitPass& P, int ng, const McDims& d, i
   return npass;
 }
 
-// Step 4: one listed cell per lane (the vertices on the node's own edges, the cell's
-// triangles), from the pass's records and list in P.
+// One vertex per lane: vertex ve = (listed cell t << 2) | axis of the pass, on the edge
+// from the cell's node along the axis (an edge with a crossing, so both ends are inside
+// the grid); its id is the id table's entry for that edge.  Position and normal use the
+// float operations of the reference's per-edge interpolation: t = (iso - f0) / (f1 - f0)
+// along the axis, the central-difference gradients at both ends (one-sided on the grid
+// faces) interpolated by t and normalised.  Field reads use offsets clamped to the grid.
+// vertex_load issues the reads, vertex_store computes and writes, so that other work
+// can run while the reads are in flight.
+struct VertexIn {
+  float c0, f1;
+  float up[3], dn[3];  // the node's neighbours one step up / down each axis (clamped)
+  float fp[3], fm[3];  // the far end's: beyond it along the axis, or up / down the others
+  int ijk[3], ax;
+  uint32_t vid;
+};
+
+__device__ __forceinline__ VertexIn vertex_load(const float* __restrict__ f, const McDims& d, const EmitPass& P,
+                                                int ve, uint32_t vbase, bool normals) {
+  VertexIn v;
+  const int t = ve >> 2, a = ve & 3;
+  const int e = P.item[t];
+  const int b = e >> 6;
+  v.ijk[0] = P.xb[b] * 64 + (e & 63);
+  v.ijk[1] = P.j[b];
+  v.ijk[2] = P.k[b];
+  v.ax = a;
+  v.vid = vbase + (a == 0 ? P.eid[0][t] : (a == 1 ? P.eid[3][t] : P.eid[8][t]));
+  const int nn[3] = {d.nx, d.ny, d.nz};
+  const int64_t st[3] = {1, d.nx, d.nxy};
+  const float* fn = f + ((int64_t)v.ijk[2] * d.nxy + (int64_t)v.ijk[1] * d.nx + v.ijk[0]);
+  const int ia = a == 0 ? v.ijk[0] : (a == 1 ? v.ijk[1] : v.ijk[2]);
+  const int na = a == 0 ? nn[0] : (a == 1 ? nn[1] : nn[2]);
+  const int64_t sa = a == 0 ? st[0] : (a == 1 ? st[1] : st[2]);
+  v.c0 = fn[0];
+  v.f1 = fn[sa];
+#pragma unroll
+  for (int q = 0; q < 3; q++) v.up[q] = v.dn[q] = v.fp[q] = v.fm[q] = 0.0f;
+  if (normals) {
+    const int64_t up2 = ia + 2 < na ? sa : 0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int64_t up = v.ijk[q] + 1 < nn[q] ? st[q] : 0, dn = v.ijk[q] > 0 ? st[q] : 0;
+      v.up[q] = fn[up];
+      v.dn[q] = fn[-dn];
+      const bool along = q == a;
+      v.fp[q] = fn[sa + (along ? up2 : up)];
+      v.fm[q] = along ? v.c0 : fn[sa - dn];
+    }
+  }
+  return v;
+}
+
+__device__ __forceinline__ void vertex_store(const VertexIn& v, const McDims& d, const DGrid& g, float iso,
+                                             float* __restrict__ pos, float* __restrict__ nrm, int64_t cap_v) {
+  const int a = v.ax;
+  const int nn[3] = {d.nx, d.ny, d.nz};
+  const float tt = (iso - v.c0) / (v.f1 - v.c0);
+  float p[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) p[q] = node_coord(g.o[q], g.s[q], v.ijk[q]);
+  const int ia = a == 0 ? v.ijk[0] : (a == 1 ? v.ijk[1] : v.ijk[2]);
+  const float oa = a == 0 ? g.o[0] : (a == 1 ? g.o[1] : g.o[2]);
+  const float spa = a == 0 ? g.s[0] : (a == 1 ? g.s[1] : g.s[2]);
+  const float pa = a == 0 ? p[0] : (a == 1 ? p[1] : p[2]);
+  const float pe = pa + tt * (node_coord(oa, spa, ia + 1) - pa);
+  if ((int64_t)v.vid >= cap_v) return;
+  pos[3 * (size_t)v.vid + 0] = a == 0 ? pe : p[0];
+  pos[3 * (size_t)v.vid + 1] = a == 1 ? pe : p[1];
+  pos[3 * (size_t)v.vid + 2] = a == 2 ? pe : p[2];
+  if (!nrm) return;
+  float gv[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const float g0 = grad_axis(v.ijk[q], nn[q], v.up[q], v.c0, v.dn[q], g.s[q]);
+    const bool along = q == a;
+    const float g1 = grad_axis(along ? v.ijk[q] + 1 : v.ijk[q], nn[q], v.fp[q], v.f1, v.fm[q], g.s[q]);
+    gv[q] = g0 + tt * (g1 - g0);
+  }
+  const float len = __builtin_sqrtf(gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2]);
+  const float inv = len > 0.0f ? 1.0f / len : 0.0f;
+  nrm[3 * (size_t)v.vid + 0] = gv[0] * inv;
+  nrm[3 * (size_t)v.vid + 1] = gv[1] * inv;
+  nrm[3 * (size_t)v.vid + 2] = gv[2] * inv;
+}
+
+// Step 4: one listed cell per lane for the edge id table and the cell's triangles; the
+// vertices on the listed nodes' own edges listed in id order, one per lane.
 __device__ __forceinline__ void emit_cells(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
                                            EmitPass& P, int nitems, uint32_t vbase, float* __restrict__ pos,
                                            float* __restrict__ nrm, uint32_t* __restrict__ idx, int64_t cap_v,
@@ -350,85 +436,6 @@ __device__ __forceinline__ void emit_cells(const float* __restrict__ f, const Mc
   const Cross* nb = P.nb[b];
   const int xb = P.xb[b], j = P.j[b], k = P.k[b];
   const int i = xb * 64 + l;
-  const uint64_t lt = (1ull << l) - 1;
-
-  // vertices on the node's own edges
-  {
-    const Cross cc = nb[0];
-    const bool own = act_lane && (((cc.bx | cc.by | cc.bz) >> l) & 1ull);
-    if (own) {
-      uint32_t vid = cc.voff + (uint32_t)(__popcll(cc.bx & lt) + __popcll(cc.by & lt) + __popcll(cc.bz & lt));
-      const uint64_t bits[3] = {cc.bx, cc.by, cc.bz};
-      const int ic = min(i, d.nx - 1);
-      const int64_t sy = d.nx, sz = d.nxy;
-      const int xs[4] = {max(ic - 1, 0), ic, min(ic + 1, d.nx - 1), min(ic + 2, d.nx - 1)};  // i-1 .. i+2
-      const int ys[4] = {max(j - 1, 0), j, min(j + 1, d.ny - 1), min(j + 2, d.ny - 1)};
-      const int zs[4] = {max(k - 1, 0), k, min(k + 1, d.nz - 1), min(k + 2, d.nz - 1)};
-      auto at = [&](int xi, int yi, int zi) { return f[(int64_t)zs[zi] * sz + (int64_t)ys[yi] * sy + xs[xi]]; };
-      const float c0 = at(1, 1, 1), c1 = at(2, 1, 1), c3 = at(1, 2, 1), c4 = at(1, 1, 2);
-      float c2 = 0.f, c5 = 0.f, c7 = 0.f;
-      float xm = 0.f, ym = 0.f, zm = 0.f, x2 = 0.f, xym = 0.f, xzm = 0.f, y2 = 0.f, yxm = 0.f, yzm = 0.f;
-      float z2 = 0.f, zxm = 0.f, zym = 0.f;
-      if (nrm) {
-        c2 = at(2, 2, 1); c5 = at(2, 1, 2); c7 = at(1, 2, 2);
-        xm = at(0, 1, 1); ym = at(1, 0, 1); zm = at(1, 1, 0);
-        x2 = at(3, 1, 1); xym = at(2, 0, 1); xzm = at(2, 1, 0);
-        y2 = at(1, 3, 1); yxm = at(0, 2, 1); yzm = at(1, 2, 0);
-        z2 = at(1, 1, 3); zxm = at(0, 1, 2); zym = at(1, 0, 2);
-      }
-      const float f0 = c0;
-      const float p0[3] = {node_coord(g.o[0], g.s[0], i), node_coord(g.o[1], g.s[1], j),
-                           node_coord(g.o[2], g.s[2], k)};
-      const float f1s[3] = {c1, c3, c4};
-      const int ijk[3] = {i, j, k};
-      float g0[3] = {0.0f, 0.0f, 0.0f};
-      if (nrm) {
-        g0[0] = grad_axis(i, d.nx, c1, c0, xm, g.s[0]);
-        g0[1] = grad_axis(j, d.ny, c3, c0, ym, g.s[1]);
-        g0[2] = grad_axis(k, d.nz, c4, c0, zm, g.s[2]);
-      }
-#pragma unroll
-      for (int ax = 0; ax < 3; ax++) {
-        if (!((bits[ax] >> l) & 1ull)) continue;
-        const float f1 = f1s[ax];
-        const float t = (iso - f0) / (f1 - f0);
-        const float p1a = node_coord(g.o[ax], g.s[ax], ijk[ax] + 1);
-        float p[3] = {p0[0], p0[1], p0[2]};
-        p[ax] = p0[ax] + t * (p1a - p0[ax]);
-        if ((int64_t)vid < cap_v) {
-          pos[3 * (size_t)vid + 0] = p[0];
-          pos[3 * (size_t)vid + 1] = p[1];
-          pos[3 * (size_t)vid + 2] = p[2];
-          if (nrm) {
-            // gradient at the edge's far end
-            float g1[3];
-            if (ax == 0) {
-              g1[0] = grad_axis(i + 1, d.nx, x2, c1, c0, g.s[0]);
-              g1[1] = grad_axis(j, d.ny, c2, c1, xym, g.s[1]);
-              g1[2] = grad_axis(k, d.nz, c5, c1, xzm, g.s[2]);
-            } else if (ax == 1) {
-              g1[0] = grad_axis(i, d.nx, c2, c3, yxm, g.s[0]);
-              g1[1] = grad_axis(j + 1, d.ny, y2, c3, c0, g.s[1]);
-              g1[2] = grad_axis(k, d.nz, c7, c3, yzm, g.s[2]);
-            } else {
-              g1[0] = grad_axis(i, d.nx, c5, c4, zxm, g.s[0]);
-              g1[1] = grad_axis(j, d.ny, c7, c4, zym, g.s[1]);
-              g1[2] = grad_axis(k + 1, d.nz, z2, c4, c0, g.s[2]);
-            }
-            const float gx = g0[0] + t * (g1[0] - g0[0]);
-            const float gy = g0[1] + t * (g1[1] - g0[1]);
-            const float gz = g0[2] + t * (g1[2] - g0[2]);
-            const float len = __builtin_sqrtf(gx * gx + gy * gy + gz * gz);
-            const float inv = len > 0.0f ? 1.0f / len : 0.0f;
-            nrm[3 * (size_t)vid + 0] = gx * inv;
-            nrm[3 * (size_t)vid + 1] = gy * inv;
-            nrm[3 * (size_t)vid + 2] = gz * inv;
-          }
-        }
-        vid++;
-      }
-    }
-  }
 
   // triangles of the cell: the case code from the sign words of its 4 corner rows and
   // their +x neighbour groups; ids from the in-group prefix of the triangle counts
@@ -491,6 +498,28 @@ __device__ __forceinline__ void emit_cells(const float* __restrict__ f, const Mc
     P.eid[6][lane] = node(0, 1, 1).base;                                                     // x
     P.eid[10][lane] = [&] { const Node n = node(1, 1, 0); return n.base + n.xb + n.yb; }();  // z
   }
+  // the node's own vertices (x, y, z crossings in that order), listed in id order: the
+  // listed cells are in group order and, within a group, in node order
+  int vinc_all;
+  {
+    uint32_t ob = 0u;
+    if (act_lane)
+      ob = (uint32_t)((nb[0].bx >> l) & 1ull) | ((uint32_t)((nb[0].by >> l) & 1ull) << 1) |
+           ((uint32_t)((nb[0].bz >> l) & 1ull) << 2);
+    const int nvo = __popc(ob);
+    const int vinc = wave_inclusive_sum(nvo);
+    vinc_all = vinc;
+    int vp = vinc - nvo;
+    if (ob & 1u) P.vitem[vp++] = (uint16_t)(lane << 2);
+    if (ob & 2u) P.vitem[vp++] = (uint16_t)((lane << 2) | 1);
+    if (ob & 4u) P.vitem[vp] = (uint16_t)((lane << 2) | 2);
+    emit_sync();
+  }
+  // the first 64 vertices' reads in flight while the triangles are written
+  const int nverts = uniform(__builtin_amdgcn_readlane(vinc_all, 63));
+  const bool vfirst = lane < nverts;
+  VertexIn vin;
+  if (vfirst) vin = vertex_load(f, d, P, P.vitem[lane], vbase, nrm != nullptr);
   if (nt) {
     const uint32_t tid0 = P.toff[b] + (uint32_t)(exc - seg);
     // the cell's triangulation row in one 16-byte load
@@ -508,6 +537,10 @@ __device__ __forceinline__ void emit_cells(const float* __restrict__ f, const Mc
       }
     }
   }
+  if (vfirst) vertex_store(vin, d, g, iso, pos, nrm, cap_v);
+  for (int v0 = 64; v0 < nverts; v0 += 64)
+    if (v0 + lane < nverts)
+      vertex_store(vertex_load(f, d, P, P.vitem[v0 + lane], vbase, nrm != nullptr), d, g, iso, pos, nrm, cap_v);
 }
 
 __device__ __forceinline__ int mc_emit_pass(const float* __restrict__ f, const McDims& d, const DGrid& g, float iso,
