@@ -99,7 +99,7 @@ __device__ __forceinline__ uint32_t pack_counts(int nv, int nt) {
 // different grids (batched chunks): everything but the pass bookkeeping is per lane.
 
 struct SignPass {  // one wave's LDS
-  uint64_t w[64][8];
+  uint64_t w[8][64];  // word q of lane (group) t at w[q][t]: consecutive lanes, no bank conflicts
   uint32_t nt[64];
   uint16_t item[64];
 };
@@ -137,9 +137,9 @@ __device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restr
   const uint64_t all = (a & a1 & ay & ay1 & b & b1 & by & by1) & cells;
   const uint64_t mixed = any & ~all;
   const int nmix = __popcll(mixed);
-  // the words in code-bit order: bit q of cell c's code is bit c of w[q]
-  uint64_t* w = P.w[lane];
-  w[0] = a; w[1] = a1; w[2] = ay1; w[3] = ay; w[4] = b; w[5] = b1; w[6] = by1; w[7] = by;
+  // the words in code-bit order: bit q of cell c's code is bit c of word q
+  P.w[0][lane] = a; P.w[1][lane] = a1; P.w[2][lane] = ay1; P.w[3][lane] = ay;
+  P.w[4][lane] = b; P.w[5][lane] = b1; P.w[6][lane] = by1; P.w[7][lane] = by;
   P.nt[lane] = 0u;
   const int incl = wave_inclusive_sum(nmix);
   const int total = __builtin_amdgcn_readlane(incl, 63);
@@ -149,13 +149,13 @@ __device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restr
     const bool take = nmix > 0 && incl - nmix >= base && incl <= base + 64;
     const uint64_t tk = ballot64(take);
     const int end = __builtin_amdgcn_readlane(incl, 63 - __clzll((long long)tk));
-    if (take) {
-      uint64_t m = mixed;
-      int p = incl - nmix - base;
-      while (m) {
-        P.item[p++] = (uint16_t)((lane << 6) | (__ffsll((unsigned long long)m) - 1));
-        m &= m - 1;
-      }
+    // list the taken groups' mixed cells: one taken group at a time, each lane placing
+    // the cell of its own bit position
+    for (uint64_t tm = tk; tm; tm &= tm - 1) {
+      const int src = __ffsll((unsigned long long)tm) - 1;
+      const uint64_t mm = readlane64(mixed, src);
+      const int p0 = __builtin_amdgcn_readlane(incl - nmix, src) - base;
+      if ((mm >> lane) & 1ull) P.item[p0 + __popcll(mm & ((1ull << lane) - 1))] = (uint16_t)((src << 6) | lane);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -163,10 +163,9 @@ __device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restr
     if (lane < end - base) {
       const int e = P.item[lane];
       const int grp = e >> 6, c = e & 63;
-      const uint64_t* gw = P.w[grp];
       int code = 0;
 #pragma unroll
-      for (int q = 0; q < 8; q++) code |= (int)((gw[q] >> c) & 1ull) << q;
+      for (int q = 0; q < 8; q++) code |= (int)((P.w[q][grp] >> c) & 1ull) << q;
       atomicAdd(&P.nt[grp], (uint32_t)s_ntris[code]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
