@@ -91,16 +91,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                       s_pass[wave], pos, nrm, idx, cap_v, cap_t);
 }
 
+constexpr int kClassifyGroups = 4;  // groups per thread of k_mc_classify_signs
+
 __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __restrict__ S, const McDims d,
                                                            uint32_t* __restrict__ counts) {
   __shared__ uint8_t s_ntris[256];
   __shared__ SignPass s_pass[4];
   s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
   __syncthreads();
-  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const bool in = g < d.groups;
-  const uint32_t c = classify_signs_group(S, d, g, in, s_ntris, s_pass[threadIdx.x >> 6]);
-  if (in) counts[g] = c;
+  // kClassifyGroups groups per thread, 256 apart; the next group's words are loaded
+  // before the current group is classified
+  const int64_t g0 = (int64_t)blockIdx.x * (256 * kClassifyGroups) + threadIdx.x;
+  GroupWords w = load_group_words(S, d, g0, g0 < d.groups);
+#pragma unroll
+  for (int u = 0; u < kClassifyGroups; u++) {
+    const int64_t g = g0 + 256 * u;
+    const bool in = g < d.groups;
+    GroupWords next = w;
+    if (u + 1 < kClassifyGroups) next = load_group_words(S, d, g + 256, g + 256 < d.groups);
+    const uint32_t c = classify_group_words(w, d, in, s_ntris, s_pass[threadIdx.x >> 6]);
+    if (in) counts[g] = c;
+    w = next;
+  }
 }
 
 // ------------------------------------------------------------------ scan
@@ -286,8 +298,9 @@ int mc_classify_signs_scan(const DGrid& g, void* d_ws, size_t ws_bytes, hipStrea
   const McLayout L = mc_layout(g);
   if (!d_ws || ws_bytes < L.bytes) return UPF_ERR_WORKSPACE;
   const McPtrs P = mc_ptrs(L, d_ws);
-  hipLaunchKernelGGL(k_mc_classify_signs, dim3((unsigned)((L.d.groups + 255) / 256)), dim3(256), 0, s, P.S, L.d,
-                     P.counts);
+  const int64_t per_block = 256 * kClassifyGroups;
+  hipLaunchKernelGGL(k_mc_classify_signs, dim3((unsigned)((L.d.groups + per_block - 1) / per_block)), dim3(256), 0, s,
+                     P.S, L.d, P.counts);
   UPF_LAUNCH_CHECK();
   return mc_scan_launch(P.counts, L.d.groups, P.tiles, P.total, P.voff, P.act, nullptr, s);
 }

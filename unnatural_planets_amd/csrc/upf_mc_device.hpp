@@ -104,32 +104,57 @@ struct SignPass {  // one wave's LDS
   uint16_t item[64];
 };
 
-__device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restrict__ S, const McDims& d, int64_t g,
-                                                         bool in, const uint8_t* s_ntris, SignPass& P) {
-  const int lane = threadIdx.x & 63;
-  uint64_t a = 0, a1 = 0, ay = 0, ay1 = 0, b = 0, b1 = 0, by = 0, by1 = 0, cells = 0;
-  int nv = 0;
+// The 8 sign words a group's classification reads (zero past the grid's faces), with
+// the row flags; loaded apart from the classification so that a lane can have the next
+// group's words in flight while it classifies the current one.
+struct GroupWords {
+  uint64_t a, ax, ay, axy, b, bx, by, bxy;
+  int xb;
+  bool hy, hz;
+};
+
+__device__ __forceinline__ GroupWords load_group_words(const uint64_t* __restrict__ S, const McDims& d, int64_t g,
+                                                       bool in) {
+  GroupWords w;
+  w.a = w.ax = w.ay = w.axy = w.b = w.bx = w.by = w.bxy = 0ull;
+  w.xb = 0;
+  w.hy = w.hz = false;
   if (in) {
     const uint32_t row = (uint32_t)(g / d.gpr);
     const int xb = (int)(g - (int64_t)row * d.gpr);
     const int k = (int)(row / (uint32_t)d.ny), j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
     const bool nx1 = xb + 1 < d.gpr, hy = j + 1 < d.ny, hz = k + 1 < d.nz;
     const int64_t plane = (int64_t)d.ny * d.gpr;
-    a = S[g];
-    const uint64_t ax = nx1 ? S[g + 1] : 0ull;
-    ay = hy ? S[g + d.gpr] : 0ull;
-    const uint64_t axy = (hy && nx1) ? S[g + d.gpr + 1] : 0ull;
-    b = hz ? S[g + plane] : 0ull;
-    const uint64_t bx = (hz && nx1) ? S[g + plane + 1] : 0ull;
-    by = (hz && hy) ? S[g + plane + d.gpr] : 0ull;
-    const uint64_t bxy = (hz && hy && nx1) ? S[g + plane + d.gpr + 1] : 0ull;
-    const int first = xb * 64;
+    w.xb = xb;
+    w.hy = hy;
+    w.hz = hz;
+    w.a = S[g];
+    w.ax = nx1 ? S[g + 1] : 0ull;
+    w.ay = hy ? S[g + d.gpr] : 0ull;
+    w.axy = (hy && nx1) ? S[g + d.gpr + 1] : 0ull;
+    w.b = hz ? S[g + plane] : 0ull;
+    w.bx = (hz && nx1) ? S[g + plane + 1] : 0ull;
+    w.by = (hz && hy) ? S[g + plane + d.gpr] : 0ull;
+    w.bxy = (hz && hy && nx1) ? S[g + plane + d.gpr + 1] : 0ull;
+  }
+  return w;
+}
+
+__device__ __forceinline__ uint32_t classify_group_words(const GroupWords& W, const McDims& d, bool in,
+                                                         const uint8_t* s_ntris, SignPass& P) {
+  const int lane = threadIdx.x & 63;
+  uint64_t a = 0, a1 = 0, ay = 0, ay1 = 0, b = 0, b1 = 0, by = 0, by1 = 0, cells = 0;
+  int nv = 0;
+  if (in) {
+    const bool hy = W.hy, hz = W.hz;
+    a = W.a; ay = W.ay; b = W.b; by = W.by;
+    const int first = W.xb * 64;
     const int nval = min(64, d.nx - first), nvx = min(64, d.nx - 1 - first);
     const uint64_t vmask = nval >= 64 ? ~0ull : ((1ull << nval) - 1);
     const uint64_t xmask = nvx >= 64 ? ~0ull : (nvx > 0 ? ((1ull << nvx) - 1) : 0ull);
     // bit i of *1 = node i + 1 of the same row
-    a1 = (a >> 1) | (ax << 63); ay1 = (ay >> 1) | (axy << 63);
-    b1 = (b >> 1) | (bx << 63); by1 = (by >> 1) | (bxy << 63);
+    a1 = (a >> 1) | (W.ax << 63); ay1 = (ay >> 1) | (W.axy << 63);
+    b1 = (b >> 1) | (W.bx << 63); by1 = (by >> 1) | (W.bxy << 63);
     nv = __popcll((a ^ a1) & xmask) + (hy ? __popcll((a ^ ay) & vmask) : 0) + (hz ? __popcll((a ^ b) & vmask) : 0);
     cells = (hy && hz) ? xmask : 0ull;
   }
@@ -174,6 +199,11 @@ __device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restr
     base = end;
   }
   return pack_counts(nv, (int)P.nt[lane]);
+}
+
+__device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restrict__ S, const McDims& d, int64_t g,
+                                                         bool in, const uint8_t* s_ntris, SignPass& P) {
+  return classify_group_words(load_group_words(S, d, g, in), d, in, s_ntris, P);
 }
 
 // ------------------------------------------------------------------ emit
