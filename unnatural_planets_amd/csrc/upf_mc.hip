@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                       s_pass[wave], pos, nrm, idx, cap_v, cap_t);
 }
 
-constexpr int kClassifyGroups = 4;  // groups per thread of k_mc_classify_signs
+constexpr int kClassifyGroups = 8;  // groups per thread of k_mc_classify_signs
 
 __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __restrict__ S, const McDims d,
                                                            uint32_t* __restrict__ counts) {
