@@ -91,30 +91,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                       s_pass[wave], pos, nrm, idx, cap_v, cap_t);
 }
 
-constexpr int kClassifyGroups = 8;  // groups per thread of k_mc_classify_signs
-
-__global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __restrict__ S, const McDims d,
-                                                           uint32_t* __restrict__ counts) {
-  __shared__ uint8_t s_ntris[256];
-  __shared__ SignPass s_pass[4];
-  s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
-  __syncthreads();
-  // kClassifyGroups groups per thread, 256 apart; the next group's words are loaded
-  // before the current group is classified
-  const int64_t g0 = (int64_t)blockIdx.x * (256 * kClassifyGroups) + threadIdx.x;
-  GroupWords w = load_group_words(S, d, g0, g0 < d.groups);
-#pragma unroll
-  for (int u = 0; u < kClassifyGroups; u++) {
-    const int64_t g = g0 + 256 * u;
-    const bool in = g < d.groups;
-    GroupWords next = w;
-    if (u + 1 < kClassifyGroups) next = load_group_words(S, d, g + 256, g + 256 < d.groups);
-    const uint32_t c = classify_group_words(w, d, in, s_ntris, s_pass[threadIdx.x >> 6]);
-    if (in) counts[g] = c;
-    w = next;
-  }
-}
-
 // ------------------------------------------------------------------ scan
 // Three running sums at once: x = vertices, y = triangles, z = active groups.
 constexpr int kScanThreads = 256;
@@ -147,6 +123,47 @@ __device__ __forceinline__ uint4 block_exclusive_scan(uint4 v, uint4* total) {
   __syncthreads();
   *total = all;
   return make_uint4(before.x + inc.x - v.x, before.y + inc.y - v.y, before.z + inc.z - v.z, 0);
+}
+
+constexpr int kClassifyGroups = 8;  // groups per thread of k_mc_classify_signs
+static_assert((256 * kClassifyGroups) % kScanTile == 0 && kScanTile % 256 == 0, "whole scan tiles per workgroup");
+
+// Packed counts of every group, and the sums of the scan tiles the workgroup covers
+// (the scan's first pass): thread t takes groups t + 256 u, so tile q of the workgroup
+// is u in [q * kScanTile / 256, (q + 1) * kScanTile / 256).
+__global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __restrict__ S, const McDims d,
+                                                           uint32_t* __restrict__ counts,
+                                                           uint4* __restrict__ tile_sums) {
+  constexpr int kTiles = 256 * kClassifyGroups / kScanTile, kPerTile = kScanTile / 256;
+  __shared__ uint8_t s_ntris[256];
+  __shared__ SignPass s_pass[4];
+  s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
+  __syncthreads();
+  // the next group's words are loaded before the current group is classified
+  const int64_t g0 = (int64_t)blockIdx.x * (256 * kClassifyGroups) + threadIdx.x;
+  uint4 sum[kTiles];
+#pragma unroll
+  for (int q = 0; q < kTiles; q++) sum[q] = make_uint4(0, 0, 0, 0);
+  GroupWords w = load_group_words(S, d, g0, g0 < d.groups);
+#pragma unroll
+  for (int u = 0; u < kClassifyGroups; u++) {
+    const int64_t g = g0 + 256 * u;
+    const bool in = g < d.groups;
+    GroupWords next = w;
+    if (u + 1 < kClassifyGroups) next = load_group_words(S, d, g + 256, g + 256 < d.groups);
+    const uint32_t c = classify_group_words(w, d, in, s_ntris, s_pass[threadIdx.x >> 6]);
+    if (in) counts[g] = c;
+    sum[u / kPerTile] = add4(sum[u / kPerTile], unpack_counts<kScanMc>(in ? c : 0u));
+    w = next;
+  }
+  const int64_t ntiles = (d.groups + kScanTile - 1) / kScanTile;
+#pragma unroll
+  for (int q = 0; q < kTiles; q++) {
+    uint4 total;
+    (void)block_exclusive_scan(sum[q], &total);
+    const int64_t tile = (int64_t)blockIdx.x * kTiles + q;
+    if (threadIdx.x == 0 && tile < ntiles) tile_sums[tile] = total;
+  }
 }
 
 template <int FMT>
@@ -275,10 +292,12 @@ int mc_classify_scan(const float* d_field, const DGrid& g, float iso, void* d_ws
 }
 
 int mc_scan_launch(const uint32_t* counts, int64_t groups, uint4* tiles, uint4* total, uint32_t* voff, uint2* act,
-                   uint32_t* toff, hipStream_t s, int fmt) {
+                   uint32_t* toff, hipStream_t s, int fmt, bool tiles_ready) {
   const int64_t ntiles = (groups + kScanTile - 1) / kScanTile;
   if (ntiles == 0) return UPF_ERR_BAD_GRID;
-  if (fmt == kScanMc)
+  if (tiles_ready) {
+    // the tile sums were written by the kernel that produced the counts
+  } else if (fmt == kScanMc)
     hipLaunchKernelGGL(k_scan_tiles<kScanMc>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups, tiles);
   else
     hipLaunchKernelGGL(k_scan_tiles<kScanBlocks>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups,
@@ -300,9 +319,9 @@ int mc_classify_signs_scan(const DGrid& g, void* d_ws, size_t ws_bytes, hipStrea
   const McPtrs P = mc_ptrs(L, d_ws);
   const int64_t per_block = 256 * kClassifyGroups;
   hipLaunchKernelGGL(k_mc_classify_signs, dim3((unsigned)((L.d.groups + per_block - 1) / per_block)), dim3(256), 0, s,
-                     P.S, L.d, P.counts);
+                     P.S, L.d, P.counts, P.tiles);
   UPF_LAUNCH_CHECK();
-  return mc_scan_launch(P.counts, L.d.groups, P.tiles, P.total, P.voff, P.act, nullptr, s);
+  return mc_scan_launch(P.counts, L.d.groups, P.tiles, P.total, P.voff, P.act, nullptr, s, kScanMc, true);
 }
 
 int64_t mc_scan_tiles(int64_t groups) { return (groups + kScanTile - 1) / kScanTile; }

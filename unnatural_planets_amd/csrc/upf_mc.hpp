@@ -47,9 +47,10 @@ constexpr int kScanMc = 0;      // vertices bits 0-7, triangles 8-16, active bit
 constexpr int kScanBlocks = 1;  // vertices bits 0-11, triangles 12-23, active bit 24
 // exclusive scan of packed counts (vertices, triangles, active) into voff / toff (dense
 // offsets, each optional), act (compacted (index, first triangle) of the active entries,
-// optional) and total
+// optional) and total; tiles_ready: the sums of the scan tiles are already in `tiles`
+// (written by the kernel that produced the counts)
 int mc_scan_launch(const uint32_t* counts, int64_t groups, uint4* tiles, uint4* total, uint32_t* voff, uint2* act,
-                   uint32_t* toff, hipStream_t s, int fmt = kScanMc);
+                   uint32_t* toff, hipStream_t s, int fmt = kScanMc, bool tiles_ready = false);
 int64_t mc_scan_tiles(int64_t groups);
 
 }  // namespace upf
