@@ -137,7 +137,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
 }
 
 // Counts of every group of every chunk (global id g: chunk, then its local group) from
-// the sign words; lanes of one wave may belong to two chunks.
+// the sign words; lanes of one wave may belong to two chunks.  One group per thread: the
+// batch's small chunks have several times the 512^3 grid's mixed cells per group, and
+// their classification passes need the occupancy (four or eight groups per thread with
+// the next group's words loaded ahead and the tile sums written, as the single-grid
+// kernel does, took 67 and 82 us against 50 + 5.6 us for this kernel and the separate
+// tile pass).
 __global__ __launch_bounds__(256) void k_batch_classify(const BChunk* __restrict__ ch, int n_chunks, int64_t groups,
                                                         const uint64_t* __restrict__ S,
                                                         uint32_t* __restrict__ counts) {
@@ -152,7 +157,8 @@ __global__ __launch_bounds__(256) void k_batch_classify(const BChunk* __restrict
   int c = uniform(find_chunk<&BChunk::group_off>(ch, n_chunks, (int64_t)blockIdx.x * 256));
   while (c + 1 < n_chunks && ch[c + 1].group_off <= (in ? g : groups - 1)) c++;
   const BChunk& C = ch[c];
-  const uint32_t cnt = classify_signs_group(S + C.group_off, C.md, g - C.group_off, in, s_ntris, s_pass[threadIdx.x >> 6]);
+  const GroupWords w = load_group_words(S + C.group_off, C.md, g - C.group_off, in);
+  const uint32_t cnt = classify_group_words(w, C.md.nx, in, s_ntris, s_pass[threadIdx.x >> 6]);
   if (in) counts[g] = cnt;
 }
 

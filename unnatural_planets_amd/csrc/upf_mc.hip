@@ -91,43 +91,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                       s_pass[wave], pos, nrm, idx, cap_v, cap_t);
 }
 
-// ------------------------------------------------------------------ scan
-// Three running sums at once: x = vertices, y = triangles, z = active groups.
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
-__device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, 0); }
-// Count words: kScanMc = marching-cubes groups (vertices bits 0-7, triangles 8-16,
-// active bit 17); kScanBlocks = narrow-band blocks (vertices 0-11, triangles 12-23,
-// active bit 24).
-template <int FMT>
-__device__ __forceinline__ uint4 unpack_counts(uint32_t c) {
-  if (FMT == kScanMc) return make_uint4(c & 0xFFu, (c >> 8) & 0x1FFu, (c >> 17) & 1u, 0);
-  return make_uint4(c & 0xFFFu, (c >> 12) & 0xFFFu, (c >> 24) & 1u, 0);
-}
-
-// Block-wide exclusive scan of one uint4 per thread; the block total goes to *total.
-__device__ __forceinline__ uint4 block_exclusive_scan(uint4 v, uint4* total) {
-  __shared__ uint4 wsum[kScanThreads / 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint4 inc = make_uint4((unsigned)wave_inclusive_sum((int)v.x), (unsigned)wave_inclusive_sum((int)v.y),
-                               (unsigned)wave_inclusive_sum((int)v.z), 0u);
-  if (lane == 63) wsum[wave] = inc;
-  __syncthreads();
-  uint4 before = make_uint4(0, 0, 0, 0), all = make_uint4(0, 0, 0, 0);
-  for (int w = 0; w < kScanThreads / 64; w++) {
-    if (w < wave) before = add4(before, wsum[w]);
-    all = add4(all, wsum[w]);
-  }
-  __syncthreads();
-  *total = all;
-  return make_uint4(before.x + inc.x - v.x, before.y + inc.y - v.y, before.z + inc.z - v.z, 0);
-}
-
-constexpr int kClassifyGroups = 8;  // groups per thread of k_mc_classify_signs
-static_assert((256 * kClassifyGroups) % kScanTile == 0 && kScanTile % 256 == 0, "whole scan tiles per workgroup");
-
 // Packed counts of every group, and the sums of the scan tiles the workgroup covers
 // (the scan's first pass): thread t takes groups t + 256 u, so tile q of the workgroup
 // is u in [q * kScanTile / 256, (q + 1) * kScanTile / 256).
@@ -151,7 +114,7 @@ __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __res
     const bool in = g < d.groups;
     GroupWords next = w;
     if (u + 1 < kClassifyGroups) next = load_group_words(S, d, g + 256, g + 256 < d.groups);
-    const uint32_t c = classify_group_words(w, d, in, s_ntris, s_pass[threadIdx.x >> 6]);
+    const uint32_t c = classify_group_words(w, d.nx, in, s_ntris, s_pass[threadIdx.x >> 6]);
     if (in) counts[g] = c;
     sum[u / kPerTile] = add4(sum[u / kPerTile], unpack_counts<kScanMc>(in ? c : 0u));
     w = next;
@@ -166,6 +129,7 @@ __global__ __launch_bounds__(256) void k_mc_classify_signs(const uint64_t* __res
   }
 }
 
+// ------------------------------------------------------------------ scan
 template <int FMT>
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint32_t* __restrict__ counts, int64_t n,
                                                              uint4* __restrict__ tile_sums) {

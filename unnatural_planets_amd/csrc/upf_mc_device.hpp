@@ -85,18 +85,56 @@ __device__ __forceinline__ uint32_t pack_counts(int nv, int nt) {
   return (uint32_t)nv | ((uint32_t)nt << 8) | ((nv | nt) ? (1u << 17) : 0u);
 }
 
+// ------------------------------------------------------------------ scan helpers
+// Three running sums at once: x = vertices, y = triangles, z = active groups.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint4 add4(uint4 a, uint4 b) { return make_uint4(a.x + b.x, a.y + b.y, a.z + b.z, 0); }
+// Count words: kScanMc = marching-cubes groups (vertices bits 0-7, triangles 8-16,
+// active bit 17); kScanBlocks = narrow-band blocks (vertices 0-11, triangles 12-23,
+// active bit 24).
+template <int FMT>
+__device__ __forceinline__ uint4 unpack_counts(uint32_t c) {
+  if (FMT == kScanMc) return make_uint4(c & 0xFFu, (c >> 8) & 0x1FFu, (c >> 17) & 1u, 0);
+  return make_uint4(c & 0xFFFu, (c >> 12) & 0xFFFu, (c >> 24) & 1u, 0);
+}
+
+// Block-wide exclusive scan of one uint4 per thread; the block total goes to *total.
+__device__ __forceinline__ uint4 block_exclusive_scan(uint4 v, uint4* total) {
+  __shared__ uint4 wsum[kScanThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint4 inc = make_uint4((unsigned)wave_inclusive_sum((int)v.x), (unsigned)wave_inclusive_sum((int)v.y),
+                               (unsigned)wave_inclusive_sum((int)v.z), 0u);
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  uint4 before = make_uint4(0, 0, 0, 0), all = make_uint4(0, 0, 0, 0);
+  for (int w = 0; w < kScanThreads / 64; w++) {
+    if (w < wave) before = add4(before, wsum[w]);
+    all = add4(all, wsum[w]);
+  }
+  __syncthreads();
+  *total = all;
+  return make_uint4(before.x + inc.x - v.x, before.y + inc.y - v.y, before.z + inc.z - v.z, 0);
+}
+
 // ------------------------------------------------------------------ classify
-// Packed counts of group g (local id in a grid with dims d; `in` false past the end)
-// from the sign words alone (written by the field kernel or k_mc_signs), one lane per
-// group.  Crossing masks come from S of the group, of its +x neighbour group and of the
-// +y / +z rows (8 words), so vertex counts are a few 64-bit operations per lane.
-// Triangle counts need the codes of the mixed cells (corners not all on one side; the
-// other codes have no triangles).  Each lane lists its group's mixed cells in LDS; in
-// passes of at most 64 cells (a group's cells never split), lane t takes listed cell t,
-// builds its code from the group's 8 words (in LDS) and adds the code's triangle count
-// to the group's LDS counter.  A wave with ~6 active groups of ~7 mixed cells each needs
+// Packed counts of a group from the sign words alone (written by the field kernel or
+// k_mc_signs), one lane per group.  Crossing masks come from S of the group, of its +x
+// neighbour group and of the +y / +z rows (8 words, load_group_words), so vertex counts
+// are a few 64-bit operations per lane.  Triangle counts need the codes of the mixed
+// cells (corners not all on one side; the other codes have no triangles).  The lanes
+// list their groups' mixed cells in LDS; in passes of at most 64 cells (a group's cells
+// never split), lane t takes listed cell t, builds its code from the group's 8 words (in
+// LDS) and adds the code's triangle count to the group's LDS counter.  A wave with ~6 active groups of ~7 mixed cells each needs
 // one pass, instead of one wave-wide step per active group.  Lanes may belong to
 // different grids (batched chunks): everything but the pass bookkeeping is per lane.
+
+// groups per thread of the classification kernels (thread t of a workgroup takes groups
+// t + 256 u), a whole number of scan tiles per workgroup so that they write the tile sums
+constexpr int kClassifyGroups = 8;
+static_assert((256 * kClassifyGroups) % kScanTile == 0 && kScanTile % 256 == 0, "whole scan tiles per workgroup");
 
 struct SignPass {  // one wave's LDS
   uint64_t w[8][64];  // word q of lane (group) t at w[q][t]: consecutive lanes, no bank conflicts
@@ -109,25 +147,21 @@ struct SignPass {  // one wave's LDS
 // group's words in flight while it classifies the current one.
 struct GroupWords {
   uint64_t a, ax, ay, axy, b, bx, by, bxy;
-  int xb;
-  bool hy, hz;
+  uint32_t seg;  // the group's segment in its row | (a +y row) << 30 | (a +z row) << 31
 };
 
 __device__ __forceinline__ GroupWords load_group_words(const uint64_t* __restrict__ S, const McDims& d, int64_t g,
                                                        bool in) {
   GroupWords w;
   w.a = w.ax = w.ay = w.axy = w.b = w.bx = w.by = w.bxy = 0ull;
-  w.xb = 0;
-  w.hy = w.hz = false;
+  w.seg = 0u;
   if (in) {
     const uint32_t row = (uint32_t)(g / d.gpr);
     const int xb = (int)(g - (int64_t)row * d.gpr);
     const int k = (int)(row / (uint32_t)d.ny), j = (int)(row - (uint32_t)k * (uint32_t)d.ny);
     const bool nx1 = xb + 1 < d.gpr, hy = j + 1 < d.ny, hz = k + 1 < d.nz;
     const int64_t plane = (int64_t)d.ny * d.gpr;
-    w.xb = xb;
-    w.hy = hy;
-    w.hz = hz;
+    w.seg = (uint32_t)xb | ((uint32_t)hy << 30) | ((uint32_t)hz << 31);
     w.a = S[g];
     w.ax = nx1 ? S[g + 1] : 0ull;
     w.ay = hy ? S[g + d.gpr] : 0ull;
@@ -140,16 +174,17 @@ __device__ __forceinline__ GroupWords load_group_words(const uint64_t* __restric
   return w;
 }
 
-__device__ __forceinline__ uint32_t classify_group_words(const GroupWords& W, const McDims& d, bool in,
+// W: the group's words (load_group_words), nx: its grid's row length
+__device__ __forceinline__ uint32_t classify_group_words(const GroupWords& W, int nx, bool in,
                                                          const uint8_t* s_ntris, SignPass& P) {
   const int lane = threadIdx.x & 63;
   uint64_t a = 0, a1 = 0, ay = 0, ay1 = 0, b = 0, b1 = 0, by = 0, by1 = 0, cells = 0;
   int nv = 0;
   if (in) {
-    const bool hy = W.hy, hz = W.hz;
+    const bool hy = (W.seg >> 30) & 1u, hz = W.seg >> 31;
     a = W.a; ay = W.ay; b = W.b; by = W.by;
-    const int first = W.xb * 64;
-    const int nval = min(64, d.nx - first), nvx = min(64, d.nx - 1 - first);
+    const int first = (int)(W.seg & 0x3FFFFFFFu) * 64;
+    const int nval = min(64, nx - first), nvx = min(64, nx - 1 - first);
     const uint64_t vmask = nval >= 64 ? ~0ull : ((1ull << nval) - 1);
     const uint64_t xmask = nvx >= 64 ? ~0ull : (nvx > 0 ? ((1ull << nvx) - 1) : 0ull);
     // bit i of *1 = node i + 1 of the same row
@@ -199,11 +234,6 @@ __device__ __forceinline__ uint32_t classify_group_words(const GroupWords& W, co
     base = end;
   }
   return pack_counts(nv, (int)P.nt[lane]);
-}
-
-__device__ __forceinline__ uint32_t classify_signs_group(const uint64_t* __restrict__ S, const McDims& d, int64_t g,
-                                                         bool in, const uint8_t* s_ntris, SignPass& P) {
-  return classify_group_words(load_group_words(S, d, g, in), d, in, s_ntris, P);
 }
 
 // ------------------------------------------------------------------ emit
