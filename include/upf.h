@@ -676,8 +676,8 @@ int upf_image_special_to_gltf_pbr(const uint8_t* d_special, int64_t n_texels, ui
 
 /* ---- batched chunks ---------------------------------------------------------------
  * Many chunks (planets) of uneven grid sizes, shapes and seeds evaluated and meshed by
- * one set of persistent launches: field rows, classify columns and active groups of
- * all chunks are spread over a fixed grid of workgroups, and one scan numbers the
+ * one set of launches: the field rows, marching-cubes groups and active groups of all
+ * chunks are each spread over one launch, and one scan numbers the
  * vertices and triangles of all chunks (chunk after chunk).  upf_batch_run makes no
  * allocation, no memset and no host synchronisation, so it can be captured into a
  * HIP graph (upf_batch_capture / upf_batch_replay do that).  No counterpart in the
@@ -692,7 +692,9 @@ typedef struct upf_chunk {
 /* Launch knobs searched by upf_batch_autotune; every setting gives identical results. */
 typedef struct upf_batch_opts {
   int32_t field_variant;  /* upf_eval_variant                                  */
-  int32_t blocks_per_cu;  /* persistent workgroups per compute unit (0 = 4)    */
+  int32_t blocks_per_cu;  /* workgroups per compute unit of the per-lane field
+                             kernel's fixed grid (0 = 4); the row form runs
+                             one workgroup per 4 field items                */
   int32_t classify_z;     /* accepted values 0, 16, 32, 64; no effect (the
                              classification reads the sign words the field
                              kernel stores, one lane per group)           */

@@ -1,15 +1,17 @@
 // upf_batch.hip -- many chunks of uneven sizes evaluated and meshed by one set of
-// persistent launches (no counterpart in the reference, which meshes one chunk per
-// task, sources/generator.cpp:159).
+// launches (no counterpart in the reference, which meshes one chunk per task,
+// sources/generator.cpp:159).
 //
-// Work items (a wavefront each), enumerated chunk after chunk:
+// Work items, enumerated chunk after chunk:
 //   field     64 x nodes of a pair of grid rows            (chunk, xb, j / 2, k), which
-//             also stores the rows' sign words
+//             also stores the rows' sign words: one item per wave over all chunks'
+//             items (row form), or a fixed grid of blocks_per_cu workgroups per CU
+//             striding over them (per-lane form)
 //   classify  one marching-cubes group per lane, from the sign words alone
-//   emit      packed passes over the active groups of the concatenated scan
-// A fixed grid of workgroups (blocks_per_cu per compute unit) strides over each list,
-// so small chunks share the machine with large ones and nothing waits on the host.
-// The marching-cubes groups of all chunks are concatenated, so a single scan numbers
+//   emit      packed passes over the active groups of the concatenated scan, on a
+//             fixed grid
+// Small chunks share the machine with large ones and nothing waits on the host.  The
+// marching-cubes groups of all chunks are concatenated, so a single scan numbers
 // vertices and triangles batch-wide; per-chunk ranges come out of the same scan.
 #include <hip/hip_runtime.h>
 
@@ -52,12 +54,57 @@ __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ ch, int n, 
 
 constexpr int kFieldSpan = 8;  // consecutive field items per wave step
 
-// Seven waves per SIMD (at most 72 VGPRs), as the single-grid row kernel; with the
-// autotuner's 7 workgroups per CU, 2.12 -> 2.09 ms for the 64-chunk batch (six waves and
-// 8 workgroups per CU before).
+// One field item: nodes xb * 64 + lane of rows 2 jp and 2 jp + 1 (when it exists) of
+// layer k of chunk C, their values into the chunk's field and their sign words into S.
+template <int VARIANT, bool EXT, bool COMPOSED>
+__device__ __forceinline__ void batch_field_item(const RowCtx& ctx, const BChunk& C, int xb, int jp, int k,
+                                                 float* __restrict__ field, uint64_t* __restrict__ S) {
+  const int lane = threadIdx.x & 63;
+  const int j = 2 * jp;
+  const bool two = j + 1 < C.grid.n[1];
+  const int ic = min(xb * 64 + lane, C.grid.n[0] - 1);
+  const float x = node_coord(C.grid.o[0], C.grid.s[0], ic);
+  const float y[2] = {node_coord(C.grid.o[1], C.grid.s[1], j),
+                      node_coord(C.grid.o[1], C.grid.s[1], two ? j + 1 : j)};
+  const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
+  float v[2];
+  if (VARIANT == UPF_EVAL_PER_LANE) {
+    // a chunk rounds as its own single-grid evaluation would: the exact form for an
+    // elevation mode or a further shape, the fused form otherwise (the choice is
+    // uniform across the wave: one chunk per row pair)
+    if (EXT && spec_ext(C.spec)) {
+      v[0] = field_eval<true, true>(C.spec, x, y[0], z);
+      v[1] = two ? field_eval<true, true>(C.spec, x, y[1], z) : v[0];
+    } else {
+      v[0] = field_eval<EXT, false>(C.spec, x, y[0], z);
+      v[1] = two ? field_eval<EXT, false>(C.spec, x, y[1], z) : v[0];
+    }
+  } else {
+    row_field<2, COMPOSED>(ctx, C.spec, x, y, z, v);
+  }
+  // the node index recomputed from the context's lane: not live across the layers
+  const int in = xb * 64 + (VARIANT == UPF_EVAL_PER_LANE ? lane : lane_of(ctx));
+  const bool valid = in < C.grid.n[0];
+  if (valid) {
+    const int64_t o = C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + in;
+    field[o] = v[0];
+    if (two) field[o + C.grid.n[0]] = v[1];
+  }
+  // the rows' sign words (bit = node exists and f < iso), as the single-grid kernels
+  const uint64_t m0 = ballot64(valid && v[0] < C.iso), m1 = ballot64(valid && v[1] < C.iso);
+  if (lane == 0) {
+    const int64_t w = C.group_off + ((int64_t)k * C.grid.n[1] + j) * C.gx + xb;
+    S[w] = m0;
+    if (two) S[w + C.gx] = m1;
+  }
+}
+
+// The per-lane form on a fixed grid of workgroups: each wave takes spans of kFieldSpan
+// consecutive items.  Seven waves per SIMD (at most 72 VGPRs).
 template <int VARIANT, bool EXT, bool COMPOSED = true>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_batch_field(
     const BChunk* __restrict__ ch, int n_chunks, int64_t n_items, float* __restrict__ field, uint64_t* __restrict__ S) {
+  static_assert(VARIANT == UPF_EVAL_PER_LANE, "the row form runs one item per wave (k_batch_field_items)");
   __shared__ float4 s_slice[4][kRowSlice];
   __shared__ float4 s_gtab[16];
   __shared__ float4 s_oct[4][kRowOctTab];
@@ -87,42 +134,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
       int jp = (int)(rest % (uint32_t)pairs);
       int k = (int)(rest / (uint32_t)pairs);
       for (; r < e; r++) {
-        const int j = 2 * jp;
-        const bool two = j + 1 < C.grid.n[1];
-        const int i = xb * 64 + lane;
-        const int ic = min(i, C.grid.n[0] - 1);
-        const float x = node_coord(C.grid.o[0], C.grid.s[0], ic);
-        const float y[2] = {node_coord(C.grid.o[1], C.grid.s[1], j),
-                            node_coord(C.grid.o[1], C.grid.s[1], two ? j + 1 : j)};
-        const float z = node_coord(C.grid.o[2], C.grid.s[2], k);
-        float v[2];
-        if (VARIANT == UPF_EVAL_PER_LANE) {
-          // a chunk rounds as its own single-grid evaluation would: the exact form for an
-          // elevation mode or a further shape, the fused form otherwise (the choice is
-          // uniform across the wave: one chunk per row pair)
-          if (EXT && spec_ext(C.spec)) {
-            v[0] = field_eval<true, true>(C.spec, x, y[0], z);
-            v[1] = two ? field_eval<true, true>(C.spec, x, y[1], z) : v[0];
-          } else {
-            v[0] = field_eval<EXT, false>(C.spec, x, y[0], z);
-            v[1] = two ? field_eval<EXT, false>(C.spec, x, y[1], z) : v[0];
-          }
-        } else {
-          row_field<2, COMPOSED>(ctx, C.spec, x, y, z, v);
-        }
-        const bool valid = i < C.grid.n[0];
-        if (valid) {
-          const int64_t o = C.node_off + ((int64_t)k * C.grid.n[1] + j) * C.grid.n[0] + i;
-          field[o] = v[0];
-          if (two) field[o + C.grid.n[0]] = v[1];
-        }
-        // the rows' sign words (bit = node exists and f < iso), as the single-grid kernels
-        const uint64_t m0 = ballot64(valid && v[0] < C.iso), m1 = ballot64(valid && v[1] < C.iso);
-        if (lane == 0) {
-          const int64_t w = C.group_off + ((int64_t)k * C.grid.n[1] + j) * C.gx + xb;
-          S[w] = m0;
-          if (two) S[w + C.gx] = m1;
-        }
+        batch_field_item<VARIANT, EXT, COMPOSED>(ctx, C, xb, jp, k, field, S);
         if (++xb == C.gx) {
           xb = 0;
           if (++jp == pairs) {
@@ -134,6 +146,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
       if (r < s1) c++;  // the span continues in the next chunk
     }
   }
+}
+
+// The row form with one field item per wave, as the single-grid row kernel runs: a
+// workgroup takes 4 consecutive items, blk_chunk[workgroup] is the chunk of its first
+// (a chunk has at least 16 items, so the 4 lie in that chunk or the next).  On the 17
+// 192^3 chunks of the 64-chunk batch the persistent loop above ran 1.06-1.08 ms against
+// 0.83 ms for the single-grid kernel over the same chunks one by one (counters: ~23%
+// fewer VALU instructions issued per busy cycle).
+template <bool COMPOSED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_batch_field_items(
+    const BChunk* __restrict__ ch, int n_chunks, const int32_t* __restrict__ blk_chunk, int64_t n_items,
+    float* __restrict__ field, uint64_t* __restrict__ S) {
+  __shared__ float4 s_slice[4][kRowSlice];
+  __shared__ float4 s_gtab[16];
+  __shared__ float4 s_oct[4][kRowOctTab];
+  if (threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
+  __syncthreads();
+  const int wave = uniform(threadIdx.x >> 6);
+  const int64_t r = (int64_t)blockIdx.x * 4 + wave;
+  if (r >= n_items) return;  // whole waves
+  int c = blk_chunk[blockIdx.x];
+  if (c + 1 < n_chunks && ch[c + 1].row_off <= r) c++;
+  c = uniform(c);
+  const BChunk& C = ch[c];
+  const int pairs = (C.grid.n[1] + 1) / 2;
+  const uint32_t local = (uint32_t)(r - C.row_off);  // < 2^31: a chunk's groups fit 31 bits
+  const int xb = (int)(local % (uint32_t)C.gx);
+  const uint32_t rest = local / (uint32_t)C.gx;
+  const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], threadIdx.x & 63);
+  batch_field_item<UPF_EVAL_ROWS, false, COMPOSED>(ctx, C, xb, (int)(rest % (uint32_t)pairs),
+                                                   (int)(rest / (uint32_t)pairs), field, S);
 }
 
 // Counts of every group of every chunk (global id g: chunk, then its local group) from
@@ -216,6 +259,8 @@ size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct upf_batch {
   std::vector<upf::BChunk> host;
   upf::BChunk* dev = nullptr;
+  int32_t* blk_chunk = nullptr;  // chunk of the first field item of each workgroup of 4 items
+  int64_t item_blocks = 0;
   int device = 0;
   int n = 0;
   int cus = 256;
@@ -271,11 +316,11 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
     hipLaunchKernelGGL((k_batch_field<UPF_EVAL_PER_LANE, false>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
                        field, S);
   else if (b->any_composed)
-    hipLaunchKernelGGL((k_batch_field<UPF_EVAL_ROWS, false, true>), dim3(grid), dim3(256), 0, s, b->dev, b->n,
-                       b->rows, field, S);
+    hipLaunchKernelGGL((k_batch_field_items<true>), dim3((unsigned)b->item_blocks), dim3(256), 0, s, b->dev, b->n,
+                       b->blk_chunk, b->rows, field, S);
   else
-    hipLaunchKernelGGL((k_batch_field<UPF_EVAL_ROWS, false, false>), dim3(grid), dim3(256), 0, s, b->dev, b->n,
-                       b->rows, field, S);
+    hipLaunchKernelGGL((k_batch_field_items<false>), dim3((unsigned)b->item_blocks), dim3(256), 0, s, b->dev, b->n,
+                       b->blk_chunk, b->rows, field, S);
   hipLaunchKernelGGL(k_batch_classify, dim3((unsigned)((b->groups + 255) / 256)), dim3(256), 0, s, b->dev, b->n,
                      b->groups, S, counts);
   UPF_LAUNCH_CHECK();
@@ -346,6 +391,22 @@ int upf_batch_create(const upf_chunk* chunks, int32_t n_chunks, const upf_batch_
     return UPF_ERR_HIP;
   }
   {
+    // workgroup -> chunk of its first field item, for the one-item-per-wave row kernel
+    b->item_blocks = (b->rows + 3) / 4;
+    if (b->item_blocks >= ((int64_t)1 << 31)) { upf_batch_destroy(b); return UPF_ERR_BAD_GRID; }
+    std::vector<int32_t> tab((size_t)b->item_blocks);
+    int c = 0;
+    for (int64_t q = 0; q < b->item_blocks; q++) {
+      while (c + 1 < n_chunks && b->host[c + 1].row_off <= 4 * q) c++;
+      tab[(size_t)q] = c;
+    }
+    if (hipMalloc(&b->blk_chunk, sizeof(int32_t) * tab.size()) != hipSuccess ||
+        hipMemcpy(b->blk_chunk, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      upf_batch_destroy(b);
+      return UPF_ERR_HIP;
+    }
+  }
+  {
     const int per_cu = std::max(1, resident_blocks(reinterpret_cast<const void*>(&k_batch_emit), 256) / b->cus);
     b->emit_blocks = b->cus * per_cu * std::max(1, 32 / per_cu);
   }
@@ -359,6 +420,7 @@ void upf_batch_destroy(upf_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->dev) (void)hipFree(b->dev);
+  if (b->blk_chunk) (void)hipFree(b->blk_chunk);
   delete b;
 }
 
@@ -443,13 +505,14 @@ int upf_batch_autotune(upf_batch* b, float* d_field, void* d_ws, size_t ws_bytes
   upf_batch_opts best_o = start;
   float best_ms = 1e30f;
   int st = UPF_OK;
-  const int variants[2] = {UPF_EVAL_ROWS, UPF_EVAL_PER_LANE};
-  // 7 = the row variant's resident workgroups per CU (seven waves per SIMD), 8 the
-  // per-lane variant's: persistent waves take equal shares, so a grid of exactly the
-  // resident size has no tail of late waves
+  // the row form (one item per wave: blocks_per_cu has no effect on it), then the
+  // per-lane form on grids of 4, 6, 7 and 8 workgroups per CU (8 = its resident count:
+  // persistent waves take equal shares, so a grid of exactly the resident size has no
+  // tail of late waves)
   const int bpcs[4] = {4, 6, 7, 8};
-  for (int cand = 0; cand < 8 && !st; cand++) {
-    const upf_batch_opts o{variants[cand / 4], bpcs[cand % 4], start.classify_z, 0};
+  for (int cand = 0; cand < 5 && !st; cand++) {
+    const upf_batch_opts o = cand == 0 ? upf_batch_opts{UPF_EVAL_ROWS, start.blocks_per_cu, start.classify_z, 0}
+                                       : upf_batch_opts{UPF_EVAL_PER_LANE, bpcs[cand - 1], start.classify_z, 0};
     if ((st = upf_batch_set_opts(b, &o))) break;
     if ((st = run(b, d_field, d_ws, ws_bytes, d_pos, d_nrm, d_idx, cap_v, cap_t, d_meshes, s))) break;  // warm-up
     if (hipEventRecord(e0, s) != hipSuccess) { st = UPF_ERR_HIP; break; }
