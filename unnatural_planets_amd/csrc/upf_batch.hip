@@ -220,7 +220,9 @@ __global__ void k_batch_meshes(const BChunk* __restrict__ ch, int n_chunks, cons
   meshes[c].n_triangles = (int64_t)t1 - t0;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_emit(const BChunk* __restrict__ ch, int n_chunks,
+// At least five waves per SIMD, as k_mc_emit (the same pass).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_emit(
+                                                    const BChunk* __restrict__ ch, int n_chunks,
                                                     const float* __restrict__ field, const uint64_t* __restrict__ S,
                                                     const uint32_t* __restrict__ voff, const uint2* __restrict__ act,
                                                     const uint4* __restrict__ total, float* __restrict__ pos,

@@ -65,7 +65,10 @@ __global__ __launch_bounds__(256) void k_mc_signs(const float* __restrict__ f, c
   if (lane < kSignGroups && g0 + lane < d.gpr) S[row * d.gpr + g0 + lane] = mine;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_mc_emit(const float* __restrict__ f, const McDims d, const DGrid g,
+// At least five waves per SIMD (at most 102 VGPRs): the edge-id table's corners took the
+// kernel to 112 VGPRs and four waves, 0.224 against 0.211 ms stand-alone at 512^3.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_mc_emit(
+                                                 const float* __restrict__ f, const McDims d, const DGrid g,
                                                  float iso, const uint64_t* __restrict__ S,
                                                  const uint32_t* __restrict__ voff, const uint2* __restrict__ act,
                                                  const uint4* __restrict__ total, float* __restrict__ pos,
@@ -78,10 +81,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   // each wave takes a contiguous run of the active-group list (neighbouring groups share
   // sign words and field values in cache) in passes of up to kEmitPassGroups groups; the
   // grid is sized to keep the CUs full, the list's length is known only on the device.
-  // Counters at 512^3 (185,747 active groups): ~1,480 VALU instructions per pass, so the
-  // kernel is VALU-bound (49 M wave instructions, ~80 us of issue at four waves per
-  // SIMD); a software-pipelined variant that issued the next pass's record loads before
-  // the current pass's field reads ran no faster (112 against 106 us).
+  // At 512^3 (185,747 active groups) the passes issue ~32 M VALU wave instructions
+  // (~52 us of issue) in ~69 us: issue and the pass's dependent reads (records, then the
+  // vertices' field values) share the time.  A software-pipelined variant that issued
+  // the next pass's record loads before the current pass's field reads ran no faster.
   const int64_t nact = (int64_t)total->z, nw = (int64_t)gridDim.x * 4;
   const int64_t per = (nact + nw - 1) / nw;
   int64_t a = ((int64_t)blockIdx.x * 4 + wave) * per;
