@@ -76,6 +76,28 @@ def test_batch_with_compositions_matches_per_chunk(upf):
     _check(upf, b, ref)
 
 
+def test_batch_with_tiny_chunks_matches_per_chunk(upf):
+    """Chunks of tiny grids (2^3 has two field items) between ordinary ones, so that the
+    4 field items of one workgroup span up to four chunks."""
+    from unnatural_planets_amd.batch import Batch
+    sizes = [32, 2, 2, 3, 2, 4, 5, 2, 48, 2, 3, 9, 2, 2, 17]
+    shapes = ["sphere", "box", "torus", "capsule"]
+    chunks = (S.Chunk * len(sizes))()
+    for i, n in enumerate(sizes):
+        shape = shapes[i % 4]
+        chunks[i].spec = S.field_spec(shape, S.BATCH_SHAPE_PARAMS[shape],
+                                      [S.layer("gradient", "fbm", 3 + i % 4, 2.0, 2.0, 0.5, 0.05, 300 + i)])
+        chunks[i].grid = S.uniform_grid(-1.6, 1.6, n)
+        chunks[i].iso = 0.0
+    print("case: batch with tiny chunks", sizes, flush=True)
+    b = Batch(chunks)
+    b.size_outputs()
+    b.run()
+    ref = _reference(upf, chunks)
+    assert sum(m.n_triangles for _, m in ref) > 1000
+    _check(upf, b, ref)
+
+
 def test_batch_graph_replays_and_autotune(upf):
     import torch
 

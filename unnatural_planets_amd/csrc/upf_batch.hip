@@ -149,8 +149,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
 }
 
 // The row form with one field item per wave, as the single-grid row kernel runs: a
-// workgroup takes 4 consecutive items, blk_chunk[workgroup] is the chunk of its first
-// (a chunk has at least 16 items, so the 4 lie in that chunk or the next).  On the 17
+// workgroup takes 4 consecutive items, blk_chunk[workgroup] is the chunk of its first,
+// and a wave walks on from there past the chunks that end before its item (at most 3
+// steps: chunks of tiny grids hold fewer than 4 items, empty ones none).  On the 17
 // 192^3 chunks of the 64-chunk batch the persistent loop above ran 1.06-1.08 ms against
 // 0.83 ms for the single-grid kernel over the same chunks one by one (counters: ~23%
 // fewer VALU instructions issued per busy cycle).
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const int64_t r = (int64_t)blockIdx.x * 4 + wave;
   if (r >= n_items) return;  // whole waves
   int c = blk_chunk[blockIdx.x];
-  if (c + 1 < n_chunks && ch[c + 1].row_off <= r) c++;
+  while (c + 1 < n_chunks && ch[c + 1].row_off <= r) c++;
   c = uniform(c);
   const BChunk& C = ch[c];
   const int pairs = (C.grid.n[1] + 1) / 2;
