@@ -692,9 +692,8 @@ typedef struct upf_chunk {
 /* Launch knobs searched by upf_batch_autotune; every setting gives identical results. */
 typedef struct upf_batch_opts {
   int32_t field_variant;  /* upf_eval_variant                                  */
-  int32_t blocks_per_cu;  /* workgroups per compute unit of the per-lane field
-                             kernel's fixed grid (0 = 4); the row form runs
-                             one workgroup per 4 field items                */
+  int32_t blocks_per_cu;  /* accepted values 0 .. 32; no effect (the field pass
+                             launches one workgroup per 4 field items)      */
   int32_t classify_z;     /* accepted values 0, 16, 32, 64; no effect (the
                              classification reads the sign words the field
                              kernel stores, one lane per group)           */

@@ -4,9 +4,7 @@
 //
 // Work items, enumerated chunk after chunk:
 //   field     64 x nodes of a pair of grid rows            (chunk, xb, j / 2, k), which
-//             also stores the rows' sign words: one item per wave over all chunks'
-//             items (row form), or a fixed grid of blocks_per_cu workgroups per CU
-//             striding over them (per-lane form)
+//             also stores the rows' sign words: one item per wave over all chunks' items
 //   classify  one marching-cubes group per lane, from the sign words alone
 //   emit      packed passes over the active groups of the concatenated scan, on a
 //             fixed grid
@@ -51,8 +49,6 @@ __device__ __forceinline__ int find_chunk(const BChunk* __restrict__ ch, int n, 
   }
   return lo;
 }
-
-constexpr int kFieldSpan = 8;  // consecutive field items per wave step
 
 // One field item: nodes xb * 64 + lane of rows 2 jp and 2 jp + 1 (when it exists) of
 // layer k of chunk C, their values into the chunk's field and their sign words into S.
@@ -99,64 +95,18 @@ __device__ __forceinline__ void batch_field_item(const RowCtx& ctx, const BChunk
   }
 }
 
-// The per-lane form on a fixed grid of workgroups: each wave takes spans of kFieldSpan
-// consecutive items.  Seven waves per SIMD (at most 72 VGPRs).
-template <int VARIANT, bool EXT, bool COMPOSED = true>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_batch_field(
-    const BChunk* __restrict__ ch, int n_chunks, int64_t n_items, float* __restrict__ field, uint64_t* __restrict__ S) {
-  static_assert(VARIANT == UPF_EVAL_PER_LANE, "the row form runs one item per wave (k_batch_field_items)");
-  __shared__ float4 s_slice[4][kRowSlice];
-  __shared__ float4 s_gtab[16];
-  __shared__ float4 s_oct[4][kRowOctTab];
-  if (threadIdx.x < 16) s_gtab[threadIdx.x] = gradient_entry(threadIdx.x);
-  __syncthreads();
-  const int wave = uniform(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], lane);
-  // Each wave takes spans of kFieldSpan consecutive items, span after span with the
-  // grid's stride: the chunk is found once and then advanced (the item index only
-  // grows), and within a span the row pair is decoded once and then stepped.
-  const int64_t step = (int64_t)gridDim.x * 4 * kFieldSpan;
-  int64_t s0 = ((int64_t)blockIdx.x * 4 + wave) * kFieldSpan;
-  int c = s0 < n_items ? uniform(find_chunk<&BChunk::row_off>(ch, n_chunks, s0)) : 0;
-  for (; s0 < n_items; s0 += step) {
-    const int64_t s1 = min(s0 + kFieldSpan, n_items);
-    while (c + 1 < n_chunks && ch[c + 1].row_off <= s0) c++;
-    c = uniform(c);
-    int64_t r = s0;
-    while (r < s1) {
-      const BChunk& C = ch[c];
-      const int64_t e = c + 1 < n_chunks ? min(s1, ch[c + 1].row_off) : s1;  // the span's part in chunk c
-      const int pairs = (C.grid.n[1] + 1) / 2;
-      const uint32_t local = (uint32_t)(r - C.row_off);  // < 2^31: a chunk's groups fit 31 bits
-      int xb = (int)(local % (uint32_t)C.gx);
-      const uint32_t rest = local / (uint32_t)C.gx;
-      int jp = (int)(rest % (uint32_t)pairs);
-      int k = (int)(rest / (uint32_t)pairs);
-      for (; r < e; r++) {
-        batch_field_item<VARIANT, EXT, COMPOSED>(ctx, C, xb, jp, k, field, S);
-        if (++xb == C.gx) {
-          xb = 0;
-          if (++jp == pairs) {
-            jp = 0;
-            k++;
-          }
-        }
-      }
-      if (r < s1) c++;  // the span continues in the next chunk
-    }
-  }
-}
-
-// The row form with one field item per wave, as the single-grid row kernel runs: a
-// workgroup takes 4 consecutive items, blk_chunk[workgroup] is the chunk of its first,
-// and a wave walks on from there past the chunks that end before its item (at most 3
-// steps: chunks of tiny grids hold fewer than 4 items, empty ones none).  On the 17
-// 192^3 chunks of the 64-chunk batch the persistent loop above ran 1.06-1.08 ms against
-// 0.83 ms for the single-grid kernel over the same chunks one by one (counters: ~23%
-// fewer VALU instructions issued per busy cycle).
-template <bool COMPOSED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_batch_field_items(
+// The field pass: one field item per wave over all chunks' items, as the single-grid
+// kernels run.  A workgroup takes 4 consecutive items, blk_chunk[workgroup] is the chunk
+// of its first, and a wave walks on from there past the chunks that end before its item
+// (at most 3 steps: chunks of tiny grids hold fewer than 4 items, empty ones none).  A
+// persistent loop of blocks_per_cu workgroups per CU striding over spans of items ran
+// the row form at 1.06-1.08 ms on the 17 192^3 chunks of the 64-chunk batch against
+// 0.83 ms for the single-grid kernel over the same chunks one by one (20 B per lane
+// spilled in the loop); this launch takes the 64-chunk batch's row-form pass from 1.33
+// to 1.09 ms and its per-lane form from 2.52 to 2.23 ms.  Seven waves per SIMD as the
+// single-grid row kernel, six for the per-lane elevation modes and further shapes.
+template <int VARIANT, bool EXT, bool COMPOSED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXT ? 6 : 7))) void k_batch_field_items(
     const BChunk* __restrict__ ch, int n_chunks, const int32_t* __restrict__ blk_chunk, int64_t n_items,
     float* __restrict__ field, uint64_t* __restrict__ S) {
   __shared__ float4 s_slice[4][kRowSlice];
@@ -176,8 +126,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   const int xb = (int)(local % (uint32_t)C.gx);
   const uint32_t rest = local / (uint32_t)C.gx;
   const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], threadIdx.x & 63);
-  batch_field_item<UPF_EVAL_ROWS, false, COMPOSED>(ctx, C, xb, (int)(rest % (uint32_t)pairs),
-                                                   (int)(rest / (uint32_t)pairs), field, S);
+  batch_field_item<VARIANT, EXT, COMPOSED>(ctx, C, xb, (int)(rest % (uint32_t)pairs), (int)(rest / (uint32_t)pairs),
+                                           field, S);
 }
 
 // Counts of every group of every chunk (global id g: chunk, then its local group) from
@@ -311,18 +261,18 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
   uint2* act = reinterpret_cast<uint2*>(w + b->off_act);
   uint4* tiles = reinterpret_cast<uint4*>(w + b->off_tiles);
   uint4* total = reinterpret_cast<uint4*>(w + b->off_total);
-  const unsigned grid = (unsigned)(b->cus * b->opts.blocks_per_cu);
+  const dim3 items((unsigned)b->item_blocks);
   if (b->any_ext)
-    hipLaunchKernelGGL((k_batch_field<UPF_EVAL_PER_LANE, true>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
-                       field, S);
+    hipLaunchKernelGGL((k_batch_field_items<UPF_EVAL_PER_LANE, true, true>), items, dim3(256), 0, s, b->dev, b->n,
+                       b->blk_chunk, b->rows, field, S);
   else if (b->opts.field_variant == UPF_EVAL_PER_LANE)
-    hipLaunchKernelGGL((k_batch_field<UPF_EVAL_PER_LANE, false>), dim3(grid), dim3(256), 0, s, b->dev, b->n, b->rows,
-                       field, S);
+    hipLaunchKernelGGL((k_batch_field_items<UPF_EVAL_PER_LANE, false, true>), items, dim3(256), 0, s, b->dev, b->n,
+                       b->blk_chunk, b->rows, field, S);
   else if (b->any_composed)
-    hipLaunchKernelGGL((k_batch_field_items<true>), dim3((unsigned)b->item_blocks), dim3(256), 0, s, b->dev, b->n,
+    hipLaunchKernelGGL((k_batch_field_items<UPF_EVAL_ROWS, false, true>), items, dim3(256), 0, s, b->dev, b->n,
                        b->blk_chunk, b->rows, field, S);
   else
-    hipLaunchKernelGGL((k_batch_field_items<false>), dim3((unsigned)b->item_blocks), dim3(256), 0, s, b->dev, b->n,
+    hipLaunchKernelGGL((k_batch_field_items<UPF_EVAL_ROWS, false, false>), items, dim3(256), 0, s, b->dev, b->n,
                        b->blk_chunk, b->rows, field, S);
   hipLaunchKernelGGL(k_batch_classify, dim3((unsigned)((b->groups + 255) / 256)), dim3(256), 0, s, b->dev, b->n,
                      b->groups, S, counts);
@@ -508,14 +458,9 @@ int upf_batch_autotune(upf_batch* b, float* d_field, void* d_ws, size_t ws_bytes
   upf_batch_opts best_o = start;
   float best_ms = 1e30f;
   int st = UPF_OK;
-  // the row form (one item per wave: blocks_per_cu has no effect on it), then the
-  // per-lane form on grids of 4, 6, 7 and 8 workgroups per CU (8 = its resident count:
-  // persistent waves take equal shares, so a grid of exactly the resident size has no
-  // tail of late waves)
-  const int bpcs[4] = {4, 6, 7, 8};
-  for (int cand = 0; cand < 5 && !st; cand++) {
-    const upf_batch_opts o = cand == 0 ? upf_batch_opts{UPF_EVAL_ROWS, start.blocks_per_cu, start.classify_z, 0}
-                                       : upf_batch_opts{UPF_EVAL_PER_LANE, bpcs[cand - 1], start.classify_z, 0};
+  // the row form and the per-lane form (blocks_per_cu has no effect on either)
+  for (int cand = 0; cand < 2 && !st; cand++) {
+    const upf_batch_opts o{cand == 0 ? UPF_EVAL_ROWS : UPF_EVAL_PER_LANE, start.blocks_per_cu, start.classify_z, 0};
     if ((st = upf_batch_set_opts(b, &o))) break;
     if ((st = run(b, d_field, d_ws, ws_bytes, d_pos, d_nrm, d_idx, cap_v, cap_t, d_meshes, s))) break;  // warm-up
     if (hipEventRecord(e0, s) != hipSuccess) { st = UPF_ERR_HIP; break; }
