@@ -187,10 +187,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   ThetaAcc acc{th[wave]};
   const int gpr = (g.n[0] + 63) / 64;
   const int64_t stride = (int64_t)gridDim.x * 4;
-  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < items; r += stride) {
-    const int xb = (int)(r % gpr);
-    const int64_t row = r / gpr;
-    const int j = (int)(row % g.n[1]), k = (int)(row / g.n[1]);
+  // item r = (k * ny + j) * gpr + xb, decomposed once; each step adds the stride's own
+  // digits (sk, sj, sx) with carries instead of dividing r again
+  int64_t r = (int64_t)blockIdx.x * 4 + wave;
+  int xb = (int)(r % gpr), j = (int)((r / gpr) % g.n[1]), k = (int)(r / gpr / g.n[1]);
+  const int sx = (int)(stride % gpr), sj = (int)((stride / gpr) % g.n[1]), sk = (int)(stride / gpr / g.n[1]);
+  for (; r < items; r += stride) {
     const int i = xb * 64 + lane;
     const bool valid = i < g.n[0];
     const int ic = min(i, g.n[0] - 1);
@@ -205,6 +207,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       if (fout) fout[node] = f;
       if (dfdx) { dfdx[3 * node] = gr[0]; dfdx[3 * node + 1] = gr[1]; dfdx[3 * node + 2] = gr[2]; }
     }
+    xb += sx;
+    const int cx = xb >= gpr;
+    xb -= cx ? gpr : 0;
+    j += sj + cx;
+    const int cj = j >= g.n[1];
+    j -= cj ? g.n[1] : 0;
+    k += sk + cj;
   }
   if (THETA) {
     __syncthreads();
