@@ -193,17 +193,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   int xb = (int)(r % gpr), j = (int)((r / gpr) % g.n[1]), k = (int)(r / gpr / g.n[1]);
   const int sx = (int)(stride % gpr), sj = (int)((stride / gpr) % g.n[1]), sk = (int)(stride / gpr / g.n[1]);
   for (; r < items; r += stride) {
+    // the row's first node (uniform) and this lane's 32-bit offset in it
+    const int64_t row0 = ((int64_t)k * g.n[1] + j) * g.n[0];
     const int i = xb * 64 + lane;
     const bool valid = i < g.n[0];
     const int ic = min(i, g.n[0] - 1);
-    const int64_t node = ((int64_t)k * g.n[1] + j) * g.n[0] + ic;
     const float x = node_coord(g.o[0], g.s[0], ic);
     const float y = node_coord(g.o[1], g.s[1], j);
     const float z = node_coord(g.o[2], g.s[2], k);
-    const float w = (THETA && valid) ? dLdf[node] : 0.0f;
+    const float w = (THETA && valid) ? dLdf[row0 + ic] : 0.0f;
     float gr[3];
     const float f = row_field_grad<THETA>(c, s, x, y, z, w, gr, acc);
     if (valid) {
+      const int64_t node = row0 + ic;
       if (fout) fout[node] = f;
       if (dfdx) { dfdx[3 * node] = gr[0]; dfdx[3 * node + 1] = gr[1]; dfdx[3 * node + 2] = gr[2]; }
     }
