@@ -96,9 +96,12 @@ __device__ __forceinline__ void batch_field_item(const RowCtx& ctx, const BChunk
 }
 
 // The field pass: one field item per wave over all chunks' items, as the single-grid
-// kernels run.  A workgroup takes 4 consecutive items, blk_chunk[workgroup] is the chunk
-// of its first, and a wave walks on from there past the chunks that end before its item
-// (at most 3 steps: chunks of tiny grids hold fewer than 4 items, empty ones none).  A
+// kernels run.  A workgroup takes 4 consecutive items; blk_start[workgroup] holds its
+// first item's chunk, segment, row pair and plane (16 bits each), from which a wave
+// steps to its own item with carries.  An item in a later chunk than the first (chunks
+// of tiny grids hold fewer than 4 items, empty ones none) walks on to its chunk and is
+// decoded by division (two 32-bit divisions per item cost ~50 VALU instructions, ~9% of
+// an item of the 64-chunk batch).  A
 // persistent loop of blocks_per_cu workgroups per CU striding over spans of items ran
 // the row form at 1.06-1.08 ms on the 17 192^3 chunks of the 64-chunk batch against
 // 0.83 ms for the single-grid kernel over the same chunks one by one (20 B per lane
@@ -107,7 +110,7 @@ __device__ __forceinline__ void batch_field_item(const RowCtx& ctx, const BChunk
 // single-grid row kernel, six for the per-lane elevation modes and further shapes.
 template <int VARIANT, bool EXT, bool COMPOSED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXT ? 6 : 7))) void k_batch_field_items(
-    const BChunk* __restrict__ ch, int n_chunks, const int32_t* __restrict__ blk_chunk, int64_t n_items,
+    const BChunk* __restrict__ ch, int n_chunks, const uint64_t* __restrict__ blk_start, int64_t n_items,
     float* __restrict__ field, uint64_t* __restrict__ S) {
   __shared__ float4 s_slice[4][kRowSlice];
   __shared__ float4 s_gtab[16];
@@ -117,17 +120,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXT ? 6 : 7
   const int wave = uniform(threadIdx.x >> 6);
   const int64_t r = (int64_t)blockIdx.x * 4 + wave;
   if (r >= n_items) return;  // whole waves
-  int c = blk_chunk[blockIdx.x];
-  while (c + 1 < n_chunks && ch[c + 1].row_off <= r) c++;
+  const uint64_t st = blk_start[blockIdx.x];
+  int c = (int)(st & 0xFFFFu);
+  int xb = (int)((st >> 16) & 0xFFFFu), jp = (int)((st >> 32) & 0xFFFFu), k = (int)(st >> 48);
+  if (c + 1 < n_chunks && ch[c + 1].row_off <= r) {
+    do c++; while (c + 1 < n_chunks && ch[c + 1].row_off <= r);
+    c = uniform(c);
+    const BChunk& C = ch[c];
+    const uint32_t local = (uint32_t)(r - C.row_off);  // < 2^31: a chunk's groups fit 31 bits
+    const uint32_t pairs = (uint32_t)(C.grid.n[1] + 1) / 2, rest = local / (uint32_t)C.gx;
+    xb = (int)(local % (uint32_t)C.gx);
+    jp = (int)(rest % pairs);
+    k = (int)(rest / pairs);
+  } else {
+    const BChunk& C = ch[c];
+    const int pairs = (C.grid.n[1] + 1) / 2;
+    for (int w = 0; w < wave; w++)
+      if (++xb == C.gx) {
+        xb = 0;
+        if (++jp == pairs) {
+          jp = 0;
+          k++;
+        }
+      }
+  }
   c = uniform(c);
-  const BChunk& C = ch[c];
-  const int pairs = (C.grid.n[1] + 1) / 2;
-  const uint32_t local = (uint32_t)(r - C.row_off);  // < 2^31: a chunk's groups fit 31 bits
-  const int xb = (int)(local % (uint32_t)C.gx);
-  const uint32_t rest = local / (uint32_t)C.gx;
   const RowCtx ctx = make_row_ctx(s_slice[wave], s_gtab, s_oct[wave], threadIdx.x & 63);
-  batch_field_item<VARIANT, EXT, COMPOSED>(ctx, C, xb, (int)(rest % (uint32_t)pairs), (int)(rest / (uint32_t)pairs),
-                                           field, S);
+  batch_field_item<VARIANT, EXT, COMPOSED>(ctx, ch[c], uniform(xb), uniform(jp), uniform(k), field, S);
 }
 
 // Counts of every group of every chunk (global id g: chunk, then its local group) from
@@ -212,7 +231,7 @@ size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct upf_batch {
   std::vector<upf::BChunk> host;
   upf::BChunk* dev = nullptr;
-  int32_t* blk_chunk = nullptr;  // chunk of the first field item of each workgroup of 4 items
+  uint64_t* blk_start = nullptr;  // first field item of each workgroup of 4: chunk | xb << 16 | jp << 32 | k << 48
   int64_t item_blocks = 0;
   int device = 0;
   int n = 0;
@@ -264,16 +283,16 @@ int run(upf_batch* b, float* field, void* ws, size_t ws_bytes, float* pos, float
   const dim3 items((unsigned)b->item_blocks);
   if (b->any_ext)
     hipLaunchKernelGGL((k_batch_field_items<UPF_EVAL_PER_LANE, true, true>), items, dim3(256), 0, s, b->dev, b->n,
-                       b->blk_chunk, b->rows, field, S);
+                       b->blk_start, b->rows, field, S);
   else if (b->opts.field_variant == UPF_EVAL_PER_LANE)
     hipLaunchKernelGGL((k_batch_field_items<UPF_EVAL_PER_LANE, false, true>), items, dim3(256), 0, s, b->dev, b->n,
-                       b->blk_chunk, b->rows, field, S);
+                       b->blk_start, b->rows, field, S);
   else if (b->any_composed)
     hipLaunchKernelGGL((k_batch_field_items<UPF_EVAL_ROWS, false, true>), items, dim3(256), 0, s, b->dev, b->n,
-                       b->blk_chunk, b->rows, field, S);
+                       b->blk_start, b->rows, field, S);
   else
     hipLaunchKernelGGL((k_batch_field_items<UPF_EVAL_ROWS, false, false>), items, dim3(256), 0, s, b->dev, b->n,
-                       b->blk_chunk, b->rows, field, S);
+                       b->blk_start, b->rows, field, S);
   hipLaunchKernelGGL(k_batch_classify, dim3((unsigned)((b->groups + 255) / 256)), dim3(256), 0, s, b->dev, b->n,
                      b->groups, S, counts);
   UPF_LAUNCH_CHECK();
@@ -344,17 +363,21 @@ int upf_batch_create(const upf_chunk* chunks, int32_t n_chunks, const upf_batch_
     return UPF_ERR_HIP;
   }
   {
-    // workgroup -> chunk of its first field item, for the one-item-per-wave row kernel
+    // each workgroup's first field item (chunk, segment, row pair, plane; all < 2^16: at
+    // most 65536 chunks of at most 65535 nodes per axis), for the field kernel
     b->item_blocks = (b->rows + 3) / 4;
     if (b->item_blocks >= ((int64_t)1 << 31)) { upf_batch_destroy(b); return UPF_ERR_BAD_GRID; }
-    std::vector<int32_t> tab((size_t)b->item_blocks);
+    std::vector<uint64_t> tab((size_t)b->item_blocks);
     int c = 0;
     for (int64_t q = 0; q < b->item_blocks; q++) {
       while (c + 1 < n_chunks && b->host[c + 1].row_off <= 4 * q) c++;
-      tab[(size_t)q] = c;
+      const BChunk& C = b->host[c];
+      const int64_t local = 4 * q - C.row_off, pairs = (C.grid.n[1] + 1) / 2;
+      const uint64_t xb = (uint64_t)(local % C.gx), rest = (uint64_t)(local / C.gx);
+      tab[(size_t)q] = (uint64_t)c | xb << 16 | (rest % pairs) << 32 | (rest / pairs) << 48;
     }
-    if (hipMalloc(&b->blk_chunk, sizeof(int32_t) * tab.size()) != hipSuccess ||
-        hipMemcpy(b->blk_chunk, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMalloc(&b->blk_start, sizeof(uint64_t) * tab.size()) != hipSuccess ||
+        hipMemcpy(b->blk_start, tab.data(), sizeof(uint64_t) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
       upf_batch_destroy(b);
       return UPF_ERR_HIP;
     }
@@ -373,7 +396,7 @@ void upf_batch_destroy(upf_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->dev) (void)hipFree(b->dev);
-  if (b->blk_chunk) (void)hipFree(b->blk_chunk);
+  if (b->blk_start) (void)hipFree(b->blk_start);
   delete b;
 }
 
