@@ -12,8 +12,9 @@ __constant__ int8_t c_edge_owner[12][4] = {
     {0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
     {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
 
-// The same tables as compile-time data for the emit: the triangulation with rows padded
-// to 16 bytes (one 16-byte load per cell), and the edge owners to check the emit's order.
+// The same tables as compile-time data for the emits: the triangulation with rows padded
+// to 16 bytes (one 16-byte load per cell), and the edge owners (the id table's order
+// check, and a 64-bit immediate for the narrow band's emit).
 namespace mc_ct {
 #define UPF_MC_QUAL static constexpr
 #include "mc_table.inc"
@@ -42,8 +43,16 @@ constexpr bool owner_order_ok() {
       if (owner[e][q] != want[e][q]) return false;
   return true;
 }
+// the edge owners packed 5 bits per edge (dx | dy << 1 | dz << 2 | axis << 3)
+constexpr uint64_t pack_owner() {
+  uint64_t r = 0;
+  for (int e = 0; e < 12; e++)
+    r |= (uint64_t)(owner[e][0] | (owner[e][1] << 1) | (owner[e][2] << 2) | (owner[e][3] << 3)) << (5 * e);
+  return r;
+}
 }  // namespace mc_ct
 __constant__ __align__(16) mc_ct::Tris16 c_tris16 = mc_ct::pad_tris();
+constexpr uint64_t kEdgeOwner = mc_ct::pack_owner();
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 

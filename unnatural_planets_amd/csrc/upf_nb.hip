@@ -259,12 +259,15 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
                                                         uint64_t* __restrict__ keys, int64_t cap_v, int64_t cap_t) {
   __shared__ float s_f[kBrick];
   __shared__ uint16_t s_tab[kBrick];
+  __shared__ uint8_t s_ntris[256];  // triangles per case code (per-lane table reads stay in LDS)
   const int64_t a = blockIdx.x;
   const uint32_t b = act[a].x;
   int bx, by, bz;
   block_coords(d, b, bx, by, bz);
   const int gi0 = bx * kB, gj0 = by * kB, gk0 = bz * kB;
   const bool lx = bx == d.bx - 1, ly = by == d.by - 1, lz = bz == d.bz - 1;
+  static_assert(kNbThreads == 256, "one case code per thread");
+  s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
   for (int n = threadIdx.x; n < kBrick; n += kNbThreads) {
     s_f[n] = bricks[a * kBrick + n];
     s_tab[n] = tabs[a * kBrick + n];
@@ -317,20 +320,31 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
               ((s_f[n + kBN] < iso) << 3) | ((s_f[n + kBN * kBN] < iso) << 4) |
               ((s_f[n + kBN * kBN + 1] < iso) << 5) | ((s_f[n + kBN * kBN + kBN + 1] < iso) << 6) |
               ((s_f[n + kBN * kBN + kBN] < iso) << 7);
-    nt += upf_mc_ntris[code[q]];
+    nt += s_ntris[code[q]];
   }
   int ntot;
   uint32_t tid = toff[a] + (uint32_t)wg_exclusive_sum(nt, &ntot);
+  // the triangles' corners: the case's triangulation row in one 16-byte load, the edge
+  // owners from a 64-bit immediate (per-element reads of the constant tables at per-lane
+  // indices were a chain of dependent memory reads per triangle corner: 528 -> 213 us
+  // for this kernel without the whole section)
   for (int q = 0; q < 2; q++) {
     const int cc = 2 * threadIdx.x + q;
     const int i = cc & 7, j = (cc >> 3) & 7, k = cc >> 6;
-    const int ntc = upf_mc_ntris[code[q]];
-    for (int tr = 0; tr < ntc; tr++, tid++) {
+    const int ntc = s_ntris[code[q]];
+    if (!ntc) continue;
+    const int4 row = *reinterpret_cast<const int4*>(&c_tris16.v[code[q]][0]);
+    const uint32_t rw[4] = {(uint32_t)row.x, (uint32_t)row.y, (uint32_t)row.z, (uint32_t)row.w};
+#pragma unroll
+    for (int tr = 0; tr < UPF_MC_MAX_TRIS; tr++, tid++) {
+      if (tr >= ntc) break;
 #pragma unroll
       for (int v = 0; v < 3; v++) {
-        const int e = upf_mc_tris[code[q]][3 * tr + v];
-        const int li = i + c_edge_owner[e][0], lj = j + c_edge_owner[e][1], lk = k + c_edge_owner[e][2];
-        const int axis = c_edge_owner[e][3];
+        const int qq = 3 * tr + v;
+        const int e = (int)((rw[qq >> 2] >> (8 * (qq & 3))) & 0xFu);  // 0..11 on a triangle row
+        const int own = (int)((kEdgeOwner >> (5 * e)) & 31u);
+        const int li = i + (own & 1), lj = j + ((own >> 1) & 1), lk = k + ((own >> 2) & 1);
+        const int axis = own >> 3;
         const int ox = (li == kB && !lx), oy = (lj == kB && !ly), oz = (lk == kB && !lz);
         const int nn = ((lk - kB * oz) * kBN + (lj - kB * oy)) * kBN + (li - kB * ox);
         uint32_t id = 0xFFFFFFFFu;
