@@ -135,6 +135,8 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
   // the (row, octave) tables of the 4 waves, 2 * kBrickRows * kBrickGroups * oct_max float4
   // each (launch-sized: the spec's largest octave count sets the LDS, hence the occupancy)
   extern __shared__ float4 s_oct[];
+  __shared__ uint8_t s_ntris[256];  // triangles per case code
+  s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];  // read after the barriers below
   const int64_t a = blockIdx.x;
   const uint32_t b = act[a].x;
   int bx, by, bz;
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_fine(const DSpec s, const DGr
                      ((s_f[n + kBN] < iso) << 3) | ((s_f[n + kBN * kBN] < iso) << 4) |
                      ((s_f[n + kBN * kBN + 1] < iso) << 5) | ((s_f[n + kBN * kBN + kBN + 1] < iso) << 6) |
                      ((s_f[n + kBN * kBN + kBN] < iso) << 7);
-    nt += upf_mc_ntris[code];
+    nt += s_ntris[code];
   }
   int ntot;
   (void)wg_exclusive_sum(nt, &ntot);
