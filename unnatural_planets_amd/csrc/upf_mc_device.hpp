@@ -7,11 +7,6 @@
 #include "mc_table.inc"
 #undef UPF_MC_QUAL
 
-// edge -> (owner corner offset x, y, z, axis)
-__constant__ int8_t c_edge_owner[12][4] = {
-    {0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
-    {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
-
 // The same tables as compile-time data for the emits: the triangulation with rows padded
 // to 16 bytes (one 16-byte load per cell), and the edge owners (the id table's order
 // check, and a 64-bit immediate for the narrow band's emit).
