@@ -217,6 +217,25 @@ def test_texture_post_dilation_and_u8_bit_exact(upf, oracle):
     assert int(special[..., 1].max()) == 0  # metallic
 
 
+@pytest.mark.parametrize("w,h,c", [(1, 1, 1), (1, 7, 2), (9, 1, 3), (65, 5, 4), (333, 200, 3), (100, 37, 1),
+                                   (64, 64, 2)])
+def test_dilation_rounds_and_shapes_bit_exact(upf, oracle, w, h, c):
+    """Dilation of random images with large NaN regions (fills travel the whole radius)
+    against the oracle, bit for bit: rounds within one launch (1, 6, 7), across two and
+    three (8, 14, 15), on tiles cut by the image's edges."""
+    rng = np.random.default_rng(w * 7919 + h * 31 + c)
+    img = rng.standard_normal((h, w, c)).astype(np.float32)
+    # NaN everywhere except sparse seeds and a few blobs
+    mask = rng.random((h, w)) < 0.97
+    img[mask] = np.nan
+    for rounds in (1, 6, 7, 8, 14, 15):
+        print("case: dilate", w, h, c, rounds, flush=True)
+        ref = oracle.image_dilate(img.copy(), rounds)
+        got = upfm.image_dilate(torch_from(img), rounds).cpu().numpy()
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+        np.testing.assert_array_equal(got[~np.isnan(got)], ref[~np.isnan(ref)])
+
+
 def torch_from(a):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()

@@ -110,7 +110,9 @@ def test_side_stream_calls_match_current_stream(upf):
     img = torch.full((64, 64, 3), float("nan"), device="cuda")
     img[20:40, 20:40] = 0.5
     ref_img = upf.image_dilate(img.clone(), 7)
-    got = upf.image_dilate(img.clone(), 7, stream=side)
+    src = img.clone()  # made on the current stream: the side stream waits for it
+    side.wait_stream(torch.cuda.current_stream())
+    got = upf.image_dilate(src, 7, stream=side)
     side.synchronize()
     assert torch.equal(torch.nan_to_num(got, -1.0), torch.nan_to_num(ref_img, -1.0))
     pts = torch.from_numpy(np.random.default_rng(1).uniform(-1.2, 1.2, (5000, 3)).astype(np.float32)).cuda()

@@ -367,39 +367,106 @@ __global__ __launch_bounds__(256) void k_bake(const DBake b, const float* __rest
   bump[o] = sat(0.5f + b.bump[0] * d + b.bump[1] * elev);
 }
 
-// One dilation pass (layout [h][w][C], validity by channel 0).
+// Up to kDilR dilation rounds in one launch (layout [h][w][C], validity by channel 0).
+// A workgroup loads its kDilT x kDilT tile with a kDilR-texel halo into LDS and runs the
+// rounds there: in round k a texel whose first channel is still NaN takes, per channel,
+// the mean of its 8 neighbours valid before round k (summed in the order dy = -1..1,
+// dx = -1..1), as one pass over the whole image would.  Valid texels never change and a
+// texel is filled at most once, so one copy of the values and a tag per texel suffice:
+// tag 0 = valid from the start, k = filled in round k (so not yet visible to round k's
+// other texels), kTagNone = still NaN, kTagOut = outside the image.  Round k computes
+// the texels at least k from the loaded border, whose neighbours are right after round
+// k - 1; the tile itself lies kDilR in.  One read of the tile and its halo and one write
+// of the tile, instead of a read and a write of the whole image per round.
+constexpr int kDilT = 32;                   // output tile side
+constexpr int kDilR = 7;                    // rounds per launch (the halo)
+constexpr int kDilS = kDilT + 2 * kDilR;    // loaded side
+constexpr uint8_t kTagOut = 254, kTagNone = 255;
+
+// only_nan: write only the texels that were NaN when loaded (the last launch of an
+// in-place dilation; k_dilate_merge then copies them back).
 template <int C>
-__global__ __launch_bounds__(256) void k_dilate(const float* __restrict__ in, float* __restrict__ out, int w, int h) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (x >= w || y >= h) return;
-  const int64_t o = ((int64_t)y * w + x) * C;
-  const float v0 = in[o];
-  if (!isnan(v0)) {
+__global__ __launch_bounds__(256) void k_dilate(const float* __restrict__ in, float* __restrict__ out, int w, int h,
+                                                int rounds, bool only_nan) {
+  __shared__ float s_v[C][kDilS * kDilS];
+  __shared__ uint8_t s_tag[kDilS * kDilS];
+  __shared__ uint16_t s_nan[kDilS * kDilS];  // the loaded texels that are NaN (only they can change)
+  __shared__ int s_nnan;
+  const int x0 = (int)blockIdx.x * kDilT - kDilR, y0 = (int)blockIdx.y * kDilT - kDilR;
+  if (threadIdx.x == 0) s_nnan = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (int t0 = 0; t0 < kDilS * kDilS; t0 += 256) {  // uniform trip count: whole waves ballot
+    const int t = t0 + (int)threadIdx.x;
+    bool nan_here = false;
+    if (t < kDilS * kDilS) {
+      const int ly = t / kDilS, lx = t - ly * kDilS;
+      const int x = x0 + lx, y = y0 + ly;
+      uint8_t tag = kTagOut;
+      if (x >= 0 && x < w && y >= 0 && y < h) {
+        const int64_t o = ((int64_t)y * w + x) * C;
 #pragma unroll
-    for (int ch = 0; ch < C; ch++) out[o + ch] = in[o + ch];
-    return;
+        for (int ch = 0; ch < C; ch++) s_v[ch][t] = in[o + ch];
+        nan_here = isnan(s_v[0][t]);
+        tag = nan_here ? kTagNone : 0;
+      }
+      s_tag[t] = tag;
+    }
+    const uint64_t m = __ballot(nan_here);
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(&s_nnan, __popcll(m));
+    base = __shfl(base, 0, 64);
+    if (nan_here) s_nan[base + __popcll(m & ((1ull << lane) - 1))] = (uint16_t)t;
   }
-  float sum[C];
+  __syncthreads();
+  const int nnan = s_nnan;
+  for (int k = 1; k <= rounds; k++) {
+    for (int e = threadIdx.x; e < nnan; e += 256) {
+      const int idx = s_nan[e];
+      const int ly = idx / kDilS, lx = idx - ly * kDilS;  // constant divisor
+      if (lx < k || lx >= kDilS - k || ly < k || ly >= kDilS - k || s_tag[idx] != kTagNone) continue;
+      float sum[C];
 #pragma unroll
-  for (int ch = 0; ch < C; ch++) sum[ch] = 0.0f;
-  int cnt = 0;
-  for (int dy = -1; dy <= 1; dy++) {
-    const int yy = y + dy;
-    if (yy < 0 || yy >= h) continue;
-    for (int dx = -1; dx <= 1; dx++) {
-      const int xx = x + dx;
-      if ((dx == 0 && dy == 0) || xx < 0 || xx >= w) continue;
-      const int64_t q = ((int64_t)yy * w + xx) * C;
-      if (isnan(in[q])) continue;
+      for (int ch = 0; ch < C; ch++) sum[ch] = 0.0f;
+      int cnt = 0;
 #pragma unroll
-      for (int ch = 0; ch < C; ch++) sum[ch] = sum[ch] + in[q + ch];
-      cnt++;
+      for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          if (dx == 0 && dy == 0) continue;
+          const int q = idx + dy * kDilS + dx;
+          if (s_tag[q] >= k) continue;  // NaN, outside, or filled in this round
+#pragma unroll
+          for (int ch = 0; ch < C; ch++) sum[ch] = sum[ch] + s_v[ch][q];
+          cnt++;
+        }
+      if (cnt) {
+#pragma unroll
+        for (int ch = 0; ch < C; ch++) s_v[ch][idx] = sum[ch] / (float)cnt;
+        s_tag[idx] = (uint8_t)k;
+      }
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < kDilT * kDilT; t += 256) {
+    const int ly = kDilR + t / kDilT, lx = kDilR + t % kDilT;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x < w && y < h && !(only_nan && s_tag[ly * kDilS + lx] == 0)) {
+      const int64_t o = ((int64_t)y * w + x) * C;
+#pragma unroll
+      for (int ch = 0; ch < C; ch++) out[o + ch] = s_v[ch][ly * kDilS + lx];
     }
   }
-  const float nan = __builtin_nanf("");
+}
+
+// img takes the values k_dilate wrote into filled (only_nan) where its first channel is NaN.
+template <int C>
+__global__ __launch_bounds__(256) void k_dilate_merge(float* __restrict__ img, const float* __restrict__ filled,
+                                                      int64_t texels) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= texels || !isnan(img[t * C])) return;
 #pragma unroll
-  for (int ch = 0; ch < C; ch++) out[o + ch] = cnt ? sum[ch] / (float)cnt : nan;
+  for (int ch = 0; ch < C; ch++) img[t * C + ch] = filled[t * C + ch];
 }
 
 __global__ __launch_bounds__(256) void k_copy_f32(const float* __restrict__ in, float* __restrict__ out, int64_t n) {
@@ -578,20 +645,37 @@ int upf_image_dilate(float* d_img, float* d_scratch, int32_t w, int32_t h, int32
   if (!d_img || !d_scratch) return UPF_ERR_NULL;
   hipStream_t s = as_stream(stream);
   const int64_t n = (int64_t)w * h * channels;
+  // launches of up to kDilR rounds each, alternating between d_img and d_scratch; the
+  // last one reads d_img and writes only the texels that were NaN into d_scratch, which
+  // k_dilate_merge copies back.  An odd number of launches before it starts from a copy
+  // in d_scratch, so that they end in d_img.
+  const int launches = (rounds + kDilR - 1) / kDilR;
   float* a = d_img;
   float* b = d_scratch;
-  if (rounds & 1) {  // an odd number of passes starts from the scratch copy and ends in d_img
+  if ((launches - 1) & 1) {
     hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_img, d_scratch, n);
     a = d_scratch;
     b = d_img;
   }
-  const dim3 grid((w + 63) / 64, (h + 3) / 4);
-  for (int r = 0; r < rounds; r++) {
+  const dim3 grid((unsigned)((w + kDilT - 1) / kDilT), (unsigned)((h + kDilT - 1) / kDilT));
+  const int64_t texels = (int64_t)w * h;
+  const dim3 mgrid((unsigned)((texels + 255) / 256));
+  for (int left = rounds; left > 0; left -= kDilR) {
+    const int r = left < kDilR ? left : kDilR;
+    const bool last = left <= kDilR;  // then a == d_img
     switch (channels) {
-      case 1: hipLaunchKernelGGL(k_dilate<1>, grid, dim3(256), 0, s, a, b, w, h); break;
-      case 2: hipLaunchKernelGGL(k_dilate<2>, grid, dim3(256), 0, s, a, b, w, h); break;
-      case 3: hipLaunchKernelGGL(k_dilate<3>, grid, dim3(256), 0, s, a, b, w, h); break;
-      default: hipLaunchKernelGGL(k_dilate<4>, grid, dim3(256), 0, s, a, b, w, h); break;
+      case 1: hipLaunchKernelGGL(k_dilate<1>, grid, dim3(256), 0, s, a, b, w, h, r, last); break;
+      case 2: hipLaunchKernelGGL(k_dilate<2>, grid, dim3(256), 0, s, a, b, w, h, r, last); break;
+      case 3: hipLaunchKernelGGL(k_dilate<3>, grid, dim3(256), 0, s, a, b, w, h, r, last); break;
+      default: hipLaunchKernelGGL(k_dilate<4>, grid, dim3(256), 0, s, a, b, w, h, r, last); break;
+    }
+    if (last) {
+      switch (channels) {
+        case 1: hipLaunchKernelGGL(k_dilate_merge<1>, mgrid, dim3(256), 0, s, a, b, texels); break;
+        case 2: hipLaunchKernelGGL(k_dilate_merge<2>, mgrid, dim3(256), 0, s, a, b, texels); break;
+        case 3: hipLaunchKernelGGL(k_dilate_merge<3>, mgrid, dim3(256), 0, s, a, b, texels); break;
+        default: hipLaunchKernelGGL(k_dilate_merge<4>, mgrid, dim3(256), 0, s, a, b, texels); break;
+      }
     }
     float* t = a;
     a = b;
