@@ -236,6 +236,26 @@ def test_dilation_rounds_and_shapes_bit_exact(upf, oracle, w, h, c):
         np.testing.assert_array_equal(got[~np.isnan(got)], ref[~np.isnan(ref)])
 
 
+def test_to_u8_lengths_and_alignment_bit_exact(upf, oracle):
+    """U8 conversion four values per thread with a scalar tail, and the scalar path for
+    an input that is not 16-byte aligned, against the oracle: NaN, infinities, values
+    outside [0, 1] and exact half steps included."""
+    rng = np.random.default_rng(11)
+    base = rng.uniform(-0.5, 1.5, 4099).astype(np.float32)
+    base[::7] = np.nan
+    base[1::11] = np.inf
+    base[2::13] = -np.inf
+    base[3::17] = (np.arange(len(base[3::17])) % 256 + 0.5) / 255.0
+    for n in (1, 3, 4, 5, 1023, 4096, 4099):
+        a = base[:n]
+        np.testing.assert_array_equal(upfm.image_to_u8(torch_from(a)).cpu().numpy(), oracle.image_to_u8(a))
+    t = torch_from(base)
+    for off in (1, 2, 3):  # views that start 4, 8 and 12 bytes into the buffer
+        v = t[off:]
+        assert v.data_ptr() % 16 != 0
+        np.testing.assert_array_equal(upfm.image_to_u8(v).cpu().numpy(), oracle.image_to_u8(base[off:]))
+
+
 def torch_from(a):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
