@@ -382,11 +382,12 @@ constexpr int kDilT = 32;                   // output tile side
 constexpr int kDilR = 7;                    // rounds per launch (the halo)
 constexpr int kDilS = kDilT + 2 * kDilR;    // loaded side
 constexpr uint8_t kTagOut = 254, kTagNone = 255;
+constexpr int kDilThreads = 512;  // 8 waves share a tile's rounds
 
 // only_nan: write only the texels that were NaN when loaded (the last launch of an
 // in-place dilation; k_dilate_merge then copies them back).
 template <int C>
-__global__ __launch_bounds__(256) void k_dilate(const float* __restrict__ in, float* __restrict__ out, int w, int h,
+__global__ __launch_bounds__(kDilThreads) void k_dilate(const float* __restrict__ in, float* __restrict__ out, int w, int h,
                                                 int rounds, bool only_nan) {
   __shared__ float s_v[C][kDilS * kDilS];
   __shared__ uint8_t s_tag[kDilS * kDilS];
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(256) void k_dilate(const float* __restrict__ in, fl
   if (threadIdx.x == 0) s_nnan = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  for (int t0 = 0; t0 < kDilS * kDilS; t0 += 256) {  // uniform trip count: whole waves ballot
+  for (int t0 = 0; t0 < kDilS * kDilS; t0 += kDilThreads) {  // uniform trip count: whole waves ballot
     const int t = t0 + (int)threadIdx.x;
     bool nan_here = false;
     if (t < kDilS * kDilS) {
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(256) void k_dilate(const float* __restrict__ in, fl
   __syncthreads();
   const int nnan = s_nnan;
   for (int k = 1; k <= rounds; k++) {
-    for (int e = threadIdx.x; e < nnan; e += 256) {
+    for (int e = threadIdx.x; e < nnan; e += kDilThreads) {
       const int idx = s_nan[e];
       const int ly = idx / kDilS, lx = idx - ly * kDilS;  // constant divisor
       if (lx < k || lx >= kDilS - k || ly < k || ly >= kDilS - k || s_tag[idx] != kTagNone) continue;
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(256) void k_dilate(const float* __restrict__ in, fl
     }
     __syncthreads();
   }
-  for (int t = threadIdx.x; t < kDilT * kDilT; t += 256) {
+  for (int t = threadIdx.x; t < kDilT * kDilT; t += kDilThreads) {
     const int ly = kDilR + t / kDilT, lx = kDilR + t % kDilT;
     const int x = x0 + lx, y = y0 + ly;
     if (x < w && y < h && !(only_nan && s_tag[ly * kDilS + lx] == 0)) {
@@ -677,10 +678,10 @@ int upf_image_dilate(float* d_img, float* d_scratch, int32_t w, int32_t h, int32
     const int r = left < kDilR ? left : kDilR;
     const bool last = left <= kDilR;  // then a == d_img
     switch (channels) {
-      case 1: hipLaunchKernelGGL(k_dilate<1>, grid, dim3(256), 0, s, a, b, w, h, r, last); break;
-      case 2: hipLaunchKernelGGL(k_dilate<2>, grid, dim3(256), 0, s, a, b, w, h, r, last); break;
-      case 3: hipLaunchKernelGGL(k_dilate<3>, grid, dim3(256), 0, s, a, b, w, h, r, last); break;
-      default: hipLaunchKernelGGL(k_dilate<4>, grid, dim3(256), 0, s, a, b, w, h, r, last); break;
+      case 1: hipLaunchKernelGGL(k_dilate<1>, grid, dim3(kDilThreads), 0, s, a, b, w, h, r, last); break;
+      case 2: hipLaunchKernelGGL(k_dilate<2>, grid, dim3(kDilThreads), 0, s, a, b, w, h, r, last); break;
+      case 3: hipLaunchKernelGGL(k_dilate<3>, grid, dim3(kDilThreads), 0, s, a, b, w, h, r, last); break;
+      default: hipLaunchKernelGGL(k_dilate<4>, grid, dim3(kDilThreads), 0, s, a, b, w, h, r, last); break;
     }
     if (last) {
       switch (channels) {
