@@ -262,6 +262,8 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
   __shared__ float s_f[kBrick];
   __shared__ uint16_t s_tab[kBrick];
   __shared__ uint8_t s_ntris[256];  // triangles per case code (per-lane table reads stay in LDS)
+  __shared__ int32_t s_nrow[8];     // upper neighbour (dx | dy << 1 | dz << 2): its active row, or -1
+  __shared__ uint32_t s_nvoff[8];   // and its first vertex id
   const int64_t a = blockIdx.x;
   const uint32_t b = act[a].x;
   int bx, by, bz;
@@ -270,12 +272,20 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
   const bool lx = bx == d.bx - 1, ly = by == d.by - 1, lz = bz == d.bz - 1;
   static_assert(kNbThreads == 256, "one case code per thread");
   s_ntris[threadIdx.x] = upf_mc_ntris[threadIdx.x];
+  // threads 1..7 look up the upper neighbours that own the vertices of the brick's upper
+  // faces: the rank read overlaps the brick's reads, the neighbour's first vertex id the
+  // vertex section; both are published before the triangle section's barrier
+  const int no = (int)threadIdx.x;
+  const bool nlook = no >= 1 && no < 8 && !((no & 1) && lx) && !((no & 2) && ly) && !((no & 4) && lz);
+  const int32_t nrow =
+      nlook ? map[(int64_t)b + (no & 1) + (int64_t)((no >> 1) & 1) * d.bx + (int64_t)(no >> 2) * d.bx * d.by] : -1;
   for (int n = threadIdx.x; n < kBrick; n += kNbThreads) {
     s_f[n] = bricks[a * kBrick + n];
     s_tab[n] = tabs[a * kBrick + n];
   }
   __syncthreads();
   const uint32_t vb = voff[a];
+  const uint32_t nvoff = nrow >= 0 ? voff[nrow] : 0u;
 
   // ---- vertices of the owned nodes 3t .. 3t+2 (normals follow in k_nb_normals)
   for (int q = 0; q < 3; q++) {
@@ -311,7 +321,12 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
     }
   }
 
-  // ---- triangles of cells 2t, 2t+1
+  if (no >= 1 && no < 8) {
+    s_nrow[no] = nrow;
+    s_nvoff[no] = nvoff;
+  }
+
+  // ---- triangles of cells 2t, 2t+1 (wg_exclusive_sum's barrier publishes s_nrow)
   int code[2], nt = 0;
 #pragma unroll
   for (int q = 0; q < 2; q++) {
@@ -354,11 +369,11 @@ __global__ __launch_bounds__(kNbThreads) void k_nb_emit(const DSpec s, const DGr
           const int entry = s_tab[nn];
           id = vb + (uint32_t)(entry & 0xFFF) + (uint32_t)__popc((entry >> 12) & ((1 << axis) - 1));
         } else {
-          const int64_t nb = (int64_t)b + ox + (int64_t)oy * d.bx + (int64_t)oz * d.bx * d.by;
-          const int32_t rnb = map[nb];
+          const int o = ox | (oy << 1) | (oz << 2);
+          const int32_t rnb = s_nrow[o];
           if (rnb >= 0) {
             const int entry = tabs[(int64_t)rnb * kBrick + nn];
-            id = voff[rnb] + (uint32_t)(entry & 0xFFF) + (uint32_t)__popc((entry >> 12) & ((1 << axis) - 1));
+            id = s_nvoff[o] + (uint32_t)(entry & 0xFFF) + (uint32_t)__popc((entry >> 12) & ((1 << axis) - 1));
           }
         }
         if ((int64_t)tid < cap_t) idx[3 * (size_t)tid + v] = id;
