@@ -171,11 +171,25 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(uint4* __restrict__ s
   if (threadIdx.x == 0) *total = carry;
 }
 
-template <int FMT>
+// OWN_PREFIX: tiles holds the tiles' sums (not yet scanned) and each workgroup adds up
+// those before its own tile itself (at most kScanOwnPrefixTiles of them, L2-resident),
+// the last one writing the grand total -- no separate top-level scan launch.  Otherwise
+// tiles holds the tiles' exclusive offsets from k_scan_top.
+constexpr int64_t kScanOwnPrefixTiles = 4096;
+
+template <int FMT, bool OWN_PREFIX>
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ counts, int64_t n,
-                                                             const uint4* __restrict__ tile_off,
+                                                             const uint4* __restrict__ tiles,
                                                              uint32_t* __restrict__ voff, uint2* __restrict__ act,
-                                                             uint32_t* __restrict__ toff) {
+                                                             uint32_t* __restrict__ toff, uint4* __restrict__ grand) {
+  uint4 before;
+  if (OWN_PREFIX) {
+    uint4 p = make_uint4(0, 0, 0, 0);
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += kScanThreads) p = add4(p, tiles[t]);
+    (void)block_exclusive_scan(p, &before);
+  } else {
+    before = tiles[blockIdx.x];
+  }
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   uint4 v[kScanItems];
   uint4 s = make_uint4(0, 0, 0, 0);
@@ -185,7 +199,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __r
     s = add4(s, v[t]);
   }
   uint4 total;
-  uint4 ex = add4(block_exclusive_scan(s, &total), tile_off[blockIdx.x]);
+  uint4 ex = add4(block_exclusive_scan(s, &total), before);
+  if (OWN_PREFIX && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *grand = add4(before, total);
 #pragma unroll
   for (int t = 0; t < kScanItems; t++) {
     if (base + t < n) {
@@ -269,13 +284,23 @@ int mc_scan_launch(const uint32_t* counts, int64_t groups, uint4* tiles, uint4* 
   else
     hipLaunchKernelGGL(k_scan_tiles<kScanBlocks>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups,
                        tiles);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tiles, ntiles, total);
-  if (fmt == kScanMc)
-    hipLaunchKernelGGL(k_scan_apply<kScanMc>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups, tiles,
-                       voff, act, toff);
-  else
-    hipLaunchKernelGGL(k_scan_apply<kScanBlocks>, dim3((unsigned)ntiles), dim3(kScanThreads), 0, s, counts, groups,
-                       tiles, voff, act, toff);
+  const dim3 grid((unsigned)ntiles);
+  if (ntiles <= kScanOwnPrefixTiles) {
+    if (fmt == kScanMc)
+      hipLaunchKernelGGL((k_scan_apply<kScanMc, true>), grid, dim3(kScanThreads), 0, s, counts, groups, tiles, voff,
+                         act, toff, total);
+    else
+      hipLaunchKernelGGL((k_scan_apply<kScanBlocks, true>), grid, dim3(kScanThreads), 0, s, counts, groups, tiles,
+                         voff, act, toff, total);
+  } else {
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, tiles, ntiles, total);
+    if (fmt == kScanMc)
+      hipLaunchKernelGGL((k_scan_apply<kScanMc, false>), grid, dim3(kScanThreads), 0, s, counts, groups, tiles, voff,
+                         act, toff, total);
+    else
+      hipLaunchKernelGGL((k_scan_apply<kScanBlocks, false>), grid, dim3(kScanThreads), 0, s, counts, groups, tiles,
+                         voff, act, toff, total);
+  }
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
