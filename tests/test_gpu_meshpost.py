@@ -69,3 +69,67 @@ def test_mesh_post_edge_cases(upf):
     m = upfm.mesh_remove_disconnected(upfm.Mesh(pos, pos.clone(), idx))
     assert m.n_vertices == 4 and m.n_triangles == 2
     assert torch.equal(m.indices.cpu(), torch.tensor([[0, 1, 2], [2, 1, 3]], dtype=torch.int32))
+
+
+def _soup(rng, nv, nt):
+    import torch
+    idx = rng.integers(0, nv, size=(nt, 3)).astype(np.int32)
+    pos = rng.random((nv, 3)).astype(np.float32)
+    return (upfm.Mesh(torch.from_numpy(pos).cuda(), torch.from_numpy(pos * 2).cuda(),
+                      torch.from_numpy(idx).cuda()), pos, idx.view(np.uint32))
+
+
+@pytest.mark.parametrize("nv,nt,seed", [(50, 20, 1), (5000, 1500, 2), (200000, 90000, 3),
+                                        (300000, 600000, 4), (1 << 20, 1 << 19, 5)])
+def test_remove_disconnected_random_soups(upf, oracle, nv, nt, seed):
+    """Random triangles over random vertex ids: from many small components (ties between
+    equal triangle counts included) to one giant component, with edges joined in an order
+    unrelated to vertex ids, against the oracle's keep flags."""
+    m, pos, idx = _soup(np.random.default_rng(seed), nv, nt)
+    kv, kt = oracle.remove_disconnected_flags(nv, idx)
+    ref = compact_mesh(pos, pos * 2, idx, kv, kt)
+    got = _np(upfm.mesh_remove_disconnected(m))
+    for a, b in zip(got, ref):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_remove_disconnected_long_chain(upf, oracle):
+    """A strip of 2 x 100000 vertices whose triangles run from the far end back to vertex 0,
+    so every link joins a long tree to the next one: one component, all kept."""
+    import torch
+    n = 100000
+    tris = []
+    for i in reversed(range(n - 1)):
+        tris += [(2 * i, 2 * i + 1, 2 * i + 2), (2 * i + 1, 2 * i + 3, 2 * i + 2)]
+    idx = np.array(tris, dtype=np.int32)
+    pos = np.random.default_rng(9).random((2 * n + 5, 3)).astype(np.float32)  # 5 unreferenced
+    m = upfm.Mesh(torch.from_numpy(pos).cuda(), None, torch.from_numpy(idx).cuda())
+    kv, kt = oracle.remove_disconnected_flags(len(pos), idx.view(np.uint32))
+    assert kt.all() and kv.sum() == 2 * n
+    got = upfm.mesh_remove_disconnected(m)
+    ref = compact_mesh(pos, None, idx.view(np.uint32), kv, kt)
+    np.testing.assert_array_equal(got.positions.cpu().numpy(), ref[0])
+    np.testing.assert_array_equal(got.indices.cpu().numpy().view(np.uint32), ref[2])
+
+
+def test_mesh_post_capacity_error_leaves_output_untouched(upf):
+    """An output too small for the kept mesh: UPF_ERR_CAPACITY with the counts, and the
+    output buffers are not written."""
+    import ctypes
+    import torch
+    s, g = islands_spec(), S.uniform_grid(-1.5, 1.5, 64)
+    m = upfm.mesh_generate_generic(s, g, remove_disconnected=False)
+    full = upfm.mesh_remove_disconnected(m)
+    nbytes = int(upfm.lib().upf_mesh_post_workspace_bytes(m.n_vertices, m.n_triangles))
+    ws = torch.empty((nbytes,), dtype=torch.uint8, device="cuda")
+    for cap_v, cap_t in ((full.n_vertices - 1, full.n_triangles), (full.n_vertices, full.n_triangles - 1)):
+        pos = torch.full((m.n_vertices, 3), 7.0, device="cuda")
+        idx = torch.full((m.n_triangles, 3), -3, dtype=torch.int32, device="cuda")
+        mb = S.MeshBuffers()
+        mb.positions, mb.normals, mb.indices = pos.data_ptr(), None, idx.data_ptr()
+        mb.cap_vertices, mb.cap_triangles = cap_v, cap_t
+        st = upfm.lib().upf_mesh_remove_disconnected(ctypes.byref(upfm._mesh_buffers(m)), ctypes.byref(mb),
+                                                     ctypes.c_void_p(ws.data_ptr()), nbytes, None)
+        assert st == S.ERR_CAPACITY
+        assert (mb.n_vertices, mb.n_triangles) == (full.n_vertices, full.n_triangles)
+        assert bool((pos == 7.0).all()) and bool((idx == -3).all())

@@ -2,14 +2,19 @@
 // connected component (meshRemoveDisconnected, sources/meshGeneration.cpp:41) and the
 // water-mesh vertex mask with its two-ring expansion (sources/meshGeneration.cpp:62-95).
 //
-// Connected components by label propagation over the triangles' edges: every vertex
-// starts with its own id as label; each pass hooks the larger of two differing labels
-// of an edge to the smaller (atomicMin) and then halves every label chain (pointer
-// jumping), until a pass changes nothing.  The final label of a vertex is the smallest
-// vertex id of its component, whatever the order the passes ran in.  The kept component
-// is the one with the most triangles (ties: the smaller label).  Both operations end in
-// one order-preserving compaction driven by the shared scan: vertex i keeps its order
-// among the kept vertices, triangle t among the kept triangles.
+// Connected components by a lock-free union-find over the triangles' edges (the
+// compare-and-swap linking of ECL-CC, Jaiganesh and Burtscher, HPDC 2018): every vertex
+// starts as its own root; an edge finds the roots of its two ends (halving the paths it
+// walks) and links the larger root under the smaller one by compare-and-swap, retrying
+// from the new roots when another edge linked first.  A label only ever points to a
+// smaller id, so a root is the smallest vertex id of its tree.  A flattening pass then
+// points every vertex at its root.  Passes repeat until one links nothing and flattens
+// nothing (the first pass normally does all the work; the second confirms it), so the
+// final label of a vertex is the smallest vertex id of its component, whatever order the
+// edges ran in.  The kept component is the one with the most triangles (ties: the
+// smaller label).  Both operations end in one order-preserving compaction driven by the
+// shared scan: vertex i keeps its order among the kept vertices, triangle t among the
+// kept triangles.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,7 +47,7 @@ PostLayout post_layout(int64_t nv, int64_t nt) {
   L.off_toff = o; o = al256p(o + 4 * (size_t)L.n);
   L.off_tiles = o; o = al256p(o + sizeof(uint4) * (size_t)mc_scan_tiles(L.n));
   L.off_total = o; o = al256p(o + sizeof(uint4));
-  L.off_misc = o; o = al256p(o + 16);  // best component key (u64), changed flag (u32)
+  L.off_misc = o; o = al256p(o + 16);  // best component key (u64), two changed flags (u32)
   L.bytes = o;
   return L;
 }
@@ -57,11 +62,55 @@ __global__ void k_pp_zero_u32(uint32_t* __restrict__ p, int64_t n) {
   if (i < n) p[i] = 0u;
 }
 
-__device__ __forceinline__ void hook(int32_t* label, int32_t a, int32_t b, uint32_t* changed) {
-  const int32_t la = label[a], lb = label[b];
-  if (la == lb) return;
-  const int32_t hi = max(la, lb), lo = min(la, lb);
-  if (atomicMin(label + hi, lo) > lo) *changed = 1u;
+// Labels change under other workgroups during a union-find pass.  The walks read them
+// through the caches (workgroup-scope relaxed accesses) and may see an older value: a
+// label only ever moves to an ancestor, so an older value is still an ancestor (or the
+// vertex itself while it was a root), and a compare-and-swap against an outdated root
+// fails and returns the current one.  Agent-scope loads, which skip the caches, made
+// the first pass over the 1.33 M-triangle torus mesh 1.7x slower (and a walk of the
+// 2.6 MB label array without the caches' locality, lanes striding over distant
+// triangles, 4.7x slower).
+__device__ __forceinline__ int32_t ld_label(const int32_t* label, int32_t v) {
+  return __hip_atomic_load(label + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void st_label(int32_t* label, int32_t v, int32_t x) {
+  __hip_atomic_store(label + v, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Root of v's tree, halving the path on the way: each visited vertex is pointed at its
+// grandparent, an ancestor, so no tree changes.  Every step goes to a smaller id.  Only
+// vertices that are no longer roots are written (a root is written by uf_unite's
+// compare-and-swap alone, and a vertex never becomes a root again).
+__device__ __forceinline__ int32_t uf_root(int32_t* label, int32_t v) {
+  int32_t p = ld_label(label, v);
+  while (p != v) {
+    const int32_t g = ld_label(label, p);
+    if (g == p) return p;
+    st_label(label, v, g);
+    v = g;
+    p = ld_label(label, v);
+  }
+  return v;
+}
+
+// Joins the trees of a and b; true when this call linked two roots.  Every failed
+// compare-and-swap lowers ra + rb (the failing root is replaced by its new root, a
+// smaller id), so the loop ends.
+__device__ __forceinline__ bool uf_unite(int32_t* label, int32_t a, int32_t b) {
+  int32_t ra = uf_root(label, a), rb = uf_root(label, b);
+  while (ra != rb) {
+    if (ra < rb) {
+      const int32_t t = ra;
+      ra = rb;
+      rb = t;
+    }
+    const int32_t old = atomicCAS(label + ra, ra, rb);
+    if (old == ra) return true;
+    ra = uf_root(label, old);
+    rb = uf_root(label, rb);
+  }
+  return false;
 }
 
 // Triangles naming a vertex outside [0, nv) are skipped by every kernel (and never kept).
@@ -69,43 +118,94 @@ __device__ __forceinline__ bool tri_ok(const uint32_t* idx, int64_t t, int64_t n
   return idx[3 * t] < nv && idx[3 * t + 1] < nv && idx[3 * t + 2] < nv;
 }
 
-__global__ void k_pp_hook(const uint32_t* __restrict__ idx, int64_t nt, int64_t nv, int32_t* label,
-                          uint32_t* changed) {
+// Edges a-b and b-c join all three vertices of a triangle.
+__global__ void k_pp_unite(const uint32_t* __restrict__ idx, int64_t nt, int64_t nv, int32_t* label,
+                           uint32_t* changed) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= nt || !tri_ok(idx, t, nv)) return;
   const int32_t a = (int32_t)idx[3 * t], b = (int32_t)idx[3 * t + 1], c = (int32_t)idx[3 * t + 2];
-  hook(label, a, b, changed);
-  hook(label, b, c, changed);
+  const bool ab = uf_unite(label, a, b);
+  const bool bc = uf_unite(label, b, c);
+  if (ab || bc) *changed = 1u;
 }
 
-__global__ void k_pp_jump(int32_t* label, int64_t nv, uint32_t* changed) {
+// Points every vertex at its root (no links run meanwhile, so the roots stay put).
+__global__ void k_pp_flatten(int32_t* label, int64_t nv, uint32_t* changed) {
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= nv) return;
-  const int32_t l = label[v], ll = label[l];
-  if (ll != l) {
-    label[v] = ll;
+  const int32_t l = ld_label(label, (int32_t)v);
+  int32_t r = l, p;
+  while ((p = ld_label(label, r)) != r) r = p;
+  if (r != l) {
+    st_label(label, (int32_t)v, r);
     *changed = 1u;
   }
 }
 
 // Triangles per component label.  A mesh is mostly one component, so nearly every
-// triangle adds to the same counter; one atomic per triangle would serialise on that
-// address.  Each wave instead adds once per distinct label among its lanes: the lowest
-// pending lane's label is broadcast, the lanes holding it are counted by ballot, and
-// its owner adds the count (one round for a wave inside one component).
-__global__ void k_pp_count(const uint32_t* __restrict__ idx, int64_t nt, int64_t nv,
-                           const int32_t* __restrict__ label, uint32_t* __restrict__ cnt) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  int32_t l = -1;
-  if (t < nt && tri_ok(idx, t, nv)) l = label[idx[3 * t]];
-  const int lane = (int)(threadIdx.x & 63);
-  uint64_t todo = __ballot(l >= 0);
-  while (todo) {
-    const int lead = __builtin_ctzll(todo);
-    const int32_t key = __shfl(l, lead);
-    const uint64_t same = __ballot(l == key) & todo;
-    if (lane == lead) atomicAdd(cnt + key, (uint32_t)__popcll(same));
-    todo &= ~same;
+// triangle adds to the same counter, and atomics on one address serialise (one per
+// wave of 64 triangles took 238 us for 1.33 M triangles).  A wave counts 1024
+// consecutive triangles as runs of equal labels: the distinct labels of each 64 are
+// found by ballot (the lowest pending lane's label is broadcast, the lanes holding it
+// counted), a run is added to its counter when the label changes, and the four waves of
+// a workgroup merge their last runs, so a one-component mesh adds once per workgroup.
+constexpr int kCountRun = 16;  // triangles per lane
+constexpr int64_t kCountTris = 4 * 64 * kCountRun;  // triangles per workgroup
+
+__global__ void __launch_bounds__(256) k_pp_count(const uint32_t* __restrict__ idx, int64_t nt, int64_t nv,
+                                                  const int32_t* __restrict__ label,
+                                                  uint32_t* __restrict__ cnt) {
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+  const int64_t t0 = ((int64_t)blockIdx.x * 4 + wave) * (64 * kCountRun) + lane;
+  int32_t l[kCountRun];
+#pragma unroll
+  for (int j = 0; j < kCountRun; j++) {
+    const int64_t t = t0 + 64 * j;
+    l[j] = (t < nt && tri_ok(idx, t, nv)) ? (int32_t)idx[3 * t] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kCountRun; j++)
+    if (l[j] >= 0) l[j] = label[l[j]];
+  int32_t cur = -1;  // the open run: label and count (wave-uniform)
+  uint32_t run = 0;
+#pragma unroll
+  for (int j = 0; j < kCountRun; j++) {
+    uint64_t todo = __ballot(l[j] >= 0);
+    while (todo) {
+      const int lead = __builtin_ctzll(todo);
+      const int32_t key = __shfl(l[j], lead);
+      const uint64_t same = __ballot(l[j] == key) & todo;
+      const uint32_t n = (uint32_t)__popcll(same);
+      if (key == cur) {
+        run += n;
+      } else {
+        if (run && lane == 0) atomicAdd(cnt + cur, run);
+        cur = key;
+        run = n;
+      }
+      todo &= ~same;
+    }
+  }
+  __shared__ int32_t s_key[4];
+  __shared__ uint32_t s_run[4];
+  if (lane == 0) {
+    s_key[wave] = cur;
+    s_run[wave] = run;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t k = s_key[0];
+    uint32_t r = s_run[0];
+    for (int w = 1; w < 4; w++) {
+      if (s_key[w] == k) {
+        r += s_run[w];
+      } else {
+        if (r) atomicAdd(cnt + k, r);
+        k = s_key[w];
+        r = s_run[w];
+      }
+    }
+    if (r) atomicAdd(cnt + k, r);
   }
 }
 
@@ -165,10 +265,15 @@ __global__ void k_pp_flags_valid(const uint32_t* __restrict__ idx, int64_t nv, i
 __global__ void k_pp_compact(const float* __restrict__ pos, const float* __restrict__ nrm,
                              const uint32_t* __restrict__ idx, int64_t nv, int64_t nt,
                              const uint32_t* __restrict__ counts, const uint32_t* __restrict__ voff,
-                             const uint32_t* __restrict__ toff, int64_t n, float* __restrict__ opos,
-                             float* __restrict__ onrm, uint32_t* __restrict__ oidx) {
+                             const uint32_t* __restrict__ toff, int64_t n, const uint4* __restrict__ total,
+                             int64_t cap_v, int64_t cap_t, float* __restrict__ opos, float* __restrict__ onrm,
+                             uint32_t* __restrict__ oidx) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
+  // The kept counts are checked against the output here, before the host reads them
+  // back: an output too small (or missing) for them is left untouched.
+  const uint4 tot = *total;
+  if (tot.x > cap_v || tot.y > cap_t || (tot.x && !opos) || (tot.y && !oidx)) return;
   const uint32_t c = counts[i];
   if (c & 1u) {
     const uint32_t o = voff[i];
@@ -200,7 +305,7 @@ int check_mesh(const upf_mesh_buffers* in, const upf_mesh_buffers* out) {
   return UPF_OK;
 }
 
-// scan the keep flags, read the totals back, check capacity and compact
+// scan the keep flags, compact (the kernel checks capacity), read the totals back
 int finish(const PostLayout& L, char* w, const upf_mesh_buffers* in, upf_mesh_buffers* out, hipStream_t s) {
   uint32_t* counts = reinterpret_cast<uint32_t*>(w + L.off_counts);
   uint32_t* voff = reinterpret_cast<uint32_t*>(w + L.off_voff);
@@ -209,6 +314,11 @@ int finish(const PostLayout& L, char* w, const upf_mesh_buffers* in, upf_mesh_bu
   int st = mc_scan_launch(counts, L.n, reinterpret_cast<uint4*>(w + L.off_tiles), total, voff, nullptr, toff, s,
                           kScanMc);
   if (st) return st;
+  if (L.n > 0)
+    hipLaunchKernelGGL(k_pp_compact, blocks_for(L.n), dim3(256), 0, s, in->positions, in->normals, in->indices,
+                       L.nv, L.nt, counts, voff, toff, L.n, total, out->cap_vertices, out->cap_triangles,
+                       out->positions, out->normals, out->indices);
+  UPF_LAUNCH_CHECK();
   uint4 h;
   UPF_HIP_CHECK(hipMemcpyAsync(&h, total, sizeof(uint4), hipMemcpyDeviceToHost, s));
   UPF_HIP_CHECK(hipStreamSynchronize(s));
@@ -216,10 +326,6 @@ int finish(const PostLayout& L, char* w, const upf_mesh_buffers* in, upf_mesh_bu
   out->n_triangles = h.y;
   if (out->n_vertices > out->cap_vertices || out->n_triangles > out->cap_triangles) return UPF_ERR_CAPACITY;
   if ((out->n_vertices && !out->positions) || (out->n_triangles && !out->indices)) return UPF_ERR_NULL;
-  if (L.n > 0 && (h.x || h.y))
-    hipLaunchKernelGGL(k_pp_compact, blocks_for(L.n), dim3(256), 0, s, in->positions, in->normals, in->indices,
-                       L.nv, L.nt, counts, voff, toff, L.n, out->positions, out->normals, out->indices);
-  UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
 
@@ -247,30 +353,35 @@ int upf_mesh_remove_disconnected(const upf_mesh_buffers* in, upf_mesh_buffers* o
   uint32_t* cnt = reinterpret_cast<uint32_t*>(w + L.off_cnt);
   uint32_t* misc = reinterpret_cast<uint32_t*>(w + L.off_misc);
   unsigned long long* best = reinterpret_cast<unsigned long long*>(misc);
-  uint32_t* changed = misc + 2;
+  uint32_t* changed = misc + 2;  // two flags
   const int64_t nv = L.nv, nt = L.nt;
   if (nt == 0) {  // no triangles: nothing is kept
     out->n_vertices = out->n_triangles = 0;
     return UPF_OK;
   }
   hipLaunchKernelGGL(k_pp_iota, blocks_for(nv), dim3(256), 0, s, label, nv);
-  // hook + jump until a pass changes nothing; every pass lowers a label or stops, so the
-  // loop ends within nv passes (in practice a few tens)
-  for (int64_t pass = 0;; pass++) {
-    if (pass > nv + 1) return UPF_ERR_HIP;
-    hipLaunchKernelGGL(k_pp_zero_u32, dim3(1), dim3(256), 0, s, changed, (int64_t)1);
-    hipLaunchKernelGGL(k_pp_hook, blocks_for(nt), dim3(256), 0, s, in->indices, nt, nv, label, changed);
-    hipLaunchKernelGGL(k_pp_jump, blocks_for(nv), dim3(256), 0, s, label, nv, changed);
-    hipLaunchKernelGGL(k_pp_jump, blocks_for(nv), dim3(256), 0, s, label, nv, changed);
+  // Two union-find passes per read-back, each with its own changed flag; the loop ends
+  // when the second pass of a round changed nothing.  A round that goes on linked two
+  // roots in its second pass (after a pass without links the flattening leaves nothing
+  // to change), so there are at most nv rounds; the first pass normally joins every
+  // component and the second confirms it.
+  for (int64_t round = 0;; round++) {
+    if (round > nv + 1) return UPF_ERR_HIP;
+    hipLaunchKernelGGL(k_pp_zero_u32, dim3(1), dim3(256), 0, s, changed, (int64_t)2);
+    for (int p = 0; p < 2; p++) {
+      hipLaunchKernelGGL(k_pp_unite, blocks_for(nt), dim3(256), 0, s, in->indices, nt, nv, label, changed + p);
+      hipLaunchKernelGGL(k_pp_flatten, blocks_for(nv), dim3(256), 0, s, label, nv, changed + p);
+    }
     UPF_LAUNCH_CHECK();
     uint32_t h = 0;
-    UPF_HIP_CHECK(hipMemcpyAsync(&h, changed, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    UPF_HIP_CHECK(hipMemcpyAsync(&h, changed + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     UPF_HIP_CHECK(hipStreamSynchronize(s));
     if (!h) break;
   }
   hipLaunchKernelGGL(k_pp_zero_u32, blocks_for(nv), dim3(256), 0, s, cnt, nv);
   hipLaunchKernelGGL(k_pp_zero_u32, dim3(1), dim3(256), 0, s, misc, (int64_t)2);
-  hipLaunchKernelGGL(k_pp_count, blocks_for(nt), dim3(256), 0, s, in->indices, nt, nv, label, cnt);
+  hipLaunchKernelGGL(k_pp_count, dim3((unsigned)((nt + kCountTris - 1) / kCountTris)), dim3(256), 0, s,
+                     in->indices, nt, nv, label, cnt);
   hipLaunchKernelGGL(k_pp_best, blocks_for(nv), dim3(256), 0, s, cnt, nv, best);
   hipLaunchKernelGGL(k_pp_flags_component, blocks_for(L.n), dim3(256), 0, s, in->indices, nv, nt, label, best,
                      reinterpret_cast<uint32_t*>(w + L.off_counts), L.n);
