@@ -3,10 +3,11 @@
 // water-mesh vertex mask with its two-ring expansion (sources/meshGeneration.cpp:62-95).
 //
 // Connected components by a lock-free union-find over the triangles' edges (the
-// compare-and-swap linking of ECL-CC, Jaiganesh and Burtscher, HPDC 2018): every vertex
-// starts as its own root; an edge finds the roots of its two ends (halving the paths it
-// walks) and links the larger root under the smaller one by compare-and-swap, retrying
-// from the new roots when another edge linked first.  A label only ever points to a
+// initialisation and compare-and-swap linking of ECL-CC, Jaiganesh and Burtscher, HPDC
+// 2018): every vertex starts pointing at the smallest vertex of its triangles; an edge
+// finds the roots of its two ends (halving the paths it walks) and links the larger
+// root under the smaller one by compare-and-swap, retrying from the new roots when
+// another edge linked first.  A label only ever points to a
 // smaller id, so a root is the smallest vertex id of its tree.  A flattening pass then
 // points every vertex at its root.  Passes repeat until one links nothing and flattens
 // nothing (the first pass normally does all the work; the second confirms it), so the
@@ -78,10 +79,26 @@ __device__ __forceinline__ void st_label(int32_t* label, int32_t v, int32_t x) {
   __hip_atomic_store(label + v, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// What the atomics wrote (labels, triangle counts, the best key) is read at agent scope
+// wherever a stale copy would change the result: cached loads, even in a later kernel,
+// have returned a label from before a compare-and-swap of an earlier pass, which left
+// a vertex pointing at a non-root after a flattening pass that saw nothing to change
+// (1 run in 20 on a 1 M-vertex soup).  The walks of k_pp_unite tolerate such values;
+// the flattening, which decides that the labels are final, and the kernels after it do
+// not.
+template <typename T>
+__device__ __forceinline__ T ld_fresh(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_fresh(int32_t* p, int32_t x) {
+  __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Root of v's tree, halving the path on the way: each visited vertex is pointed at its
 // grandparent, an ancestor, so no tree changes.  Every step goes to a smaller id.  Only
-// vertices that are no longer roots are written (a root is written by uf_unite's
-// compare-and-swap alone, and a vertex never becomes a root again).
+// vertices that are no longer roots are written (during a pass a root is written by
+// uf_unite's compare-and-swap alone, and a vertex never becomes a root again).
 __device__ __forceinline__ int32_t uf_root(int32_t* label, int32_t v) {
   int32_t p = ld_label(label, v);
   while (p != v) {
@@ -96,9 +113,21 @@ __device__ __forceinline__ int32_t uf_root(int32_t* label, int32_t v) {
 
 // Joins the trees of a and b; true when this call linked two roots.  Every failed
 // compare-and-swap lowers ra + rb (the failing root is replaced by its new root, a
-// smaller id), so the loop ends.
+// smaller id), so the loop ends.  Neighbouring lanes take neighbouring triangles, which
+// share vertices and so often the same two roots: a lane whose previous lane holds the
+// same pair leaves the link to it (that lane's loop ends only once the two trees are
+// one).  60% of the compare-and-swaps of the first pass over a marching-cubes mesh
+// failed before.  Must be called by the whole wave's active lanes.
 __device__ __forceinline__ bool uf_unite(int32_t* label, int32_t a, int32_t b) {
   int32_t ra = uf_root(label, a), rb = uf_root(label, b);
+  {
+    const int32_t hi = max(ra, rb), lo = min(ra, rb);
+    const int lane = (int)(threadIdx.x & 63);
+    const int prev = (lane + 63) & 63;
+    const int32_t phi = __shfl(hi, prev), plo = __shfl(lo, prev);
+    const uint64_t act = __ballot(1);
+    if (lane > 0 && ((act >> prev) & 1ull) && phi == hi && plo == lo) return false;
+  }
   while (ra != rb) {
     if (ra < rb) {
       const int32_t t = ra;
@@ -129,15 +158,32 @@ __global__ void k_pp_unite(const uint32_t* __restrict__ idx, int64_t nt, int64_t
   if (ab || bc) *changed = 1u;
 }
 
+// Before the first union-find pass, every vertex points at the smallest vertex of the
+// triangles it belongs to (ECL-CC's initialisation, by atomicMin): a forest, since every
+// label is a smaller id, whose stars already join most of a marching-cubes mesh's
+// neighbourhoods.  With it and uf_unite's shared links, a call over the 1.33 M-triangle
+// torus mesh takes 0.52-0.53 ms instead of 0.65 (a soup of random triangles over random
+// vertex ids, whose neighbouring triangles share no vertices, got slower: 1.5 -> 1.7-1.8
+// ms).
+__global__ void k_pp_init_min(const uint32_t* __restrict__ idx, int64_t nt, int64_t nv, int32_t* label) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nt || !tri_ok(idx, t, nv)) return;
+  const int32_t a = (int32_t)idx[3 * t], b = (int32_t)idx[3 * t + 1], c = (int32_t)idx[3 * t + 2];
+  const int32_t m = min(a, min(b, c));
+  if (a != m) atomicMin(label + a, m);
+  if (b != m) atomicMin(label + b, m);
+  if (c != m) atomicMin(label + c, m);
+}
+
 // Points every vertex at its root (no links run meanwhile, so the roots stay put).
 __global__ void k_pp_flatten(int32_t* label, int64_t nv, uint32_t* changed) {
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= nv) return;
-  const int32_t l = ld_label(label, (int32_t)v);
+  const int32_t l = ld_fresh(label + v);
   int32_t r = l, p;
-  while ((p = ld_label(label, r)) != r) r = p;
+  while ((p = ld_fresh(label + r)) != r) r = p;
   if (r != l) {
-    st_label(label, (int32_t)v, r);
+    st_fresh(label + v, r);
     *changed = 1u;
   }
 }
@@ -165,7 +211,7 @@ __global__ void __launch_bounds__(256) k_pp_count(const uint32_t* __restrict__ i
   }
 #pragma unroll
   for (int j = 0; j < kCountRun; j++)
-    if (l[j] >= 0) l[j] = label[l[j]];
+    if (l[j] >= 0) l[j] = ld_fresh(label + l[j]);
   int32_t cur = -1;  // the open run: label and count (wave-uniform)
   uint32_t run = 0;
 #pragma unroll
@@ -211,8 +257,9 @@ __global__ void __launch_bounds__(256) k_pp_count(const uint32_t* __restrict__ i
 
 __global__ void k_pp_best(const uint32_t* __restrict__ cnt, int64_t nv, unsigned long long* best) {
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (v >= nv || cnt[v] == 0u) return;
-  atomicMax(best, ((unsigned long long)cnt[v] << 32) | (unsigned long long)(0xFFFFFFFFu - (uint32_t)v));
+  if (v >= nv) return;
+  const uint32_t c = ld_fresh(cnt + v);
+  if (c) atomicMax(best, ((unsigned long long)c << 32) | (unsigned long long)(0xFFFFFFFFu - (uint32_t)v));
 }
 
 // keep flags in the kScanMc count format: vertex bit 0, triangle bit 8
@@ -221,9 +268,9 @@ __global__ void k_pp_flags_component(const uint32_t* __restrict__ idx, int64_t n
                                      uint32_t* __restrict__ counts, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int32_t keep = (int32_t)(0xFFFFFFFFu - (uint32_t)(*best & 0xFFFFFFFFull));
-  const uint32_t kv = (i < nv && label[i] == keep) ? 1u : 0u;
-  const uint32_t kt = (i < nt && tri_ok(idx, i, nv) && label[idx[3 * i]] == keep) ? 1u : 0u;
+  const int32_t keep = (int32_t)(0xFFFFFFFFu - (uint32_t)(ld_fresh(best) & 0xFFFFFFFFull));
+  const uint32_t kv = (i < nv && ld_fresh(label + i) == keep) ? 1u : 0u;
+  const uint32_t kt = (i < nt && tri_ok(idx, i, nv) && ld_fresh(label + idx[3 * i]) == keep) ? 1u : 0u;
   counts[i] = kv | (kt << 8);
 }
 
@@ -360,6 +407,7 @@ int upf_mesh_remove_disconnected(const upf_mesh_buffers* in, upf_mesh_buffers* o
     return UPF_OK;
   }
   hipLaunchKernelGGL(k_pp_iota, blocks_for(nv), dim3(256), 0, s, label, nv);
+  hipLaunchKernelGGL(k_pp_init_min, blocks_for(nt), dim3(256), 0, s, in->indices, nt, nv, label);
   // Two union-find passes per read-back, each with its own changed flag; the loop ends
   // when the second pass of a round changed nothing.  A round that goes on linked two
   // roots in its second pass (after a pass without links the flattening leaves nothing
