@@ -236,6 +236,20 @@ def test_dilation_rounds_and_shapes_bit_exact(upf, oracle, w, h, c):
         np.testing.assert_array_equal(got[~np.isnan(got)], ref[~np.isnan(ref)])
 
 
+def test_dilation_blocky_holes_bit_exact(upf, oracle):
+    """The texture-post workload's pattern, 30% of the image NaN in 8 x 8 blocks, on a
+    1024 x 768 three-channel image, by 7 rounds (one launch) and 10 (two), bit for bit."""
+    rng = np.random.default_rng(21)
+    img = rng.random((768, 1024, 3)).astype(np.float32)
+    holes = np.repeat(np.repeat(rng.random((96, 128)) < 0.3, 8, 0), 8, 1)
+    img[holes] = np.nan
+    for rounds in (7, 10):
+        ref = oracle.image_dilate(img.copy(), rounds)
+        got = upfm.image_dilate(torch_from(img), rounds).cpu().numpy()
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+        np.testing.assert_array_equal(got[~np.isnan(got)], ref[~np.isnan(ref)])
+
+
 def test_to_u8_lengths_and_alignment_bit_exact(upf, oracle):
     """U8 conversion four values per thread with a scalar tail, and the scalar path for
     an input that is not 16-byte aligned, against the oracle: NaN, infinities, values
