@@ -275,7 +275,7 @@ def torch_from(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize("w,h", [(1, 1), (1, 7), (9, 1), (65, 5), (1024, 768), (333, 200)])
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 7), (9, 1), (65, 5), (1024, 768), (333, 200), (4, 3), (8, 2), (4, 1)])
 def test_height_to_normal_bit_exact(upf, oracle, w, h):
     """Sobel height-to-normal of U8 height maps, GPU against the oracle bit for bit,
     including the one-texel and one-row edge cases of the clamp-to-edge rule."""
@@ -284,6 +284,30 @@ def test_height_to_normal_bit_exact(upf, oracle, w, h):
     for strength in (1.0, 0.25, 3.0):
         got = upfm.image_height_to_normal(torch_from(hm), strength).cpu().numpy()
         np.testing.assert_array_equal(got, oracle.image_height_to_normal(hm, strength))
+
+
+def test_normal_and_pbr_unaligned_and_short(upf, oracle):
+    """Both conversions four texels per lane where the buffers allow 4-byte accesses and
+    one per lane otherwise: views starting 1-3 bytes into a buffer, and PBR lengths that
+    leave 1-3 texels after the groups of four, against the oracle."""
+    import torch
+    rng = np.random.default_rng(17)
+    hm = rng.integers(0, 256, size=(24 * 16 + 3,), dtype=np.uint8)
+    base = torch_from(hm)
+    for off in (1, 2, 3):
+        v = base[off:off + 24 * 16].view(16, 24)
+        assert v.data_ptr() % 4 != 0
+        np.testing.assert_array_equal(upfm.image_height_to_normal(v, 1.0).cpu().numpy(),
+                                      oracle.image_height_to_normal(hm[off:off + 24 * 16].reshape(16, 24), 1.0))
+    sp = rng.integers(0, 256, size=(40, 2), dtype=np.uint8)
+    t = torch_from(sp.reshape(-1))
+    for n in range(1, 10):
+        np.testing.assert_array_equal(upfm.image_special_to_gltf_pbr(t[:2 * n].view(n, 2)).cpu().numpy(),
+                                      oracle.image_special_to_pbr(sp[:n]))
+    for off in (1, 2, 3):  # texel-aligned views whose start is not 8-byte aligned
+        v = t[2 * off:2 * off + 2 * 30].view(30, 2)
+        np.testing.assert_array_equal(upfm.image_special_to_gltf_pbr(v).cpu().numpy(),
+                                      oracle.image_special_to_pbr(sp[off:off + 30]))
 
 
 def test_special_to_pbr_and_export_maps(upf, oracle):

@@ -493,12 +493,28 @@ __global__ __launch_bounds__(256) void k_copy_f32(const float* __restrict__ in, 
 
 __device__ __forceinline__ uint8_t unit_to_u8(float n) { return (uint8_t)((n * 0.5f + 0.5f) * 255.0f + 0.5f); }
 
-// Sobel height-to-normal, one texel per lane over a 64 x 4 tile (rules at
-// upf_image_height_to_normal).  Neighbour bytes come through L1/L2: 1 B read and 3 B
-// written per texel from HBM.
+// Sobel normal of one texel from its 3 x 3 heights (rules at upf_image_height_to_normal),
+// shared by both kernels below so that they round alike.
+__device__ __forceinline__ void sobel_normal(uint32_t a8, uint32_t b8, uint32_t c8, uint32_t d8, uint32_t f8,
+                                             uint32_t g8, uint32_t h8, uint32_t i8, float strength, uint8_t* o) {
+  const float k = 1.0f / 255.0f;
+  const float a = (float)a8 * k, b = (float)b8 * k, c = (float)c8 * k;
+  const float d = (float)d8 * k, f = (float)f8 * k;
+  const float g = (float)g8 * k, hh = (float)h8 * k, i = (float)i8 * k;
+  const float gx = (c + 2.0f * f + i) - (a + 2.0f * d + g);
+  const float gy = (g + 2.0f * hh + i) - (a + 2.0f * b + c);
+  const float vx = -strength * gx, vy = -strength * gy;
+  const float inv = 1.0f / sqrtf(vx * vx + vy * vy + 1.0f);
+  o[0] = unit_to_u8(vx * inv);
+  o[1] = unit_to_u8(vy * inv);
+  o[2] = unit_to_u8(inv);
+}
+
+// One texel per lane over a 64 x 4 tile, texels [x_first, w) of each row.  Neighbour
+// bytes come through L1/L2: 1 B read and 3 B written per texel from HBM.
 __global__ __launch_bounds__(256) void k_height_to_normal(const uint8_t* __restrict__ in, int w, int h, float strength,
-                                                          uint8_t* __restrict__ out) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+                                                          uint8_t* __restrict__ out, int x_first) {
+  const int x = x_first + blockIdx.x * 64 + (threadIdx.x & 63);
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (x >= w || y >= h) return;
   const int xm = x > 0 ? x - 1 : 0, xp = x + 1 < w ? x + 1 : w - 1;
@@ -506,28 +522,71 @@ __global__ __launch_bounds__(256) void k_height_to_normal(const uint8_t* __restr
   const uint8_t* rm = in + (int64_t)ym * w;
   const uint8_t* r0 = in + (int64_t)y * w;
   const uint8_t* rp = in + (int64_t)yp * w;
-  const float k = 1.0f / 255.0f;
-  const float a = rm[xm] * k, b = rm[x] * k, c = rm[xp] * k;
-  const float d = r0[xm] * k, f = r0[xp] * k;
-  const float g = rp[xm] * k, hh = rp[x] * k, i = rp[xp] * k;
-  const float gx = (c + 2.0f * f + i) - (a + 2.0f * d + g);
-  const float gy = (g + 2.0f * hh + i) - (a + 2.0f * b + c);
-  const float vx = -strength * gx, vy = -strength * gy;
-  const float inv = 1.0f / sqrtf(vx * vx + vy * vy + 1.0f);
-  uint8_t* o = out + ((int64_t)y * w + x) * 3;
-  o[0] = unit_to_u8(vx * inv);
-  o[1] = unit_to_u8(vy * inv);
-  o[2] = unit_to_u8(inv);
+  uint8_t o[3];
+  sobel_normal(rm[xm], rm[x], rm[xp], r0[xm], r0[xp], rp[xm], rp[x], rp[xp], strength, o);
+  uint8_t* q = out + ((int64_t)y * w + x) * 3;
+  q[0] = o[0];
+  q[1] = o[1];
+  q[2] = o[2];
 }
 
-// (roughness, metallic) -> (255, roughness, metallic), one texel per lane.
-__global__ __launch_bounds__(256) void k_special_to_pbr(const uint8_t* __restrict__ in, int64_t n,
+// Four texels per lane (x = 4X .. 4X + 3, all inside the row): a 4-byte read per row plus
+// the two bytes beside it, and the 12 output bytes as three 4-byte stores (the rows and
+// the output 4-byte aligned).  With one byte store per channel the 4096^2 map took 45 us.
+__global__ __launch_bounds__(256) void k_height_to_normal_x4(const uint8_t* __restrict__ in, int w, int h, float strength,
+                                                             uint32_t* __restrict__ out) {
+  const int X = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (X >= (w >> 2) || y >= h) return;
+  const int x0 = 4 * X;
+  const int ym = y > 0 ? y - 1 : 0, yp = y + 1 < h ? y + 1 : h - 1;
+  const int xl = x0 > 0 ? x0 - 1 : 0, xr = x0 + 4 < w ? x0 + 4 : w - 1;
+  uint32_t row[3][6];  // bytes x0 - 1 .. x0 + 4 of rows ym, y, yp (clamped)
+  const int rows[3] = {ym, y, yp};
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    const uint8_t* p = in + (int64_t)rows[r] * w;
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(p + x0);
+    row[r][0] = p[xl];
+#pragma unroll
+    for (int j = 0; j < 4; j++) row[r][1 + j] = (v >> (8 * j)) & 0xFFu;
+    row[r][5] = p[xr];
+  }
+  uint8_t o[12];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    sobel_normal(row[0][j], row[0][j + 1], row[0][j + 2], row[1][j], row[1][j + 2], row[2][j], row[2][j + 1],
+                 row[2][j + 2], strength, o + 3 * j);
+  uint32_t* q = out + ((int64_t)y * w + x0) * 3 / 4;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+    q[k] = (uint32_t)o[4 * k] | ((uint32_t)o[4 * k + 1] << 8) | ((uint32_t)o[4 * k + 2] << 16) |
+           ((uint32_t)o[4 * k + 3] << 24);
+}
+
+// (roughness, metallic) -> (255, roughness, metallic), texels [first, n), one per lane.
+__global__ __launch_bounds__(256) void k_special_to_pbr(const uint8_t* __restrict__ in, int64_t first, int64_t n,
                                                         uint8_t* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t i = first + (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   out[3 * i] = 255;
   out[3 * i + 1] = in[2 * i];
   out[3 * i + 2] = in[2 * i + 1];
+}
+
+// Four texels per lane: one 8-byte read and three 4-byte stores (in 8-byte and out
+// 4-byte aligned; n4 groups of four).
+__global__ __launch_bounds__(256) void k_special_to_pbr_x4(const uint2* __restrict__ in, int64_t n4,
+                                                           uint32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n4) return;
+  const uint2 v = in[i];  // r0 m0 r1 m1 | r2 m2 r3 m3
+  const uint32_t r0 = v.x & 0xFFu, m0 = (v.x >> 8) & 0xFFu, r1 = (v.x >> 16) & 0xFFu, m1 = v.x >> 24;
+  const uint32_t r2 = v.y & 0xFFu, m2 = (v.y >> 8) & 0xFFu, r3 = (v.y >> 16) & 0xFFu, m3 = v.y >> 24;
+  // bytes: 255 r0 m0 255 | r1 m1 255 r2 | m2 255 r3 m3
+  out[3 * i] = 0xFFu | (r0 << 8) | (m0 << 16) | (0xFFu << 24);
+  out[3 * i + 1] = r1 | (m1 << 8) | (0xFFu << 16) | (r2 << 24);
+  out[3 * i + 2] = m2 | (0xFFu << 8) | (r3 << 16) | (m3 << 24);
 }
 
 __device__ __forceinline__ uint32_t to_u8(float v) {
@@ -739,8 +798,19 @@ int upf_image_height_to_normal(const uint8_t* d_height, int32_t w, int32_t h, fl
   if (w < 1 || h < 1) return UPF_ERR_BAD_GRID;
   if (!(strength == strength) || isinf(strength)) return UPF_ERR_INVALID_VALUE;
   if (!d_height || !d_out) return UPF_ERR_NULL;
-  hipLaunchKernelGGL(k_height_to_normal, dim3((w + 63) / 64, (h + 3) / 4), dim3(256), 0, as_stream(stream), d_height,
-                     w, h, strength, d_out);
+  hipStream_t s = as_stream(stream);
+  // four texels per lane where the rows and the output allow 4-byte accesses (w a
+  // multiple of 4, both buffers 4-byte aligned), one per lane otherwise
+  int x_first = 0;
+  if (w >= 4 && w % 4 == 0 && reinterpret_cast<uintptr_t>(d_height) % 4 == 0 &&
+      reinterpret_cast<uintptr_t>(d_out) % 4 == 0) {
+    hipLaunchKernelGGL(k_height_to_normal_x4, dim3((w / 4 + 63) / 64, (h + 3) / 4), dim3(256), 0, s, d_height, w, h,
+                       strength, reinterpret_cast<uint32_t*>(d_out));
+    x_first = w;
+  }
+  if (x_first < w)
+    hipLaunchKernelGGL(k_height_to_normal, dim3((w - x_first + 63) / 64, (h + 3) / 4), dim3(256), 0, s, d_height, w,
+                       h, strength, d_out, x_first);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
@@ -749,8 +819,18 @@ int upf_image_special_to_gltf_pbr(const uint8_t* d_special, int64_t n, uint8_t* 
   if (n < 0) return UPF_ERR_BAD_GRID;
   if (n == 0) return UPF_OK;
   if (!d_special || !d_out) return UPF_ERR_NULL;
-  hipLaunchKernelGGL(k_special_to_pbr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, as_stream(stream), d_special,
-                     n, d_out);
+  hipStream_t s = as_stream(stream);
+  int64_t first = 0;
+  if (reinterpret_cast<uintptr_t>(d_special) % 8 == 0 && reinterpret_cast<uintptr_t>(d_out) % 4 == 0) {
+    const int64_t n4 = n / 4;
+    if (n4)
+      hipLaunchKernelGGL(k_special_to_pbr_x4, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, s,
+                         reinterpret_cast<const uint2*>(d_special), n4, reinterpret_cast<uint32_t*>(d_out));
+    first = 4 * n4;
+  }
+  if (first < n)
+    hipLaunchKernelGGL(k_special_to_pbr, dim3((unsigned)((n - first + 255) / 256)), dim3(256), 0, s, d_special, first,
+                       n, d_out);
   UPF_LAUNCH_CHECK();
   return UPF_OK;
 }
