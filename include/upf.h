@@ -437,7 +437,8 @@ int upf_mc_case_codes(const float* d_field, const upf_grid* grid, float iso,
  * choices.  Both write an order-preserving compaction of `in` into `out` (kept vertex i
  * keeps its order among the kept vertices, kept triangle t among the kept triangles;
  * normals copied when both normal pointers are set), return UPF_ERR_CAPACITY with the
- * counts in out when out is too small, and synchronise the stream.  meshFlipNormals
+ * counts in out when out is too small (its buffers then left untouched), and synchronise
+ * the stream.  meshFlipNormals
  * (:42) has no counterpart: upf_mc_emit already writes outward normals and CCW-outward
  * winding.  Triangles naming a vertex outside [0, in->n_vertices) are dropped. */
 size_t upf_mesh_post_workspace_bytes(int64_t n_vertices, int64_t n_triangles);
